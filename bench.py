@@ -1,0 +1,377 @@
+#!/usr/bin/env python3
+"""Benchmark of the HipSimulator sweep path on one MI355X.
+
+Default workload (the configuration the headline GLUPS figure is quoted on):
+Conway's Game of Life, 16384 x 16384 uint8 cells, Bernoulli(0.3) from seed,
+constant dead edge (Cube topology), 1000 generations.  One benchmark step is
+one whole run: restore the pristine uint8 input (device copy), pack, 1000
+generations, unpack -- uint8 grid in HBM in, uint8 grid in HBM out.
+
+Other workloads (`--workload jacobi|lbm`) measure the Jacobi3D FP64 and the
+D3Q19 FP32 families on their configured shapes.
+
+Prints ONE JSON line (see the keys below).  GPU time comes from HIP events
+recorded on the stream the kernels run on; the dominant kernel's average
+launch duration is measured the same way over a block of launches.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+
+from libgeodecomp_amd import lgdhip, synthetic  # noqa: E402
+
+METRIC = "GLUPS + % HBM roofline; learned vs plain Jacobi time to 1e-6 residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def timed(stream, fn, reps=1):
+    """Run fn() `reps` times between two events on `stream`; return ms."""
+    e0, e1 = lgdhip.Event(), lgdhip.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.sync()
+    return e0.elapsed_ms(e1)
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch of the dominant kernel from the committed rocprof
+    PMC summary (profiles/<name>.json), or None."""
+    p = REPO / "profiles" / f"{name}.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def cpu_threads() -> int:
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def load_oracle():
+    """The CPU oracle, timed as the CPU baseline (never as the product)."""
+    sys.path.insert(0, str(REPO / "tests"))
+    import oracle_lib  # noqa: E402
+
+    oracle_lib.load()
+    return oracle_lib
+
+
+# ----------------------------------------------------------------------
+# Game of Life 16384^2 x 1000
+# ----------------------------------------------------------------------
+class Life:
+    name = "life"
+    dtype = "u8"
+    bytes_per_lup = 2.0  # 1 B read + 1 B write of the uint8 cell format
+
+    def __init__(self, n=16384, generations=1000, seed=synthetic.DEFAULT_SEED, per_launch=16, variant=0):
+        self.n, self.gens, self.seed = n, generations, seed
+        self.tile = lgdhip.tile(steps_per_launch=per_launch, variant=variant)
+        self.per_launch = per_launch
+        self.stream = lgdhip.Stream()
+        self.g = lgdhip.make_grid(2, n, n, 1, halo=1, elem_bytes=1)
+        t0 = time.time()
+        host = synthetic.life_initial(n, n, 0.3, seed)
+        log(f"life input {n}x{n} generated in {time.time() - t0:.1f}s, density {host.mean():.4f}")
+        self.pristine = lgdhip.DeviceGrid(self.g, np.uint8)
+        self.pristine.fill_edge(0, self.stream.h)
+        self.pristine.load(host, self.stream.h)
+        self.work = lgdhip.DeviceGrid(self.g, np.uint8)
+        self.nbytes = lgdhip.grid_bytes(self.g)
+        self.ws_bytes = lgdhip.load().lgdhip_life2d_workspace_bytes(C.byref(self.g))
+        self.ws = lgdhip.DeviceBuffer(self.ws_bytes)
+        self.host = host
+
+    def config(self):
+        return {"workload": f"Conway Game of Life 2D uint8 {self.n}x{self.n}, Bernoulli(0.3), dead Cube edge, "
+                            f"{self.gens} generations", "grid": [self.n, self.n], "generations": self.gens,
+                "seed": self.seed, "generations_per_launch": self.per_launch, "parallelism": "single GPU"}
+
+    def lups_per_step(self):
+        return float(self.n) * self.n * self.gens
+
+    def step(self):
+        s = self.stream.h
+        lgdhip.call("lgdhip_memcpy_d2d", self.work.ptr, self.pristine.ptr, self.nbytes, s)
+        lgdhip.call("lgdhip_life2d_run", self.work.ptr, C.byref(self.g), 0, self.gens, self.ws.ptr,
+                    self.ws_bytes, C.byref(self.tile), s)
+
+    def dominant_kernel(self):
+        """(kernel name, avg ms per launch, algorithmic bytes per launch)."""
+        s = self.stream.h
+        half = self.ws_bytes // 2
+        pa, pb = self.ws.value, self.ws.value + half
+        lgdhip.call("lgdhip_life2d_pack", self.pristine.ptr, C.byref(self.g), C.c_void_p(pa), s)
+        launches = 64
+        gens = launches * self.per_launch
+        in_b = C.c_int()
+        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_life2d_packed_run", C.c_void_p(pa), C.c_void_p(pb),
+                                                    C.byref(self.g), 0, gens, C.byref(self.tile),
+                                                    C.byref(in_b), s))
+        per_launch_bytes = self.bytes_per_lup * self.n * self.n * self.per_launch
+        return "life_packed_tb", ms / launches, per_launch_bytes
+
+    def cpu_baseline(self, budget_s=15.0):
+        oracle = load_oracle()
+        th = cpu_threads()
+        sample = self.host[:4096, :4096].copy()
+        gens = 8
+        t0 = time.perf_counter()
+        oracle.life2d(sample, gens, 0, th)
+        dt = time.perf_counter() - t0
+        # scale the sample to roughly the time budget
+        gens = max(8, int(gens * budget_s / max(dt, 1e-3)))
+        gens = min(gens, 2000)
+        t0 = time.perf_counter()
+        oracle.life2d(sample, gens, 0, th)
+        dt = time.perf_counter() - t0
+        return {"value": 1e-9 * sample.size * gens / dt, "unit": "GLUPS", "cores": th, "kind": "port",
+                "sample": f"C oracle (OpenMP, {th} threads) on the top-left 4096x4096 of the same input, "
+                          f"{gens} generations, {dt:.1f}s"}
+
+
+# ----------------------------------------------------------------------
+# Jacobi3D FP64
+# ----------------------------------------------------------------------
+class Jacobi:
+    name = "jacobi"
+    dtype = "f64"
+    bytes_per_lup = 16.0
+
+    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, **_):
+        self.n, self.sweeps, self.seed = n, steps, seed
+        self.stream = lgdhip.Stream()
+        self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=8)
+        host = synthetic.jacobi_initial(n, n, n, seed)
+        self.pristine = lgdhip.DeviceGrid(self.g, np.float64)
+        self.a = lgdhip.DeviceGrid(self.g, np.float64)
+        self.b = lgdhip.DeviceGrid(self.g, np.float64)
+        for d in (self.pristine, self.a, self.b):
+            d.fill_edge(0.0, self.stream.h)
+        self.pristine.load(host, self.stream.h)
+        self.nbytes = lgdhip.grid_bytes(self.g)
+        self.host = host
+
+    def config(self):
+        return {"workload": f"Jacobi3D 7-point FP64 {self.n}^3, zero Cube edge, U[0,1) interior, {self.sweeps} steps",
+                "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed, "parallelism": "single GPU"}
+
+    def lups_per_step(self):
+        return float(self.n) ** 3 * self.sweeps
+
+    def step(self):
+        s = self.stream.h
+        lgdhip.call("lgdhip_memcpy_d2d", self.a.ptr, self.pristine.ptr, self.nbytes, s)
+        lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps, None, None, s)
+
+    def dominant_kernel(self):
+        launches = 200
+        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr,
+                                                    C.byref(self.g), launches, None, None, self.stream.h))
+        return "jacobi3d_f64_zmarch", ms / launches, self.bytes_per_lup * float(self.n) ** 3
+
+    def cpu_baseline(self, budget_s=15.0):
+        oracle = load_oracle()
+        th = cpu_threads()
+        steps = 4
+        t0 = time.perf_counter()
+        oracle.jacobi3d(self.host, steps, 0.0, th)
+        dt = time.perf_counter() - t0
+        steps = max(4, min(2000, int(steps * budget_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.jacobi3d(self.host, steps, 0.0, th)
+        dt = time.perf_counter() - t0
+        return {"value": 1e-9 * self.host.size * steps / dt, "unit": "GLUPS", "cores": th, "kind": "port",
+                "sample": f"C oracle (OpenMP, {th} threads), same {self.n}^3 input, {steps} steps, {dt:.1f}s"}
+
+
+# ----------------------------------------------------------------------
+# D3Q19 LBM FP32
+# ----------------------------------------------------------------------
+class LBM:
+    name = "lbm"
+    dtype = "f32"
+    bytes_per_lup = 153.0  # 19 x 4 B read + 19 x 4 B write + 1 B wall flags
+
+    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, **_):
+        self.n, self.sweeps, self.seed = n, steps, seed
+        self.stream = lgdhip.Stream()
+        s = self.stream.h
+        self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19)
+        self.fg = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=1, members=1)
+        sys.path.insert(0, str(REPO / "tests"))
+        ux, uy, uz = synthetic.lbm_initial_velocity(n, n, n, 1e-3, seed)
+        f0 = equilibrium_f32(np.ones_like(ux), ux, uy, uz)
+        self.pristine = lgdhip.DeviceGrid(self.g, np.float32)
+        self.a = lgdhip.DeviceGrid(self.g, np.float32)
+        self.b = lgdhip.DeviceGrid(self.g, np.float32)
+        for d in (self.pristine, self.a, self.b):
+            d.fill_edge(np.zeros(19, np.float32), s)
+        self.pristine.load(f0, s)
+        del f0
+        self.flags = lgdhip.DeviceGrid(self.fg, np.uint8)
+        self.flags.fill_edge(0, s)
+        lgdhip.call("lgdhip_lbm_cavity_flags", self.flags.ptr, C.byref(self.fg), s)
+        self.p = lgdhip.lgdhip_lbm_params(1.7, 0.05, 0.0, 0.0)
+        self.nbytes = lgdhip.grid_bytes(self.g)
+        self.ux = ux
+
+    def config(self):
+        return {"workload": f"LBM D3Q19 BGK lid-driven cavity FP32 {self.n}^3, omega 1.7, lid 0.05, "
+                            f"{self.sweeps} steps", "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed,
+                "parallelism": "single GPU"}
+
+    def lups_per_step(self):
+        return float(self.n) ** 3 * self.sweeps
+
+    def step(self):
+        s = self.stream.h
+        lgdhip.call("lgdhip_memcpy_d2d", self.a.ptr, self.pristine.ptr, self.nbytes, s)
+        lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr, self.flags.ptr, C.byref(self.g),
+                    C.byref(self.p), self.sweeps, None, None, s)
+
+    def dominant_kernel(self):
+        launches = 20
+        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
+                                                    self.flags.ptr, C.byref(self.g), C.byref(self.p), launches,
+                                                    None, None, self.stream.h))
+        return "lbm_d3q19_pull", ms / launches, self.bytes_per_lup * float(self.n) ** 3
+
+    def cpu_baseline(self, budget_s=15.0):
+        oracle = load_oracle()
+        th = cpu_threads()
+        m = 64
+        ux, uy, uz = synthetic.lbm_initial_velocity(m, m, m, 1e-3, self.seed)
+        f0 = oracle.lbm_equilibrium(np.ones_like(ux), ux, uy, uz)
+        fl = oracle.lbm_cavity_flags(m, m, m)
+        steps = 2
+        t0 = time.perf_counter()
+        oracle.lbm_run(f0, fl, steps, 1.7, (0.05, 0.0, 0.0), th)
+        dt = time.perf_counter() - t0
+        steps = max(2, min(2000, int(steps * budget_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.lbm_run(f0, fl, steps, 1.7, (0.05, 0.0, 0.0), th)
+        dt = time.perf_counter() - t0
+        return {"value": 1e-9 * m ** 3 * steps / dt, "unit": "GLUPS", "cores": th, "kind": "port",
+                "sample": f"C oracle FP64 (OpenMP, {th} threads), 64^3 cavity, {steps} steps, {dt:.1f}s"}
+
+
+def equilibrium_f32(rho, jx, jy, jz):
+    """Incompressible D3Q19 equilibrium (same formula as the cell) in numpy."""
+    ex = [0, 0, 1, -1, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0]
+    ey = [0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1]
+    ez = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1]
+    w = [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12
+    d = rho / 3.0 - 0.5 * (jx * jx + jy * jy + jz * jz)
+    f = np.empty((19,) + rho.shape, dtype=np.float32)
+    for i in range(19):
+        eu = ex[i] * jx + ey[i] * jy + ez[i] * jz
+        f[i] = 3.0 * w[i] * (d + eu + 1.5 * eu * eu)
+    return f
+
+
+WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="life")
+    ap.add_argument("--size", type=int, default=0, help="override the grid edge length")
+    ap.add_argument("--iters", type=int, default=0, help="override generations / steps per run")
+    ap.add_argument("--per-launch", type=int, default=16, help="life: generations per launch")
+    ap.add_argument("--variant", type=int, default=0, help="life: packed kernel variant")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    lgdhip.load()
+    if lgdhip.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    lgdhip.call("lgdhip_set_device", 0)
+    log("device:", lgdhip.device_name(0))
+
+    kw = {}
+    if args.size:
+        kw["n"] = args.size
+    if args.iters:
+        kw["generations" if args.workload == "life" else "steps"] = args.iters
+    if args.workload == "life":
+        kw.update(per_launch=args.per_launch, variant=args.variant)
+    wl = WORKLOADS[args.workload](**kw)
+
+    for i in range(args.warmup):
+        wl.step()
+    wl.stream.sync()
+    log(f"warmup {args.warmup} done")
+
+    lgdhip.call("lgdhip_device_synchronize")
+    t0 = time.perf_counter()
+    ms = timed(wl.stream, wl.step, reps=args.steps)
+    lgdhip.call("lgdhip_device_synchronize")
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    value = 1e-9 * wl.lups_per_step() / (ms_per_step * 1e-3)
+    log(f"timed {args.steps} steps: {ms:.3f} ms (wall {wall * 1e3:.1f} ms), {value:.1f} GLUPS")
+
+    kname, k_ms, k_bytes = wl.dominant_kernel()
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9
+    log(f"dominant kernel {kname}: {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
+    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname),
+                "avg_launch_us": round(k_ms * 1e3, 3), "algorithmic_bytes_per_launch": k_bytes}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = wl.cpu_baseline()
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": wl.dtype,
+        "data": "synthetic (SplitMix64 seeded initializer)",
+        "config": wl.config(),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,212 @@
+/*
+ * lgdhip.h -- C-ABI of the MI355X (gfx950) sweep library behind HipSimulator.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, never
+ * throws and never takes ownership of a host pointer.  Every function that
+ * can fail returns an int: 0 on success, a negative code otherwise
+ * (lgdhip_error_string() turns it into text, lgdhip_last_error() gives the
+ * detailed message of the last failure on the calling thread).
+ *
+ * Device buffers are plain device pointers (from lgdhip_malloc or any other
+ * HIP allocator).  Streams and events are opaque handles (hipStream_t and
+ * hipEvent_t underneath); a NULL stream is the null stream.
+ *
+ * Reference interfaces replaced (paths relative to the libgeodecomp tree):
+ *   - device allocation / copies / launch loop of
+ *     src/libgeodecomp/parallelization/cudasimulator.h:337-340 (cudaMalloc),
+ *     :490-497 (getGrid D2H), :515-516 (run H2D), :467-470 (nano-step loop)
+ *   - the per-cell sweep kernels kernel2D/kernel3D
+ *     (cudasimulator.h:82-155, :160-236) for the user cells
+ *       JacobiCellFixedHood  src/testbed/performancetests/main.cpp:1182-1208
+ *       ConwayCell           src/examples/gameoflife/main.cpp:25-62
+ *       D3Q19 BGK Cell       src/examples/latticeboltzmann/main.cpp:16-237
+ *   - CUDASoAGrid's device SoA storage and edge fill
+ *     src/libgeodecomp/storage/cudasoagrid.h:16-57, :113-506
+ */
+#ifndef LGDHIP_H
+#define LGDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* error codes                                                         */
+/* ------------------------------------------------------------------ */
+#define LGDHIP_OK                 0
+#define LGDHIP_ERR_INVALID_ARG   -1   /* bad pointer, size or descriptor     */
+#define LGDHIP_ERR_UNSUPPORTED   -2   /* shape/tile the kernel cannot run    */
+#define LGDHIP_ERR_WORKSPACE     -3   /* workspace too small                 */
+#define LGDHIP_ERR_NO_DEVICE     -4   /* no HIP device visible               */
+#define LGDHIP_ERR_HIP         -100   /* HIP runtime error: -100 - hipError  */
+
+typedef void *lgdhip_stream_t;  /* hipStream_t */
+typedef void *lgdhip_event_t;   /* hipEvent_t  */
+
+/*
+ * Device grid descriptor.  A grid stores `members` planes (struct of
+ * arrays, member-major) of sx*sy*sz elements each; the logical cell
+ * (x,y,z) of member m lives at element
+ *     m*sx*sy*sz + (z+oz)*sx*sy + (y+oy)*sx + (x+ox).
+ * The `halo` layers around the logical box hold the topology's edge cell
+ * (Cube topology: the constant boundary value), like the padding the
+ * reference adds on non-wrapping axes (cudasimulator.h:328-335,
+ * soagrid.h:578-584).  For 2D grids nz == 1, oz == 0, sz == 1.
+ */
+typedef struct lgdhip_grid {
+    int32_t dim;         /* 2 or 3                                  */
+    int32_t halo;        /* padding layers on every side (>= 1)     */
+    int32_t elem_bytes;  /* bytes per element of one member         */
+    int32_t members;     /* number of SoA member planes             */
+    int64_t nx, ny, nz;  /* logical extents                         */
+    int64_t ox, oy, oz;  /* element offset of logical (0,0,0)       */
+    int64_t sx, sy, sz;  /* allocated extents (row pitch, rows, planes) */
+} lgdhip_grid;
+
+/* Tuning knobs picked by the autotuner; NULL selects the defaults. */
+typedef struct lgdhip_tile {
+    int32_t bx, by, bz;        /* work-group shape (threads)           */
+    int32_t steps_per_launch;  /* temporal blocking depth              */
+    int32_t variant;           /* kernel variant id (family specific)  */
+    int32_t reserved;
+} lgdhip_tile;
+
+/* Parameters of the D3Q19 lid-driven cavity kernel family. */
+typedef struct lgdhip_lbm_params {
+    float omega;      /* BGK relaxation rate                     */
+    float lid_ux;     /* lid velocity (x component, +y face)     */
+    float lid_uy;
+    float lid_uz;
+} lgdhip_lbm_params;
+
+/* ------------------------------------------------------------------ */
+/* runtime                                                             */
+/* ------------------------------------------------------------------ */
+const char *lgdhip_version(void);
+const char *lgdhip_error_string(int code);
+const char *lgdhip_last_error(void);
+int lgdhip_get_device_count(int *count);
+int lgdhip_set_device(int device);
+int lgdhip_device_name(int device, char *buf, int len);
+int lgdhip_device_synchronize(void);
+
+int lgdhip_malloc(void **ptr, size_t bytes);
+int lgdhip_free(void *ptr);
+int lgdhip_host_malloc(void **ptr, size_t bytes);   /* pinned host memory */
+int lgdhip_host_free(void *ptr);
+/* host<->device copies return after the copy has completed */
+int lgdhip_memcpy_h2d(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
+int lgdhip_memcpy_d2h(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
+int lgdhip_memcpy_d2d(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
+/* byte-pattern fill by a kernel (safe inside graph capture) */
+int lgdhip_fill_u8(void *dst, uint8_t value, size_t bytes, lgdhip_stream_t stream);
+/* streaming copy kernel (roofline / PMC calibration); bytes % 16 == 0 */
+int lgdhip_copy_kernel(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
+
+int lgdhip_stream_create(lgdhip_stream_t *stream);
+int lgdhip_stream_destroy(lgdhip_stream_t stream);
+int lgdhip_stream_synchronize(lgdhip_stream_t stream);
+int lgdhip_event_create(lgdhip_event_t *event);
+int lgdhip_event_destroy(lgdhip_event_t event);
+int lgdhip_event_record(lgdhip_event_t event, lgdhip_stream_t stream);
+int lgdhip_event_synchronize(lgdhip_event_t event);
+int lgdhip_event_elapsed_ms(float *ms, lgdhip_event_t start, lgdhip_event_t stop);
+
+/* Graph capture of a launch sequence on `stream` (hipGraph). */
+typedef void *lgdhip_graph_t;
+int lgdhip_graph_begin(lgdhip_stream_t stream);
+int lgdhip_graph_end(lgdhip_stream_t stream, lgdhip_graph_t *graph);
+int lgdhip_graph_launch(lgdhip_graph_t graph, lgdhip_stream_t stream);
+int lgdhip_graph_destroy(lgdhip_graph_t graph);
+
+/* ------------------------------------------------------------------ */
+/* grids (device SoA storage)                                          */
+/* ------------------------------------------------------------------ */
+/* Fill a descriptor with this library's padding/alignment convention. */
+int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz,
+                     int halo, int elem_bytes, int members);
+/* Elements per member plane (sx*sy*sz) and total bytes of a grid. */
+int64_t lgdhip_grid_plane_elems(const lgdhip_grid *g);
+int64_t lgdhip_grid_bytes(const lgdhip_grid *g);
+/* Write `edge` (members*elem_bytes bytes, member-major) into every halo
+ * element of every member plane.  Replaces the padding fill of
+ * cudasimulator.h:508-513 and cudasoagrid.h:16-57. */
+int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge,
+                          lgdhip_stream_t stream);
+/* Copy the logical box between a dense host array (member-major, x
+ * fastest, nx*ny*nz elements per member) and a device grid. */
+int lgdhip_grid_load(void *grid, const lgdhip_grid *g, const void *host,
+                     lgdhip_stream_t stream);
+int lgdhip_grid_store(const void *grid, const lgdhip_grid *g, void *host,
+                      lgdhip_stream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* Jacobi3D, FP64, 7-point, Cube topology                              */
+/*   T' = (T[z-1]+T[y-1]+T[x-1]+T+T[x+1]+T[y+1]+T[z+1]) * (1.0/7.0)     */
+/*   JacobiCellFixedHood::update, main.cpp:1194-1205                   */
+/* ------------------------------------------------------------------ */
+/* Runs `steps` sweeps ping-ponging between a and b (both carry the edge
+ * value in their halo).  *result_in_b is set to 1 when the final state is
+ * in b, 0 when it is in a. */
+int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
+                            const lgdhip_tile *tile, int *result_in_b,
+                            lgdhip_stream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* Game of Life, uint8 cells (0/1), Moore<2,1>, Cube topology          */
+/*   ConwayCell::update, gameoflife/main.cpp:36-59                     */
+/* ------------------------------------------------------------------ */
+/* Byte-per-cell sweep, one generation per launch; ping-pongs a/b. */
+int lgdhip_life2d_u8_run(uint8_t *a, uint8_t *b, const lgdhip_grid *g, int steps,
+                         int *result_in_b, lgdhip_stream_t stream);
+/* Bit-packed state: ceil(nx/32) 32-bit words per row, ny rows, bit i of
+ * word w is cell x = 32*w + i. */
+int64_t lgdhip_life2d_packed_words(const lgdhip_grid *g);
+int lgdhip_life2d_pack(const uint8_t *grid, const lgdhip_grid *g, uint32_t *packed,
+                       lgdhip_stream_t stream);
+int lgdhip_life2d_unpack(const uint32_t *packed, const lgdhip_grid *g, uint8_t *grid,
+                         lgdhip_stream_t stream);
+/* `steps` generations on the packed state (register temporal blocking,
+ * tile->steps_per_launch generations per launch); edge_alive is the
+ * constant state of every cell outside the grid. */
+int lgdhip_life2d_packed_run(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g,
+                             int edge_alive, int steps, const lgdhip_tile *tile,
+                             int *result_in_b, lgdhip_stream_t stream);
+/* uint8 grid in, uint8 grid out (in place): pack -> packed_run -> unpack.
+ * `workspace` must hold lgdhip_life2d_workspace_bytes(g) bytes. */
+size_t lgdhip_life2d_workspace_bytes(const lgdhip_grid *g);
+int lgdhip_life2d_run(uint8_t *grid, const lgdhip_grid *g, int edge_alive, int steps,
+                      void *workspace, size_t workspace_bytes,
+                      const lgdhip_tile *tile, lgdhip_stream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* D3Q19 BGK lid-driven cavity, FP32 populations (19 SoA planes)       */
+/*   fused pull stream+collide after latticeboltzmann/main.cpp:92-229  */
+/* ------------------------------------------------------------------ */
+/* `flags` is a uint8 grid (1 member, same extents) of wall bits, see
+ * lgdhip_lbm_flag_* below.  Ping-pongs a/b like the other families. */
+#define LGDHIP_LBM_WALL_XLO 0x01
+#define LGDHIP_LBM_WALL_XHI 0x02
+#define LGDHIP_LBM_WALL_YLO 0x04
+#define LGDHIP_LBM_WALL_YHI 0x08
+#define LGDHIP_LBM_WALL_ZLO 0x10
+#define LGDHIP_LBM_WALL_ZHI 0x20
+#define LGDHIP_LBM_LID_YHI  0x40
+int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags,
+                             const lgdhip_grid *g, const lgdhip_lbm_params *p,
+                             int steps, const lgdhip_tile *tile, int *result_in_b,
+                             lgdhip_stream_t stream);
+/* Cavity wall flags for the whole box (lid on the +y face). */
+int lgdhip_lbm_cavity_flags(uint8_t *flags, const lgdhip_grid *fg, lgdhip_stream_t stream);
+/* rho, ux, uy, uz (dense nx*ny*nz float arrays, device) from populations. */
+int lgdhip_lbm_d3q19_f32_moments(const float *f, const lgdhip_grid *g, float *rho,
+                                 float *ux, float *uy, float *uz, lgdhip_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LGDHIP_H */

@@ -1,0 +1,213 @@
+// D3Q19 BGK lattice Boltzmann, lid-driven cavity, FP32 populations.
+//
+// Structure after the reference LBM cell (src/examples/latticeboltzmann/main.cpp:92-229):
+// fused pull stream + collide, incompressible equilibrium built from the
+// momentum j (not u = j/rho), weights 1/3, 1/18, 1/36, direction names and
+// numbering C N E W S T B NW SW NE SE TW BW TE BE TN BN TS BS (:36-55).
+// Differences, all deliberate and shared with the oracle cell:
+//  * relaxation rate omega is a parameter (the reference hard-codes 1/1.7);
+//  * walls are half-way bounce-back decided per link from the cell's wall
+//    flags (the reference uses extra boundary cells with a State enum);
+//  * the +y face may be a moving lid: a population coming back from it gets
+//    6 w_i (e_i . u_lid), the moving-wall bounce-back, where the reference
+//    injects a fixed momentum on two links (:206-220).
+// Layout: 19 member planes of the padded SoA grid (one per population), a
+// separate uint8 flag grid with the same logical extents.
+#include "lgdhip_internal.h"
+
+namespace {
+
+// e_i, opposite direction, weight class (0: rest, 1: axis, 2: diagonal)
+struct Dir {
+    int ex, ey, ez, opp, wc;
+};
+// C N E W S T B NW SW NE SE TW BW TE BE TN BN TS BS
+constexpr Dir kDirs[19] = {
+    {0, 0, 0, 0, 0},    // C
+    {0, 1, 0, 4, 1},    // N
+    {1, 0, 0, 3, 1},    // E
+    {-1, 0, 0, 2, 1},   // W
+    {0, -1, 0, 1, 1},   // S
+    {0, 0, 1, 6, 1},    // T
+    {0, 0, -1, 5, 1},   // B
+    {-1, 1, 0, 10, 2},  // NW
+    {-1, -1, 0, 9, 2},  // SW
+    {1, 1, 0, 8, 2},    // NE
+    {1, -1, 0, 7, 2},   // SE
+    {-1, 0, 1, 14, 2},  // TW
+    {-1, 0, -1, 13, 2}, // BW
+    {1, 0, 1, 12, 2},   // TE
+    {1, 0, -1, 11, 2},  // BE
+    {0, 1, 1, 18, 2},   // TN
+    {0, 1, -1, 17, 2},  // BN
+    {0, -1, 1, 16, 2},  // TS
+    {0, -1, -1, 15, 2}, // BS
+};
+
+__device__ __forceinline__ constexpr float weight(int wc) {
+    return wc == 0 ? 1.0f / 3.0f : (wc == 1 ? 1.0f / 18.0f : 1.0f / 36.0f);
+}
+
+// is the link feeding direction d blocked by a wall, given the cell flags?
+__device__ __forceinline__ bool blocked(const Dir &d, uint32_t fl) {
+    return (d.ex == 1 && (fl & LGDHIP_LBM_WALL_XLO)) || (d.ex == -1 && (fl & LGDHIP_LBM_WALL_XHI)) ||
+           (d.ey == 1 && (fl & LGDHIP_LBM_WALL_YLO)) || (d.ey == -1 && (fl & LGDHIP_LBM_WALL_YHI)) ||
+           (d.ez == 1 && (fl & LGDHIP_LBM_WALL_ZLO)) || (d.ez == -1 && (fl & LGDHIP_LBM_WALL_ZHI));
+}
+
+template <int BY>
+__global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restrict__ src, float *__restrict__ dst,
+                                                          const uint8_t *__restrict__ flags, int nx, int ny,
+                                                          int nz, int64_t sx, int64_t sxy, int64_t plane,
+                                                          int64_t origin, int64_t fsx, int64_t fsxy,
+                                                          int64_t forigin, float omega, float lux, float luy,
+                                                          float luz) {
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx || y >= ny || z >= nz) return;
+    const int64_t idx = origin + (int64_t)z * sxy + (int64_t)y * sx + x;
+    const uint32_t fl = flags[forigin + (int64_t)z * fsxy + (int64_t)y * fsx + x];
+
+    float f[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const Dir d = kDirs[i];
+        const bool b = blocked(d, fl);
+        const int64_t pull = idx - (d.ex + d.ey * sx + d.ez * sxy);
+        const int64_t at = b ? (int64_t)d.opp * plane + idx : (int64_t)i * plane + pull;
+        float v = src[at];
+        if (d.ey == -1 && b && (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI)) {
+            v += 6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
+        }
+        f[i] = v;
+    }
+
+    float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        rho += f[i];
+        if (kDirs[i].ex) jx += (float)kDirs[i].ex * f[i];
+        if (kDirs[i].ey) jy += (float)kDirs[i].ey * f[i];
+        if (kDirs[i].ez) jz += (float)kDirs[i].ez * f[i];
+    }
+    const float dterm = rho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
+    const float keep = 1.0f - omega;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const Dir d = kDirs[i];
+        const float eu = (float)d.ex * jx + (float)d.ey * jy + (float)d.ez * jz;
+        const float w3 = 3.0f * weight(d.wc) * omega;
+        dst[(int64_t)i * plane + idx] = keep * f[i] + w3 * (dterm + eu + 1.5f * eu * eu);
+    }
+}
+
+__global__ void lbm_cavity_flags_kernel(uint8_t *__restrict__ flags, int nx, int ny, int nz, int64_t sx,
+                                        int64_t sxy, int64_t origin) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx) return;
+    uint32_t fl = 0;
+    if (x == 0) fl |= LGDHIP_LBM_WALL_XLO;
+    if (x == nx - 1) fl |= LGDHIP_LBM_WALL_XHI;
+    if (y == 0) fl |= LGDHIP_LBM_WALL_YLO;
+    if (y == ny - 1) fl |= LGDHIP_LBM_WALL_YHI | LGDHIP_LBM_LID_YHI;
+    if (z == 0) fl |= LGDHIP_LBM_WALL_ZLO;
+    if (z == nz - 1) fl |= LGDHIP_LBM_WALL_ZHI;
+    flags[origin + (int64_t)z * sxy + (int64_t)y * sx + x] = (uint8_t)fl;
+}
+
+__global__ void lbm_moments_kernel(const float *__restrict__ f, int nx, int ny, int nz, int64_t sx, int64_t sxy,
+                                   int64_t plane, int64_t origin, float *__restrict__ rho, float *__restrict__ ux,
+                                   float *__restrict__ uy, float *__restrict__ uz) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y = blockIdx.y;
+    const int z = blockIdx.z;
+    if (x >= nx) return;
+    const int64_t idx = origin + (int64_t)z * sxy + (int64_t)y * sx + x;
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const float v = f[(int64_t)i * plane + idx];
+        r += v;
+        if (kDirs[i].ex) jx += (float)kDirs[i].ex * v;
+        if (kDirs[i].ey) jy += (float)kDirs[i].ey * v;
+        if (kDirs[i].ez) jz += (float)kDirs[i].ez * v;
+    }
+    const int64_t o = ((int64_t)z * ny + y) * nx + x;
+    rho[o] = r;
+    ux[o] = jx;
+    uy[o] = jy;
+    uz[o] = jz;
+}
+
+int flags_grid(const lgdhip_grid &g, lgdhip_grid *fg) {
+    return lgdhip_grid_make(fg, 3, g.nx, g.ny, g.nz, g.halo, 1, 1);
+}
+
+} // namespace
+
+extern "C" {
+
+int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgdhip_grid *g,
+                             const lgdhip_lbm_params *p, int steps, const lgdhip_tile *tile, int *result_in_b,
+                             lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(g, 3, 4, 19);
+    if (rc) return rc;
+    LGD_REQUIRE(a && b && a != b && flags && p, "NULL or aliased argument");
+    LGD_REQUIRE(steps >= 0, "steps must be >= 0");
+    LGD_REQUIRE(g->nz <= 65535, "nz must be <= 65535");
+    (void)tile;
+    lgdhip_grid fg;
+    rc = flags_grid(*g, &fg);
+    if (rc) return rc;
+    constexpr int BY = 4;
+    dim3 grid(lgdhip::ceil_div(g->nx, 64), lgdhip::ceil_div(g->ny, BY), (unsigned)g->nz);
+    dim3 block(64, BY);
+    const int64_t sxy = g->sx * g->sy;
+    const int64_t plane = lgdhip::plane_elems(*g);
+    const int64_t origin = lgdhip::origin_index(*g);
+    const int64_t forigin = lgdhip::origin_index(fg);
+    hipStream_t s = lgdhip::as_stream(stream);
+    float *cur = a, *nxt = b;
+    for (int t = 0; t < steps; ++t) {
+        lbm_d3q19_pull<BY><<<grid, block, 0, s>>>(cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz, g->sx, sxy,
+                                                  plane, origin, fg.sx, fg.sx * fg.sy, forigin, p->omega,
+                                                  p->lid_ux, p->lid_uy, p->lid_uz);
+        LGD_LAUNCHED("lbm_d3q19_pull");
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    if (result_in_b) *result_in_b = (cur == b) ? 1 : 0;
+    return 0;
+}
+
+int lgdhip_lbm_cavity_flags(uint8_t *flags, const lgdhip_grid *fg, lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(fg, 3, 1, 1);
+    if (rc) return rc;
+    LGD_REQUIRE(flags, "NULL pointer");
+    LGD_REQUIRE(fg->ny <= 65535 && fg->nz <= 65535, "ny, nz must be <= 65535");
+    dim3 grid(lgdhip::ceil_div(fg->nx, 256), (unsigned)fg->ny, (unsigned)fg->nz);
+    lbm_cavity_flags_kernel<<<grid, 256, 0, lgdhip::as_stream(stream)>>>(
+        flags, (int)fg->nx, (int)fg->ny, (int)fg->nz, fg->sx, fg->sx * fg->sy, lgdhip::origin_index(*fg));
+    LGD_LAUNCHED("lbm_cavity_flags");
+    return 0;
+}
+
+int lgdhip_lbm_d3q19_f32_moments(const float *f, const lgdhip_grid *g, float *rho, float *ux, float *uy, float *uz,
+                                 lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(g, 3, 4, 19);
+    if (rc) return rc;
+    LGD_REQUIRE(f && rho && ux && uy && uz, "NULL pointer");
+    LGD_REQUIRE(g->ny <= 65535 && g->nz <= 65535, "ny, nz must be <= 65535");
+    dim3 grid(lgdhip::ceil_div(g->nx, 256), (unsigned)g->ny, (unsigned)g->nz);
+    lbm_moments_kernel<<<grid, 256, 0, lgdhip::as_stream(stream)>>>(
+        f, (int)g->nx, (int)g->ny, (int)g->nz, g->sx, g->sx * g->sy, lgdhip::plane_elems(*g),
+        lgdhip::origin_index(*g), rho, ux, uy, uz);
+    LGD_LAUNCHED("lbm_moments");
+    return 0;
+}
+
+} // extern "C"

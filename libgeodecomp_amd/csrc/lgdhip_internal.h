@@ -1,0 +1,51 @@
+// Shared helpers of the lgdhip C-ABI implementation (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/lgdhip.h"
+
+namespace lgdhip {
+
+// Records a detailed message for lgdhip_last_error() and returns `code`.
+int set_error(int code, const std::string &msg);
+
+inline int hip_code(hipError_t e) { return e == hipSuccess ? 0 : LGDHIP_ERR_HIP - (int)e; }
+
+int hip_fail(hipError_t e, const char *what);
+
+#define LGD_HIP(call)                                              \
+    do {                                                           \
+        hipError_t lgd_e_ = (call);                                \
+        if (lgd_e_ != hipSuccess) return ::lgdhip::hip_fail(lgd_e_, #call); \
+    } while (0)
+
+// Checks the launch status of the kernel just enqueued.
+#define LGD_LAUNCHED(name)                                         \
+    do {                                                           \
+        hipError_t lgd_e_ = hipGetLastError();                     \
+        if (lgd_e_ != hipSuccess) return ::lgdhip::hip_fail(lgd_e_, name); \
+    } while (0)
+
+#define LGD_REQUIRE(cond, msg)                                     \
+    do {                                                           \
+        if (!(cond)) return ::lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, msg); \
+    } while (0)
+
+inline hipStream_t as_stream(lgdhip_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Validates the internal consistency of a descriptor (extents, offsets and
+// pitches leave room for the halo).  Returns 0 or an error code.
+int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members);
+
+inline int64_t plane_elems(const lgdhip_grid &g) { return g.sx * g.sy * g.sz; }
+inline int64_t origin_index(const lgdhip_grid &g) {
+    return (g.oz * g.sy + g.oy) * g.sx + g.ox;
+}
+
+inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
+
+} // namespace lgdhip

@@ -1,0 +1,69 @@
+/*
+ * lgd_oracle.h -- the CPU oracle of liblgdhip (TEST INFRASTRUCTURE ONLY).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this library, and only as the checker / the timed CPU baseline.  No product
+ * code path (liblgdhip, HipSimulator) calls it.
+ *
+ * Dense host arrays, x fastest: index (z*ny + y)*nx + x; LBM populations are
+ * 19 such planes, member-major.  `threads` > 1 runs the OpenMP variant
+ * (the reference's OpenMPSimulator splits planes the same way,
+ * src/libgeodecomp/storage/updatefunctormacros.h:9-73).
+ *
+ * Parity pinning (tests/test_oracle.py), per function:
+ *  - lgdo_jacobi3d_f64: the known-answer neighbour sums of
+ *    src/libgeodecomp/storage/test/unit/fixedneighborhoodupdatefunctortest.h
+ *    (testCube :304-369: value x + 1000y + 1e6z, edge -1001001, expected
+ *    encoded coordinate sums, exact in FP64) times 1.0/7.0; constant field
+ *    with equal edge is a fixed point; maximum principle; symmetry under
+ *    axis permutations; brute-force pure-Python loop on tiny grids.
+ *  - lgdo_life2d_u8: blinker period 2, glider translation (1,1) per 4
+ *    generations, block still life, the reference example's glider /
+ *    diehard / acorn seeds (src/examples/gameoflife/main.cpp:70-108) with
+ *    their published population counts (diehard vanishes after 130
+ *    generations; acorn reaches 633 cells after 5206 generations on an
+ *    unbounded plane); brute force against scipy-free numpy convolution.
+ *  - lgdo_lbm_d3q19_*: rest state is a fixed point; mass conservation with
+ *    bounce-back walls and a moving lid; equilibrium moments sum rules
+ *    (sum f_eq = rho, sum e f_eq = j, closed form); mirror symmetry of the
+ *    cavity about its x-y plane; FP32 and FP64 versions agree to FP32
+ *    rounding.
+ */
+#ifndef LGD_ORACLE_H
+#define LGD_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* wall flag bits of the cavity cell (same values as liblgdhip's) */
+#define LGDO_WALL_XLO 0x01
+#define LGDO_WALL_XHI 0x02
+#define LGDO_WALL_YLO 0x04
+#define LGDO_WALL_YHI 0x08
+#define LGDO_WALL_ZLO 0x10
+#define LGDO_WALL_ZHI 0x20
+#define LGDO_LID_YHI 0x40
+
+int lgdo_jacobi3d_f64(double *grid, long nx, long ny, long nz, double edge, int steps, int threads);
+int lgdo_life2d_u8(uint8_t *grid, long nx, long ny, int edge, int steps, int threads);
+
+int lgdo_lbm_direction(int i, int *ex, int *ey, int *ez, int *opp);
+void lgdo_lbm_cavity_flags(uint8_t *flags, long nx, long ny, long nz);
+int lgdo_lbm_d3q19_double(double *f, const uint8_t *flags, long nx, long ny, long nz, double omega, double lux,
+                          double luy, double luz, int steps, int threads);
+int lgdo_lbm_d3q19_float(float *f, const uint8_t *flags, long nx, long ny, long nz, float omega, float lux,
+                         float luy, float luz, int steps, int threads);
+void lgdo_lbm_equilibrium_f64(double *f, const double *rho, const double *jx, const double *jy, const double *jz,
+                              long ncells);
+void lgdo_lbm_moments_f64(const double *f, long ncells, double *rho, double *jx, double *jy, double *jz);
+
+int lgdo_max_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,104 @@
+"""ctypes binding of the CPU oracle (oracle/lib/liblgd_oracle.so).
+
+Test infrastructure: used by tests/ (and by __graft_entry__.smoke / the
+benchmark's CPU-baseline leg through their own loaders) as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+REPO = Path(__file__).resolve().parents[1]
+ORACLE_LIB = REPO / "oracle" / "lib" / "liblgd_oracle.so"
+
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not ORACLE_LIB.exists():
+            raise FileNotFoundError(f"{ORACLE_LIB} missing: run `make -C oracle`")
+        lib = C.CDLL(str(ORACLE_LIB))
+        P, L, I, D, F = C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_float
+        lib.lgdo_jacobi3d_f64.argtypes = [P, L, L, L, D, I, I]
+        lib.lgdo_life2d_u8.argtypes = [P, L, L, I, I, I]
+        lib.lgdo_lbm_cavity_flags.argtypes = [P, L, L, L]
+        lib.lgdo_lbm_cavity_flags.restype = None
+        lib.lgdo_lbm_d3q19_double.argtypes = [P, P, L, L, L, D, D, D, D, I, I]
+        lib.lgdo_lbm_d3q19_float.argtypes = [P, P, L, L, L, F, F, F, F, I, I]
+        lib.lgdo_lbm_equilibrium_f64.argtypes = [P, P, P, P, P, L]
+        lib.lgdo_lbm_equilibrium_f64.restype = None
+        lib.lgdo_lbm_moments_f64.argtypes = [P, L, P, P, P, P]
+        lib.lgdo_lbm_moments_f64.restype = None
+        lib.lgdo_lbm_direction.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        _lib = lib
+    return _lib
+
+
+def _p(a: np.ndarray):
+    assert a.flags.c_contiguous
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def jacobi3d(grid: np.ndarray, steps: int, edge: float = 0.0, threads: int = 1) -> np.ndarray:
+    out = np.ascontiguousarray(grid, dtype=np.float64).copy()
+    nz, ny, nx = out.shape
+    rc = load().lgdo_jacobi3d_f64(_p(out), nx, ny, nz, edge, steps, threads)
+    assert rc == 0
+    return out
+
+
+def life2d(grid: np.ndarray, steps: int, edge: int = 0, threads: int = 1) -> np.ndarray:
+    out = np.ascontiguousarray(grid, dtype=np.uint8).copy()
+    ny, nx = out.shape
+    rc = load().lgdo_life2d_u8(_p(out), nx, ny, edge, steps, threads)
+    assert rc == 0
+    return out
+
+
+def lbm_directions():
+    lib = load()
+    out = []
+    for i in range(19):
+        ex, ey, ez, op = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib.lgdo_lbm_direction(i, C.byref(ex), C.byref(ey), C.byref(ez), C.byref(op))
+        out.append((ex.value, ey.value, ez.value, op.value))
+    return out
+
+
+def lbm_cavity_flags(nx: int, ny: int, nz: int) -> np.ndarray:
+    fl = np.zeros((nz, ny, nx), dtype=np.uint8)
+    load().lgdo_lbm_cavity_flags(_p(fl), nx, ny, nz)
+    return fl
+
+
+def lbm_run(f: np.ndarray, flags: np.ndarray, steps: int, omega: float, lid=(0.0, 0.0, 0.0),
+            threads: int = 1) -> np.ndarray:
+    out = np.ascontiguousarray(f).copy()
+    _, nz, ny, nx = out.shape
+    lib = load()
+    if out.dtype == np.float64:
+        rc = lib.lgdo_lbm_d3q19_double(_p(out), _p(flags), nx, ny, nz, omega, *lid, steps, threads)
+    else:
+        assert out.dtype == np.float32
+        rc = lib.lgdo_lbm_d3q19_float(_p(out), _p(flags), nx, ny, nz, omega, *lid, steps, threads)
+    assert rc == 0
+    return out
+
+
+def lbm_equilibrium(rho, jx, jy, jz) -> np.ndarray:
+    rho, jx, jy, jz = (np.ascontiguousarray(a, dtype=np.float64) for a in (rho, jx, jy, jz))
+    f = np.empty((19,) + rho.shape, dtype=np.float64)
+    load().lgdo_lbm_equilibrium_f64(_p(f), _p(rho), _p(jx), _p(jy), _p(jz), rho.size)
+    return f
+
+
+def lbm_moments(f: np.ndarray):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    shp = f.shape[1:]
+    out = [np.empty(shp, dtype=np.float64) for _ in range(4)]
+    load().lgdo_lbm_moments_f64(_p(f), int(np.prod(shp)), *(_p(o) for o in out))
+    return tuple(out)

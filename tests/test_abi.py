@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every symbol include/lgdhip.h declares
+(CPU only: no compute call is made)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+from libgeodecomp_amd import lgdhip
+
+REPO = Path(__file__).resolve().parents[1]
+
+
+def declared_functions():
+    text = (REPO / "include" / "lgdhip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lgdhip_\w+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    assert "lgdhip_jacobi3d_f64_run" in names and "lgdhip_life2d_run" in names
+    assert len(names) >= 40
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(str(lgdhip.library_path()))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol():
+    assert set(declared_functions()) == set(lgdhip.SIGNATURES)
+
+
+def test_grid_descriptor_without_gpu():
+    # pure host logic of the C-ABI: descriptor layout and error reporting
+    g = lgdhip.make_grid(3, 130, 7, 5, halo=1, elem_bytes=8)
+    assert (g.ox * 8) % 64 == 0 and (g.sx * 8) % 128 == 0
+    assert g.sx >= g.ox + g.nx + 1 and g.sy == 9 and g.sz == 7
+    assert lgdhip.grid_bytes(g) == g.sx * g.sy * g.sz * 8
+    g2 = lgdhip.make_grid(2, 33, 5, 1, halo=1, elem_bytes=1)
+    assert g2.sz == 1 and g2.oz == 0 and g2.sx >= g2.ox + 64 + 1
+    bad = lgdhip.lgdhip_grid()
+    try:
+        lgdhip.call("lgdhip_grid_make", C.byref(bad), 4, 1, 1, 1, 1, 8, 1)
+    except lgdhip.LgdHipError as e:
+        assert e.code == -1 and "dim" in str(e)
+    else:
+        raise AssertionError("invalid dim accepted")
+    lib = lgdhip.load()
+    assert lib.lgdhip_error_string(-1) == b"invalid argument"
+    assert lib.lgdhip_life2d_packed_words(C.byref(g2)) == 2 * 5
+
+
+def test_struct_layouts_match_header():
+    assert C.sizeof(lgdhip.lgdhip_grid) == 4 * 4 + 9 * 8
+    assert C.sizeof(lgdhip.lgdhip_tile) == 6 * 4
+    assert C.sizeof(lgdhip.lgdhip_lbm_params) == 4 * 4
