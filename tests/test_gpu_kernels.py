@@ -172,12 +172,22 @@ def test_lbm_vs_oracle(hip, oracle, shape, steps):
     ref64 = oracle.lbm_run(f0, flags, steps, 1.7, (0.05, 0.0, 0.0), threads=4)
     # same FP32 formula, only contraction/ordering differs: a few ulps of 1/18
     assert np.abs(out - ref32).max() < 1e-6
-    r, jx, jy, jz = oracle.lbm_moments(out.astype(np.float64))
-    R, JX, JY, JZ = oracle.lbm_moments(ref64)
-    err_rho = np.abs(r - R).max()
-    err_j = max(np.abs(a - b).max() for a, b in ((jx, JX), (jy, JY), (jz, JZ)))
-    say(f"lbm max-abs vs FP64 oracle: rho {err_rho:.3e}  j {err_j:.3e}")
-    assert err_rho < 2e-6 and err_j < 2e-6
+
+    def moment_errors(f):
+        r, jx, jy, jz = oracle.lbm_moments(f.astype(np.float64))
+        R, JX, JY, JZ = oracle.lbm_moments(ref64)
+        return np.abs(r - R).max(), max(np.abs(a - b).max() for a, b in ((jx, JX), (jy, JY), (jz, JZ)))
+
+    err_rho, err_j = moment_errors(out)
+    o_rho, o_j = moment_errors(ref32)
+    say(f"lbm max-abs vs FP64 oracle after {steps} steps: rho {err_rho:.3e} j {err_j:.3e} "
+        f"(FP32 oracle: rho {o_rho:.3e} j {o_j:.3e})")
+    # FP32 storage: rho = sum of 19 values near 1/19..1/3 rounds at 2^-24
+    # per add, so the error grows at most ~19*2^-24 per step; the kernel
+    # may not be worse than the FP32 restatement of the same cell by more
+    # than a rounding-order margin
+    assert err_rho <= 19 * 2.0 ** -24 * steps and err_j <= 19 * 2.0 ** -24 * steps
+    assert err_rho <= 1.5 * o_rho + 1e-6 and err_j <= 1.5 * o_j + 1e-7
     # mass is conserved by bounce-back and the lid term
     assert abs(out.astype(np.float64).sum() - f0.sum()) < 1e-4
 
@@ -185,5 +195,8 @@ def test_lbm_vs_oracle(hip, oracle, shape, steps):
 def test_lbm_rest_state(hip, oracle):
     n = (16, 8, 8)
     f0 = oracle.lbm_equilibrium(np.ones(n[::-1]), *(np.zeros(n[::-1]),) * 3)
-    out, _ = gpu_lbm(hip, f0, 20, lid=(0.0, 0.0, 0.0))
-    assert np.abs(out - f0).max() < 1e-7
+    out, fl = gpu_lbm(hip, f0, 20, lid=(0.0, 0.0, 0.0))
+    ref32 = oracle.lbm_run(f0.astype(np.float32), fl, 20, 1.7, (0.0, 0.0, 0.0))
+    # the rest state is a fixed point up to FP32 rounding of rho = sum f
+    assert np.abs(out - f0).max() < 20 * 19 * 2.0 ** -24
+    assert np.abs(out - ref32).max() < 1e-7
