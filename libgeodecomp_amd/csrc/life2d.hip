@@ -116,12 +116,13 @@ constexpr int kLanes = 64;
 constexpr int kOutCols = kLanes - 2;  // lanes 0 and 63 only feed their neighbours
 constexpr int kWavesPerBlock = 4;     // stacked row bands
 
-// word of lane-1 (left neighbour column) and lane+1 (right neighbour column)
+// word of lane-1 (left neighbour column) and lane+1 (right neighbour column);
+// lanes 0 and 63 receive 0 (bound_ctrl) and only ever feed halo columns
 template <bool DPP>
 __device__ __forceinline__ uint32_t from_left(uint32_t v) {
     if constexpr (DPP) {
         // DPP wave_shr:1 -- lane i reads lane i-1
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x138, 0xF, 0xF, true);
     } else {
         return (uint32_t)__shfl_up((int)v, 1, kLanes);
     }
@@ -130,11 +131,29 @@ template <bool DPP>
 __device__ __forceinline__ uint32_t from_right(uint32_t v) {
     if constexpr (DPP) {
         // DPP wave_shl:1 -- lane i reads lane i+1
-        return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x130, 0xF, 0xF, false);
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x130, 0xF, 0xF, true);
     } else {
         return (uint32_t)__shfl_down((int)v, 1, kLanes);
     }
 }
+
+// v_bitop3_b32 truth tables: bit k of the immediate is f(a,b,c) for the
+// input bits (a,b,c) of k = 4a + 2b + c, i.e. f evaluated on 0xF0, 0xCC, 0xAA
+template <typename F>
+constexpr uint32_t truth_table(F f) {
+    return f(0xF0u, 0xCCu, 0xAAu) & 0xFFu;
+}
+constexpr uint32_t kXor3 = truth_table([](uint32_t a, uint32_t b, uint32_t c) { return a ^ b ^ c; });
+constexpr uint32_t kMaj3 = truth_table([](uint32_t a, uint32_t b, uint32_t c) { return (a & b) | (a & c) | (b & c); });
+// (p1, p0, k1) -> q == 1 and q == 2 where q = p0 + k1 + 2 p1
+constexpr uint32_t kQ1 = truth_table([](uint32_t p1, uint32_t p0, uint32_t k1) { return ~p1 & (p0 ^ k1); });
+constexpr uint32_t kQ2 = truth_table(
+    [](uint32_t p1, uint32_t p0, uint32_t k1) { return (p1 & ~(p0 | k1)) | (~p1 & p0 & k1); });
+// (ones, alive, q2) -> even-count survivor term, (ones, q1, t) -> result
+constexpr uint32_t kSurvive = truth_table([](uint32_t o, uint32_t a, uint32_t q2) { return ~o & a & q2; });
+constexpr uint32_t kResult = truth_table([](uint32_t o, uint32_t q1, uint32_t t) { return (o & q1) | t; });
+
+#define LGD_BITOP3(a, b, c, tt) __builtin_amdgcn_bitop3_b32((a), (b), (c), (tt))
 
 // Horizontal 3-cell sums of one row word: s + 2c = west + centre + east.
 template <bool DPP>
@@ -143,24 +162,22 @@ __device__ __forceinline__ void row_sums(uint32_t v, uint32_t &s, uint32_t &c) {
     const uint32_t rw = from_right<DPP>(v);
     const uint32_t west = __builtin_amdgcn_alignbit(v, lw, 31);  // (v << 1) | (lw >> 31)
     const uint32_t east = __builtin_amdgcn_alignbit(rw, v, 1);   // (v >> 1) | (rw << 31)
-    const uint32_t t = west ^ v;
-    s = t ^ east;
-    c = (west & v) | (east & t);
+    s = LGD_BITOP3(west, v, east, kXor3);
+    c = LGD_BITOP3(west, v, east, kMaj3);
 }
 
 // s9 = (s0 + 2c0) + (s1 + 2c1) + (s2 + 2c2); alive' = s9==3 | (alive & s9==4)
 __device__ __forceinline__ uint32_t life_rule(uint32_t s0, uint32_t c0, uint32_t s1, uint32_t c1, uint32_t s2,
                                               uint32_t c2, uint32_t alive) {
-    const uint32_t t = s0 ^ s1;
-    const uint32_t ones = t ^ s2;                  // weight-1 digit of s9
-    const uint32_t k1 = (s0 & s1) | (s2 & t);      // weight-2 carry
-    const uint32_t u = c0 ^ c1;
-    const uint32_t p0 = u ^ c2;                    // c0+c1+c2 = p0 + 2 p1
-    const uint32_t p1 = (c0 & c1) | (c2 & u);
+    const uint32_t ones = LGD_BITOP3(s0, s1, s2, kXor3);  // weight-1 digit of s9
+    const uint32_t k1 = LGD_BITOP3(s0, s1, s2, kMaj3);    // weight-2 carry
+    const uint32_t p0 = LGD_BITOP3(c0, c1, c2, kXor3);    // c0+c1+c2 = p0 + 2 p1
+    const uint32_t p1 = LGD_BITOP3(c0, c1, c2, kMaj3);
     // q = p0 + k1 + 2 p1 counts the weight-2 units: s9 = ones + 2q
-    const uint32_t q1 = ~p1 & (p0 ^ k1);                        // q == 1
-    const uint32_t q2 = (p1 & ~(p0 | k1)) | (~p1 & p0 & k1);    // q == 2
-    return (ones & q1) | (~ones & alive & q2);
+    const uint32_t q1 = LGD_BITOP3(p1, p0, k1, kQ1);      // q == 1
+    const uint32_t q2 = LGD_BITOP3(p1, p0, k1, kQ2);      // q == 2
+    const uint32_t t = LGD_BITOP3(ones, alive, q2, kSurvive);
+    return LGD_BITOP3(ones, q1, t, kResult);
 }
 
 template <int R, bool DPP>

@@ -1,0 +1,241 @@
+// HipSimulator<CELL>: single-GPU (MI355X, gfx950) drop-in for the
+// reference's CUDASimulator step loop
+// (src/libgeodecomp/parallelization/cudasimulator.h:297-596).
+//
+// Same user-facing contract as CUDASimulator/SerialSimulator: constructed
+// from an Initializer it owns, Writers and Steerers added with
+// addWriter()/addSteerer(), run()/step()/getGrid()/getStep()/
+// gatherStatistics().  Event order follows SerialSimulator
+// (serialsimulator.h:76-117, :144-180): STEERER_INITIALIZED and
+// WRITER_INITIALIZED at the start of run(), steerers before every step whose
+// number is a multiple of their period, writers after steps that are a
+// multiple of their period and at the last step, STEERER_ALL_DONE at the
+// end.
+//
+// The grid lives on the device between plugin callbacks.  The sweeps are
+// the hand-written kernels of the cell's registered kernel family
+// (libgeodecomp/hip/kernelfamilies.h), reached through the lgdhip C-ABI; the
+// host only holds the Grid that Writers and Steerers see.  Differences from
+// CUDASimulator, all in the user's favour:
+//  * steerer edits are uploaded before the next sweep (the reference leaves
+//    them on the host, cudasimulator.h:450-465);
+//  * between two plugin callbacks run() advances all steps in one call into
+//    the family (temporal blocking / multi-step launches), instead of one
+//    launch per nano-step;
+//  * the device state is downloaded only when a plugin looks at it.
+#ifndef LIBGEODECOMP_PARALLELIZATION_HIPSIMULATOR_H
+#define LIBGEODECOMP_PARALLELIZATION_HIPSIMULATOR_H
+
+#include <libgeodecomp/hip/kernelfamilies.h>
+#include <libgeodecomp/parallelization/monolithicsimulator.h>
+#include <libgeodecomp/storage/grid.h>
+
+#include <memory>
+#include <type_traits>
+
+namespace LibGeoDecomp {
+
+template<typename CELL_TYPE>
+class HipSimulator : public MonolithicSimulator<CELL_TYPE>
+{
+public:
+    typedef typename MonolithicSimulator<CELL_TYPE>::Topology Topology;
+    typedef typename APITraits::SelectHipKernel<CELL_TYPE>::Value Family;
+    typedef Grid<CELL_TYPE, Topology> GridType;
+    typedef typename Steerer<CELL_TYPE>::SteererFeedback SteererFeedback;
+    static const int DIM = Topology::DIM;
+    static const unsigned NANO_STEPS = APITraits::SelectNanoSteps<CELL_TYPE>::VALUE;
+
+    static_assert(!std::is_void<Family>::value,
+                  "HipSimulator needs a HIP kernel family for this cell: derive its API from "
+                  "APITraits::HasHipKernel<...> or use LIBGEODECOMP_REGISTER_HIP_KERNEL");
+
+    using MonolithicSimulator<CELL_TYPE>::chronometer;
+    using MonolithicSimulator<CELL_TYPE>::initializer;
+    using MonolithicSimulator<CELL_TYPE>::steerers;
+    using MonolithicSimulator<CELL_TYPE>::stepNum;
+    using MonolithicSimulator<CELL_TYPE>::writers;
+    using MonolithicSimulator<CELL_TYPE>::gridDim;
+
+    // blockSize keeps the reference constructor's shape
+    // (cudasimulator.h:314-316); a zero component leaves the family's
+    // tuned default for that knob.  stepsPerLaunch > 0 sets the temporal
+    // blocking depth of families that have one.
+    explicit HipSimulator(Initializer<CELL_TYPE> *initializer, Coord<3> blockSize = Coord<3>(0, 0, 0),
+                          int stepsPerLaunch = 0, int variant = 0) :
+        MonolithicSimulator<CELL_TYPE>(initializer),
+        hostGrid(initializer->gridBox()),
+        hostCurrent(true),
+        deviceCurrent(false)
+    {
+        HipKernels::Tuning tuning;
+        if (blockSize.x() || blockSize.y() || blockSize.z() || stepsPerLaunch || variant) {
+            tuning.useTile = true;
+            tuning.tile.bx = blockSize.x();
+            tuning.tile.by = blockSize.y();
+            tuning.tile.bz = blockSize.z();
+            tuning.tile.steps_per_launch = stepsPerLaunch;
+            tuning.tile.variant = variant;
+        }
+        device.reset(new Family(initializer->gridBox().dimensions, tuning));
+        stepNum = initializer->startStep();
+        initializer->grid(&hostGrid);
+    }
+
+    virtual void step()
+    {
+        SteererFeedback feedback;
+        step(&feedback);
+    }
+
+    virtual void step(SteererFeedback *feedback)
+    {
+        TimeTotal t(&chronometer);
+        handleInput(STEERER_NEXT_STEP, feedback);
+        advance(1);
+        ++stepNum;
+        handleOutput(stepNum == initializer->maxSteps() ? WRITER_ALL_DONE : WRITER_STEP_FINISHED);
+    }
+
+    virtual void run()
+    {
+        initializer->grid(&hostGrid);
+        hostCurrent = true;
+        deviceCurrent = false;
+        stepNum = initializer->startStep();
+        setIORegions();
+        SteererFeedback feedback;
+        handleInput(STEERER_INITIALIZED, &feedback);
+        handleOutput(WRITER_INITIALIZED);
+
+        while (stepNum < initializer->maxSteps()) {
+            if (feedback.simulationEnded()) {
+                break;
+            }
+            const unsigned quiet = quietSteps();
+            if (quiet <= 1) {
+                step(&feedback);
+                continue;
+            }
+            // no steerer wakes up before, and no writer after, any of the
+            // next `quiet` steps but the last one: run them as one batch
+            TimeTotal t(&chronometer);
+            advance(quiet);
+            stepNum += quiet;
+            handleOutput(stepNum == initializer->maxSteps() ? WRITER_ALL_DONE : WRITER_STEP_FINISHED);
+        }
+        handleInput(STEERER_ALL_DONE, &feedback);
+    }
+
+    virtual const typename Simulator<CELL_TYPE>::GridType *getGrid()
+    {
+        syncHost();
+        return &hostGrid;
+    }
+
+    // number of device sweeps issued so far (for tests and tracing)
+    unsigned long long sweepsIssued() const { return sweeps; }
+
+protected:
+    GridType hostGrid;
+    std::unique_ptr<Family> device;
+    bool hostCurrent;
+    bool deviceCurrent;
+    unsigned long long sweeps = 0;
+
+    void syncHost()
+    {
+        if (!hostCurrent) {
+            device->download(&hostGrid);
+            hostCurrent = true;
+        }
+    }
+
+    void syncDevice()
+    {
+        if (!deviceCurrent) {
+            device->upload(hostGrid);
+            deviceCurrent = true;
+        }
+    }
+
+    void advance(unsigned steps)
+    {
+        TimeCompute t(&chronometer);
+        syncDevice();
+        device->run(steps * NANO_STEPS);
+        device->sync();
+        sweeps += (unsigned long long)steps * NANO_STEPS;
+        hostCurrent = false;
+    }
+
+    // how many steps from stepNum on can run without a plugin callback in
+    // between (at least 1)
+    unsigned quietSteps() const
+    {
+        const unsigned last = initializer->maxSteps();
+        unsigned k = 0;
+        for (unsigned s = stepNum; s < last; ++s) {
+            if (s != stepNum) {
+                for (std::size_t i = 0; i < steerers.size(); ++i) {
+                    if (s % steerers[i]->getPeriod() == 0) {
+                        return k;
+                    }
+                }
+            } else {
+                for (std::size_t i = 0; i < steerers.size(); ++i) {
+                    if (s % steerers[i]->getPeriod() == 0) {
+                        return 0;
+                    }
+                }
+            }
+            ++k;
+            const unsigned after = s + 1;
+            for (std::size_t i = 0; i < writers.size(); ++i) {
+                if (after % writers[i]->getPeriod() == 0 || after == last) {
+                    return k;
+                }
+            }
+        }
+        return k;
+    }
+
+    void handleOutput(WriterEvent event)
+    {
+        TimeOutput t(&chronometer);
+        for (std::size_t i = 0; i < writers.size(); ++i) {
+            if ((event != WRITER_STEP_FINISHED) || ((stepNum % writers[i]->getPeriod()) == 0)) {
+                syncHost();
+                writers[i]->stepFinished(hostGrid, stepNum, event);
+            }
+        }
+    }
+
+    void handleInput(SteererEvent event, SteererFeedback *feedback)
+    {
+        TimeInput t(&chronometer);
+        Region<DIM> simArea;
+        simArea << hostGrid.boundingBox();
+        for (std::size_t i = 0; i < steerers.size(); ++i) {
+            if ((event != STEERER_NEXT_STEP) || (stepNum % steerers[i]->getPeriod() == 0)) {
+                syncHost();
+                steerers[i]->nextStep(&hostGrid, simArea, gridDim, stepNum, event, 0, true, feedback);
+                // the steerer may have changed cells: upload before the next sweep
+                deviceCurrent = false;
+            }
+        }
+    }
+
+    void setIORegions()
+    {
+        Region<DIM> simArea;
+        simArea << hostGrid.boundingBox();
+        for (std::size_t i = 0; i < steerers.size(); ++i) {
+            steerers[i]->setRegion(simArea);
+        }
+    }
+};
+
+}
+
+#endif

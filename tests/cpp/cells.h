@@ -1,0 +1,404 @@
+// User-model cells for the C++ host tests, written against the
+// reference-compatible API exactly as a LibGeoDecomp user would, plus the
+// one line that registers their HIP kernel family.
+#ifndef LGD_TEST_CELLS_H
+#define LGD_TEST_CELLS_H
+
+#include <libgeodecomp/geometry/fixedcoord.h>
+#include <libgeodecomp/io/simpleinitializer.h>
+#include <libgeodecomp/io/steerer.h>
+#include <libgeodecomp/io/writer.h>
+#include <libgeodecomp/misc/apitraits.h>
+
+#include <cstdint>
+#include <vector>
+
+#ifdef LGD_WITH_HIP
+#include <libgeodecomp/hip/kernelfamilies.h>
+#endif
+
+using namespace LibGeoDecomp;
+
+// SplitMix64 (the seeded initializer shared with libgeodecomp_amd/synthetic.py)
+inline uint64_t splitmix64(uint64_t seed, uint64_t index)
+{
+    uint64_t z = seed + (index + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+inline double uniform01(uint64_t seed, uint64_t index)
+{
+    return (double)(splitmix64(seed, index) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------
+// Conway's Game of Life, as src/examples/gameoflife/main.cpp:25-62
+// (default Moore<2,1> stencil and Cube<2> topology)
+// ---------------------------------------------------------------------
+class ConwayCell
+{
+public:
+    class API
+    {};
+
+    explicit ConwayCell(bool alive = false) : alive(alive) {}
+
+    template<typename COORD_MAP>
+    int countLivingNeighbors(const COORD_MAP& neighborhood)
+    {
+        int ret = 0;
+        for (int y = -1; y < 2; ++y) {
+            for (int x = -1; x < 2; ++x) {
+                ret += neighborhood[Coord<2>(x, y)].alive;
+            }
+        }
+        ret -= neighborhood[Coord<2>(0, 0)].alive;
+        return ret;
+    }
+
+    template<typename COORD_MAP>
+    void update(const COORD_MAP& neighborhood, unsigned)
+    {
+        int living = countLivingNeighbors(neighborhood);
+        alive = neighborhood[Coord<2>(0, 0)].alive;
+        if (alive) {
+            alive = (2 <= living) && (living <= 3);
+        } else {
+            alive = (living == 3);
+        }
+    }
+
+    bool operator==(const ConwayCell& o) const { return alive == o.alive; }
+
+    bool alive;
+};
+
+#ifdef LGD_WITH_HIP
+LIBGEODECOMP_REGISTER_HIP_KERNEL(ConwayCell, HipKernels::Life2D<ConwayCell, bool, &ConwayCell::alive>)
+#endif
+
+// The reference example's start pattern: glider, diehard and acorn on a
+// 160x90 board (coordinates as in gameoflife/main.cpp:70-108).
+class ExampleLifeInitializer : public SimpleInitializer<ConwayCell>
+{
+public:
+    explicit ExampleLifeInitializer(unsigned steps = 800) : SimpleInitializer<ConwayCell>(Coord<2>(160, 90), steps) {}
+
+    virtual void grid(GridBase<ConwayCell, 2> *ret)
+    {
+        CoordBox<2> rect = ret->boundingBox();
+        const int cells[][2] = {
+            {11, 10}, {12, 11}, {10, 12}, {11, 12}, {12, 12},                        // glider
+            {55, 70}, {56, 70}, {56, 71}, {60, 71}, {61, 71}, {62, 71}, {61, 69},    // diehard
+            {111, 30}, {113, 31}, {110, 32}, {111, 32}, {113, 32}, {114, 32}, {115, 32}  // acorn
+        };
+        for (CoordBox<2>::Iterator i = rect.begin(); i != rect.end(); ++i) {
+            ret->set(*i, ConwayCell(false));
+        }
+        for (const auto& c : cells) {
+            Coord<2> p(c[0], c[1]);
+            if (rect.inBounds(p)) {
+                ret->set(p, ConwayCell(true));
+            }
+        }
+    }
+};
+
+class RandomLifeInitializer : public SimpleInitializer<ConwayCell>
+{
+public:
+    RandomLifeInitializer(Coord<2> dim, unsigned steps, uint64_t seed, double density = 0.3, unsigned start = 0,
+                          bool edgeAlive = false) :
+        SimpleInitializer<ConwayCell>(dim, steps), seed(seed), density(density), start(start), edgeAlive(edgeAlive)
+    {}
+
+    virtual void grid(GridBase<ConwayCell, 2> *ret)
+    {
+        CoordBox<2> box = ret->boundingBox();
+        ret->setEdge(ConwayCell(edgeAlive));
+        for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+            ret->set(*i, ConwayCell(uniform01(seed, i->toIndex(dimensions)) < density));
+        }
+    }
+
+    unsigned startStep() const { return start; }
+
+private:
+    uint64_t seed;
+    double density;
+    unsigned start;
+    bool edgeAlive;
+};
+
+// ---------------------------------------------------------------------
+// Jacobi 7-point, as JacobiCellFixedHood (src/testbed/performancetests/main.cpp:1182-1208)
+// ---------------------------------------------------------------------
+class JacobiCell
+{
+public:
+    class API :
+        public APITraits::HasFixedCoordsOnlyUpdate,
+        public APITraits::HasStencil<Stencils::VonNeumann<3, 1> >,
+        public APITraits::HasCubeTopology<3>
+    {};
+
+    explicit JacobiCell(double t = 0) : temp(t) {}
+
+    template<typename NEIGHBORHOOD>
+    void update(const NEIGHBORHOOD& hood, int /* nanoStep */)
+    {
+        temp = (hood[FixedCoord< 0,  0, -1>()].temp +
+                hood[FixedCoord< 0, -1,  0>()].temp +
+                hood[FixedCoord<-1,  0,  0>()].temp +
+                hood[FixedCoord< 0,  0,  0>()].temp +
+                hood[FixedCoord< 1,  0,  0>()].temp +
+                hood[FixedCoord< 0,  1,  0>()].temp +
+                hood[FixedCoord< 0,  0,  1>()].temp) * (1.0 / 7.0);
+    }
+
+    bool operator==(const JacobiCell& o) const { return temp == o.temp; }
+
+    double temp;
+};
+
+#ifdef LGD_WITH_HIP
+LIBGEODECOMP_REGISTER_HIP_KERNEL(JacobiCell, HipKernels::Jacobi3DF64<JacobiCell, double, &JacobiCell::temp>)
+#endif
+
+class JacobiInitializer : public SimpleInitializer<JacobiCell>
+{
+public:
+    JacobiInitializer(Coord<3> dim, unsigned steps, uint64_t seed, double edge = 0.0) :
+        SimpleInitializer<JacobiCell>(dim, steps), seed(seed), edge(edge)
+    {}
+
+    virtual void grid(GridBase<JacobiCell, 3> *ret)
+    {
+        ret->setEdge(JacobiCell(edge));
+        CoordBox<3> box = ret->boundingBox();
+        for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+            ret->set(*i, JacobiCell(uniform01(seed, i->toIndex(dimensions))));
+        }
+    }
+
+private:
+    uint64_t seed;
+    double edge;
+};
+
+// ---------------------------------------------------------------------
+// D3Q19 BGK lid-driven cavity cell (structure of
+// src/examples/latticeboltzmann/main.cpp:16-237; walls as per-link
+// bounce-back from the cell's wall flags, +y lid moving with lidVelocity)
+// ---------------------------------------------------------------------
+enum LBMFlags {
+    WALL_XLO = 0x01, WALL_XHI = 0x02, WALL_YLO = 0x04, WALL_YHI = 0x08,
+    WALL_ZLO = 0x10, WALL_ZHI = 0x20, LID_YHI = 0x40
+};
+
+template<typename T>
+class LBMCavityCell
+{
+public:
+    class API :
+        public APITraits::HasFixedCoordsOnlyUpdate,
+        public APITraits::HasStencil<Stencils::Moore<3, 1> >,
+        public APITraits::HasCubeTopology<3>
+    {};
+
+    static T omega;
+    static T lidVelocity[3];
+
+    // C N E W S T B NW SW NE SE TW BW TE BE TN BN TS BS
+    static constexpr int EX[19] = {0, 0, 1, -1, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0};
+    static constexpr int EY[19] = {0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1};
+    static constexpr int EZ[19] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1};
+    static constexpr int OPP[19] = {0, 4, 3, 2, 1, 6, 5, 10, 9, 8, 7, 14, 13, 12, 11, 18, 17, 16, 15};
+
+    static T weight(int i) { return i == 0 ? T(1) / T(3) : (i < 7 ? T(1) / T(18) : T(1) / T(36)); }
+
+    LBMCavityCell() : flags(0)
+    {
+        for (int i = 0; i < 19; ++i) {
+            f[i] = 0;
+        }
+    }
+
+    bool blocked(int i) const
+    {
+        return (EX[i] == 1 && (flags & WALL_XLO)) || (EX[i] == -1 && (flags & WALL_XHI)) ||
+               (EY[i] == 1 && (flags & WALL_YLO)) || (EY[i] == -1 && (flags & WALL_YHI)) ||
+               (EZ[i] == 1 && (flags & WALL_ZLO)) || (EZ[i] == -1 && (flags & WALL_ZHI));
+    }
+
+    template<int I, typename HOOD>
+    static T pulled(const HOOD& hood, const LBMCavityCell& self)
+    {
+        if (self.blocked(I)) {
+            T v = self.f[OPP[I]];
+            if (EY[I] == -1 && (self.flags & WALL_YHI) && (self.flags & LID_YHI)) {
+                v += T(6) * weight(I) * (T(EX[I]) * lidVelocity[0] + T(EY[I]) * lidVelocity[1] +
+                                         T(EZ[I]) * lidVelocity[2]);
+            }
+            return v;
+        }
+        return hood[FixedCoord<-EX[I], -EY[I], -EZ[I]>()].f[I];
+    }
+
+    template<typename HOOD, int... I>
+    static void gather(const HOOD& hood, const LBMCavityCell& self, T *out, std::integer_sequence<int, I...>)
+    {
+        ((out[I] = pulled<I>(hood, self)), ...);
+    }
+
+    template<typename HOOD>
+    void update(const HOOD& hood, unsigned /* nanoStep */)
+    {
+        *this = hood[FixedCoord<0, 0, 0>()];
+        T g[19];
+        gather(hood, *this, g, std::make_integer_sequence<int, 19>());
+        T rho = 0, jx = 0, jy = 0, jz = 0;
+        for (int i = 0; i < 19; ++i) {
+            rho += g[i];
+            if (EX[i]) jx += T(EX[i]) * g[i];
+            if (EY[i]) jy += T(EY[i]) * g[i];
+            if (EZ[i]) jz += T(EZ[i]) * g[i];
+        }
+        const T dterm = rho * (T(1) / T(3)) - T(0.5) * (jx * jx + jy * jy + jz * jz);
+        const T keep = T(1) - omega;
+        for (int i = 0; i < 19; ++i) {
+            const T eu = T(EX[i]) * jx + T(EY[i]) * jy + T(EZ[i]) * jz;
+            const T w3 = T(3) * weight(i) * omega;
+            f[i] = keep * g[i] + w3 * (dterm + eu + T(1.5) * eu * eu);
+        }
+    }
+
+    T f[19];
+    unsigned char flags;
+};
+
+template<typename T> T LBMCavityCell<T>::omega = T(1.7);
+template<typename T> T LBMCavityCell<T>::lidVelocity[3] = {T(0.05), T(0), T(0)};
+template<typename T> constexpr int LBMCavityCell<T>::EX[19];
+template<typename T> constexpr int LBMCavityCell<T>::EY[19];
+template<typename T> constexpr int LBMCavityCell<T>::EZ[19];
+template<typename T> constexpr int LBMCavityCell<T>::OPP[19];
+
+#ifdef LGD_WITH_HIP
+LIBGEODECOMP_REGISTER_HIP_KERNEL(LBMCavityCell<float>, HipKernels::LBMD3Q19F32<LBMCavityCell<float> >)
+#endif
+
+template<typename T>
+class CavityInitializer : public SimpleInitializer<LBMCavityCell<T> >
+{
+public:
+    CavityInitializer(Coord<3> dim, unsigned steps, uint64_t seed, double amplitude = 1e-3) :
+        SimpleInitializer<LBMCavityCell<T> >(dim, steps), seed(seed), amplitude(amplitude)
+    {}
+
+    virtual void grid(GridBase<LBMCavityCell<T>, 3> *ret)
+    {
+        typedef LBMCavityCell<T> C;
+        const Coord<3> d = this->dimensions;
+        CoordBox<3> box = ret->boundingBox();
+        const std::size_t n = (std::size_t)d.prod();
+        for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+            const std::size_t idx = i->toIndex(d);
+            // rho = 1, j = uniform[-a, a] per component, f = f_eq(rho, j) in FP64
+            double j[3];
+            for (int k = 0; k < 3; ++k) {
+                j[k] = (2.0 * uniform01(seed + 1 + k, idx) - 1.0) * amplitude;
+            }
+            (void)n;
+            C c;
+            const double dterm = 1.0 / 3.0 - 0.5 * (j[0] * j[0] + j[1] * j[1] + j[2] * j[2]);
+            for (int q = 0; q < 19; ++q) {
+                const double eu = C::EX[q] * j[0] + C::EY[q] * j[1] + C::EZ[q] * j[2];
+                const double w = q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0);
+                c.f[q] = (T)(3.0 * w * (dterm + eu + 1.5 * eu * eu));
+            }
+            unsigned fl = 0;
+            if (i->x() == 0) fl |= WALL_XLO;
+            if (i->x() == d.x() - 1) fl |= WALL_XHI;
+            if (i->y() == 0) fl |= WALL_YLO;
+            if (i->y() == d.y() - 1) fl |= WALL_YHI | LID_YHI;
+            if (i->z() == 0) fl |= WALL_ZLO;
+            if (i->z() == d.z() - 1) fl |= WALL_ZHI;
+            c.flags = (unsigned char)fl;
+            ret->set(*i, c);
+        }
+    }
+
+private:
+    uint64_t seed;
+    double amplitude;
+};
+
+// ---------------------------------------------------------------------
+// plugin doubles: event-recording writer and steerer
+// ---------------------------------------------------------------------
+struct Event
+{
+    unsigned step;
+    int event;
+    bool operator==(const Event& o) const { return step == o.step && event == o.event; }
+};
+
+template<typename CELL>
+class RecordingWriter : public Writer<CELL>
+{
+public:
+    typedef typename Writer<CELL>::GridType GridType;
+    typedef void (*Hook)(const GridType&, unsigned, void *);
+
+    RecordingWriter(std::vector<Event> *events, unsigned period, Hook hook = 0, void *ctx = 0) :
+        Writer<CELL>("recording", period), events(events), hook(hook), ctx(ctx)
+    {}
+
+    Writer<CELL> *clone() const { return new RecordingWriter(*this); }
+
+    void stepFinished(const GridType& grid, unsigned step, WriterEvent event)
+    {
+        events->push_back(Event{step, (int)event});
+        if (hook) {
+            hook(grid, step, ctx);
+        }
+    }
+
+private:
+    std::vector<Event> *events;
+    Hook hook;
+    void *ctx;
+};
+
+template<typename CELL>
+class RecordingSteerer : public Steerer<CELL>
+{
+public:
+    typedef typename Steerer<CELL>::GridType GridType;
+    typedef typename Steerer<CELL>::SteererFeedback SteererFeedback;
+    typedef Coord<Steerer<CELL>::Topology::DIM> CoordType;
+    typedef void (*Hook)(GridType *, unsigned, SteererFeedback *, void *);
+
+    RecordingSteerer(std::vector<Event> *events, unsigned period, Hook hook = 0, void *ctx = 0) :
+        Steerer<CELL>(period), events(events), hook(hook), ctx(ctx)
+    {}
+
+    void nextStep(GridType *grid, const Region<Steerer<CELL>::Topology::DIM>&, const CoordType&, unsigned step,
+                  SteererEvent event, std::size_t, bool, SteererFeedback *feedback)
+    {
+        events->push_back(Event{step, (int)event});
+        if (hook && event == STEERER_NEXT_STEP) {
+            hook(grid, step, feedback, ctx);
+        }
+    }
+
+private:
+    std::vector<Event> *events;
+    Hook hook;
+    void *ctx;
+};
+
+#endif

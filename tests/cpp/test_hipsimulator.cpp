@@ -1,0 +1,282 @@
+// GPU tests of HipSimulator: the same user cells and plugins run through
+// HipSimulator (kernels via the lgdhip C-ABI) and through the oracle
+// SerialSimulator; grids at every writer callback and the plugin event
+// sequences must agree (bit-exact for Life and Jacobi, FP32 tolerance for
+// the LBM cell).  Mirrors the cases of
+// src/libgeodecomp/parallelization/test/unit/cudasimulatortest.h:69-209 and
+// serialsimulatortest.h:47-232 that apply to a single-GPU Cube simulator.
+#include "cells.h"
+#include "harness.h"
+
+#include <libgeodecomp/parallelization/hipsimulator.h>
+#include <serialsimulator.h>
+
+#include <cmath>
+#include <memory>
+
+template<typename CELL>
+struct Snapshots
+{
+    typedef typename Writer<CELL>::GridType GridType;
+    std::vector<std::vector<CELL> > grids;
+
+    static void hook(const GridType& g, unsigned, void *ctx)
+    {
+        Snapshots *s = static_cast<Snapshots *>(ctx);
+        std::vector<CELL> cells;
+        CoordBox<GridType::DIM> box = g.boundingBox();
+        for (typename CoordBox<GridType::DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+            cells.push_back(g.get(*i));
+        }
+        s->grids.push_back(cells);
+    }
+};
+
+template<typename CELL, typename INIT>
+void runBoth(INIT *(*make)(), unsigned writerPeriod, std::vector<Event> *evHip, std::vector<Event> *evRef,
+             Snapshots<CELL> *snapHip, Snapshots<CELL> *snapRef, unsigned steererPeriod = 0,
+             typename RecordingSteerer<CELL>::Hook steer = 0)
+{
+    HipSimulator<CELL> hip(make());
+    SerialSimulator<CELL> ref(make());
+    hip.addWriter(new RecordingWriter<CELL>(evHip, writerPeriod, &Snapshots<CELL>::hook, snapHip));
+    ref.addWriter(new RecordingWriter<CELL>(evRef, writerPeriod, &Snapshots<CELL>::hook, snapRef));
+    if (steererPeriod) {
+        hip.addSteerer(new RecordingSteerer<CELL>(evHip, steererPeriod, steer));
+        ref.addSteerer(new RecordingSteerer<CELL>(evRef, steererPeriod, steer));
+    }
+    hip.run();
+    ref.run();
+    CHECK_EQ(hip.getStep(), ref.getStep());
+}
+
+// ---------------------------------------------------------------------
+// Game of Life
+// ---------------------------------------------------------------------
+static ExampleLifeInitializer *makeExample() { return new ExampleLifeInitializer(800); }
+
+LGD_TEST(life_reference_example_800_steps)
+{
+    std::vector<Event> eh, er;
+    Snapshots<ConwayCell> sh, sr;
+    runBoth<ConwayCell>(&makeExample, 10, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK_EQ(sh.grids.size(), (std::size_t)(800 / 10 + 1));
+    CHECK(sh.grids == sr.grids);
+}
+
+static RandomLifeInitializer *makeRandom() { return new RandomLifeInitializer(Coord<2>(517, 301), 95, 42, 0.3, 13); }
+
+// plant a glider into the grid every time the steerer runs: the edits must
+// reach the device (the reference CUDASimulator drops them)
+static void plantGlider(GridBase<ConwayCell, 2> *grid, unsigned step, Steerer<ConwayCell>::SteererFeedback *,
+                        void *)
+{
+    const int x0 = 20 + (int)(step * 7) % 400, y0 = 10 + (int)(step * 3) % 250;
+    const int cells[5][2] = {{1, 0}, {2, 1}, {0, 2}, {1, 2}, {2, 2}};
+    for (int y = 0; y < 5; ++y) {
+        for (int x = 0; x < 5; ++x) {
+            grid->set(Coord<2>(x0 + x - 1, y0 + y - 1), ConwayCell(false));
+        }
+    }
+    for (const auto& c : cells) {
+        grid->set(Coord<2>(x0 + c[0], y0 + c[1]), ConwayCell(true));
+    }
+}
+
+LGD_TEST(life_random_with_steering)
+{
+    std::vector<Event> eh, er;
+    Snapshots<ConwayCell> sh, sr;
+    runBoth<ConwayCell>(&makeRandom, 3, &eh, &er, &sh, &sr, 5, &plantGlider);
+    CHECK(eh == er);
+    CHECK(!sh.grids.empty());
+    CHECK(sh.grids == sr.grids);
+}
+
+static RandomLifeInitializer *makeLiveEdge()
+{
+    return new RandomLifeInitializer(Coord<2>(97, 1000), 70, 7, 0.3, 0, true);
+}
+
+LGD_TEST(life_alive_edge)
+{
+    std::vector<Event> eh, er;
+    Snapshots<ConwayCell> sh, sr;
+    runBoth<ConwayCell>(&makeLiveEdge, 35, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
+static void endAt19(GridBase<ConwayCell, 2> *, unsigned step, Steerer<ConwayCell>::SteererFeedback *fb, void *)
+{
+    if (step == 19) {
+        fb->endSimulation();
+    }
+}
+
+LGD_TEST(life_steerer_terminates)
+{
+    std::vector<Event> eh, er;
+    Snapshots<ConwayCell> sh, sr;
+    runBoth<ConwayCell>(&makeRandom, 4, &eh, &er, &sh, &sr, 1, &endAt19);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
+// manual step() calls: the device state is downloaded lazily by getGrid()
+LGD_TEST(life_manual_steps)
+{
+    HipSimulator<ConwayCell> hip(makeRandom());
+    SerialSimulator<ConwayCell> ref(makeRandom());
+    CHECK(*hip.getGrid() == *ref.getGrid());
+    for (int i = 0; i < 9; ++i) {
+        hip.step();
+        ref.step();
+        if (i % 4 == 0) {
+            CHECK(*hip.getGrid() == *ref.getGrid());
+        }
+    }
+    CHECK_EQ(hip.getStep(), 13u + 9u);
+    CHECK(*hip.getGrid() == *ref.getGrid());
+    CHECK_EQ(hip.sweepsIssued(), 9ull);
+}
+
+// run() twice re-initialises the grid (serialsimulatortest.h
+// testRunMustResetGridPriorToSimulation)
+LGD_TEST(life_run_twice)
+{
+    HipSimulator<ConwayCell> hip(makeExample());
+    std::vector<Event> e1, e2;
+    Snapshots<ConwayCell> s1, s2;
+    hip.addWriter(new RecordingWriter<ConwayCell>(&e1, 100, &Snapshots<ConwayCell>::hook, &s1));
+    hip.run();
+    hip.addWriter(new RecordingWriter<ConwayCell>(&e2, 100, &Snapshots<ConwayCell>::hook, &s2));
+    e1.clear();
+    s1.grids.clear();
+    hip.run();
+    CHECK(e1 == e2);
+    CHECK(s1.grids == s2.grids);
+}
+
+// a whole run without plugins is one batch of sweeps
+LGD_TEST(life_batched_run)
+{
+    HipSimulator<ConwayCell> hip(makeExample());
+    SerialSimulator<ConwayCell> ref(makeExample());
+    hip.run();
+    ref.run();
+    CHECK_EQ(hip.sweepsIssued(), 800ull);
+    CHECK(*hip.getGrid() == *ref.getGrid());
+}
+
+// tuned launch parameters do not change results
+LGD_TEST(life_tuned_variants)
+{
+    SerialSimulator<ConwayCell> ref(makeRandom());
+    ref.run();
+    for (int variant = 0; variant < 4; ++variant) {
+        for (int spl : {1, 5, 32}) {
+            if ((variant & 2) && 2 * spl >= 64) {
+                continue;
+            }
+            HipSimulator<ConwayCell> hip(makeRandom(), Coord<3>(), spl, variant);
+            hip.run();
+            CHECK(*hip.getGrid() == *ref.getGrid());
+        }
+    }
+}
+
+// ---------------------------------------------------------------------
+// Jacobi3D FP64
+// ---------------------------------------------------------------------
+static JacobiInitializer *makeJacobi() { return new JacobiInitializer(Coord<3>(64, 48, 40), 30, 3, 0.25); }
+
+LGD_TEST(jacobi_bit_exact)
+{
+    std::vector<Event> eh, er;
+    Snapshots<JacobiCell> sh, sr;
+    runBoth<JacobiCell>(&makeJacobi, 7, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK_EQ(sh.grids.size(), (std::size_t)(1 + 4 + 1));
+    CHECK(sh.grids == sr.grids);
+}
+
+static void heatCorner(GridBase<JacobiCell, 3> *grid, unsigned, Steerer<JacobiCell>::SteererFeedback *, void *)
+{
+    grid->set(Coord<3>(1, 2, 3), JacobiCell(100.0));
+}
+
+LGD_TEST(jacobi_steered)
+{
+    std::vector<Event> eh, er;
+    Snapshots<JacobiCell> sh, sr;
+    runBoth<JacobiCell>(&makeJacobi, 10, &eh, &er, &sh, &sr, 6, &heatCorner);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
+// ---------------------------------------------------------------------
+// D3Q19 LBM, FP32 device vs FP32 and FP64 oracle cells
+// ---------------------------------------------------------------------
+LGD_TEST(lbm_cavity)
+{
+    const Coord<3> dim(24, 20, 16);
+    const unsigned steps = 40;
+    HipSimulator<LBMCavityCell<float> > hip(new CavityInitializer<float>(dim, steps, 9));
+    SerialSimulator<LBMCavityCell<float> > ref32(new CavityInitializer<float>(dim, steps, 9));
+    SerialSimulator<LBMCavityCell<double> > ref64(new CavityInitializer<double>(dim, steps, 9));
+    hip.run();
+    ref32.run();
+    ref64.run();
+    double err32 = 0, errRho = 0, errJ = 0, errRho32 = 0;
+    double mass0 = 0, mass = 0;
+    CavityInitializer<double> init(dim, steps, 9);
+    Grid<LBMCavityCell<double>, Topologies::Cube<3>::Topology> g0(CoordBox<3>(Coord<3>(), dim));
+    init.grid(&g0);
+    CoordBox<3> box(Coord<3>(), dim);
+    typedef LBMCavityCell<double> C;
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const LBMCavityCell<float> a = hip.getGrid()->get(*i);
+        const LBMCavityCell<float> b = ref32.getGrid()->get(*i);
+        const LBMCavityCell<double> c = ref64.getGrid()->get(*i);
+        double ra = 0, rc = 0, rb = 0, ja[3] = {0, 0, 0}, jc[3] = {0, 0, 0};
+        for (int q = 0; q < 19; ++q) {
+            err32 = std::max(err32, (double)std::fabs(a.f[q] - b.f[q]));
+            ra += a.f[q];
+            rb += b.f[q];
+            rc += c.f[q];
+            mass += a.f[q];
+            mass0 += g0[*i].f[q];
+            ja[0] += C::EX[q] * (double)a.f[q];
+            ja[1] += C::EY[q] * (double)a.f[q];
+            ja[2] += C::EZ[q] * (double)a.f[q];
+            jc[0] += C::EX[q] * c.f[q];
+            jc[1] += C::EY[q] * c.f[q];
+            jc[2] += C::EZ[q] * c.f[q];
+        }
+        errRho = std::max(errRho, std::fabs(ra - rc));
+        errRho32 = std::max(errRho32, std::fabs(rb - rc));
+        for (int k = 0; k < 3; ++k) {
+            errJ = std::max(errJ, std::fabs(ja[k] - jc[k]));
+        }
+    }
+    std::printf("  lbm %dx%dx%d x %u: max|f_gpu - f_oracle32| %.3e, max|rho - rho64| %.3e (oracle32 %.3e), "
+                "max|j - j64| %.3e, mass drift %.3e\n",
+                dim.x(), dim.y(), dim.z(), steps, err32, errRho, errRho32, errJ, std::fabs(mass - mass0));
+    CHECK(err32 < 1e-6);
+    CHECK(errRho <= 1.5 * errRho32 + 1e-6);
+    CHECK(errRho < 19 * steps * std::ldexp(1.0, -24));
+    CHECK(errJ < 1e-5);
+    CHECK(std::fabs(mass - mass0) < 1e-3);
+}
+
+int main(int argc, char **argv)
+{
+    int n = 0;
+    if (lgdhip_get_device_count(&n) != 0 || n < 1) {
+        std::printf("no HIP device\n");
+        return 2;
+    }
+    return lgdtest::runAll(argc, argv);
+}

@@ -1,0 +1,238 @@
+// CPU tests of the reference-compatible API layer and of the oracle
+// SerialSimulator's plugin semantics, restating the expectations of
+// src/libgeodecomp/parallelization/test/unit/serialsimulatortest.h.
+#include "cells.h"
+#include "harness.h"
+
+#include <libgeodecomp/geometry/region.h>
+#include <serialsimulator.h>
+
+// A cell that counts its updates and checks that every neighbour inside the
+// grid has seen the same number of updates (after misc/testcell.h).
+template<int DIM>
+class CountingCell
+{
+public:
+    class API : public APITraits::HasCubeTopology<DIM>
+    {};
+
+    explicit CountingCell(int cycle = 0, bool edge = false) : cycle(cycle), valid(true), edge(edge) {}
+
+    template<typename HOOD>
+    void update(const HOOD& hood, unsigned)
+    {
+        *this = hood[Coord<DIM>()];
+        for (int dy = -1; dy <= 1; ++dy) {
+            for (int dx = -1; dx <= 1; ++dx) {
+                Coord<DIM> c;
+                c[0] = dx;
+                if (DIM > 1) {
+                    c[1 % DIM] = dy;
+                }
+                const CountingCell& n = hood[c];
+                if (!n.edge && n.cycle != cycle) {
+                    valid = false;
+                }
+            }
+        }
+        ++cycle;
+    }
+
+    bool operator==(const CountingCell& o) const { return cycle == o.cycle && valid == o.valid; }
+
+    int cycle;
+    bool valid;
+    bool edge;
+};
+
+template<int DIM>
+class CountingInitializer : public SimpleInitializer<CountingCell<DIM> >
+{
+public:
+    CountingInitializer(Coord<DIM> dim, unsigned start, unsigned steps) :
+        SimpleInitializer<CountingCell<DIM> >(dim, steps), start(start)
+    {}
+
+    virtual void grid(GridBase<CountingCell<DIM>, DIM> *ret)
+    {
+        ret->setEdge(CountingCell<DIM>(-1, true));
+        CoordBox<DIM> box = ret->boundingBox();
+        for (typename CoordBox<DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+            ret->set(*i, CountingCell<DIM>((int)start));
+        }
+    }
+
+    unsigned startStep() const { return start; }
+
+private:
+    unsigned start;
+};
+
+LGD_TEST(geometry_basics)
+{
+    Coord<3> a(1, 2, 3), b(4, 5, 6);
+    CHECK_EQ((a + b), Coord<3>(5, 7, 9));
+    CHECK_EQ(Coord<3>(2, 3, 4).prod(), 24);
+    CHECK_EQ(Coord<3>(1, 2, 3).toIndex(Coord<3>(10, 20, 30)), (std::size_t)(1 + 10 * 2 + 200 * 3));
+    CoordBox<2> box(Coord<2>(3, 4), Coord<2>(5, 2));
+    int n = 0;
+    Coord<2> last;
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        CHECK(box.inBounds(*i));
+        last = *i;
+        ++n;
+    }
+    CHECK_EQ(n, 10);
+    CHECK_EQ(last, Coord<2>(7, 5));
+    CHECK_EQ((Stencils::VonNeumann<3, 1>::VOLUME), 7);
+    CHECK_EQ((Stencils::VonNeumann<2, 2>::VOLUME), 13);
+    CHECK_EQ((Stencils::Moore<2, 1>::VOLUME), 9);
+    CHECK_EQ((Stencils::Moore<3, 1>::VOLUME), 27);
+    typedef Topologies::Torus<2>::Topology Torus;
+    typedef Topologies::Cube<2>::Topology Cube;
+    CHECK_EQ(Torus::normalize(Coord<2>(-1, 12), Coord<2>(10, 12)), Coord<2>(9, 0));
+    CHECK(Cube::isOutOfBounds(Coord<2>(-1, 3), Coord<2>(10, 12)));
+    CHECK(!Torus::isOutOfBounds(Coord<2>(-1, 3), Coord<2>(10, 12)));
+    CHECK((Torus::WrapsAxis<1>::VALUE) && !(Cube::WrapsAxis<0>::VALUE));
+}
+
+LGD_TEST(region_basics)
+{
+    Region<2> r;
+    r << CoordBox<2>(Coord<2>(0, 0), Coord<2>(4, 3));
+    r << Streak<2>(Coord<2>(2, 1), 7);
+    CHECK_EQ(r.size(), (std::size_t)(12 + 3));
+    CHECK_EQ(r.boundingBox(), CoordBox<2>(Coord<2>(0, 0), Coord<2>(7, 3)));
+    CHECK(r.count(Coord<2>(6, 1)) && !r.count(Coord<2>(6, 0)));
+}
+
+LGD_TEST(apitraits_selection)
+{
+    CHECK((std::is_same<APITraits::SelectTopology<ConwayCell>::Value, Topologies::Cube<2>::Topology>::value));
+    CHECK((std::is_same<APITraits::SelectStencil<ConwayCell>::Value, Stencils::Moore<2, 1> >::value));
+    CHECK((std::is_same<APITraits::SelectTopology<JacobiCell>::Value, Topologies::Cube<3>::Topology>::value));
+    CHECK((std::is_same<APITraits::SelectFixedCoordsOnlyUpdate<JacobiCell>::Value, APITraits::TrueType>::value));
+    CHECK((std::is_same<APITraits::SelectFixedCoordsOnlyUpdate<ConwayCell>::Value, APITraits::FalseType>::value));
+    CHECK((APITraits::SelectNanoSteps<ConwayCell>::VALUE == 1));
+    CHECK((std::is_void<APITraits::SelectHipKernel<CountingCell<2> >::Value>::value));
+}
+
+LGD_TEST(writer_rejects_zero_period)
+{
+    std::vector<Event> ev;
+    CHECK_THROWS(RecordingWriter<ConwayCell>(&ev, 0), std::invalid_argument);
+}
+
+// serialsimulatortest.h testSerialSimulatorShouldCallBackWriter: dim 17x12,
+// start 13, end 21, writer period 3
+LGD_TEST(serial_writer_events)
+{
+    std::vector<Event> ev;
+    SerialSimulator<CountingCell<2> > sim(new CountingInitializer<2>(Coord<2>(17, 12), 13, 21));
+    sim.addWriter(new RecordingWriter<CountingCell<2> >(&ev, 3));
+    sim.run();
+    std::vector<Event> exp;
+    exp.push_back(Event{13, WRITER_INITIALIZED});
+    for (unsigned i = 13 + 2; i < 21; i += 3) {
+        exp.push_back(Event{i, WRITER_STEP_FINISHED});
+    }
+    exp.push_back(Event{21, WRITER_ALL_DONE});
+    CHECK(ev == exp);
+    CHECK_EQ(sim.getStep(), 21u);
+}
+
+// serialsimulatortest.h testSteererCallback: period 5
+LGD_TEST(serial_steerer_events)
+{
+    std::vector<Event> ev;
+    SerialSimulator<CountingCell<2> > sim(new CountingInitializer<2>(Coord<2>(17, 12), 13, 21));
+    sim.addSteerer(new RecordingSteerer<CountingCell<2> >(&ev, 5));
+    sim.run();
+    std::vector<Event> exp;
+    exp.push_back(Event{13, STEERER_INITIALIZED});
+    for (unsigned t = 13; t < 21; ++t) {
+        if (t % 5 == 0) {
+            exp.push_back(Event{t, STEERER_NEXT_STEP});
+        }
+    }
+    exp.push_back(Event{21, STEERER_ALL_DONE});
+    CHECK(ev == exp);
+}
+
+// every cell sees neighbours of the same age at every step (testcell.h)
+template<int DIM>
+void countingRun(Coord<DIM> dim)
+{
+    SerialSimulator<CountingCell<DIM> > sim(new CountingInitializer<DIM>(dim, 0, 9));
+    sim.run();
+    const GridBase<CountingCell<DIM>, DIM> *g = sim.getGrid();
+    CoordBox<DIM> box = g->boundingBox();
+    for (typename CoordBox<DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+        CHECK(g->get(*i).valid);
+        CHECK_EQ(g->get(*i).cycle, 9);
+    }
+}
+
+LGD_TEST(serial_counting_cells)
+{
+    countingRun<2>(Coord<2>(17, 12));
+    countingRun<3>(Coord<3>(7, 5, 4));
+}
+
+// testSteererCanTerminateSimulation: the run stops after the step during
+// which a steerer requested the end
+static void endAt19(GridBase<CountingCell<2>, 2> *, unsigned step, Steerer<CountingCell<2> >::SteererFeedback *fb,
+                    void *)
+{
+    if (step == 19) {
+        fb->endSimulation();
+    }
+}
+
+LGD_TEST(serial_steerer_terminates)
+{
+    std::vector<Event> ev;
+    SerialSimulator<CountingCell<2> > sim(new CountingInitializer<2>(Coord<2>(17, 12), 13, 30));
+    sim.addSteerer(new RecordingSteerer<CountingCell<2> >(&ev, 1, endAt19));
+    sim.run();
+    CHECK_EQ(sim.getStep(), 20u);
+    CHECK_EQ(sim.getGrid()->get(Coord<2>(3, 3)).cycle, 20);
+}
+
+// run() re-initialises: two runs give the same events and grid
+LGD_TEST(serial_run_twice)
+{
+    std::vector<Event> ev1, ev2;
+    SerialSimulator<ConwayCell> sim(new ExampleLifeInitializer(60));
+    sim.addWriter(new RecordingWriter<ConwayCell>(&ev1, 7));
+    sim.run();
+    Grid<ConwayCell> first(sim.getGrid()->boundingBox());
+    CoordBox<2> box = first.boundingBox();
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        first.set(*i, sim.getGrid()->get(*i));
+    }
+    sim.addWriter(new RecordingWriter<ConwayCell>(&ev2, 7));
+    ev1.clear();
+    sim.run();
+    CHECK(ev1 == ev2);
+    CHECK(*sim.getGrid() == first);
+}
+
+// the reference example's glider crosses the board: after 4k generations it
+// sits k cells further along +x +y
+LGD_TEST(serial_life_glider)
+{
+    SerialSimulator<ConwayCell> sim(new ExampleLifeInitializer(40));
+    for (int i = 0; i < 40; ++i) {
+        sim.step();
+    }
+    const int glider[5][2] = {{11, 10}, {12, 11}, {10, 12}, {11, 12}, {12, 12}};
+    for (const auto& c : glider) {
+        CHECK(sim.getGrid()->get(Coord<2>(c[0] + 10, c[1] + 10)).alive);
+    }
+}
+
+int main(int argc, char **argv)
+{
+    return lgdtest::runAll(argc, argv);
+}
