@@ -221,7 +221,7 @@ class LBM:
         self.fg = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=1, members=1)
         sys.path.insert(0, str(REPO / "tests"))
         ux, uy, uz = synthetic.lbm_initial_velocity(n, n, n, 1e-3, seed)
-        f0 = equilibrium_f32(np.ones_like(ux), ux, uy, uz)
+        f0 = equilibrium_f32(np.ones_like(ux), ux, uy, uz)  # shifted populations
         self.pristine = lgdhip.DeviceGrid(self.g, np.float32)
         self.a = lgdhip.DeviceGrid(self.g, np.float32)
         self.b = lgdhip.DeviceGrid(self.g, np.float32)
@@ -277,7 +277,9 @@ class LBM:
 
 
 def equilibrium_f32(rho, jx, jy, jz):
-    """Incompressible D3Q19 equilibrium (same formula as the cell) in numpy."""
+    """Incompressible D3Q19 equilibrium (same formula as the cell) in numpy,
+    returned in the device's shifted storage g = f_eq - w (FP64 math, FP32
+    result)."""
     ex = [0, 0, 1, -1, 0, 0, 0, -1, -1, 1, 1, -1, -1, 1, 1, 0, 0, 0, 0]
     ey = [0, 1, 0, 0, -1, 0, 0, 1, -1, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1]
     ez = [0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1]
@@ -286,7 +288,7 @@ def equilibrium_f32(rho, jx, jy, jz):
     f = np.empty((19,) + rho.shape, dtype=np.float32)
     for i in range(19):
         eu = ex[i] * jx + ey[i] * jy + ez[i] * jz
-        f[i] = 3.0 * w[i] * (d + eu + 1.5 * eu * eu)
+        f[i] = 3.0 * w[i] * (d + eu + 1.5 * eu * eu) - w[i]
     return f
 
 

@@ -186,8 +186,12 @@ int lgdhip_life2d_run(uint8_t *grid, const lgdhip_grid *g, int edge_alive, int s
 /* D3Q19 BGK lid-driven cavity, FP32 populations (19 SoA planes)       */
 /*   fused pull stream+collide after latticeboltzmann/main.cpp:92-229  */
 /* ------------------------------------------------------------------ */
-/* `flags` is a uint8 grid (1 member, same extents) of wall bits, see
- * lgdhip_lbm_flag_* below.  Ping-pongs a/b like the other families. */
+/* Populations are stored shifted, g_i = f_i - w_i (w = 1/3, 1/18, 1/36 for
+ * rest, axis and diagonal links), 19 member planes in the order
+ * C N E W S T B NW SW NE SE TW BW TE BE TN BN TS BS.  `flags` is a uint8
+ * grid made with lgdhip_grid_make(3, nx, ny, nz, halo, 1, 1) holding the
+ * LGDHIP_LBM_* wall bits of every cell.  Ping-pongs a/b like the other
+ * families. */
 #define LGDHIP_LBM_WALL_XLO 0x01
 #define LGDHIP_LBM_WALL_XHI 0x02
 #define LGDHIP_LBM_WALL_YLO 0x04
@@ -201,7 +205,8 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags,
                              lgdhip_stream_t stream);
 /* Cavity wall flags for the whole box (lid on the +y face). */
 int lgdhip_lbm_cavity_flags(uint8_t *flags, const lgdhip_grid *fg, lgdhip_stream_t stream);
-/* rho, ux, uy, uz (dense nx*ny*nz float arrays, device) from populations. */
+/* rho = 1 + sum g and the momentum j = sum e g (the incompressible model's
+ * velocity) as dense nx*ny*nz float device arrays. */
 int lgdhip_lbm_d3q19_f32_moments(const float *f, const lgdhip_grid *g, float *rho,
                                  float *ux, float *uy, float *uz, lgdhip_stream_t stream);
 

@@ -13,6 +13,13 @@
 //    injects a fixed momentum on two links (:206-220).
 // Layout: 19 member planes of the padded SoA grid (one per population), a
 // separate uint8 flag grid with the same logical extents.
+//
+// Storage is shifted: the device holds g_i = f_i - w_i (deviation from the
+// rest state).  Bounce-back and streaming are unchanged (w_i = w_opp(i)),
+// rho = 1 + sum g and j = sum e g, and the equilibrium deviation is
+// 3 w_i (drho/3 - j.j/2 + e.j + 1.5 (e.j)^2) with drho = sum g.  Summing
+// the small deviations instead of values near 1/3 removes the systematic
+// FP32 mass drift of the unshifted form (about one ulp of rho per step).
 #include "lgdhip_internal.h"
 
 namespace {
@@ -55,50 +62,62 @@ __device__ __forceinline__ bool blocked(const Dir &d, uint32_t fl) {
            (d.ez == 1 && (fl & LGDHIP_LBM_WALL_ZLO)) || (d.ez == -1 && (fl & LGDHIP_LBM_WALL_ZHI));
 }
 
-template <int BY>
+// IDX is int32_t when every element offset of the 19 planes fits 31 bits
+// (then addresses are a uniform base plus a 32-bit lane offset), else int64_t.
+template <int BY, typename IDX>
 __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restrict__ src, float *__restrict__ dst,
                                                           const uint8_t *__restrict__ flags, int nx, int ny,
-                                                          int nz, int64_t sx, int64_t sxy, int64_t plane,
-                                                          int64_t origin, int64_t fsx, int64_t fsxy,
-                                                          int64_t forigin, float omega, float lux, float luy,
-                                                          float luz) {
+                                                          int nz, IDX sx, IDX sxy, IDX plane, IDX origin,
+                                                          IDX fsx, IDX fsxy, IDX forigin, float omega, float lux,
+                                                          float luy, float luz) {
     const int x = blockIdx.x * 64 + threadIdx.x;
     const int y = blockIdx.y * BY + threadIdx.y;
     const int z = blockIdx.z;
     if (x >= nx || y >= ny || z >= nz) return;
-    const int64_t idx = origin + (int64_t)z * sxy + (int64_t)y * sx + x;
-    const uint32_t fl = flags[forigin + (int64_t)z * fsxy + (int64_t)y * fsx + x];
+    const IDX idx = origin + (IDX)z * sxy + (IDX)y * sx + x;
+    const uint32_t fl = flags[forigin + (IDX)z * fsxy + (IDX)y * fsx + x];
 
-    float f[19];
+    // all 19 source offsets first (selects, no branches), then all loads,
+    // so every wave has its 19 loads in flight at once
+    IDX at[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const Dir d = kDirs[i];
-        const bool b = blocked(d, fl);
-        const int64_t pull = idx - (d.ex + d.ey * sx + d.ez * sxy);
-        const int64_t at = b ? (int64_t)d.opp * plane + idx : (int64_t)i * plane + pull;
-        float v = src[at];
-        if (d.ey == -1 && b && (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI)) {
-            v += 6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
-        }
-        f[i] = v;
+        const IDX pulled = (IDX)i * plane + idx - (d.ex + d.ey * sx + d.ez * sxy);
+        const IDX bounced = (IDX)d.opp * plane + idx;
+        at[i] = blocked(d, fl) ? bounced : pulled;
     }
-
-    float rho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    float f[19];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) f[i] = src[at[i]];
+    // moving lid: populations reflected from the +y face gain 6 w_i e_i.u
+    const bool lid = (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI);
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
-        rho += f[i];
+        const Dir d = kDirs[i];
+        if (d.ey == -1) {
+            const float term = 6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
+            f[i] += lid ? term : 0.0f;
+        }
+    }
+
+    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        drho += f[i];
         if (kDirs[i].ex) jx += (float)kDirs[i].ex * f[i];
         if (kDirs[i].ey) jy += (float)kDirs[i].ey * f[i];
         if (kDirs[i].ez) jz += (float)kDirs[i].ez * f[i];
     }
-    const float dterm = rho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
+    const float dterm = drho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
     const float keep = 1.0f - omega;
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const Dir d = kDirs[i];
         const float eu = (float)d.ex * jx + (float)d.ey * jy + (float)d.ez * jz;
         const float w3 = 3.0f * weight(d.wc) * omega;
-        dst[(int64_t)i * plane + idx] = keep * f[i] + w3 * (dterm + eu + 1.5f * eu * eu);
+        // streamed once per sweep: keep it out of the caches' way
+        __builtin_nontemporal_store(keep * f[i] + w3 * (dterm + eu + 1.5f * eu * eu), &dst[(IDX)i * plane + idx]);
     }
 }
 
@@ -126,7 +145,7 @@ __global__ void lbm_moments_kernel(const float *__restrict__ f, int nx, int ny, 
     const int z = blockIdx.z;
     if (x >= nx) return;
     const int64_t idx = origin + (int64_t)z * sxy + (int64_t)y * sx + x;
-    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+    float r = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;  // r = drho (shifted storage)
 #pragma unroll
     for (int i = 0; i < 19; ++i) {
         const float v = f[(int64_t)i * plane + idx];
@@ -136,7 +155,7 @@ __global__ void lbm_moments_kernel(const float *__restrict__ f, int nx, int ny, 
         if (kDirs[i].ez) jz += (float)kDirs[i].ez * v;
     }
     const int64_t o = ((int64_t)z * ny + y) * nx + x;
-    rho[o] = r;
+    rho[o] = 1.0f + r;
     ux[o] = jx;
     uy[o] = jy;
     uz[o] = jz;
@@ -169,12 +188,21 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     const int64_t plane = lgdhip::plane_elems(*g);
     const int64_t origin = lgdhip::origin_index(*g);
     const int64_t forigin = lgdhip::origin_index(fg);
+    // 32-bit element offsets when all 19 planes (and the flag grid) fit 31 bits
+    const bool small_index = 19 * plane < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
     hipStream_t s = lgdhip::as_stream(stream);
     float *cur = a, *nxt = b;
     for (int t = 0; t < steps; ++t) {
-        lbm_d3q19_pull<BY><<<grid, block, 0, s>>>(cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz, g->sx, sxy,
-                                                  plane, origin, fg.sx, fg.sx * fg.sy, forigin, p->omega,
-                                                  p->lid_ux, p->lid_uy, p->lid_uz);
+        if (small_index) {
+            lbm_d3q19_pull<BY, int32_t><<<grid, block, 0, s>>>(
+                cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz, (int32_t)g->sx, (int32_t)sxy, (int32_t)plane,
+                (int32_t)origin, (int32_t)fg.sx, (int32_t)(fg.sx * fg.sy), (int32_t)forigin, p->omega, p->lid_ux,
+                p->lid_uy, p->lid_uz);
+        } else {
+            lbm_d3q19_pull<BY, int64_t><<<grid, block, 0, s>>>(cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz,
+                                                               g->sx, sxy, plane, origin, fg.sx, fg.sx * fg.sy,
+                                                               forigin, p->omega, p->lid_ux, p->lid_uy, p->lid_uz);
+        }
         LGD_LAUNCHED("lbm_d3q19_pull");
         float *tmp = cur;
         cur = nxt;

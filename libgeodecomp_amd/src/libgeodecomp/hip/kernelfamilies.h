@@ -260,6 +260,10 @@ public:
         flags.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&fg)));
     }
 
+    // lattice weight of population q (rest, axis, diagonal)
+    static double weight(int q) { return q == 0 ? 1.0 / 3.0 : (q < 7 ? 1.0 / 18.0 : 1.0 / 36.0); }
+
+    // the device keeps the shifted populations g = f - w (see lgdhip.h)
     void upload(const Grid<CELL, Topology>& host)
     {
         const std::size_t n = host.size();
@@ -268,13 +272,13 @@ public:
         for (std::size_t i = 0; i < n; ++i) {
             const CELL& c = host.data()[i];
             for (int q = 0; q < 19; ++q) {
-                dense[q * n + i] = c.f[q];
+                dense[q * n + i] = (float)((double)c.f[q] - weight(q));
             }
             fl[i] = c.flags;
         }
         float edge[19];
         for (int q = 0; q < 19; ++q) {
-            edge[q] = host.getEdge().f[q];
+            edge[q] = (float)((double)host.getEdge().f[q] - weight(q));
         }
         const uint8_t edgeFlags = host.getEdge().flags;
         check(lgdhip_grid_fill_edge(a->get(), &g, edge, stream.get()), "lgdhip_grid_fill_edge");
@@ -291,7 +295,7 @@ public:
         check(lgdhip_grid_store(a->get(), &g, dense.data(), stream.get()), "lgdhip_grid_store");
         for (std::size_t i = 0; i < n; ++i) {
             for (int q = 0; q < 19; ++q) {
-                host->data()[i].f[q] = dense[q * n + i];
+                host->data()[i].f[q] = (float)((double)dense[q * n + i] + weight(q));
             }
         }
     }

@@ -229,7 +229,7 @@ LGD_TEST(lbm_cavity)
     hip.run();
     ref32.run();
     ref64.run();
-    double err32 = 0, errRho = 0, errJ = 0, errRho32 = 0;
+    double err32 = 0, err64 = 0, errRho = 0, errJ = 0, errRho32 = 0;
     double mass0 = 0, mass = 0;
     CavityInitializer<double> init(dim, steps, 9);
     Grid<LBMCavityCell<double>, Topologies::Cube<3>::Topology> g0(CoordBox<3>(Coord<3>(), dim));
@@ -243,6 +243,7 @@ LGD_TEST(lbm_cavity)
         double ra = 0, rc = 0, rb = 0, ja[3] = {0, 0, 0}, jc[3] = {0, 0, 0};
         for (int q = 0; q < 19; ++q) {
             err32 = std::max(err32, (double)std::fabs(a.f[q] - b.f[q]));
+            err64 = std::max(err64, std::fabs((double)a.f[q] - c.f[q]));
             ra += a.f[q];
             rb += b.f[q];
             rc += c.f[q];
@@ -261,14 +262,16 @@ LGD_TEST(lbm_cavity)
             errJ = std::max(errJ, std::fabs(ja[k] - jc[k]));
         }
     }
-    std::printf("  lbm %dx%dx%d x %u: max|f_gpu - f_oracle32| %.3e, max|rho - rho64| %.3e (oracle32 %.3e), "
-                "max|j - j64| %.3e, mass drift %.3e\n",
-                dim.x(), dim.y(), dim.z(), steps, err32, errRho, errRho32, errJ, std::fabs(mass - mass0));
-    CHECK(err32 < 1e-6);
-    CHECK(errRho <= 1.5 * errRho32 + 1e-6);
-    CHECK(errRho < 19 * steps * std::ldexp(1.0, -24));
-    CHECK(errJ < 1e-5);
-    CHECK(std::fabs(mass - mass0) < 1e-3);
+    std::printf("  lbm %dx%dx%d x %u: max|f_gpu - f_oracle64| %.3e, max|f_gpu - f_oracle32| %.3e, "
+                "max|rho - rho64| %.3e (FP32 cell on the CPU: %.3e), max|j - j64| %.3e, mass drift %.3e\n",
+                dim.x(), dim.y(), dim.z(), steps, err64, err32, errRho, errRho32, errJ, std::fabs(mass - mass0));
+    // the device keeps shifted FP32 populations (g = f - w): more accurate
+    // than the same cell in plain FP32 on the CPU, which drifts by about one
+    // ulp of rho per step; the values handed back are rounded to float f
+    CHECK(err64 < 2e-7);
+    CHECK(errRho < 2e-6 && errRho <= errRho32);
+    CHECK(errJ < 1e-6);
+    CHECK(std::fabs(mass - mass0) < 1e-7 * mass0 + 19 * dim.prod() * std::ldexp(1.0, -25));
 }
 
 int main(int argc, char **argv)
