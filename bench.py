@@ -129,7 +129,9 @@ class Life:
                                                     C.byref(self.g), 0, gens, C.byref(self.tile),
                                                     C.byref(in_b), s))
         per_launch_bytes = self.bytes_per_lup * self.n * self.n * self.per_launch
-        return "life_packed_tb", ms / launches, per_launch_bytes
+        # variant 0 is the library default (a stacked-wave kernel)
+        name = "life_packed_tb" if 1 <= self.tile.variant <= 4 else "life_packed_stacked"
+        return name, ms / launches, per_launch_bytes
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -158,8 +160,9 @@ class Jacobi:
     dtype = "f64"
     bytes_per_lup = 16.0
 
-    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, **_):
+    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, variant=0, by=0, bz=0, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
+        self.tile = lgdhip.tile(by=by, bz=bz, variant=variant)
         self.stream = lgdhip.Stream()
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=8)
         host = synthetic.jacobi_initial(n, n, n, seed)
@@ -182,13 +185,16 @@ class Jacobi:
     def step(self):
         s = self.stream.h
         lgdhip.call("lgdhip_memcpy_d2d", self.a.ptr, self.pristine.ptr, self.nbytes, s)
-        lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps, None, None, s)
+        lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps,
+                    C.byref(self.tile), None, s)
 
     def dominant_kernel(self):
         launches = 200
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr,
-                                                    C.byref(self.g), launches, None, None, self.stream.h))
-        return "jacobi3d_f64_zmarch", ms / launches, self.bytes_per_lup * float(self.n) ** 3
+                                                    C.byref(self.g), launches, C.byref(self.tile), None,
+                                                    self.stream.h))
+        name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
+        return name, ms / launches, self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -304,7 +310,9 @@ def main():
     ap.add_argument("--size", type=int, default=0, help="override the grid edge length")
     ap.add_argument("--iters", type=int, default=0, help="override generations / steps per run")
     ap.add_argument("--per-launch", type=int, default=16, help="life: generations per launch")
-    ap.add_argument("--variant", type=int, default=0, help="life: packed kernel variant")
+    ap.add_argument("--variant", type=int, default=0, help="kernel variant id (0: library default)")
+    ap.add_argument("--by", type=int, default=0, help="jacobi: rows per work-group (0: default)")
+    ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -327,6 +335,8 @@ def main():
         kw["generations" if args.workload == "life" else "steps"] = args.iters
     if args.workload == "life":
         kw.update(per_launch=args.per_launch, variant=args.variant)
+    if args.workload == "jacobi":
+        kw.update(variant=args.variant, by=args.by, bz=args.bz)
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):

@@ -45,19 +45,90 @@ __global__ __launch_bounds__(kWaveX *BY) void jacobi3d_f64_zmarch(
     }
 }
 
+// Same sweep, but the z march advances U planes per iteration and issues all
+// 5U loads of a batch before the arithmetic, so each thread keeps U HBM
+// fetches (the z+1 planes) in flight instead of one: the single-plane march
+// above is latency bound (one outstanding plane per thread).  The sum order
+// per cell is unchanged.
+template <int BY, int U>
+__global__ __launch_bounds__(kWaveX *BY) void jacobi3d_f64_zbatch(
+    const double *__restrict__ src, double *__restrict__ dst, int nx, int ny, int nz, int64_t sx,
+    int64_t sxy, int64_t origin, int zchunk) {
+    const int x = blockIdx.x * kWaveX + threadIdx.x;
+    const int y = blockIdx.y * BY + threadIdx.y;
+    const int z0 = blockIdx.z * zchunk;
+    if (x >= nx || y >= ny) return;
+    int z1 = z0 + zchunk;
+    if (z1 > nz) z1 = nz;
+    int64_t i = origin + (int64_t)z0 * sxy + (int64_t)y * sx + x;
+    // q[0] = plane z-1, q[1] = plane z, q[2..U+1] = planes z+1..z+U
+    double q[U + 2];
+    q[0] = src[i - sxy];
+    q[1] = src[i];
+    int z = z0;
+    for (; z + U <= z1; z += U) {
+        double ym[U], xm[U], xp[U], yp[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            const int64_t j = i + (int64_t)k * sxy;
+            q[k + 2] = src[j + sxy];
+            ym[k] = src[j - sx];
+            xm[k] = src[j - 1];
+            xp[k] = src[j + 1];
+            yp[k] = src[j + sx];
+        }
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+            double s = q[k];  // ( 0,  0, -1)
+            s += ym[k];       // ( 0, -1,  0)
+            s += xm[k];       // (-1,  0,  0)
+            s += q[k + 1];    // ( 0,  0,  0)
+            s += xp[k];       // ( 1,  0,  0)
+            s += yp[k];       // ( 0,  1,  0)
+            s += q[k + 2];    // ( 0,  0,  1)
+            dst[i + (int64_t)k * sxy] = s * (1.0 / 7.0);
+        }
+        q[0] = q[U];
+        q[1] = q[U + 1];
+        i += (int64_t)U * sxy;
+    }
+    double below = q[0], center = q[1];
+    for (; z < z1; ++z) {
+        const double above = src[i + sxy];
+        double s = below;
+        s += src[i - sx];
+        s += src[i - 1];
+        s += center;
+        s += src[i + 1];
+        s += src[i + sx];
+        s += above;
+        dst[i] = s * (1.0 / 7.0);
+        below = center;
+        center = above;
+        i += sxy;
+    }
+}
+
+// variant ids (lgdhip_tile::variant): 0 default, 1 single-plane march,
+// 2 four-plane batches, 3 eight-plane batches
+constexpr int kJacobiDefaultVariant = 2;
+
 struct JacobiLaunch {
     dim3 grid, block;
     int zchunk;
     int by;
+    int variant;
 };
 
 JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     JacobiLaunch L;
     L.by = 4;
     L.zchunk = 16;
+    L.variant = kJacobiDefaultVariant;
     if (tile) {
         if (tile->by == 1 || tile->by == 2 || tile->by == 4 || tile->by == 8) L.by = tile->by;
         if (tile->bz > 0) L.zchunk = tile->bz;
+        if (tile->variant >= 1 && tile->variant <= 3) L.variant = tile->variant;
     }
     // aim for several thousand work-groups so all 256 CUs stay busy even on
     // small cubes: shrink the z chunk until the grid is large enough
@@ -71,10 +142,32 @@ JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     return L;
 }
 
+template <int BY>
+void launch_batched(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L, hipStream_t s) {
+    const int64_t sxy = g.sx * g.sy;
+    const int64_t origin = lgdhip::origin_index(g);
+    if (L.variant == 3)
+        jacobi3d_f64_zbatch<BY, 8><<<L.grid, L.block, 0, s>>>(src, dst, (int)g.nx, (int)g.ny, (int)g.nz, g.sx, sxy,
+                                                             origin, L.zchunk);
+    else
+        jacobi3d_f64_zbatch<BY, 4><<<L.grid, L.block, 0, s>>>(src, dst, (int)g.nx, (int)g.ny, (int)g.nz, g.sx, sxy,
+                                                             origin, L.zchunk);
+}
+
 int launch_sweep(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L,
                  hipStream_t s) {
     const int64_t sxy = g.sx * g.sy;
     const int64_t origin = lgdhip::origin_index(g);
+    if (L.variant != 1) {
+        switch (L.by) {
+        case 1: launch_batched<1>(src, dst, g, L, s); break;
+        case 2: launch_batched<2>(src, dst, g, L, s); break;
+        case 8: launch_batched<8>(src, dst, g, L, s); break;
+        default: launch_batched<4>(src, dst, g, L, s); break;
+        }
+        LGD_LAUNCHED("jacobi3d_f64_zbatch");
+        return 0;
+    }
     switch (L.by) {
     case 1:
         jacobi3d_f64_zmarch<1><<<L.grid, L.block, 0, s>>>(src, dst, (int)g.nx, (int)g.ny, (int)g.nz, g.sx, sxy,

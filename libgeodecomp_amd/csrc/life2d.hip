@@ -251,9 +251,108 @@ __global__ __launch_bounds__(kLanes *kWavesPerBlock) void life_packed_tb(
     }
 }
 
-// variant bits of lgdhip_tile::variant for the packed kernel
-constexpr int kVariantShfl = 1;    // neighbour words via ds_bpermute instead of DPP
-constexpr int kVariantRows64 = 2;  // 64 register rows per lane instead of 128
+// ---------------------------------------------------------------------------
+// stacked waves: NW waves of one work-group hold NW*R consecutive rows of the
+// same 64 word columns and hand their boundary rows to each other through
+// LDS every generation, so only the work-group's top and bottom K rows are
+// redundant halo (instead of 2K rows per wave).
+// ---------------------------------------------------------------------------
+template <int R, int NW>
+__global__ __launch_bounds__(kLanes *NW) void life_packed_stacked(const uint32_t *__restrict__ src,
+                                                                  uint32_t *__restrict__ dst, int nw, int ny,
+                                                                  uint32_t last_mask, uint32_t edge, int gens,
+                                                                  int halo_rows) {
+    // [buffer][wave][0: first row, 1: last row][lane]
+    __shared__ uint32_t xchg[2][NW][2][kLanes];
+    const int lane = threadIdx.x & (kLanes - 1);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
+    const int col = (int)blockIdx.x * kOutCols + lane - 1;
+    const int block_row0 = (int)blockIdx.y * (NW * R - 2 * halo_rows) - halo_rows;
+    if (block_row0 + halo_rows >= ny) return;  // block-uniform: no wave is left at a barrier
+    const int row0 = block_row0 + wave * R;
+
+    const bool col_in = col >= 0 && col < nw;
+    const uint32_t cmask = col_in ? (col == nw - 1 ? last_mask : 0xFFFFFFFFu) : 0u;
+    const bool rows_edge = row0 < 0 || row0 + R > ny;
+    const bool cols_edge = blockIdx.x == 0 || (int)(blockIdx.x + 1) * kOutCols + 1 > nw - 1;
+    int ilo = row0 < 0 ? (-row0 < R ? -row0 : R) : 0;
+    int ihi = ny - row0 < R ? (ny - row0 > 0 ? ny - row0 : 0) : R;
+
+    const int colc = col < 0 ? 0 : (col >= nw ? nw - 1 : col);
+    uint32_t v[R];
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        int rc = row0 + i;
+        rc = rc < 0 ? 0 : (rc >= ny ? ny - 1 : rc);
+        const uint32_t x = src[(int64_t)rc * nw + colc];
+        v[i] = (i >= ilo && i < ihi) ? ((x & cmask) | (edge & ~cmask)) : edge;
+    }
+
+    for (int g = 0; g < gens; ++g) {
+        asm volatile("" : "+s"(ilo), "+s"(ihi));
+        const int buf = g & 1;
+        xchg[buf][wave][0][lane] = v[0];
+        xchg[buf][wave][1][lane] = v[R - 1];
+        __syncthreads();
+        // the block's outermost rows have no neighbour: they decay into the
+        // K-row halo like the band edges of life_packed_tb
+        const uint32_t above = wave > 0 ? xchg[buf][wave - 1][1][lane] : v[0];
+        const uint32_t below = wave < NW - 1 ? xchg[buf][wave + 1][0][lane] : v[R - 1];
+
+        uint32_t sa, ca, sb, cb;
+        row_sums<true>(above, sa, ca);
+        row_sums<true>(v[0], sb, cb);
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            uint32_t sc, cc;
+            row_sums<true>(i + 1 < R ? v[i + 1] : below, sc, cc);
+            v[i] = life_rule(sa, ca, sb, cb, sc, cc, v[i]);
+            sa = sb;
+            ca = cb;
+            sb = sc;
+            cb = cc;
+        }
+        if (cols_edge) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[i] = (v[i] & cmask) | (edge & ~cmask);
+        }
+        if (rows_edge) {
+#pragma unroll
+            for (int i = 0; i < R; ++i) v[i] = (i >= ilo && i < ihi) ? v[i] : edge;
+        }
+    }
+
+    if (lane == 0 || lane == kLanes - 1 || !col_in) return;
+    // rows of this wave inside [block_row0 + K, block_row0 + NW*R - K) and the grid
+    const int lo = block_row0 + halo_rows - row0;
+    const int hi = block_row0 + NW * R - halo_rows - row0;
+    const int iend = ihi < hi ? ihi : hi;
+#pragma unroll
+    for (int i = 0; i < R; ++i) {
+        if (i >= lo && i < iend) dst[(int64_t)(row0 + i) * nw + col] = v[i];
+    }
+}
+
+// variant ids of lgdhip_tile::variant for the packed kernel (0 = default)
+//   1: independent waves, 128 register rows      2: same with ds_bpermute
+//   3: independent waves, 64 register rows       4: same with ds_bpermute
+//   5: 4 stacked waves x 64 rows                 6: 8 stacked waves x 64 rows
+//   7: 8 stacked waves x 32 rows                 8: 16 stacked waves x 32 rows
+constexpr int kNumVariants = 8;
+constexpr int kDefaultVariant = 6;
+constexpr int kDefaultPerLaunch = 16;
+
+// register rows one work-group column holds for a variant
+int rows_per_group(int variant) {
+    switch (variant) {
+    case 1: case 2: return 128;
+    case 3: case 4: return 64;
+    case 5: return 4 * 64;
+    case 6: return 8 * 64;
+    case 7: return 8 * 32;
+    default: return 16 * 32;
+    }
+}
 
 template <int R>
 void launch_rows(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge, int gens,
@@ -268,13 +367,24 @@ void launch_rows(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t la
         life_packed_tb<R, false><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows);
 }
 
+template <int R, int NW>
+void launch_stacked(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge, int gens,
+                    int halo_rows, hipStream_t s) {
+    const int valid = NW * R - 2 * halo_rows;
+    dim3 grid((unsigned)((nw + kOutCols - 1) / kOutCols), (unsigned)((ny + valid - 1) / valid));
+    life_packed_stacked<R, NW><<<grid, kLanes * NW, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows);
+}
+
 int packed_launch(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge,
                   int gens, int halo_rows, int variant, hipStream_t s) {
-    const bool dpp = !(variant & kVariantShfl);
-    if (variant & kVariantRows64)
-        launch_rows<64>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, dpp, s);
-    else
-        launch_rows<128>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, dpp, s);
+    switch (variant) {
+    case 1: case 2: launch_rows<128>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 1, s); break;
+    case 3: case 4: launch_rows<64>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 3, s); break;
+    case 5: launch_stacked<64, 4>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
+    case 6: launch_stacked<64, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
+    case 7: launch_stacked<32, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
+    default: launch_stacked<32, 16>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
+    }
     LGD_LAUNCHED("life_packed_tb");
     return 0;
 }
@@ -355,14 +465,14 @@ int lgdhip_life2d_packed_run(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g, i
     LGD_REQUIRE(pa && pb && pa != pb, "need two distinct device buffers");
     LGD_REQUIRE(steps >= 0, "steps must be >= 0");
     LGD_REQUIRE(edge_alive == 0 || edge_alive == 1, "edge_alive must be 0 or 1");
-    int per_launch = 16;
-    int variant = 0;
+    int per_launch = kDefaultPerLaunch;
+    int variant = kDefaultVariant;
     if (tile) {
         if (tile->steps_per_launch > 0) per_launch = tile->steps_per_launch;
-        variant = tile->variant;
+        if (tile->variant != 0) variant = tile->variant;
     }
-    if (variant < 0 || variant > 3) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "unknown life variant");
-    const int rows = (variant & kVariantRows64) ? 64 : 128;
+    if (variant < 1 || variant > kNumVariants) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "unknown life variant");
+    const int rows = rows_per_group(variant);
     // a lane's neighbour word covers 32 cells, so at most 32 generations may
     // run before the halo lanes' garbage reaches an output column; the
     // register rows must keep at least one valid row per band

@@ -175,9 +175,9 @@ LGD_TEST(life_tuned_variants)
 {
     SerialSimulator<ConwayCell> ref(makeRandom());
     ref.run();
-    for (int variant = 0; variant < 4; ++variant) {
+    for (int variant = 0; variant <= 8; ++variant) {
         for (int spl : {1, 5, 32}) {
-            if ((variant & 2) && 2 * spl >= 64) {
+            if ((variant == 3 || variant == 4) && 2 * spl >= 64) {
                 continue;
             }
             HipSimulator<ConwayCell> hip(makeRandom(), Coord<3>(), spl, variant);

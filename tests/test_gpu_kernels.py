@@ -56,10 +56,12 @@ def test_jacobi_bit_exact_vs_oracle(hip, oracle, shape, steps):
 def test_jacobi_tiles_agree(hip, oracle):
     a = synthetic.jacobi_initial(96, 70, 40, seed=9)
     exp = oracle.jacobi3d(a, 6, edge=0.5, threads=4)
-    for by in (1, 2, 4, 8):
-        for bz in (1, 3, 64):
-            say("jacobi tile", by, bz)
-            assert np.array_equal(gpu_jacobi(hip, a, 6, edge=0.5, tile=hip.tile(by=by, bz=bz)), exp)
+    for variant in (1, 2, 3):
+        for by in (1, 2, 4, 8):
+            for bz in (1, 3, 9, 64):
+                say("jacobi tile", variant, by, bz)
+                t = hip.tile(by=by, bz=bz, variant=variant)
+                assert np.array_equal(gpu_jacobi(hip, a, 6, edge=0.5, tile=t), exp)
 
 
 # ----------------------------------------------------------------------
@@ -114,11 +116,13 @@ def test_life_packed_bit_exact(hip, oracle, shape, edge):
     assert np.array_equal(gpu_life_packed(hip, g0, 45, edge), exp)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", list(range(0, 9)))
 @pytest.mark.parametrize("per_launch", [1, 7, 16, 31])
-def test_life_packed_variants(hip, oracle, variant, per_launch):
-    say("life variant", variant, per_launch)
-    g0 = synthetic.life_initial(300, 211, seed=variant * 100 + per_launch)
+@pytest.mark.parametrize("shape", [(300, 211), (130, 1500)])
+def test_life_packed_variants(hip, oracle, variant, per_launch, shape):
+    nx, ny = shape
+    say("life variant", variant, per_launch, shape)
+    g0 = synthetic.life_initial(nx, ny, seed=variant * 100 + per_launch)
     exp = oracle.life2d(g0, 40, threads=4)
     out = gpu_life_packed(hip, g0, 40, tile=hip.tile(steps_per_launch=per_launch, variant=variant))
     assert np.array_equal(out, exp)
