@@ -84,7 +84,7 @@ class Life:
     dtype = "u8"
     bytes_per_lup = 2.0  # 1 B read + 1 B write of the uint8 cell format
 
-    def __init__(self, n=16384, generations=1000, seed=synthetic.DEFAULT_SEED, per_launch=16, variant=0):
+    def __init__(self, n=16384, generations=1000, seed=synthetic.DEFAULT_SEED, per_launch=31, variant=7):
         self.n, self.gens, self.seed = n, generations, seed
         self.tile = lgdhip.tile(steps_per_launch=per_launch, variant=variant)
         self.per_launch = per_launch
@@ -309,8 +309,8 @@ def main():
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="life")
     ap.add_argument("--size", type=int, default=0, help="override the grid edge length")
     ap.add_argument("--iters", type=int, default=0, help="override generations / steps per run")
-    ap.add_argument("--per-launch", type=int, default=16, help="life: generations per launch")
-    ap.add_argument("--variant", type=int, default=0, help="kernel variant id (0: library default)")
+    ap.add_argument("--per-launch", type=int, default=31, help="life: generations per launch")
+    ap.add_argument("--variant", type=int, default=-1, help="kernel variant id (-1: tuned default, 0: library default)")
     ap.add_argument("--by", type=int, default=0, help="jacobi: rows per work-group (0: default)")
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -334,9 +334,9 @@ def main():
     if args.iters:
         kw["generations" if args.workload == "life" else "steps"] = args.iters
     if args.workload == "life":
-        kw.update(per_launch=args.per_launch, variant=args.variant)
+        kw.update(per_launch=args.per_launch, variant=7 if args.variant < 0 else args.variant)
     if args.workload == "jacobi":
-        kw.update(variant=args.variant, by=args.by, bz=args.bz)
+        kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz)
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):

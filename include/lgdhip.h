@@ -210,6 +210,65 @@ int lgdhip_lbm_cavity_flags(uint8_t *flags, const lgdhip_grid *fg, lgdhip_stream
 int lgdhip_lbm_d3q19_f32_moments(const float *f, const lgdhip_grid *g, float *rho,
                                  float *ux, float *uy, float *uz, lgdhip_stream_t stream);
 
+/* ------------------------------------------------------------------ */
+/* Batched ensemble of 3D Poisson problems, Jacobi 7-point             */
+/*   6 u_c - sum_nb u_nb = f_c, constant Dirichlet value per face.     */
+/*   No reference counterpart (the sweep is the Jacobi cell of         */
+/*   src/testbed/performancetests/main.cpp:1194-1205 plus a source).   */
+/* ------------------------------------------------------------------ */
+/* Problem k occupies a padded (n+2)^3 block (x fastest) starting at
+ * element `offset` of the pools: u (double; its halo holds the faces) and
+ * f (float; halo unused). */
+typedef struct lgdhip_poisson_problem {
+    int32_t n;          /* interior cells per edge                      */
+    int32_t reserved;
+    int64_t offset;     /* element offset of the padded block           */
+    double face[6];     /* Dirichlet values at x=-1, x=n, y=-1, y=n, z=-1, z=n */
+} lgdhip_poisson_problem;
+
+typedef struct lgdhip_poisson_options {
+    double rel_tol;              /* stop at ||r||_2 <= rel_tol * reference        */
+    int32_t chunk;               /* FP32 sweeps between convergence checks (even) */
+    int32_t max_sweeps;          /* per problem                                    */
+    double inner_reduction;      /* FP64 refresh once the correction's residual
+                                    fell by this factor (0: 1e-2)                 */
+    int32_t use_graph;           /* replay chunks as a captured hipGraph           */
+    int32_t reserved;
+    const double *reference_norms; /* host array, one per problem, or NULL: the
+                                      residual norm of the initial guess          */
+} lgdhip_poisson_options;
+
+typedef struct lgdhip_poisson_result {
+    int64_t sweeps;              /* Jacobi sweeps performed                        */
+    int32_t outer_iterations;    /* FP64 refresh steps                             */
+    int32_t converged;
+    double reference_norm;
+    double residual_norm;        /* final FP64 ||f + sum_nb u - 6u||_2             */
+} lgdhip_poisson_result;
+
+/* Assigns consecutive, 256-byte aligned offsets; returns the pool length in
+ * elements (allocate u with 8 and f with 4 bytes per element). */
+int64_t lgdhip_poisson_layout(lgdhip_poisson_problem *probs, int count);
+size_t lgdhip_poisson_workspace_bytes(const lgdhip_poisson_problem *probs, int count);
+/* Writes every problem's face values into the halo of u. */
+int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int count, void *workspace,
+                             size_t workspace_bytes, lgdhip_stream_t stream);
+/* FP64 residual norms ||f + sum_nb u - 6u||_2 into the host array norms. */
+int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_poisson_problem *probs,
+                                  int count, void *workspace, size_t workspace_bytes, double *norms,
+                                  lgdhip_stream_t stream);
+/* The hot sweep alone: `sweeps` FP32 Jacobi sweeps on A e = r for every
+ * problem (e pools with zero halos, ping-pong ea/eb). */
+int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count,
+                              int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
+                              lgdhip_stream_t stream);
+/* Solves every problem from the guess in u (interior) to opt->rel_tol:
+ * FP64 residual and update, FP32 Jacobi sweeps on the correction, all
+ * active problems swept by one launch per sweep. */
+int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
+                               const lgdhip_poisson_options *opt, void *workspace, size_t workspace_bytes,
+                               lgdhip_poisson_result *results, lgdhip_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -339,8 +339,10 @@ __global__ __launch_bounds__(kLanes *NW) void life_packed_stacked(const uint32_t
 //   5: 4 stacked waves x 64 rows                 6: 8 stacked waves x 64 rows
 //   7: 8 stacked waves x 32 rows                 8: 16 stacked waves x 32 rows
 constexpr int kNumVariants = 8;
-constexpr int kDefaultVariant = 6;
-constexpr int kDefaultPerLaunch = 16;
+// tuned on MI355X for 16384^2 (8 waves x 32 rows, 31 generations per launch:
+// 4.9 ms for 1000 generations); see the variant sweep in the profiles
+constexpr int kDefaultVariant = 7;
+constexpr int kDefaultPerLaunch = 31;
 
 // register rows one work-group column holds for a variant
 int rows_per_group(int variant) {

@@ -1,0 +1,545 @@
+// Batched ensemble of independent 3D Poisson problems solved by Jacobi.
+//
+// Problem p: n^3 interior unknowns, discrete Poisson equation
+//     6 u_c - sum_{6 neighbours} u_nb = f_c
+// (h = 1), constant Dirichlet value per face outside the cube.  Jacobi:
+//     u_c <- (sum_nb u_nb + f_c) / 6.
+// The reference has no counterpart (no source term, residual or batching,
+// SURVEY row A19); the sweep is the Jacobi cell of
+// src/testbed/performancetests/main.cpp:1194-1205 with a source term.
+//
+// Precision scheme: FP32 storage of u cannot reach a 1e-6 relative
+// residual (the rounding floor of u is ~1e-4 of the initial residual for
+// these sizes), so the solver is a mixed-precision refinement:
+//   u (FP64) -> r = f + sum_nb u - 6u in FP64, stored FP32, with the FP64
+//   residual norm;  e (FP32) <- m Jacobi sweeps on A e = r from e = 0
+//   (12 bytes per cell and sweep: read e, read r, write e);  u += e.
+// Jacobi on the correction is the same linear iteration as Jacobi on u, so
+// the sweep count to a tolerance matches plain FP64 Jacobi; only the
+// arithmetic of the hot loop is FP32.
+//
+// Layout: each problem occupies a padded (n+2)^3 block, x fastest, at its
+// element offset in the pools (u: double, f/e/r: float).  u's halo holds the
+// face values; e's halo stays zero.  All kernels walk one list of tiles
+// (problem, x0, y0, z0) covering every active problem, so one launch sweeps
+// the whole ensemble; norms are reduced deterministically (per-tile FP64
+// partials, then per-problem sums in tile order).
+#include "lgdhip_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+constexpr int kTX = 32;  // tile width (x), one 128-byte FP32 row segment
+constexpr int kTY = 8;   // tile height (y)
+constexpr int kTZ = 16;  // z planes marched per tile
+constexpr int kThreads = kTX * kTY;
+
+struct ProblemDev {
+    int32_t n;
+    int32_t pad;
+    int64_t base;  // element offset of the padded block
+};
+
+struct Tile {
+    int32_t p, x0, y0, z0;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) lds[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0) {
+        for (int w = 0; w < kThreads / 64; ++w) s += lds[w];
+    }
+    return s;
+}
+
+// r = f + sum_nb u - 6 u (FP64), r32 = (float) r, per-tile sum of r^2
+__global__ __launch_bounds__(kThreads) void poisson_residual_f64(const double *__restrict__ u,
+                                                                 const float *__restrict__ f,
+                                                                 float *__restrict__ r32, const Tile *__restrict__ tiles,
+                                                                 const ProblemDev *__restrict__ probs,
+                                                                 double *__restrict__ partial) {
+    __shared__ double lds[kThreads / 64];
+    const Tile t = tiles[blockIdx.x];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int x = t.x0 + (int)(threadIdx.x % kTX);
+    const int y = t.y0 + (int)(threadIdx.x / kTX);
+    const int64_t sx = n + 2, sxy = sx * sx;
+    double acc = 0.0;
+    if (x < n && y < n) {
+        const int z1 = min(t.z0 + kTZ, n);
+        int64_t i = P.base + ((int64_t)(t.z0 + 1) * sx + (y + 1)) * sx + (x + 1);
+        for (int z = t.z0; z < z1; ++z, i += sxy) {
+            double s = u[i - sxy];
+            s += u[i - sx];
+            s += u[i - 1];
+            s += u[i + 1];
+            s += u[i + sx];
+            s += u[i + sxy];
+            const double r = (double)f[i] + s - 6.0 * u[i];
+            r32[i] = (float)r;
+            acc += r * r;
+        }
+    }
+    const double s = block_sum(acc, lds);
+    if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// one FP32 Jacobi sweep on A e = r:  e' = (e[z-1] + e[y-1] + e[x-1] + e[x+1]
+// + e[y+1] + e[z+1] + r) * (1/6), summed in that order; when `partial` is
+// given, also the per-tile sum of the FP64 square of the residual of the
+// input iterate, r + sum_nb e - 6 e (= 6 (e' - e) up to rounding)
+__global__ __launch_bounds__(kThreads) void poisson_sweep_f32(const float *__restrict__ e, float *__restrict__ e2,
+                                                              const float *__restrict__ r,
+                                                              const Tile *__restrict__ tiles,
+                                                              const ProblemDev *__restrict__ probs,
+                                                              double *__restrict__ partial) {
+    __shared__ double lds[kThreads / 64];
+    const Tile t = tiles[blockIdx.x];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int x = t.x0 + (int)(threadIdx.x % kTX);
+    const int y = t.y0 + (int)(threadIdx.x / kTX);
+    const int64_t sx = n + 2, sxy = sx * sx;
+    double acc = 0.0;
+    if (x < n && y < n) {
+        const int z1 = min(t.z0 + kTZ, n);
+        int64_t i = P.base + ((int64_t)(t.z0 + 1) * sx + (y + 1)) * sx + (x + 1);
+        float below = e[i - sxy];
+        float center = e[i];
+        for (int z = t.z0; z < z1; ++z, i += sxy) {
+            const float above = e[i + sxy];
+            float s = below;
+            s += e[i - sx];
+            s += e[i - 1];
+            s += e[i + 1];
+            s += e[i + sx];
+            s += above;
+            s += r[i];
+            e2[i] = s * (1.0f / 6.0f);
+            if (partial) {
+                const double rr = (double)s - 6.0 * (double)center;
+                acc += rr * rr;
+            }
+            below = center;
+            center = above;
+        }
+    }
+    if (partial) {
+        const double s = block_sum(acc, lds);
+        if (threadIdx.x == 0) partial[blockIdx.x] = s;
+    }
+}
+
+// u += e over the interior and reset both correction buffers to zero
+__global__ __launch_bounds__(kThreads) void poisson_update_f64(double *__restrict__ u, float *e, float *eother,
+                                                               const Tile *__restrict__ tiles,
+                                                               const ProblemDev *__restrict__ probs) {
+    const Tile t = tiles[blockIdx.x];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int x = t.x0 + (int)(threadIdx.x % kTX);
+    const int y = t.y0 + (int)(threadIdx.x / kTX);
+    if (x >= n || y >= n) return;
+    const int64_t sx = n + 2, sxy = sx * sx;
+    const int z1 = min(t.z0 + kTZ, n);
+    int64_t i = P.base + ((int64_t)(t.z0 + 1) * sx + (y + 1)) * sx + (x + 1);
+    for (int z = t.z0; z < z1; ++z, i += sxy) {
+        u[i] += (double)e[i];
+        e[i] = 0.0f;
+        eother[i] = 0.0f;
+    }
+}
+
+// norms[k] = sum of partial over the tiles [begin[k], end[k])  (fixed order)
+__global__ __launch_bounds__(256) void poisson_reduce(const double *__restrict__ partial, const int *__restrict__ begin,
+                                                      const int *__restrict__ end, double *__restrict__ norms) {
+    __shared__ double lds[256];
+    const int k = blockIdx.x;
+    double s = 0.0;
+    for (int i = begin[k] + (int)threadIdx.x; i < end[k]; i += 256) s += partial[i];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) norms[k] = lds[0];
+}
+
+// faces into the halo of u (edges and corners are never read)
+__global__ void poisson_faces(double *__restrict__ u, const ProblemDev *__restrict__ probs,
+                              const double *__restrict__ faces) {
+    const ProblemDev P = probs[blockIdx.y];
+    const int n = P.n;
+    const int64_t sx = n + 2;
+    const double *fv = faces + 6 * blockIdx.y;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n * n; k += gridDim.x * blockDim.x) {
+        const int a = k % n + 1, b = k / n + 1;
+        u[P.base + ((int64_t)b * sx + a) * sx + 0] = fv[0];          // x = -1
+        u[P.base + ((int64_t)b * sx + a) * sx + (n + 1)] = fv[1];    // x = n
+        u[P.base + ((int64_t)b * sx + 0) * sx + a] = fv[2];          // y = -1
+        u[P.base + ((int64_t)b * sx + (n + 1)) * sx + a] = fv[3];    // y = n
+        u[P.base + ((int64_t)0 * sx + b) * sx + a] = fv[4];          // z = -1
+        u[P.base + ((int64_t)(n + 1) * sx + b) * sx + a] = fv[5];    // z = n
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+int64_t padded_elems(int n) { return (int64_t)(n + 2) * (n + 2) * (n + 2); }
+
+int64_t tiles_of(int n) {
+    return (int64_t)((n + kTX - 1) / kTX) * ((n + kTY - 1) / kTY) * ((n + kTZ - 1) / kTZ);
+}
+
+void append_tiles(std::vector<Tile> *out, int p, int n) {
+    for (int z = 0; z < n; z += kTZ)
+        for (int y = 0; y < n; y += kTY)
+            for (int x = 0; x < n; x += kTX) out->push_back(Tile{p, x, y, z});
+}
+
+int check_problems(const lgdhip_poisson_problem *probs, int count) {
+    LGD_REQUIRE(probs && count > 0, "no problems");
+    for (int k = 0; k < count; ++k) {
+        if (probs[k].n < 1 || probs[k].n > 4096)
+            return lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, "problem edge length must be in [1, 4096]");
+        if (probs[k].offset < 0) return lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, "negative problem offset");
+    }
+    return 0;
+}
+
+// workspace carve-up, every piece 256-byte aligned
+struct Workspace {
+    float *e0, *e1, *r;
+    double *partial, *norms;
+    Tile *tiles;
+    ProblemDev *probs;
+    int *begin, *end;
+    double *faces;
+    size_t bytes;
+};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+Workspace carve(void *base, int64_t pool, int64_t ntiles, int count) {
+    Workspace w;
+    char *p = (char *)base;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = p ? p + off : nullptr;
+        off += align256(bytes);
+        return q;
+    };
+    w.e0 = (float *)take(pool * 4);
+    w.e1 = (float *)take(pool * 4);
+    w.r = (float *)take(pool * 4);
+    w.partial = (double *)take(ntiles * 8);
+    w.norms = (double *)take((size_t)count * 8);
+    w.tiles = (Tile *)take(ntiles * sizeof(Tile));
+    w.probs = (ProblemDev *)take((size_t)count * sizeof(ProblemDev));
+    w.begin = (int *)take((size_t)count * 4);
+    w.end = (int *)take((size_t)count * 4);
+    w.faces = (double *)take((size_t)count * 6 * 8);
+    w.bytes = off;
+    return w;
+}
+
+int64_t pool_extent(const lgdhip_poisson_problem *probs, int count) {
+    int64_t e = 0;
+    for (int k = 0; k < count; ++k) e = std::max(e, probs[k].offset + padded_elems(probs[k].n));
+    return e;
+}
+
+int64_t total_tiles(const lgdhip_poisson_problem *probs, int count) {
+    int64_t t = 0;
+    for (int k = 0; k < count; ++k) t += tiles_of(probs[k].n);
+    return t;
+}
+
+// Uploads problem descriptors and the tile list of the problems in `set`;
+// begin/end index the tiles of set[k].
+int upload_set(const Workspace &w, const lgdhip_poisson_problem *probs, const std::vector<int> &set,
+               std::vector<Tile> *tiles, hipStream_t s) {
+    tiles->clear();
+    std::vector<int> b(set.size()), e(set.size());
+    for (size_t k = 0; k < set.size(); ++k) {
+        b[k] = (int)tiles->size();
+        append_tiles(tiles, set[k], probs[set[k]].n);
+        e[k] = (int)tiles->size();
+    }
+    if (tiles->empty()) return 0;
+    LGD_HIP(hipMemcpyAsync(w.tiles, tiles->data(), tiles->size() * sizeof(Tile), hipMemcpyHostToDevice, s));
+    LGD_HIP(hipMemcpyAsync(w.begin, b.data(), b.size() * 4, hipMemcpyHostToDevice, s));
+    LGD_HIP(hipMemcpyAsync(w.end, e.data(), e.size() * 4, hipMemcpyHostToDevice, s));
+    LGD_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int residual_norms(const double *u, const float *f, const Workspace &w, const std::vector<Tile> &tiles, int nset,
+                   std::vector<double> *norms, hipStream_t s) {
+    if (tiles.empty()) return 0;
+    poisson_residual_f64<<<(unsigned)tiles.size(), kThreads, 0, s>>>(u, f, w.r, w.tiles, w.probs, w.partial);
+    LGD_LAUNCHED("poisson_residual_f64");
+    poisson_reduce<<<(unsigned)nset, 256, 0, s>>>(w.partial, w.begin, w.end, w.norms);
+    LGD_LAUNCHED("poisson_reduce");
+    norms->resize(nset);
+    LGD_HIP(hipMemcpyAsync(norms->data(), w.norms, (size_t)nset * 8, hipMemcpyDeviceToHost, s));
+    LGD_HIP(hipStreamSynchronize(s));
+    for (double &v : *norms) v = std::sqrt(v);
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int64_t lgdhip_poisson_layout(lgdhip_poisson_problem *probs, int count) {
+    if (!probs || count <= 0) return 0;
+    int64_t off = 0;
+    for (int k = 0; k < count; ++k) {
+        probs[k].offset = off;
+        // keep every problem block 256-byte aligned in the FP32 pools
+        off += (padded_elems(probs[k].n) + 63) / 64 * 64;
+    }
+    return off;
+}
+
+size_t lgdhip_poisson_workspace_bytes(const lgdhip_poisson_problem *probs, int count) {
+    if (check_problems(probs, count)) return 0;
+    return carve(nullptr, pool_extent(probs, count), total_tiles(probs, count), count).bytes;
+}
+
+int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int count, void *workspace,
+                             size_t workspace_bytes, lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(u && workspace, "NULL pointer");
+    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    std::vector<ProblemDev> pd(count);
+    std::vector<double> faces(6 * (size_t)count);
+    int nmax = 1;
+    for (int k = 0; k < count; ++k) {
+        pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+        std::memcpy(&faces[6 * (size_t)k], probs[k].face, 6 * sizeof(double));
+        nmax = std::max(nmax, probs[k].n);
+    }
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+    LGD_HIP(hipMemcpyAsync(w.faces, faces.data(), faces.size() * 8, hipMemcpyHostToDevice, s));
+    LGD_REQUIRE(count <= 65535, "at most 65535 problems");
+    dim3 grid(lgdhip::ceil_div((int64_t)nmax * nmax, 256), (unsigned)count);
+    poisson_faces<<<grid, 256, 0, s>>>(u, w.probs, w.faces);
+    LGD_LAUNCHED("poisson_faces");
+    LGD_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count,
+                              int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
+                              lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
+    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    std::vector<ProblemDev> pd(count);
+    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    std::vector<Tile> tiles;
+    rc = upload_set(w, probs, all, &tiles, s);
+    if (rc) return rc;
+    float *cur = ea, *nxt = eb;
+    for (int t = 0; t < sweeps; ++t) {
+        poisson_sweep_f32<<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr);
+        LGD_LAUNCHED("poisson_sweep_f32");
+        std::swap(cur, nxt);
+    }
+    if (result_in_b) *result_in_b = cur == eb ? 1 : 0;
+    return 0;
+}
+
+int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
+                                  void *workspace, size_t workspace_bytes, double *norms,
+                                  lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(u && f && workspace && norms, "NULL pointer");
+    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    std::vector<ProblemDev> pd(count);
+    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    std::vector<Tile> tiles;
+    rc = upload_set(w, probs, all, &tiles, s);
+    if (rc) return rc;
+    std::vector<double> nv;
+    rc = residual_norms(u, f, w, tiles, count, &nv, s);
+    if (rc) return rc;
+    std::memcpy(norms, nv.data(), nv.size() * 8);
+    return 0;
+}
+
+int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
+                               const lgdhip_poisson_options *opt, void *workspace, size_t workspace_bytes,
+                               lgdhip_poisson_result *results, lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(u && f && workspace && results && opt, "NULL pointer");
+    LGD_REQUIRE(opt->rel_tol > 0 && opt->chunk >= 1 && opt->max_sweeps >= 1, "bad solver options");
+    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    const bool graphs = opt->use_graph != 0 && s != nullptr;
+    const double theta = opt->inner_reduction > 0 ? opt->inner_reduction : 1e-2;
+
+    std::vector<ProblemDev> pd(count);
+    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+
+    // zero the correction pools (a kernel fill, never a memset node)
+    const int64_t pool = pool_extent(probs, count);
+    rc = lgdhip_fill_u8(w.e0, 0, (size_t)pool * 4, stream);
+    if (rc) return rc;
+    rc = lgdhip_fill_u8(w.e1, 0, (size_t)pool * 4, stream);
+    if (rc) return rc;
+
+    // FP64 residual of the caller's initial guess.  The tolerance is relative
+    // to opt->reference_norms[k] when given (e.g. the zero-guess residual when
+    // u holds a predicted guess), else to this initial residual.
+    std::vector<int> active(count);
+    for (int k = 0; k < count; ++k) active[k] = k;
+    std::vector<Tile> tiles;
+    rc = upload_set(w, probs, active, &tiles, s);
+    if (rc) return rc;
+    std::vector<double> rn;
+    rc = residual_norms(u, f, w, tiles, count, &rn, s);
+    if (rc) return rc;
+
+    struct State {
+        double ref, outer, inner;
+        int64_t sweeps, inner_sweeps;
+        int outer_iters;
+        bool done;
+    };
+    std::vector<State> st(count);
+    for (int k = 0; k < count; ++k) {
+        const double ref = (opt->reference_norms && opt->reference_norms[k] > 0) ? opt->reference_norms[k] : rn[k];
+        st[k] = State{ref, rn[k], rn[k], 0, 0, 0, rn[k] <= opt->rel_tol * ref};
+    }
+
+    float *ecur = w.e0, *enxt = w.e1;  // every problem's correction lives in ecur between chunks
+    hipGraphExec_t graph = nullptr;
+    bool graph_valid = false;
+    while (true) {
+        active.clear();
+        for (int k = 0; k < count; ++k)
+            if (!st[k].done) active.push_back(k);
+        if (active.empty()) break;
+        rc = upload_set(w, probs, active, &tiles, s);
+        if (rc) return rc;
+        graph_valid = false;
+
+        bool set_changed = false;
+        while (!set_changed) {
+            // one chunk: chunk-1 plain sweeps, then a sweep that also
+            // reports the inner residual of its input iterate; an even
+            // chunk leaves the correction in ecur, so the captured graph
+            // stays valid from chunk to chunk
+            const int C = opt->chunk + (opt->chunk & 1);
+            auto enqueue = [&](float *a, float *b) -> int {
+                for (int t = 0; t < C; ++t) {
+                    poisson_sweep_f32<<<(unsigned)tiles.size(), kThreads, 0, s>>>(
+                        a, b, w.r, w.tiles, w.probs, t == C - 1 ? w.partial : nullptr);
+                    std::swap(a, b);
+                }
+                poisson_reduce<<<(unsigned)active.size(), 256, 0, s>>>(w.partial, w.begin, w.end, w.norms);
+                return 0;
+            };
+            if (graphs) {
+                if (!graph_valid) {
+                    if (graph) hipGraphExecDestroy(graph);
+                    graph = nullptr;
+                    LGD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    enqueue(ecur, enxt);
+                    hipGraph_t g = nullptr;
+                    LGD_HIP(hipStreamEndCapture(s, &g));
+                    hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                    hipGraphDestroy(g);
+                    if (e != hipSuccess) return lgdhip::hip_fail(e, "hipGraphInstantiate");
+                    graph_valid = true;
+                }
+                LGD_HIP(hipGraphLaunch(graph, s));
+            } else {
+                enqueue(ecur, enxt);
+                LGD_LAUNCHED("poisson_sweep_f32");
+            }
+            std::vector<double> inner(active.size());
+            LGD_HIP(hipMemcpyAsync(inner.data(), w.norms, inner.size() * 8, hipMemcpyDeviceToHost, s));
+            LGD_HIP(hipStreamSynchronize(s));
+
+            // problems whose correction has converged far enough get an
+            // FP64 update u += e and a fresh FP64 residual
+            std::vector<int> refresh;
+            for (size_t a = 0; a < active.size(); ++a) {
+                State &q = st[active[a]];
+                q.sweeps += C;
+                q.inner_sweeps += C;
+                q.inner = std::sqrt(inner[a]);
+                const bool inner_ok = q.inner <= theta * q.outer || q.inner <= 0.5 * opt->rel_tol * q.ref;
+                if (inner_ok || q.sweeps >= opt->max_sweeps) refresh.push_back(active[a]);
+            }
+            if (refresh.empty()) continue;
+            std::vector<Tile> rt;
+            rc = upload_set(w, probs, refresh, &rt, s);
+            if (rc) return rc;
+            poisson_update_f64<<<(unsigned)rt.size(), kThreads, 0, s>>>(u, ecur, enxt, w.tiles, w.probs);
+            LGD_LAUNCHED("poisson_update_f64");
+            rc = residual_norms(u, f, w, rt, (int)refresh.size(), &rn, s);
+            if (rc) return rc;
+            for (size_t a = 0; a < refresh.size(); ++a) {
+                State &q = st[refresh[a]];
+                q.outer = rn[a];
+                q.outer_iters += 1;
+                q.inner_sweeps = 0;
+                if (q.outer <= opt->rel_tol * q.ref || q.sweeps >= opt->max_sweeps) {
+                    q.done = true;
+                    set_changed = true;
+                }
+            }
+            // the refreshed problems restart their correction from zero; the
+            // tile list on the device was replaced, so re-upload the active set
+            rc = upload_set(w, probs, active, &tiles, s);
+            if (rc) return rc;
+            graph_valid = false;
+        }
+    }
+    if (graph) hipGraphExecDestroy(graph);
+    for (int k = 0; k < count; ++k) {
+        results[k].sweeps = st[k].sweeps;
+        results[k].outer_iterations = st[k].outer_iters;
+        results[k].converged = st[k].outer <= opt->rel_tol * st[k].ref ? 1 : 0;
+        results[k].reference_norm = st[k].ref;
+        results[k].residual_norm = st[k].outer;
+    }
+    return 0;
+}
+
+} // extern "C"

@@ -70,11 +70,12 @@ __global__ void lgd_fill_u32_kernel(uint32_t *__restrict__ dst, uint32_t v, size
     for (; i < n; i += stride) dst[i] = v;
 }
 
+// one 16-byte element per thread, no grid-stride loop: the fastest plain
+// streaming copy measured on MI355X (~6.1 TB/s read+write for 2-3 GiB)
 __global__ __launch_bounds__(256) void lgd_copy16_kernel(uint4 *__restrict__ dst,
                                                          const uint4 *__restrict__ src, size_t n) {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 // Writes the edge value into every halo element of one member plane.
@@ -222,8 +223,8 @@ int lgdhip_copy_kernel(void *dst, const void *src, size_t bytes, lgdhip_stream_t
     LGD_REQUIRE(((uintptr_t)dst & 15) == 0 && ((uintptr_t)src & 15) == 0, "pointers must be 16-byte aligned");
     const size_t n = bytes / 16;
     if (n == 0) return 0;
-    size_t blocks = (n + 255) / 256;
-    if (blocks > 8192) blocks = 8192;
+    const size_t blocks = (n + 255) / 256;
+    LGD_REQUIRE(blocks <= 0xFFFFFFFFull, "copy too large for one launch");
     lgd_copy16_kernel<<<(unsigned)blocks, 256, 0, as_stream(stream)>>>((uint4 *)dst, (const uint4 *)src, n);
     LGD_LAUNCHED("lgdhip_copy_kernel");
     return 0;

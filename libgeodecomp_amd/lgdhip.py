@@ -46,6 +46,21 @@ class lgdhip_lbm_params(C.Structure):
     _fields_ = [("omega", C.c_float), ("lid_ux", C.c_float), ("lid_uy", C.c_float), ("lid_uz", C.c_float)]
 
 
+class lgdhip_poisson_problem(C.Structure):
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("offset", C.c_int64), ("face", C.c_double * 6)]
+
+
+class lgdhip_poisson_options(C.Structure):
+    _fields_ = [("rel_tol", C.c_double), ("chunk", C.c_int32), ("max_sweeps", C.c_int32),
+                ("inner_reduction", C.c_double), ("use_graph", C.c_int32), ("reserved", C.c_int32),
+                ("reference_norms", C.POINTER(C.c_double))]
+
+
+class lgdhip_poisson_result(C.Structure):
+    _fields_ = [("sweeps", C.c_int64), ("outer_iterations", C.c_int32), ("converged", C.c_int32),
+                ("reference_norm", C.c_double), ("residual_norm", C.c_double)]
+
+
 LBM_WALL_XLO, LBM_WALL_XHI = 0x01, 0x02
 LBM_WALL_YLO, LBM_WALL_YHI = 0x04, 0x08
 LBM_WALL_ZLO, LBM_WALL_ZHI = 0x10, 0x20
@@ -106,6 +121,15 @@ SIGNATURES = {
     "lgdhip_lbm_d3q19_f32_run": (_I, [_P, _P, _P, _GP, C.POINTER(lgdhip_lbm_params), _I, _TP, _IP, _P]),
     "lgdhip_lbm_cavity_flags": (_I, [_P, _GP, _P]),
     "lgdhip_lbm_d3q19_f32_moments": (_I, [_P, _GP, _P, _P, _P, _P, _P]),
+    "lgdhip_poisson_layout": (_I64, [C.POINTER(lgdhip_poisson_problem), _I]),
+    "lgdhip_poisson_workspace_bytes": (_SZ, [C.POINTER(lgdhip_poisson_problem), _I]),
+    "lgdhip_poisson_set_faces": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ, _P]),
+    "lgdhip_poisson_residual_norms": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ,
+                                           C.POINTER(C.c_double), _P]),
+    "lgdhip_poisson_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _P, _SZ, _IP, _P]),
+    "lgdhip_poisson_batch_solve": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I,
+                                        C.POINTER(lgdhip_poisson_options), _P, _SZ,
+                                        C.POINTER(lgdhip_poisson_result), _P]),
 }
 
 # functions returning a status code (checked); the others return values

@@ -63,3 +63,32 @@ def lbm_initial_velocity(nx: int, ny: int, nz: int, amplitude: float = 1e-3,
     n = nx * ny * nz
     u = [(2.0 * uniform01(seed + k, n) - 1.0) * amplitude for k in (1, 2, 3)]
     return tuple(x.reshape(nz, ny, nx) for x in u)
+
+
+def poisson_ensemble(count: int, sizes=tuple(range(32, 193, 16)), seed: int = DEFAULT_SEED):
+    """Ensemble of Poisson problems on n^3 cubes (n drawn from `sizes`): the
+    source is a sum of 1-8 Gaussians (centres uniform in the cube, widths
+    uniform[2,12] cells, amplitudes N(0,1)) evaluated in FP64 at the cell
+    centres and stored FP32; each of the six faces gets a Dirichlet value
+    uniform[-1,1].  Returns a list of (n, faces[6], f) with f the padded
+    (n+2)^3 float32 block (halo zero), x fastest."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.choice(np.asarray(sizes)))
+        k = int(rng.integers(1, 9))
+        centres = rng.uniform(0.0, n, size=(k, 3))
+        widths = rng.uniform(2.0, 12.0, size=k)
+        amps = rng.standard_normal(k)
+        faces = rng.uniform(-1.0, 1.0, size=6)
+        ax = np.arange(n, dtype=np.float64)
+        f = np.zeros((n, n, n), dtype=np.float64)
+        for (cx, cy, cz), w, a in zip(centres, widths, amps):
+            gx = np.exp(-(ax - cx) ** 2 / (2 * w * w))
+            gy = np.exp(-(ax - cy) ** 2 / (2 * w * w))
+            gz = np.exp(-(ax - cz) ** 2 / (2 * w * w))
+            f += a * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+        fp = np.zeros((n + 2, n + 2, n + 2), dtype=np.float32)
+        fp[1:-1, 1:-1, 1:-1] = f.astype(np.float32)
+        out.append((n, faces, fp))
+    return out

@@ -21,6 +21,7 @@
  */
 #include "lgd_oracle.h"
 
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -257,4 +258,87 @@ int lgdo_max_threads(void) {
 #else
     return 1;
 #endif
+}
+
+/* ------------------------------------------------------------------ */
+/* Poisson problems (padded (n+2)^3 blocks, x fastest)                 */
+/* ------------------------------------------------------------------ */
+void lgdo_poisson_set_faces(double *u, long n, const double face[6]) {
+    const long s = n + 2;
+    for (long b = 1; b <= n; ++b)
+        for (long a = 1; a <= n; ++a) {
+            u[(b * s + a) * s + 0] = face[0];
+            u[(b * s + a) * s + (n + 1)] = face[1];
+            u[(b * s + 0) * s + a] = face[2];
+            u[(b * s + (n + 1)) * s + a] = face[3];
+            u[(0 * s + b) * s + a] = face[4];
+            u[((n + 1) * s + b) * s + a] = face[5];
+        }
+}
+
+/* one FP32 Jacobi sweep on A e = r, in the summation order of the kernel */
+void lgdo_poisson_sweep_f32(const float *e, float *e2, const float *r, long n) {
+    const long s = n + 2, sxy = s * s;
+    for (long z = 1; z <= n; ++z)
+        for (long y = 1; y <= n; ++y)
+            for (long x = 1; x <= n; ++x) {
+                const long i = (z * s + y) * s + x;
+                float v = e[i - sxy];
+                v += e[i - s];
+                v += e[i - 1];
+                v += e[i + 1];
+                v += e[i + s];
+                v += e[i + sxy];
+                v += r[i];
+                e2[i] = v * (1.0f / 6.0f);
+            }
+}
+
+double lgdo_poisson_residual_norm(const double *u, const float *f, long n) {
+    const long s = n + 2, sxy = s * s;
+    double acc = 0.0;
+    for (long z = 1; z <= n; ++z)
+        for (long y = 1; y <= n; ++y)
+            for (long x = 1; x <= n; ++x) {
+                const long i = (z * s + y) * s + x;
+                const double nb = u[i - sxy] + u[i - s] + u[i - 1] + u[i + 1] + u[i + s] + u[i + sxy];
+                const double r = (double)f[i] + nb - 6.0 * u[i];
+                acc += r * r;
+            }
+    return sqrt(acc);
+}
+
+/* plain FP64 Jacobi u <- (sum_nb u + f)/6 from the given u until
+ * ||r|| <= rel_tol * ||r(initial u)|| (or reference if > 0), checking every
+ * `check` sweeps; returns the sweep count */
+long lgdo_poisson_jacobi_f64(double *u, const float *f, long n, double rel_tol, double reference, long max_sweeps,
+                             long check, double *final_norm) {
+    const long s = n + 2, sxy = s * s, total = s * s * s;
+    double *v = (double *)malloc((size_t)total * sizeof(double));
+    if (!v) return -1;
+    memcpy(v, u, (size_t)total * sizeof(double));
+    const double ref = reference > 0 ? reference : lgdo_poisson_residual_norm(u, f, n);
+    double *cur = u, *nxt = v;
+    long sweeps = 0;
+    double rn = lgdo_poisson_residual_norm(cur, f, n);
+    while (rn > rel_tol * ref && sweeps < max_sweeps) {
+        for (long k = 0; k < check; ++k) {
+            for (long z = 1; z <= n; ++z)
+                for (long y = 1; y <= n; ++y)
+                    for (long x = 1; x <= n; ++x) {
+                        const long i = (z * s + y) * s + x;
+                        const double nb = cur[i - sxy] + cur[i - s] + cur[i - 1] + cur[i + 1] + cur[i + s] + cur[i + sxy];
+                        nxt[i] = (nb + (double)f[i]) * (1.0 / 6.0);
+                    }
+            double *t = cur;
+            cur = nxt;
+            nxt = t;
+            ++sweeps;
+        }
+        rn = lgdo_poisson_residual_norm(cur, f, n);
+    }
+    if (cur != u) memcpy(u, cur, (size_t)total * sizeof(double));
+    free(v);
+    if (final_norm) *final_norm = rn;
+    return sweeps;
 }

@@ -28,6 +28,12 @@
  *    (sum f_eq = rho, sum e f_eq = j, closed form); mirror symmetry of the
  *    cavity about its x-y plane; FP32 and FP64 versions agree to FP32
  *    rounding.
+ *  - lgdo_poisson_*: the residual decays by exactly cos(pi/(n+1)) per
+ *    sweep for the smoothest eigenvector source (closed form of the Jacobi
+ *    spectrum); integer manufactured solutions with integer faces give an
+ *    exact right-hand side and are recovered; the FP32 sweep is the FP64
+ *    sweep rounded (compared to tolerance) and a brute-force Python loop on
+ *    tiny cubes.
  */
 #ifndef LGD_ORACLE_H
 #define LGD_ORACLE_H
@@ -59,6 +65,13 @@ int lgdo_lbm_d3q19_float(float *f, const uint8_t *flags, long nx, long ny, long 
 void lgdo_lbm_equilibrium_f64(double *f, const double *rho, const double *jx, const double *jy, const double *jz,
                               long ncells);
 void lgdo_lbm_moments_f64(const double *f, long ncells, double *rho, double *jx, double *jy, double *jz);
+
+/* Poisson problems: padded (n+2)^3 blocks, x fastest, faces in u's halo */
+void lgdo_poisson_set_faces(double *u, long n, const double face[6]);
+void lgdo_poisson_sweep_f32(const float *e, float *e2, const float *r, long n);
+double lgdo_poisson_residual_norm(const double *u, const float *f, long n);
+long lgdo_poisson_jacobi_f64(double *u, const float *f, long n, double rel_tol, double reference, long max_sweeps,
+                             long check, double *final_norm);
 
 int lgdo_max_threads(void);
 

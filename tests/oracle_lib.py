@@ -34,6 +34,14 @@ def load():
         lib.lgdo_lbm_moments_f64.argtypes = [P, L, P, P, P, P]
         lib.lgdo_lbm_moments_f64.restype = None
         lib.lgdo_lbm_direction.argtypes = [I, C.POINTER(I), C.POINTER(I), C.POINTER(I), C.POINTER(I)]
+        lib.lgdo_poisson_set_faces.argtypes = [P, L, P]
+        lib.lgdo_poisson_set_faces.restype = None
+        lib.lgdo_poisson_sweep_f32.argtypes = [P, P, P, L]
+        lib.lgdo_poisson_sweep_f32.restype = None
+        lib.lgdo_poisson_residual_norm.argtypes = [P, P, L]
+        lib.lgdo_poisson_residual_norm.restype = D
+        lib.lgdo_poisson_jacobi_f64.argtypes = [P, P, L, D, D, L, L, C.POINTER(D)]
+        lib.lgdo_poisson_jacobi_f64.restype = L
         _lib = lib
     return _lib
 
@@ -102,3 +110,27 @@ def lbm_moments(f: np.ndarray):
     out = [np.empty(shp, dtype=np.float64) for _ in range(4)]
     load().lgdo_lbm_moments_f64(_p(f), int(np.prod(shp)), *(_p(o) for o in out))
     return tuple(out)
+
+
+def poisson_faces(u: np.ndarray, faces) -> None:
+    fa = np.ascontiguousarray(faces, dtype=np.float64)
+    load().lgdo_poisson_set_faces(_p(u), u.shape[0] - 2, _p(fa))
+
+
+def poisson_sweep_f32(e: np.ndarray, r: np.ndarray) -> np.ndarray:
+    out = np.zeros_like(e)
+    load().lgdo_poisson_sweep_f32(_p(e), _p(out), _p(r), e.shape[0] - 2)
+    return out
+
+
+def poisson_residual_norm(u: np.ndarray, f: np.ndarray) -> float:
+    return float(load().lgdo_poisson_residual_norm(_p(u), _p(f), u.shape[0] - 2))
+
+
+def poisson_jacobi(u: np.ndarray, f: np.ndarray, rel_tol: float, reference: float = 0.0, max_sweeps: int = 10 ** 7,
+                   check: int = 1):
+    """Plain FP64 Jacobi in place; returns (sweeps, final residual norm)."""
+    fn = C.c_double(0.0)
+    sweeps = load().lgdo_poisson_jacobi_f64(_p(u), _p(f), u.shape[0] - 2, rel_tol, reference, max_sweeps, check,
+                                            C.byref(fn))
+    return int(sweeps), fn.value
