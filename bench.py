@@ -298,7 +298,40 @@ def equilibrium_f32(rho, jx, jy, jz):
     return f
 
 
-WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM}
+class Copy:
+    """Calibration: the plain 16-byte streaming copy kernel over 2 GiB, the
+    measured HBM ceiling the stencil kernels are compared with (not a
+    workload of the simulator)."""
+    name = "copy"
+    dtype = "u8"
+    bytes_per_lup = 2.0  # one byte read + one written per copied byte
+
+    def __init__(self, n=1 << 31, **_):
+        self.n = n
+        self.stream = lgdhip.Stream()
+        self.a = lgdhip.DeviceBuffer(n)
+        self.b = lgdhip.DeviceBuffer(n)
+        lgdhip.call("lgdhip_fill_u8", self.a.ptr, 1, n, self.stream.h)
+
+    def config(self):
+        return {"workload": f"device copy calibration, {self.n} bytes", "bytes": self.n, "parallelism": "single GPU"}
+
+    def lups_per_step(self):
+        return float(self.n)
+
+    def step(self):
+        lgdhip.call("lgdhip_copy_kernel", self.b.ptr, self.a.ptr, self.n, self.stream.h)
+
+    def dominant_kernel(self):
+        launches = 10
+        ms = timed(self.stream, self.step, reps=launches)
+        return "lgd_copy16_kernel", ms / launches, 2.0 * self.n
+
+    def cpu_baseline(self, budget_s=0.0):
+        return None
+
+
+WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "copy": Copy}
 
 
 def main():
