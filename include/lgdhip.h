@@ -269,6 +269,58 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
                                const lgdhip_poisson_options *opt, void *workspace, size_t workspace_bytes,
                                lgdhip_poisson_result *results, lgdhip_stream_t stream);
 
+/* ------------------------------------------------------------------ */
+/* 3D U-Net building blocks for the learned initial guess             */
+/*   (no reference counterpart): BF16 operands on the MFMA cores,     */
+/*   FP32 accumulation.                                                */
+/* ------------------------------------------------------------------ */
+/* Activations are channels-last BF16 cubes: element (z,y,x,c) of an n^3
+ * tensor with channel stride ld and channel offset off is at
+ * ((z*n+y)*n+x)*ld + off + c.  Weights are BF16 [cout][Kp], k = tap*cin + ci,
+ * tap = (dz+1)*9 + (dy+1)*3 + (dx+1) (3x3x3) or 0 (1x1x1), zero padded to
+ * Kp = lgdhip_conv3d_kpad(cin, ksize).  Zero padding at the faces. */
+int lgdhip_conv3d_kpad(int cin, int ksize);
+/* y[v][co] (+)= act(sum_k x[v+tap][ci] w[co][k] + bias[co]); BF16 output to y
+ * (if non-NULL) and/or FP32 output to y32 (channel stride ld32).  relu != 0
+ * clamps at 0; accumulate != 0 adds the BF16 value already in y.  cout <= 192;
+ * when cin % 8 == 0, x must be 16-byte aligned and ldx, offx multiples of 8. */
+int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
+                               const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
+                               int relu, int accumulate, lgdhip_stream_t stream);
+/* weights of the input-gradient convolution: wt[ci][k'] (Kp' = kpad(cout)),
+ * flipped taps, so dx = conv(dy, wt) with cin' = cout, cout' = cin */
+int lgdhip_conv3d_dgrad_weights(const void *w, int cin, int cout, int ksize, void *wt, lgdhip_stream_t stream);
+/* dw[co][k] (+)= sum_v dy[v][co] x[v+tap(k)][ci(k)] (FP32, fixed-order
+ * reduction over voxel chunks), db[co] = sum_v dy[v][co] (if non-NULL) */
+size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, int chunk_vox);
+int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *dy, int ldy,
+                             int offy, int cout, float *dw, float *db, int accumulate, int chunk_vox,
+                             void *workspace, size_t workspace_bytes, lgdhip_stream_t stream);
+/* g *= (y > 0), in place */
+int lgdhip_relu_backward_bf16(void *g, int ldg, int offg, const void *y, int ldy, int offy, int c, int64_t nvox,
+                              lgdhip_stream_t stream);
+/* 2x2x2 average pool n^3 -> (n/2)^3, its gradient (accumulate adds) */
+int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int ldy, int offy, int c,
+                         lgdhip_stream_t stream);
+int lgdhip_avgpool2_backward_bf16(const void *dy, int n, int ldy, int offy, void *dx, int ldx, int offx, int c,
+                                  int accumulate, lgdhip_stream_t stream);
+/* nearest 2x upsample m^3 -> (2m)^3, its gradient (sum of the 8 children) */
+int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int ldy, int offy, int c,
+                          lgdhip_stream_t stream);
+int lgdhip_upsample2_backward_bf16(const void *dy, int m, int ldy, int offy, void *dx, int ldx, int offx, int c,
+                                   int accumulate, lgdhip_stream_t stream);
+/* FP32 field -> one BF16 channel of a channels-last tensor */
+int lgdhip_pack_bf16(const float *src, int64_t nvox, void *y, int ldy, int offy, lgdhip_stream_t stream);
+/* MSE: grad[v*ldg] = 2 (pred - target) / nvox (BF16), partial[b] = per-block
+ * sums of squared errors (lgdhip_mse_partials(nvox) entries) */
+int64_t lgdhip_mse_partials(int64_t nvox);
+int lgdhip_mse_loss(const float *pred, const float *target, int64_t nvox, void *grad, int ldg, float *partial,
+                    lgdhip_stream_t stream);
+/* Adam on FP32 master weights (kp > 0: [rows][kp] with the columns
+ * >= cols_used kept zero); writes BF16 copies to w_bf16 if non-NULL */
+int lgdhip_adam_step(float *w, float *m, float *v, const float *g, int64_t count, int cols_used, int kp, float lr,
+                     float beta1, float beta2, float eps, int t, void *w_bf16, lgdhip_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

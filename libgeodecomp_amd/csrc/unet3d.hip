@@ -1,0 +1,586 @@
+// 3D U-Net building blocks on the MFMA cores (BF16 operands, FP32 accumulate).
+//
+// No reference counterpart (the learned initial guess is new to this
+// build).  Conventions:
+//  * activations are channels-last BF16 cubes: element (z,y,x,c) of an n^3
+//    tensor with channel stride `ld` and channel offset `off` lives at
+//    ((z*n + y)*n + x)*ld + off + c; concatenation is two producers writing
+//    disjoint channel ranges of one buffer;
+//  * convolution weights are BF16 [cout][Kp] with k = tap*cin + ci,
+//    tap = (dz+1)*9 + (dy+1)*3 + (dx+1) for 3x3x3 (tap 0 for 1x1x1), K padded
+//    with zeros to Kp = roundup(taps*cin, 32); zero padding at the volume
+//    faces ("same" convolution);
+//  * a convolution is an implicit GEMM Y[v][co] = sum_k A(v,k) W[co][k] with
+//    A(v,k) = X[v + tap(k)][ci(k)], evaluated 16 voxels x 16 channels x 32 k
+//    at a time by v_mfma_f32_16x16x32_bf16.
+// dgrad is the same kernel with flipped, transposed weights; wgrad reduces
+// dY^T X over voxels with the same instruction and a fixed-order reduction.
+#include "lgdhip_internal.h"
+
+#include <hip/hip_bf16.h>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+// round-to-nearest-even FP32 -> BF16 (finite values)
+__device__ __forceinline__ u16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (u16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(u16 h) { return __uint_as_float((uint32_t)h << 16); }
+
+union Frag {
+    bf16x8 v;
+    uint4 q;
+    u16 h[8];
+};
+
+__device__ __forceinline__ void tap_offset(int tap, int ksize, int &dz, int &dy, int &dx) {
+    if (ksize == 1) {
+        dz = dy = dx = 0;
+    } else {
+        dz = tap / 9 - 1;
+        dy = (tap / 3) % 3 - 1;
+        dx = tap % 3 - 1;
+    }
+}
+
+// A fragment: 8 consecutive k of voxel (z, y, x) -- one 16-byte load when
+// the 8 k share a tap (cin % 8 == 0), else element-wise
+__device__ __forceinline__ Frag load_a(const u16 *__restrict__ X, int n, int ld, int off, int cin, int ksize,
+                                       int z, int y, int x, int k0, int K) {
+    Frag f;
+    f.q = make_uint4(0, 0, 0, 0);
+    if ((cin & 7) == 0) {
+        if (k0 < K) {
+            const int tap = k0 / cin, ci = k0 % cin;
+            int dz, dy, dx;
+            tap_offset(tap, ksize, dz, dy, dx);
+            const int zz = z + dz, yy = y + dy, xx = x + dx;
+            if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n) {
+                f.q = *reinterpret_cast<const uint4 *>(X + ((int64_t)((zz * n + yy) * n + xx)) * ld + off + ci);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = k0 + e;
+            if (k < K) {
+                const int tap = k / cin, ci = k % cin;
+                int dz, dy, dx;
+                tap_offset(tap, ksize, dz, dy, dx);
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n)
+                    f.h[e] = X[((int64_t)((zz * n + yy) * n + xx)) * ld + off + ci];
+            }
+        }
+    }
+    return f;
+}
+
+constexpr int kMaxNTiles = 12;  // cout up to 192 (the dgrad of a 192-channel concat input)
+
+// One wave: 16 consecutive voxels (linear order, any n), all cout (NT tiles of 16).
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
+                                                       int ksize, const u16 *__restrict__ W, int Kp,
+                                                       const float *__restrict__ bias, int cout, u16 *Y, int ldy,
+                                                       int offy, float *Y32, int ld32, int relu, int accumulate) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int64_t nvox = (int64_t)n * n * n;
+    const int64_t v0 = wave * 16;
+    if (v0 >= nvox) return;
+    const int i = lane & 15;   // voxel of the A rows / channel of the B columns
+    const int kq = lane >> 4;  // which 8 of the 32 k
+    const int K = (ksize == 1 ? 1 : 27) * cin;
+    // this lane's A voxel
+    const int64_t va = v0 + i;
+    const bool a_ok = va < nvox;
+    const int ax = (int)(va % n), ay = (int)((va / n) % n), az = (int)(va / ((int64_t)n * n));
+
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int kc = 0; kc < Kp; kc += 32) {
+        const int k0 = kc + 8 * kq;
+        Frag a;
+        a.q = make_uint4(0, 0, 0, 0);
+        if (a_ok) a = load_a(X, n, ldx, offx, cin, ksize, az, ay, ax, k0, K);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            Frag b;
+            b.q = make_uint4(0, 0, 0, 0);
+            if (t * 16 + i < cout) b.q = *reinterpret_cast<const uint4 *>(W + (int64_t)(t * 16 + i) * Kp + k0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[t], 0, 0, 0);
+        }
+    }
+    // D[r][c] in lane c + 16*(r/4), element r%4: voxel v0 + 4*(lane/16) + e, channel t*16 + lane%16
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + i;
+        if (co >= cout) continue;
+        const float b = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t v = v0 + 4 * kq + e;
+            if (v >= nvox) continue;
+            float r = acc[t][e] + b;
+            if (relu) r = r > 0.0f ? r : 0.0f;
+            if (Y) {
+                u16 *p = Y + v * ldy + offy + co;
+                if (accumulate) r += bf2f(*p);
+                *p = f2bf(r);
+            }
+            if (Y32) Y32[v * ld32 + co] = r;
+        }
+    }
+}
+
+// dW partial over a chunk of voxels: P[chunk][co][k] = sum_v dY[v][co] X[v+tap(k)][ci(k)]
+// one wave per (chunk, k tile of 16), all co tiles
+template <int NT>
+__global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
+                                                         int ksize, const u16 *__restrict__ dY, int ldy, int offy,
+                                                         int cout, int Kp, int chunk_vox, float *__restrict__ P) {
+    const int lane = threadIdx.x & 63;
+    const int ktiles = Kp / 16;
+    const int wave = (int)(blockIdx.x * 4 + threadIdx.x / 64);
+    const int nvox = n * n * n;
+    const int chunks = (nvox + chunk_vox - 1) / chunk_vox;
+    if (wave >= chunks * ktiles) return;
+    const int kt = wave % ktiles;
+    const int chunk = wave / ktiles;
+    const int i = lane & 15;
+    const int vq = lane >> 4;
+    const int K = (ksize == 1 ? 1 : 27) * cin;
+    const int k = kt * 16 + i;  // this lane's B column
+    int tap = 0, ci = 0, dz = 0, dy = 0, dx = 0;
+    if (k < K) {
+        tap = k / cin;
+        ci = k % cin;
+        tap_offset(tap, ksize, dz, dy, dx);
+    }
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int v_begin = chunk * chunk_vox;
+    const int v_end = min(v_begin + chunk_vox, nvox);
+    for (int vb = v_begin; vb < v_end; vb += 32) {
+        Frag b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int v = vb + 8 * vq + e;
+            u16 val = 0;
+            if (v < v_end && k < K) {
+                const int x = v % n, y = (v / n) % n, z = v / (n * n);
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n)
+                    val = X[((int64_t)((zz * n + yy) * n + xx)) * ldx + offx + ci];
+            }
+            b.h[e] = val;
+        }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            Frag a;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int v = vb + 8 * vq + e;
+                const int co = t * 16 + i;
+                a.h[e] = (v < v_end && co < cout) ? dY[(int64_t)v * ldy + offy + co] : (u16)0;
+            }
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[t], 0, 0, 0);
+        }
+    }
+    // D[co][k]: lane c + 16*(r/4) holds row 4*(lane/16)+e (co), column lane%16 (k)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int co = t * 16 + 4 * vq + e;
+            if (co < cout) P[((int64_t)chunk * cout + co) * Kp + kt * 16 + i] = acc[t][e];
+        }
+    }
+}
+
+// dW[co][k] = sum over chunks of P (fixed order); db[co] = sum_v dY[v][co]
+__global__ void conv3d_wgrad_reduce(const float *__restrict__ P, int chunks, int cout, int Kp, float *__restrict__ dW,
+                                    int accumulate) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)cout * Kp) return;
+    float s = 0.0f;
+    for (int c = 0; c < chunks; ++c) s += P[(int64_t)c * cout * Kp + idx];
+    dW[idx] = accumulate ? dW[idx] + s : s;
+}
+
+__global__ void conv3d_bias_grad(const u16 *__restrict__ dY, int nvox, int ldy, int offy, int cout,
+                                 float *__restrict__ db, int accumulate) {
+    __shared__ float lds[256];
+    const int co = blockIdx.x;
+    float s = 0.0f;
+    for (int v = threadIdx.x; v < nvox; v += 256) s += bf2f(dY[(int64_t)v * ldy + offy + co]);
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) db[co] = accumulate ? db[co] + lds[0] : lds[0];
+}
+
+// dgrad weights: Wt[ci][tap'*cout + co] = W[co][tap*cin + ci], tap' = 26 - tap
+__global__ void conv3d_dgrad_weights(const u16 *__restrict__ W, int cin, int cout, int ksize, int Kp,
+                                     u16 *__restrict__ Wt, int Kpt) {
+    const int taps = ksize == 1 ? 1 : 27;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)cin * Kpt) return;
+    const int ci = (int)(idx / Kpt);
+    const int kk = (int)(idx % Kpt);
+    u16 v = 0;
+    if (kk < taps * cout) {
+        const int tp = kk / cout, co = kk % cout;
+        const int tap = taps - 1 - tp;
+        v = W[(int64_t)co * Kp + tap * cin + ci];
+    }
+    Wt[idx] = v;
+}
+
+// g *= (y > 0) for the ReLU backward (in place, BF16)
+__global__ void relu_backward(u16 *g, int ldg, int offg, const u16 *__restrict__ y, int ldy, int offy, int c,
+                              int64_t nvox) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nvox * c) return;
+    const int64_t v = idx / c;
+    const int ch = (int)(idx % c);
+    if (!(bf2f(y[v * ldy + offy + ch]) > 0.0f)) g[v * ldg + offg + ch] = 0;
+}
+
+// 2x2x2 average pool: Y (n/2)^3 <- X n^3
+__global__ void avgpool2(const u16 *__restrict__ X, int n, int ldx, int offx, u16 *Y, int ldy, int offy, int c) {
+    const int m = n / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * m * m * c) return;
+    const int ch = (int)(idx % c);
+    const int64_t v = idx / c;
+    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    float s = 0.0f;
+    for (int d = 0; d < 8; ++d) {
+        const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
+        s += bf2f(X[((int64_t)((zz * n + yy) * n + xx)) * ldx + offx + ch]);
+    }
+    Y[v * ldy + offy + ch] = f2bf(s * 0.125f);
+}
+
+// gradient of the average pool: dX[child] (+)= dY[parent] / 8
+__global__ void avgpool2_backward(const u16 *__restrict__ dY, int n, int ldy, int offy, u16 *dX, int ldx, int offx,
+                                  int c, int accumulate) {
+    const int m = n / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n * n * c) return;
+    const int ch = (int)(idx % c);
+    const int64_t v = idx / c;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+    float g = bf2f(dY[p * ldy + offy + ch]) * 0.125f;
+    u16 *q = dX + v * ldx + offx + ch;
+    if (accumulate) g += bf2f(*q);
+    *q = f2bf(g);
+}
+
+// nearest 2x upsample: Y (2m)^3 <- X m^3
+__global__ void upsample2(const u16 *__restrict__ X, int m, int ldx, int offx, u16 *Y, int ldy, int offy, int c) {
+    const int n = 2 * m;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n * n * c) return;
+    const int ch = (int)(idx % c);
+    const int64_t v = idx / c;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+    Y[v * ldy + offy + ch] = X[p * ldx + offx + ch];
+}
+
+// gradient of the upsample: dX[parent] = sum of the 8 children
+__global__ void upsample2_backward(const u16 *__restrict__ dY, int m, int ldy, int offy, u16 *dX, int ldx, int offx,
+                                   int c, int accumulate) {
+    const int n = 2 * m;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * m * m * c) return;
+    const int ch = (int)(idx % c);
+    const int64_t v = idx / c;
+    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    float s = 0.0f;
+    for (int d = 0; d < 8; ++d) {
+        const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
+        s += bf2f(dY[((int64_t)((zz * n + yy) * n + xx)) * ldy + offy + ch]);
+    }
+    u16 *q = dX + v * ldx + offx + ch;
+    if (accumulate) s += bf2f(*q);
+    *q = f2bf(s);
+}
+
+// FP32 -> BF16 into a channel slot of a channels-last tensor
+__global__ void pack_bf16(const float *__restrict__ src, int64_t nvox, u16 *Y, int ldy, int offy) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nvox) Y[v * ldy + offy] = f2bf(src[v]);
+}
+
+// MSE loss: L = mean((pred - target)^2); grad = 2 (pred - target) / N as BF16;
+// per-block partial sums of the squared error (deterministic reduction)
+__global__ void mse_loss(const float *__restrict__ pred, const float *__restrict__ target, int64_t nvox, u16 *grad,
+                         int ldg, float *__restrict__ partial) {
+    __shared__ float lds[256];
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float se = 0.0f;
+    if (v < nvox) {
+        const float d = pred[v] - target[v];
+        se = d * d;
+        grad[v * ldg] = f2bf(2.0f * d / (float)nvox);
+    }
+    lds[threadIdx.x] = se;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = lds[0];
+}
+
+// Adam on FP32 master weights [cout][Kp] (padding columns stay zero); writes BF16 copies
+__global__ void adam_step(float *__restrict__ w, float *__restrict__ m, float *__restrict__ v,
+                          const float *__restrict__ g, int64_t count, int cols_used, int Kp, float lr, float b1,
+                          float b2, float eps, float c1, float c2, u16 *__restrict__ wb) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (Kp > 0 && (int)(i % Kp) >= cols_used) {
+        w[i] = 0.0f;
+        if (wb) wb[i] = 0;
+        return;
+    }
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i] = wi;
+    if (wb) wb[i] = f2bf(wi);
+}
+
+unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+
+int check_tensor(int n, int ld, int off, int c) {
+    if (n < 1 || ld < 1 || off < 0 || c < 1 || off + c > ld)
+        return lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, "bad tensor shape (n, ld, off, c)");
+    if ((int64_t)n * n * n * ld >= ((int64_t)1 << 40))
+        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "tensor too large");
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int lgdhip_conv3d_kpad(int cin, int ksize) { return (((ksize == 1 ? 1 : 27) * cin + 31) / 32) * 32; }
+
+int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
+                               const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
+                               int relu, int accumulate, lgdhip_stream_t stream) {
+    LGD_REQUIRE(x && w && (y || y32), "NULL pointer");
+    LGD_REQUIRE(ksize == 1 || ksize == 3, "ksize must be 1 or 3");
+    LGD_REQUIRE(cout >= 1 && cout <= 16 * kMaxNTiles, "cout must be in [1, 192]");
+    LGD_REQUIRE(!y32 || ld32 >= cout, "ld32 must be >= cout");
+    int rc = check_tensor(n, ldx, offx, cin);
+    if (rc) return rc;
+    if (y) {
+        rc = check_tensor(n, ldy, offy, cout);
+        if (rc) return rc;
+    }
+    if ((cin & 7) == 0) {
+        LGD_REQUIRE((ldx & 7) == 0 && (offx & 7) == 0, "ldx and offx must be multiples of 8 when cin % 8 == 0");
+        LGD_REQUIRE(((uintptr_t)x & 15) == 0, "x must be 16-byte aligned");
+    }
+    LGD_REQUIRE(((uintptr_t)w & 15) == 0, "w must be 16-byte aligned");
+    const int Kp = lgdhip_conv3d_kpad(cin, ksize);
+    const int nt = (cout + 15) / 16;
+    const int64_t waves = ((int64_t)n * n * n + 15) / 16;
+    const unsigned grid = (unsigned)((waves + 3) / 4);
+    hipStream_t s = lgdhip::as_stream(stream);
+    const u16 *X = (const u16 *)x;
+    const u16 *Wb = (const u16 *)w;
+    u16 *Y = (u16 *)y;
+#define LGD_CONV_CASE(T)                                                                                    \
+    case T:                                                                                                 \
+        conv3d_fwd_mfma<T><<<grid, 256, 0, s>>>(X, n, ldx, offx, cin, ksize, Wb, Kp, bias, cout, Y, ldy,    \
+                                                offy, y32, ld32, relu, accumulate);                         \
+        break;
+    switch (nt) {
+        LGD_CONV_CASE(1)
+        LGD_CONV_CASE(2)
+        LGD_CONV_CASE(3)
+        LGD_CONV_CASE(4)
+        LGD_CONV_CASE(5)
+        LGD_CONV_CASE(6)
+        LGD_CONV_CASE(7)
+        LGD_CONV_CASE(8)
+        LGD_CONV_CASE(9)
+        LGD_CONV_CASE(10)
+        LGD_CONV_CASE(11)
+        LGD_CONV_CASE(12)
+    }
+#undef LGD_CONV_CASE
+    LGD_LAUNCHED("conv3d_fwd_mfma");
+    return 0;
+}
+
+size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, int chunk_vox) {
+    if (chunk_vox <= 0) chunk_vox = 4096;
+    const int64_t nvox = (int64_t)n * n * n;
+    const int64_t chunks = (nvox + chunk_vox - 1) / chunk_vox;
+    return (size_t)(chunks * cout * lgdhip_conv3d_kpad(cin, ksize) * 4);
+}
+
+int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *dy, int ldy,
+                             int offy, int cout, float *dw, float *db, int accumulate, int chunk_vox,
+                             void *workspace, size_t workspace_bytes, lgdhip_stream_t stream) {
+    LGD_REQUIRE(x && dy && dw && workspace, "NULL pointer");
+    LGD_REQUIRE(ksize == 1 || ksize == 3, "ksize must be 1 or 3");
+    LGD_REQUIRE(cout >= 1 && cout <= 16 * kMaxNTiles, "cout must be in [1, 192]");
+    int rc = check_tensor(n, ldx, offx, cin);
+    if (rc) return rc;
+    rc = check_tensor(n, ldy, offy, cout);
+    if (rc) return rc;
+    if (chunk_vox <= 0) chunk_vox = 4096;
+    LGD_REQUIRE(chunk_vox % 32 == 0, "chunk_vox must be a multiple of 32");
+    if (workspace_bytes < lgdhip_conv3d_wgrad_workspace_bytes(n, cin, ksize, cout, chunk_vox))
+        return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "wgrad workspace too small");
+    const int Kp = lgdhip_conv3d_kpad(cin, ksize);
+    const int64_t nvox = (int64_t)n * n * n;
+    const int chunks = (int)((nvox + chunk_vox - 1) / chunk_vox);
+    const int64_t waves = (int64_t)chunks * (Kp / 16);
+    const unsigned grid = (unsigned)((waves + 3) / 4);
+    hipStream_t s = lgdhip::as_stream(stream);
+    float *P = (float *)workspace;
+    const int nt = (cout + 15) / 16;
+#define LGD_WG_CASE(T)                                                                                       \
+    case T:                                                                                                  \
+        conv3d_wgrad_mfma<T><<<grid, 256, 0, s>>>((const u16 *)x, n, ldx, offx, cin, ksize, (const u16 *)dy, \
+                                                  ldy, offy, cout, Kp, chunk_vox, P);                        \
+        break;
+    switch (nt) {
+        LGD_WG_CASE(1)
+        LGD_WG_CASE(2)
+        LGD_WG_CASE(3)
+        LGD_WG_CASE(4)
+        LGD_WG_CASE(5)
+        LGD_WG_CASE(6)
+        LGD_WG_CASE(7)
+        LGD_WG_CASE(8)
+        LGD_WG_CASE(9)
+        LGD_WG_CASE(10)
+        LGD_WG_CASE(11)
+        LGD_WG_CASE(12)
+    }
+#undef LGD_WG_CASE
+    LGD_LAUNCHED("conv3d_wgrad_mfma");
+    conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, dw, accumulate);
+    LGD_LAUNCHED("conv3d_wgrad_reduce");
+    if (db) {
+        conv3d_bias_grad<<<cout, 256, 0, s>>>((const u16 *)dy, (int)nvox, ldy, offy, cout, db, accumulate);
+        LGD_LAUNCHED("conv3d_bias_grad");
+    }
+    return 0;
+}
+
+int lgdhip_conv3d_dgrad_weights(const void *w, int cin, int cout, int ksize, void *wt, lgdhip_stream_t stream) {
+    LGD_REQUIRE(w && wt, "NULL pointer");
+    LGD_REQUIRE(ksize == 1 || ksize == 3, "ksize must be 1 or 3");
+    const int Kp = lgdhip_conv3d_kpad(cin, ksize);
+    const int Kpt = lgdhip_conv3d_kpad(cout, ksize);
+    conv3d_dgrad_weights<<<blocks_for((int64_t)cin * Kpt), 256, 0, lgdhip::as_stream(stream)>>>(
+        (const u16 *)w, cin, cout, ksize, Kp, (u16 *)wt, Kpt);
+    LGD_LAUNCHED("conv3d_dgrad_weights");
+    return 0;
+}
+
+int lgdhip_relu_backward_bf16(void *g, int ldg, int offg, const void *y, int ldy, int offy, int c, int64_t nvox,
+                              lgdhip_stream_t stream) {
+    LGD_REQUIRE(g && y && c > 0 && nvox > 0, "bad arguments");
+    relu_backward<<<blocks_for(nvox * c), 256, 0, lgdhip::as_stream(stream)>>>((u16 *)g, ldg, offg, (const u16 *)y,
+                                                                               ldy, offy, c, nvox);
+    LGD_LAUNCHED("relu_backward");
+    return 0;
+}
+
+int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int ldy, int offy, int c,
+                         lgdhip_stream_t stream) {
+    LGD_REQUIRE(x && y && n % 2 == 0, "bad arguments");
+    const int64_t m = n / 2;
+    avgpool2<<<blocks_for(m * m * m * c), 256, 0, lgdhip::as_stream(stream)>>>((const u16 *)x, n, ldx, offx, (u16 *)y,
+                                                                               ldy, offy, c);
+    LGD_LAUNCHED("avgpool2");
+    return 0;
+}
+
+int lgdhip_avgpool2_backward_bf16(const void *dy, int n, int ldy, int offy, void *dx, int ldx, int offx, int c,
+                                  int accumulate, lgdhip_stream_t stream) {
+    LGD_REQUIRE(dy && dx && n % 2 == 0, "bad arguments");
+    avgpool2_backward<<<blocks_for((int64_t)n * n * n * c), 256, 0, lgdhip::as_stream(stream)>>>(
+        (const u16 *)dy, n, ldy, offy, (u16 *)dx, ldx, offx, c, accumulate);
+    LGD_LAUNCHED("avgpool2_backward");
+    return 0;
+}
+
+int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int ldy, int offy, int c,
+                          lgdhip_stream_t stream) {
+    LGD_REQUIRE(x && y && m >= 1, "bad arguments");
+    const int64_t n = 2 * (int64_t)m;
+    upsample2<<<blocks_for(n * n * n * c), 256, 0, lgdhip::as_stream(stream)>>>((const u16 *)x, m, ldx, offx, (u16 *)y,
+                                                                                ldy, offy, c);
+    LGD_LAUNCHED("upsample2");
+    return 0;
+}
+
+int lgdhip_upsample2_backward_bf16(const void *dy, int m, int ldy, int offy, void *dx, int ldx, int offx, int c,
+                                   int accumulate, lgdhip_stream_t stream) {
+    LGD_REQUIRE(dy && dx && m >= 1, "bad arguments");
+    upsample2_backward<<<blocks_for((int64_t)m * m * m * c), 256, 0, lgdhip::as_stream(stream)>>>(
+        (const u16 *)dy, m, ldy, offy, (u16 *)dx, ldx, offx, c, accumulate);
+    LGD_LAUNCHED("upsample2_backward");
+    return 0;
+}
+
+int lgdhip_pack_bf16(const float *src, int64_t nvox, void *y, int ldy, int offy, lgdhip_stream_t stream) {
+    LGD_REQUIRE(src && y && nvox > 0, "bad arguments");
+    pack_bf16<<<blocks_for(nvox), 256, 0, lgdhip::as_stream(stream)>>>(src, nvox, (u16 *)y, ldy, offy);
+    LGD_LAUNCHED("pack_bf16");
+    return 0;
+}
+
+int64_t lgdhip_mse_partials(int64_t nvox) { return (nvox + 255) / 256; }
+
+int lgdhip_mse_loss(const float *pred, const float *target, int64_t nvox, void *grad, int ldg, float *partial,
+                    lgdhip_stream_t stream) {
+    LGD_REQUIRE(pred && target && grad && partial && nvox > 0, "bad arguments");
+    mse_loss<<<blocks_for(nvox), 256, 0, lgdhip::as_stream(stream)>>>(pred, target, nvox, (u16 *)grad, ldg, partial);
+    LGD_LAUNCHED("mse_loss");
+    return 0;
+}
+
+int lgdhip_adam_step(float *w, float *m, float *v, const float *g, int64_t count, int cols_used, int kp, float lr,
+                     float beta1, float beta2, float eps, int t, void *w_bf16, lgdhip_stream_t stream) {
+    LGD_REQUIRE(w && m && v && g && count > 0 && t >= 1, "bad arguments");
+    const float c1 = 1.0f - powf(beta1, (float)t);
+    const float c2 = 1.0f - powf(beta2, (float)t);
+    adam_step<<<blocks_for(count), 256, 0, lgdhip::as_stream(stream)>>>(w, m, v, g, count, cols_used, kp, lr, beta1,
+                                                                       beta2, eps, c1, c2, (u16 *)w_bf16);
+    LGD_LAUNCHED("adam_step");
+    return 0;
+}
+
+} // extern "C"

@@ -130,10 +130,28 @@ SIGNATURES = {
     "lgdhip_poisson_batch_solve": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I,
                                         C.POINTER(lgdhip_poisson_options), _P, _SZ,
                                         C.POINTER(lgdhip_poisson_result), _P]),
+    "lgdhip_conv3d_kpad": (_I, [_I, _I]),
+    "lgdhip_conv3d_bf16_forward": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _I, _P, _I, _I, _I, _P]),
+    "lgdhip_conv3d_dgrad_weights": (_I, [_P, _I, _I, _I, _P, _P]),
+    "lgdhip_conv3d_wgrad_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
+    "lgdhip_conv3d_bf16_wgrad": (_I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _I, _P, _P, _I, _I, _P, _SZ, _P]),
+    "lgdhip_relu_backward_bf16": (_I, [_P, _I, _I, _P, _I, _I, _I, _I64, _P]),
+    "lgdhip_avgpool2_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _P]),
+    "lgdhip_avgpool2_backward_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P]),
+    "lgdhip_upsample2_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _P]),
+    "lgdhip_upsample2_backward_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P]),
+    "lgdhip_pack_bf16": (_I, [_P, _I64, _P, _I, _I, _P]),
+    "lgdhip_mse_partials": (_I64, [_I64]),
+    "lgdhip_mse_loss": (_I, [_P, _P, _I64, _P, _I, _P, _P]),
+    "lgdhip_adam_step": (_I, [_P, _P, _P, _P, _I64, _I, _I, C.c_float, C.c_float, C.c_float, C.c_float, _I,
+                              _P, _P]),
 }
 
+# lgdhip_conv3d_kpad returns a value, not a status
+_VALUE_RETURNING_INT = {"lgdhip_conv3d_kpad"}
+
 # functions returning a status code (checked); the others return values
-_STATUS = {n for n, (r, _) in SIGNATURES.items() if r is _I}
+_STATUS = {n for n, (r, _) in SIGNATURES.items() if r is _I} - _VALUE_RETURNING_INT
 
 _lib = None
 

@@ -342,3 +342,88 @@ long lgdo_poisson_jacobi_f64(double *u, const float *f, long n, double rel_tol, 
     if (final_norm) *final_norm = rn;
     return sweeps;
 }
+
+/* ------------------------------------------------------------------ */
+/* 3D convolution (FP64, channels-last, zero padding), for the U-Net   */
+/* ------------------------------------------------------------------ */
+/* x[v][cin], w[cout][taps][cin], y[v][cout]; taps = 27 (ksize 3) or 1 */
+static void tap_xyz(int tap, int ksize, int *dz, int *dy, int *dx) {
+    if (ksize == 1) {
+        *dz = *dy = *dx = 0;
+    } else {
+        *dz = tap / 9 - 1;
+        *dy = (tap / 3) % 3 - 1;
+        *dx = tap % 3 - 1;
+    }
+}
+
+void lgdo_conv3d_forward(const double *x, long n, int cin, int ksize, const double *w, const double *bias, int cout,
+                         double *y) {
+    const int taps = ksize == 1 ? 1 : 27;
+    for (long z = 0; z < n; ++z)
+        for (long yy = 0; yy < n; ++yy)
+            for (long xx = 0; xx < n; ++xx) {
+                const long v = (z * n + yy) * n + xx;
+                for (int co = 0; co < cout; ++co) {
+                    double s = bias ? bias[co] : 0.0;
+                    for (int t = 0; t < taps; ++t) {
+                        int dz, dy, dx;
+                        tap_xyz(t, ksize, &dz, &dy, &dx);
+                        const long zz = z + dz, y2 = yy + dy, x2 = xx + dx;
+                        if (zz < 0 || y2 < 0 || x2 < 0 || zz >= n || y2 >= n || x2 >= n) continue;
+                        const long u = (zz * n + y2) * n + x2;
+                        for (int ci = 0; ci < cin; ++ci) s += w[((long)co * taps + t) * cin + ci] * x[u * cin + ci];
+                    }
+                    y[v * cout + co] = s;
+                }
+            }
+}
+
+/* dx[v][ci] = sum_{t,co} w[co][t][ci] dy[v - off(t)][co] */
+void lgdo_conv3d_dgrad(const double *dy, long n, int cin, int ksize, const double *w, int cout, double *dx) {
+    const int taps = ksize == 1 ? 1 : 27;
+    const long nv = n * n * n;
+    for (long i = 0; i < nv * cin; ++i) dx[i] = 0.0;
+    for (long z = 0; z < n; ++z)
+        for (long yy = 0; yy < n; ++yy)
+            for (long xx = 0; xx < n; ++xx) {
+                const long v = (z * n + yy) * n + xx;
+                for (int t = 0; t < taps; ++t) {
+                    int dz, dyy, dxx;
+                    tap_xyz(t, ksize, &dz, &dyy, &dxx);
+                    const long zz = z + dz, y2 = yy + dyy, x2 = xx + dxx;
+                    if (zz < 0 || y2 < 0 || x2 < 0 || zz >= n || y2 >= n || x2 >= n) continue;
+                    const long u = (zz * n + y2) * n + x2;
+                    /* y[v] used x[u] with tap t: dx[u] += w * dy[v] */
+                    for (int co = 0; co < cout; ++co)
+                        for (int ci = 0; ci < cin; ++ci)
+                            dx[u * cin + ci] += w[((long)co * taps + t) * cin + ci] * dy[v * cout + co];
+                }
+            }
+}
+
+/* dw[co][t][ci] = sum_v dy[v][co] x[v + off(t)][ci], db[co] = sum_v dy[v][co] */
+void lgdo_conv3d_wgrad(const double *x, long n, int cin, int ksize, const double *dy, int cout, double *dw,
+                       double *db) {
+    const int taps = ksize == 1 ? 1 : 27;
+    for (long i = 0; i < (long)cout * taps * cin; ++i) dw[i] = 0.0;
+    if (db)
+        for (int co = 0; co < cout; ++co) db[co] = 0.0;
+    for (long z = 0; z < n; ++z)
+        for (long yy = 0; yy < n; ++yy)
+            for (long xx = 0; xx < n; ++xx) {
+                const long v = (z * n + yy) * n + xx;
+                if (db)
+                    for (int co = 0; co < cout; ++co) db[co] += dy[v * cout + co];
+                for (int t = 0; t < taps; ++t) {
+                    int dz, dyy, dxx;
+                    tap_xyz(t, ksize, &dz, &dyy, &dxx);
+                    const long zz = z + dz, y2 = yy + dyy, x2 = xx + dxx;
+                    if (zz < 0 || y2 < 0 || x2 < 0 || zz >= n || y2 >= n || x2 >= n) continue;
+                    const long u = (zz * n + y2) * n + x2;
+                    for (int co = 0; co < cout; ++co)
+                        for (int ci = 0; ci < cin; ++ci)
+                            dw[((long)co * taps + t) * cin + ci] += dy[v * cout + co] * x[u * cin + ci];
+                }
+            }
+}

@@ -18,11 +18,10 @@
  *    with equal edge is a fixed point; maximum principle; symmetry under
  *    axis permutations; brute-force pure-Python loop on tiny grids.
  *  - lgdo_life2d_u8: blinker period 2, glider translation (1,1) per 4
- *    generations, block still life, the reference example's glider /
- *    diehard / acorn seeds (src/examples/gameoflife/main.cpp:70-108) with
- *    their published population counts (diehard vanishes after 130
- *    generations; acorn reaches 633 cells after 5206 generations on an
- *    unbounded plane); brute force against scipy-free numpy convolution.
+ *    generations, block still life, the reference example's glider and
+ *    diehard seeds (src/examples/gameoflife/main.cpp:70-89; diehard vanishes
+ *    after exactly 130 generations on an unbounded plane); brute force
+ *    against a numpy evaluation of the 3x3 box sum, dead and alive edges.
  *  - lgdo_lbm_d3q19_*: rest state is a fixed point; mass conservation with
  *    bounce-back walls and a moving lid; equilibrium moments sum rules
  *    (sum f_eq = rho, sum e f_eq = j, closed form); mirror symmetry of the
@@ -34,6 +33,10 @@
  *    exact right-hand side and are recovered; the FP32 sweep is the FP64
  *    sweep rounded (compared to tolerance) and a brute-force Python loop on
  *    tiny cubes.
+ *  - lgdo_conv3d_*: a delta kernel reproduces (shifted) inputs, linearity,
+ *    and the adjoint identities <conv(x), g> = <x, dgrad(g)> and
+ *    <conv_w(x), g> = <w, wgrad(x, g)> (exact up to FP64 rounding), plus a
+ *    central finite difference of a scalar loss on a tiny shape.
  */
 #ifndef LGD_ORACLE_H
 #define LGD_ORACLE_H
@@ -72,6 +75,14 @@ void lgdo_poisson_sweep_f32(const float *e, float *e2, const float *r, long n);
 double lgdo_poisson_residual_norm(const double *u, const float *f, long n);
 long lgdo_poisson_jacobi_f64(double *u, const float *f, long n, double rel_tol, double reference, long max_sweeps,
                              long check, double *final_norm);
+
+/* 3D convolution, FP64, channels-last x[v][cin], w[cout][taps][cin], zero
+ * padding; taps = 27 for ksize 3 (tap = (dz+1)*9 + (dy+1)*3 + (dx+1)), 1 for 1 */
+void lgdo_conv3d_forward(const double *x, long n, int cin, int ksize, const double *w, const double *bias, int cout,
+                         double *y);
+void lgdo_conv3d_dgrad(const double *dy, long n, int cin, int ksize, const double *w, int cout, double *dx);
+void lgdo_conv3d_wgrad(const double *x, long n, int cin, int ksize, const double *dy, int cout, double *dw,
+                       double *db);
 
 int lgdo_max_threads(void);
 

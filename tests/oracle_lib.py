@@ -134,3 +134,48 @@ def poisson_jacobi(u: np.ndarray, f: np.ndarray, rel_tol: float, reference: floa
     sweeps = load().lgdo_poisson_jacobi_f64(_p(u), _p(f), u.shape[0] - 2, rel_tol, reference, max_sweeps, check,
                                             C.byref(fn))
     return int(sweeps), fn.value
+
+
+def _conv_lib():
+    lib = load()
+    if not getattr(lib, "_conv_ready", False):
+        P, L, I = C.c_void_p, C.c_long, C.c_int
+        lib.lgdo_conv3d_forward.argtypes = [P, L, I, I, P, P, I, P]
+        lib.lgdo_conv3d_forward.restype = None
+        lib.lgdo_conv3d_dgrad.argtypes = [P, L, I, I, P, I, P]
+        lib.lgdo_conv3d_dgrad.restype = None
+        lib.lgdo_conv3d_wgrad.argtypes = [P, L, I, I, P, I, P, P]
+        lib.lgdo_conv3d_wgrad.restype = None
+        lib._conv_ready = True
+    return lib
+
+
+def conv3d_forward(x: np.ndarray, w: np.ndarray, bias=None, ksize: int = 3) -> np.ndarray:
+    """x[n,n,n,cin], w[cout,taps,cin] -> y[n,n,n,cout] in FP64."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n, cin, cout = x.shape[0], x.shape[-1], w.shape[0]
+    y = np.zeros(x.shape[:3] + (cout,))
+    b = None if bias is None else np.ascontiguousarray(bias, dtype=np.float64)
+    _conv_lib().lgdo_conv3d_forward(_p(x), n, cin, ksize, _p(w), None if b is None else _p(b), cout, _p(y))
+    return y
+
+
+def conv3d_dgrad(dy: np.ndarray, w: np.ndarray, ksize: int = 3) -> np.ndarray:
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    n, cout, cin = dy.shape[0], w.shape[0], w.shape[-1]
+    dx = np.zeros(dy.shape[:3] + (cin,))
+    _conv_lib().lgdo_conv3d_dgrad(_p(dy), n, cin, ksize, _p(w), cout, _p(dx))
+    return dx
+
+
+def conv3d_wgrad(x: np.ndarray, dy: np.ndarray, ksize: int = 3):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dy = np.ascontiguousarray(dy, dtype=np.float64)
+    n, cin, cout = x.shape[0], x.shape[-1], dy.shape[-1]
+    taps = 1 if ksize == 1 else 27
+    dw = np.zeros((cout, taps, cin))
+    db = np.zeros(cout)
+    _conv_lib().lgdo_conv3d_wgrad(_p(x), n, cin, ksize, _p(dy), cout, _p(dw), _p(db))
+    return dw, db

@@ -1,0 +1,303 @@
+"""U-Net kernels: FP64 oracle pinning (CPU) and GPU parity of the MFMA
+convolutions, their gradients, pooling, loss, Adam and the whole network."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from libgeodecomp_amd.unet import bf16_round, conv_weights_layout, from_bf16_bits, to_bf16_bits
+
+
+# ----------------------------------------------------------------------
+# oracle pinning (CPU)
+# ----------------------------------------------------------------------
+def test_oracle_conv_delta_kernel_shifts(oracle):
+    rng = np.random.default_rng(0)
+    n, c = 5, 3
+    x = rng.standard_normal((n, n, n, c))
+    # tap (dz,dy,dx) = (1,0,-1): tap index (2)*9 + 1*3 + 0 = 21; identity over channels
+    w = np.zeros((c, 27, c))
+    for k in range(c):
+        w[k, 21, k] = 1.0
+    y = oracle.conv3d_forward(x, w)
+    exp = np.zeros_like(x)
+    exp[:-1, :, 1:] = x[1:, :, :-1]      # y[z,y,x] = x[z+1, y, x-1]
+    assert np.array_equal(y, exp)
+
+
+def test_oracle_conv_adjoint_identities(oracle):
+    rng = np.random.default_rng(1)
+    n, cin, cout = 4, 3, 5
+    x = rng.standard_normal((n, n, n, cin))
+    w = rng.standard_normal((cout, 27, cin))
+    g = rng.standard_normal((n, n, n, cout))
+    y = oracle.conv3d_forward(x, w)
+    dx = oracle.conv3d_dgrad(g, w)
+    dw, db = oracle.conv3d_wgrad(x, g)
+    assert abs(np.vdot(y, g) - np.vdot(x, dx)) < 1e-10 * abs(np.vdot(y, g))
+    assert abs(np.vdot(y, g) - np.vdot(w, dw)) < 1e-10 * abs(np.vdot(y, g))
+    assert np.allclose(db, g.sum(axis=(0, 1, 2)))
+
+
+def test_oracle_conv_finite_difference(oracle):
+    rng = np.random.default_rng(2)
+    n, cin, cout = 3, 2, 2
+    x = rng.standard_normal((n, n, n, cin))
+    w = rng.standard_normal((cout, 27, cin))
+    t = rng.standard_normal((n, n, n, cout))
+
+    def loss(wv):
+        return 0.5 * np.sum((oracle.conv3d_forward(x, wv) - t) ** 2)
+
+    dw, _ = oracle.conv3d_wgrad(x, oracle.conv3d_forward(x, w) - t)
+    for idx in [(0, 13, 0), (1, 0, 1), (1, 26, 0)]:
+        h = 1e-5
+        wp, wm = w.copy(), w.copy()
+        wp[idx] += h
+        wm[idx] -= h
+        fd = (loss(wp) - loss(wm)) / (2 * h)
+        assert abs(fd - dw[idx]) < 1e-6 * max(1.0, abs(fd))
+
+
+# ----------------------------------------------------------------------
+# GPU
+# ----------------------------------------------------------------------
+def dev(hip, host):
+    b = hip.DeviceBuffer(max(host.nbytes, 16))
+    b.upload(host)
+    return b
+
+
+def run_conv(hip, x, w, bias, ksize, ld_out=None, off_out=0, relu=False, y_init=None):
+    """x[n,n,n,cin] (bf16 values), w[cout,taps,cin] -> (y32, y_bf16 buffer slice)."""
+    n, cin = x.shape[0], x.shape[-1]
+    cout = w.shape[0]
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, ksize)
+    xb = dev(hip, to_bf16_bits(x))
+    wb = dev(hip, to_bf16_bits(conv_weights_layout(w.astype(np.float32), kp)))
+    bb = dev(hip, bias.astype(np.float32))
+    ld = ld_out or cout
+    y = dev(hip, to_bf16_bits(y_init) if y_init is not None else np.zeros(n ** 3 * ld, np.uint16))
+    y32 = hip.DeviceBuffer(n ** 3 * cout * 4)
+    hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, cin, 0, cin, ksize, wb.ptr, bb.ptr, cout, y.ptr, ld, off_out,
+             y32.ptr, cout, 1 if relu else 0, 1 if y_init is not None else 0, None)
+    out32 = np.empty(n ** 3 * cout, np.float32)
+    y32.download(out32)
+    outb = np.empty(n ** 3 * ld, np.uint16)
+    y.download(outb)
+    return out32.reshape(n, n, n, cout), from_bf16_bits(outb).reshape(n, n, n, ld)
+
+
+CONV_CASES = [(8, 2, 16, 3), (16, 16, 16, 3), (12, 48, 16, 3), (8, 192, 64, 3), (16, 16, 1, 1), (4, 64, 128, 3),
+              (8, 32, 192, 3), (6, 8, 24, 3)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout,ksize", CONV_CASES)
+def test_conv_forward_vs_oracle(hip, oracle, n, cin, cout, ksize):
+    print("[case] conv fwd", n, cin, cout, ksize, flush=True)
+    rng = np.random.default_rng(n * 1000 + cin * 10 + cout)
+    taps = 27 if ksize == 3 else 1
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, taps, cin)) / np.sqrt(taps * cin)).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
+    y32, yb = run_conv(hip, x, w, b, ksize)
+    ref = oracle.conv3d_forward(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64), ksize)
+    # FP32 accumulation of K products: bound by K * 2^-24 * sum |terms|
+    mag = oracle.conv3d_forward(np.abs(x).astype(np.float64), np.abs(w).astype(np.float64), np.abs(b), ksize)
+    K = taps * cin
+    assert np.all(np.abs(y32 - ref) <= (K + 2) * 2.0 ** -24 * mag + 1e-30)
+    # the BF16 output is the FP32 result rounded to nearest even
+    assert np.array_equal(yb, bf16_round(y32))
+
+
+@pytest.mark.gpu
+def test_conv_relu_slice_accumulate(hip, oracle):
+    rng = np.random.default_rng(9)
+    n, cin, cout, ld, off = 8, 16, 16, 48, 32
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / 12).astype(np.float32))
+    b = np.zeros(cout, np.float32)
+    init = bf16_round(rng.standard_normal((n, n, n, ld)).astype(np.float32))
+    y32, yb = run_conv(hip, x, w, b, 3, ld_out=ld, off_out=off, relu=True, y_init=init)
+    ref = np.maximum(oracle.conv3d_forward(x, w.astype(np.float64), None, 3), 0)
+    # y32 = init + relu(conv) in FP32; the BF16 slice is that value rounded
+    got = y32.astype(np.float64) - init[..., off:]
+    assert np.all(np.abs(got - ref) <= 1e-5 * (1 + np.abs(init[..., off:]) + ref))
+    assert np.array_equal(yb[..., :off], init[..., :off])
+    assert np.array_equal(yb[..., off:], bf16_round(y32))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout", [(8, 16, 16), (8, 48, 16), (4, 192, 64), (8, 16, 1)])
+def test_conv_dgrad_vs_oracle(hip, oracle, n, cin, cout):
+    print("[case] conv dgrad", n, cin, cout, flush=True)
+    rng = np.random.default_rng(cin + cout)
+    ksize = 3 if cout > 1 else 1
+    taps = 27 if ksize == 3 else 1
+    w = bf16_round((rng.standard_normal((cout, taps, cin)) / 8).astype(np.float32))
+    g = bf16_round(rng.standard_normal((n, n, n, cout)).astype(np.float32))
+    lib = hip.load()
+    kp, kpt = lib.lgdhip_conv3d_kpad(cin, ksize), lib.lgdhip_conv3d_kpad(cout, ksize)
+    wb = dev(hip, to_bf16_bits(conv_weights_layout(w, kp)))
+    wt = hip.DeviceBuffer(cin * kpt * 2)
+    hip.call("lgdhip_conv3d_dgrad_weights", wb.ptr, cin, cout, ksize, wt.ptr, None)
+    gb = dev(hip, to_bf16_bits(g))
+    dx32 = hip.DeviceBuffer(n ** 3 * cin * 4)
+    hip.call("lgdhip_conv3d_bf16_forward", gb.ptr, n, cout, 0, cout, ksize, wt.ptr, None, cin, None, cin, 0,
+             dx32.ptr, cin, 0, 0, None)
+    out = np.empty(n ** 3 * cin, np.float32)
+    dx32.download(out)
+    ref = oracle.conv3d_dgrad(g, w.astype(np.float64), ksize)
+    mag = oracle.conv3d_dgrad(np.abs(g), np.abs(w).astype(np.float64), ksize)
+    assert np.all(np.abs(out.reshape(ref.shape) - ref) <= (taps * cout + 2) * 2.0 ** -24 * mag + 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout,ksize", [(8, 16, 16, 3), (12, 2, 16, 3), (4, 192, 64, 3), (16, 16, 1, 1)])
+def test_conv_wgrad_vs_oracle(hip, oracle, n, cin, cout, ksize):
+    print("[case] conv wgrad", n, cin, cout, ksize, flush=True)
+    rng = np.random.default_rng(7 * cin + cout)
+    taps = 27 if ksize == 3 else 1
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    g = bf16_round(rng.standard_normal((n, n, n, cout)).astype(np.float32))
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, ksize)
+    chunk = 256
+    wsb = lib.lgdhip_conv3d_wgrad_workspace_bytes(n, cin, ksize, cout, chunk)
+    ws = hip.DeviceBuffer(wsb)
+    xb, gb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(g))
+    dw = hip.DeviceBuffer(cout * kp * 4)
+    db = hip.DeviceBuffer(cout * 4)
+    hip.call("lgdhip_conv3d_bf16_wgrad", xb.ptr, n, cin, 0, cin, ksize, gb.ptr, cout, 0, cout, dw.ptr, db.ptr, 0,
+             chunk, ws.ptr, wsb, None)
+    out = np.empty((cout, kp), np.float32)
+    dw.download(out)
+    outb = np.empty(cout, np.float32)
+    db.download(outb)
+    ref, refb = oracle.conv3d_wgrad(x, g, ksize)
+    mag, _ = oracle.conv3d_wgrad(np.abs(x), np.abs(g), ksize)
+    got = out[:, :taps * cin].reshape(ref.shape)
+    assert np.all(out[:, taps * cin:] == 0)
+    assert np.all(np.abs(got - ref) <= (n ** 3 + 64) * 2.0 ** -24 * mag + 1e-30)
+    assert np.allclose(outb, refb, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.gpu
+def test_pool_upsample_relu_mse_adam(hip):
+    rng = np.random.default_rng(4)
+    n, c = 8, 16
+    x = bf16_round(rng.standard_normal((n, n, n, c)).astype(np.float32))
+    xb = dev(hip, to_bf16_bits(x))
+    m = n // 2
+    yb = hip.DeviceBuffer(m ** 3 * c * 2)
+    hip.call("lgdhip_avgpool2_bf16", xb.ptr, n, c, 0, yb.ptr, c, 0, c, None)
+    y = np.empty(m ** 3 * c, np.uint16)
+    yb.download(y)
+    blocks = x.reshape(m, 2, m, 2, m, 2, c)
+    s = np.zeros((m, m, m, c), np.float32)
+    for d in range(8):     # the kernel's FP32 order: children z-major, then y, then x
+        s = s + blocks[:, d >> 2, :, (d >> 1) & 1, :, d & 1, :]
+    exp = bf16_round(s * np.float32(0.125))
+    assert np.array_equal(from_bf16_bits(y).reshape(exp.shape), exp)
+    ub = hip.DeviceBuffer(n ** 3 * c * 2)
+    hip.call("lgdhip_upsample2_bf16", yb.ptr, m, c, 0, ub.ptr, c, 0, c, None)
+    u = np.empty(n ** 3 * c, np.uint16)
+    ub.download(u)
+    up = np.repeat(np.repeat(np.repeat(from_bf16_bits(y).reshape(m, m, m, c), 2, 0), 2, 1), 2, 2)
+    assert np.array_equal(from_bf16_bits(u).reshape(up.shape), up)
+    # relu backward: zero where y <= 0
+    g = bf16_round(rng.standard_normal((n, n, n, c)).astype(np.float32))
+    gb = dev(hip, to_bf16_bits(g))
+    hip.call("lgdhip_relu_backward_bf16", gb.ptr, c, 0, xb.ptr, c, 0, c, n ** 3, None)
+    gg = np.empty(n ** 3 * c, np.uint16)
+    gb.download(gg)
+    assert np.array_equal(from_bf16_bits(gg).reshape(g.shape), np.where(x > 0, g, 0))
+    # mse
+    lib = hip.load()
+    pred = rng.standard_normal(n ** 3).astype(np.float32)
+    tgt = rng.standard_normal(n ** 3).astype(np.float32)
+    pb, tb = dev(hip, pred), dev(hip, tgt)
+    grad = hip.DeviceBuffer(n ** 3 * 2)
+    parts = hip.DeviceBuffer(lib.lgdhip_mse_partials(n ** 3) * 4)
+    hip.call("lgdhip_mse_loss", pb.ptr, tb.ptr, n ** 3, grad.ptr, 1, parts.ptr, None)
+    ph = np.empty(lib.lgdhip_mse_partials(n ** 3), np.float32)
+    parts.download(ph)
+    assert abs(ph.sum() / n ** 3 - np.mean((pred - tgt) ** 2)) < 1e-5
+    gh = np.empty(n ** 3, np.uint16)
+    grad.download(gh)
+    assert np.array_equal(from_bf16_bits(gh), bf16_round((2 * (pred - tgt) / n ** 3).astype(np.float32)))
+    # adam, two steps against numpy
+    w = rng.standard_normal(64).astype(np.float32)
+    gw = rng.standard_normal(64).astype(np.float32)
+    wd, md, vd, gd = dev(hip, w), dev(hip, np.zeros(64, np.float32)), dev(hip, np.zeros(64, np.float32)), dev(hip, gw)
+    mw, vw, wr = np.zeros(64), np.zeros(64), w.astype(np.float64)
+    for t in (1, 2):
+        hip.call("lgdhip_adam_step", wd.ptr, md.ptr, vd.ptr, gd.ptr, 64, 0, 0, 1e-2, 0.9, 0.999, 1e-8, t, None, None)
+        mw = 0.9 * mw + 0.1 * gw
+        vw = 0.999 * vw + 0.001 * gw * gw
+        wr = wr - 1e-2 * (mw / (1 - 0.9 ** t)) / (np.sqrt(vw / (1 - 0.999 ** t)) + 1e-8)
+    wout = np.empty(64, np.float32)
+    wd.download(wout)
+    assert np.allclose(wout, wr, atol=1e-6)
+
+
+@pytest.mark.gpu
+def test_unet_gradients_vs_fp64_oracle(hip, oracle):
+    """Whole network at 16^3: forward values and every layer's weight and
+    bias gradient against the FP64 oracle network with the same (BF16)
+    weights.  Against the reference that rounds to BF16 where the GPU stores
+    BF16, the only difference left is FP32 accumulation order (K up to 5184
+    products: ~3e-4 relative, enough to move a BF16 rounding of many
+    elements by one unit), so the bound is 3e-2 per layer with cosine >
+    0.9995 (an index or tap mistake gives O(1));
+    against the plain FP64 network the BF16 activation and gradient storage
+    shows as a few per cent, bounded at 25 % with cosine > 0.98."""
+    import unet_ref
+    from libgeodecomp_amd.unet import UNet3D
+
+    n = 16
+    rng = np.random.default_rng(11)
+    net = UNet3D(n=n, seed=3)
+    W = {}
+    for name in net.order:
+        w, b = net.get_weights(name)
+        w = bf16_round(w)
+        b = rng.standard_normal(b.shape).astype(np.float32) * 0.05
+        net.set_weights(name, w, b)
+        W[name] = (w.astype(np.float64), b.astype(np.float64))
+    f = rng.standard_normal((n, n, n)).astype(np.float32)
+    bnd = rng.standard_normal((n, n, n)).astype(np.float32)
+    tgt = rng.standard_normal((n, n, n)).astype(np.float32)
+    fb, bb, tb = dev(hip, f), dev(hip, bnd), dev(hip, tgt)
+    net.load_input([fb, bb])
+    pred_dev = net.forward()
+    loss = net.loss(tb)
+    net.backward()
+    pred = np.empty(n ** 3, np.float32)
+    pred_dev.download(pred)
+    pred = pred.reshape(n, n, n)
+    x = np.stack([bf16_round(f), bf16_round(bnd)], axis=-1).astype(np.float64)
+
+    failures = []
+    for bf16, tol_w, tol_cos, tol_pred in ((True, 3e-2, 0.9995, 5e-3), (False, 0.25, 0.98, 3e-2)):
+        ref = unet_ref.RefUNet(W, bf16=bf16)
+        pr = ref.forward(x)
+        rel_pred = np.linalg.norm(pred - pr) / np.linalg.norm(pr)
+        loss_ref = np.mean((pr - tgt) ** 2)
+        G = ref.backward(2 * (pr - tgt) / n ** 3)
+        print(f"[unet] bf16-emulating={bf16}: forward rel err {rel_pred:.3e}, loss gpu {loss:.6e} "
+              f"ref {loss_ref:.6e}", flush=True)
+        if not (rel_pred < tol_pred and abs(loss - loss_ref) < tol_pred * loss_ref):
+            failures.append((bf16, "forward"))
+        for name in net.order:
+            gw, gb = net.grads(name)
+            rw, rb = G[name]
+            ew = np.linalg.norm(gw - rw) / max(np.linalg.norm(rw), 1e-30)
+            eb = np.linalg.norm(gb - rb) / max(np.linalg.norm(rb), 1e-30)
+            cos = np.vdot(gw, rw) / (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-300)
+            print(f"  {name}: |dW err| {ew:.3e} cos {cos:.6f} |db err| {eb:.3e}", flush=True)
+            if not (ew < tol_w and cos > tol_cos and eb < tol_w):
+                failures.append((bf16, name))
+    assert not failures, failures
