@@ -236,6 +236,9 @@ typedef struct lgdhip_poisson_options {
     int32_t reserved;
     const double *reference_norms; /* host array, one per problem, or NULL: the
                                       residual norm of the initial guess          */
+    int32_t smooth_sweeps;       /* weighted-Jacobi sweeps before the plain ones
+                                    (0: none; rounded up to even)                 */
+    float smooth_omega;          /* their weight in (0, 1], e.g. 2/3              */
 } lgdhip_poisson_options;
 
 typedef struct lgdhip_poisson_result {
@@ -320,6 +323,35 @@ int lgdhip_mse_loss(const float *pred, const float *target, int64_t nvox, void *
  * >= cols_used kept zero); writes BF16 copies to w_bf16 if non-NULL */
 int lgdhip_adam_step(float *w, float *m, float *v, const float *g, int64_t count, int cols_used, int kp, float lr,
                      float beta1, float beta2, float eps, int t, void *w_bf16, lgdhip_stream_t stream);
+
+/* ------------------------------------------------------------------ */
+/* Learned initial guess: Poisson pools <-> U-Net tensors             */
+/*   (no reference counterpart).  f and u are the pool base pointers; */
+/*   prob (host) gives n, the block offset and the faces.  symmetry   */
+/*   in [0,48) reads the problem through a cube symmetry: axis        */
+/*   permutation (symmetry % 6) and reflections (bits of symmetry/6). */
+/* ------------------------------------------------------------------ */
+/* The harmonic basis H (dense n^3 FP32, x fastest) is the solution for
+ * f = 0, value 1 on the face x = -1 and 0 on the other faces; face j's basis
+ * field is H read at (x,y,z), (n-1-x,y,z), (y,x,z), (n-1-y,x,z), (z,y,x),
+ * (n-1-z,y,x) for j = 0..5, and h = sum_j face_j H_j solves f = 0 with the
+ * problem's faces. */
+/* x[v][0] = bf16(f/s), x[v][1] = bf16(b/s) with b the face field (sum of the
+ * Dirichlet values of the faces cell v touches), x[v][2] = bf16(h/s) when
+ * harmonic != NULL, remaining channels up to ldx zero; inv_scale = 1/s */
+int lgdhip_poisson_unet_input(const float *f, const lgdhip_poisson_problem *prob, double inv_scale, int symmetry,
+                              const float *harmonic, void *x, int ldx, lgdhip_stream_t stream);
+/* t[v] = (float)((u_interior[v] - h[v]) / s), h = 0 when harmonic == NULL:
+ * the training target (with harmonic: the source response alone) */
+int lgdhip_poisson_unet_target(const double *u, const lgdhip_poisson_problem *prob, double inv_scale, int symmetry,
+                               const float *harmonic, float *t, lgdhip_stream_t stream);
+/* u_interior[v] = pred[v] * s + h[v] (h = 0 when harmonic == NULL): the
+ * initial guess; the halo is left alone */
+int lgdhip_poisson_unet_guess(const float *pred, const lgdhip_poisson_problem *prob, double scale,
+                              const float *harmonic, double *u, lgdhip_stream_t stream);
+/* u_interior = h = sum_j face_j H_j (the boundary part alone, no network) */
+int lgdhip_poisson_face_guess(const float *harmonic, const lgdhip_poisson_problem *prob, double *u,
+                              lgdhip_stream_t stream);
 
 #ifdef __cplusplus
 }

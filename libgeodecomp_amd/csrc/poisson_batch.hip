@@ -97,12 +97,16 @@ __global__ __launch_bounds__(kThreads) void poisson_residual_f64(const double *_
 // one FP32 Jacobi sweep on A e = r:  e' = (e[z-1] + e[y-1] + e[x-1] + e[x+1]
 // + e[y+1] + e[z+1] + r) * (1/6), summed in that order; when `partial` is
 // given, also the per-tile sum of the FP64 square of the residual of the
-// input iterate, r + sum_nb e - 6 e (= 6 (e' - e) up to rounding)
+// input iterate, r + sum_nb e - 6 e (= 6 (e' - e) up to rounding).
+// DAMPED: weighted Jacobi e' = e + omega (J(e) - e), which damps the
+// highest-frequency error modes by |1 - 2 omega| per sweep instead of
+// plain Jacobi's cos(pi/(n+1)) (used to smooth a predicted initial guess).
+template <bool DAMPED>
 __global__ __launch_bounds__(kThreads) void poisson_sweep_f32(const float *__restrict__ e, float *__restrict__ e2,
                                                               const float *__restrict__ r,
                                                               const Tile *__restrict__ tiles,
                                                               const ProblemDev *__restrict__ probs,
-                                                              double *__restrict__ partial) {
+                                                              double *__restrict__ partial, float omega) {
     __shared__ double lds[kThreads / 64];
     const Tile t = tiles[blockIdx.x];
     const ProblemDev P = probs[t.p];
@@ -125,7 +129,10 @@ __global__ __launch_bounds__(kThreads) void poisson_sweep_f32(const float *__res
             s += e[i + sx];
             s += above;
             s += r[i];
-            e2[i] = s * (1.0f / 6.0f);
+            if (DAMPED)
+                e2[i] = center + omega * (s * (1.0f / 6.0f) - center);
+            else
+                e2[i] = s * (1.0f / 6.0f);
             if (partial) {
                 const double rr = (double)s - 6.0 * (double)center;
                 acc += rr * rr;
@@ -365,7 +372,8 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
     if (rc) return rc;
     float *cur = ea, *nxt = eb;
     for (int t = 0; t < sweeps; ++t) {
-        poisson_sweep_f32<<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr);
+        poisson_sweep_f32<false>
+            <<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr, 1.0f);
         LGD_LAUNCHED("poisson_sweep_f32");
         std::swap(cur, nxt);
     }
@@ -404,6 +412,9 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
     if (rc) return rc;
     LGD_REQUIRE(u && f && workspace && results && opt, "NULL pointer");
     LGD_REQUIRE(opt->rel_tol > 0 && opt->chunk >= 1 && opt->max_sweeps >= 1, "bad solver options");
+    LGD_REQUIRE(opt->smooth_sweeps >= 0 && (opt->smooth_sweeps == 0 || (opt->smooth_omega > 0.0f &&
+                                                                        opt->smooth_omega <= 1.0f)),
+                "smooth_omega must be in (0, 1]");
     const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
     if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
     hipStream_t s = lgdhip::as_stream(stream);
@@ -446,6 +457,30 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
     }
 
     float *ecur = w.e0, *enxt = w.e1;  // every problem's correction lives in ecur between chunks
+
+    // smoothing phase: weighted Jacobi on the correction of every problem
+    // not yet converged (an even count leaves the correction in ecur); the
+    // main loop then continues plain Jacobi from that correction
+    const int smooth = opt->smooth_sweeps + (opt->smooth_sweeps & 1);
+    if (smooth > 0) {
+        active.clear();
+        for (int k = 0; k < count; ++k)
+            if (!st[k].done) active.push_back(k);
+        if (!active.empty()) {
+            rc = upload_set(w, probs, active, &tiles, s);
+            if (rc) return rc;
+            for (int t = 0; t < smooth; ++t) {
+                poisson_sweep_f32<true><<<(unsigned)tiles.size(), kThreads, 0, s>>>(
+                    ecur, enxt, w.r, w.tiles, w.probs, nullptr, opt->smooth_omega);
+                LGD_LAUNCHED("poisson_sweep_f32");
+                std::swap(ecur, enxt);
+            }
+            for (int k : active) {
+                st[k].sweeps += smooth;
+                st[k].inner_sweeps += smooth;
+            }
+        }
+    }
     hipGraphExec_t graph = nullptr;
     bool graph_valid = false;
     while (true) {
@@ -466,8 +501,8 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             const int C = opt->chunk + (opt->chunk & 1);
             auto enqueue = [&](float *a, float *b) -> int {
                 for (int t = 0; t < C; ++t) {
-                    poisson_sweep_f32<<<(unsigned)tiles.size(), kThreads, 0, s>>>(
-                        a, b, w.r, w.tiles, w.probs, t == C - 1 ? w.partial : nullptr);
+                    poisson_sweep_f32<false><<<(unsigned)tiles.size(), kThreads, 0, s>>>(
+                        a, b, w.r, w.tiles, w.probs, t == C - 1 ? w.partial : nullptr, 1.0f);
                     std::swap(a, b);
                 }
                 poisson_reduce<<<(unsigned)active.size(), 256, 0, s>>>(w.partial, w.begin, w.end, w.norms);

@@ -1,0 +1,312 @@
+"""Learned initial guess for the Poisson ensemble.
+
+The problem is linear, so a solution splits into u = h + u_f: h is the
+response to the Dirichlet faces with f = 0 and u_f the response to the source
+with zero faces.  h is exact and cheap: h = sum_j face_j H_j where the six
+basis fields are symmetric images of one field H (value 1 on the face
+x = -1), solved once per cube size by the batched Jacobi solver itself.  u_f
+is what the 3D U-Net (libgeodecomp_amd.unet) learns: it is trained on source
+responses of steady states that the batched Jacobi solver generates, reading
+every problem through one of the 48 symmetries of the cube and a random sign
+(exact augmentations), with bias-free layers (a positively homogeneous
+network) and per-sample normalised targets, so that every sample weighs by
+its relative error.  At inference it proposes u0 = h + s * net(f / s)
+(s = max |f|), which the same solver polishes -- a short weighted-Jacobi
+smoothing phase for the high-frequency part of the network's error, then
+plain Jacobi -- to the same relative residual as the plain solve from the
+zero guess (both relative to the zero guess's residual ||f + b||_2).
+
+No reference counterpart.  Everything on the device is a liblgdhip kernel;
+this module only orchestrates (and times) them.
+
+    python -m libgeodecomp_amd.learned_guess --n 128 --train 256 --eval 16 --steps 3000
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import time
+
+import numpy as np
+
+from . import lgdhip, synthetic
+from .unet import UNet3D
+
+
+def _sync():
+    lgdhip.call("lgdhip_device_synchronize")
+
+
+class Ensemble:
+    """n^3 Poisson problems (one size) in device pools: f (FP32), u (FP64)."""
+
+    def __init__(self, problems, stream=None):
+        self.lib = lgdhip.load()
+        self.count = len(problems)
+        self.n = problems[0][0]
+        assert all(p[0] == self.n for p in problems), "one cube size per ensemble"
+        self.stream = stream
+        self.probs = (lgdhip.lgdhip_poisson_problem * self.count)()
+        self.fmax = np.zeros(self.count)
+        self.facemax = np.zeros(self.count)
+        for k, (n, faces, f) in enumerate(problems):
+            self.probs[k].n = n
+            for j in range(6):
+                self.probs[k].face[j] = faces[j]
+            self.fmax[k] = float(np.abs(f).max())
+            self.facemax[k] = float(np.abs(np.asarray(faces)).max())
+        self.elems = self.lib.lgdhip_poisson_layout(self.probs, self.count)
+        self.ws_bytes = self.lib.lgdhip_poisson_workspace_bytes(self.probs, self.count)
+        self.ws = lgdhip.DeviceBuffer(self.ws_bytes)
+        self.u = lgdhip.DeviceBuffer(self.elems * 8)
+        self.f = lgdhip.DeviceBuffer(self.elems * 4)
+        block = (self.n + 2) ** 3
+        for k, (_, _, f) in enumerate(problems):
+            host = np.ascontiguousarray(f, dtype=np.float32).ravel()
+            assert host.size == block
+            dst = C.c_void_p(self.f.value + self.probs[k].offset * 4)
+            lgdhip.call("lgdhip_memcpy_h2d", dst, host.ctypes.data_as(C.c_void_p), host.nbytes, None)
+        self.reset()
+        self.reference = self.residual_norms()
+
+    def reset(self):
+        """u = 0 in every interior, faces in every halo."""
+        lgdhip.call("lgdhip_fill_u8", self.u.ptr, 0, self.elems * 8, self.stream)
+        lgdhip.call("lgdhip_poisson_set_faces", self.u.ptr, self.probs, self.count, self.ws.ptr, self.ws_bytes,
+                    self.stream)
+
+    def residual_norms(self) -> np.ndarray:
+        out = np.zeros(self.count)
+        lgdhip.call("lgdhip_poisson_residual_norms", self.u.ptr, self.f.ptr, self.probs, self.count, self.ws.ptr,
+                    self.ws_bytes, out.ctypes.data_as(C.POINTER(C.c_double)), self.stream)
+        return out
+
+    def solve(self, rel_tol: float = 1e-6, chunk: int = 64, use_graph: int = 1, max_sweeps: int = 10_000_000,
+              smooth_sweeps: int = 0, smooth_omega: float = 2.0 / 3.0):
+        """Solves every problem from the current u to rel_tol relative to the
+        zero guess's residual norm; returns the per-problem results."""
+        opt = lgdhip.lgdhip_poisson_options()
+        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = rel_tol, chunk, max_sweeps, use_graph
+        opt.smooth_sweeps, opt.smooth_omega = smooth_sweeps, smooth_omega
+        ref = (C.c_double * self.count)(*self.reference)
+        opt.reference_norms = ref
+        res = (lgdhip.lgdhip_poisson_result * self.count)()
+        lgdhip.call("lgdhip_poisson_batch_solve", self.u.ptr, self.f.ptr, self.probs, self.count, C.byref(opt),
+                    self.ws.ptr, self.ws_bytes, res, self.stream)
+        return res
+
+    def interior(self, k: int) -> np.ndarray:
+        """Problem k's interior of u (FP64, [z, y, x])."""
+        n = self.n
+        host = np.empty((n + 2) ** 3)
+        src = C.c_void_p(self.u.value + self.probs[k].offset * 8)
+        lgdhip.call("lgdhip_memcpy_d2h", host.ctypes.data_as(C.c_void_p), src, host.nbytes, self.stream)
+        return host.reshape((n + 2,) * 3)[1:-1, 1:-1, 1:-1].copy()
+
+
+def harmonic_basis(n: int, rel_tol: float = 1e-9, stream=None) -> "lgdhip.DeviceBuffer":
+    """H: the solution for f = 0, value 1 on the face x = -1, 0 on the other
+    faces, by the batched Jacobi solver; dense n^3 FP32 on the device."""
+    f = np.zeros((n + 2,) * 3, np.float32)
+    ens = Ensemble([(n, np.array([1.0, 0, 0, 0, 0, 0]), f)], stream=stream)
+    res = ens.solve(rel_tol)
+    assert res[0].converged == 1, "harmonic basis did not converge"
+    H = np.ascontiguousarray(ens.interior(0).astype(np.float32))
+    buf = lgdhip.DeviceBuffer(H.nbytes)
+    buf.upload(H, stream)
+    return buf
+
+
+class LearnedGuess:
+    """U-Net proposing initial guesses for n^3 problems.  With the harmonic
+    basis H (default) it learns the source response (u - h)/s from f/s
+    (s = max |f|); without, u/s from (f/s, b/s) (s = max(|f|, |faces|))."""
+
+    def __init__(self, n: int, channels=(16, 32, 64, 128), seed: int = 0, harmonic=None, stream=None):
+        self.n = n
+        self.H = harmonic
+        self.stream = stream
+        self.in_ch = 1 if harmonic is not None else 2
+        self.net = UNet3D(n=n, channels=channels, in_ch=self.in_ch, seed=seed, stream=stream, bias=False)
+        self.target = lgdhip.DeviceBuffer(n ** 3 * 4)
+        self.rng = np.random.default_rng(seed + 1)
+        self._Hp = harmonic.ptr if harmonic is not None else None
+
+    def scales(self, data: "Ensemble") -> np.ndarray:
+        s = data.fmax if self.H is not None else np.maximum(data.fmax, data.facemax)
+        return np.maximum(s, 1e-30)
+
+    def _input(self, data: Ensemble, k: int, sym: int, inv: float):
+        lgdhip.call("lgdhip_poisson_unet_input", data.f.ptr, C.byref(data.probs[k]), inv, sym, None,
+                    self.net.x.ptr, self.net.x.ld, self.stream)
+
+    def sample_norms(self, data: Ensemble) -> np.ndarray:
+        """RMS of every problem's target at unit input scale (data.u must
+        hold the solutions)."""
+        s = self.scales(data)
+        out = np.zeros(data.count)
+        for k in range(data.count):
+            lgdhip.call("lgdhip_poisson_unet_target", data.u.ptr, C.byref(data.probs[k]), 1.0 / s[k], 0, self._Hp,
+                        self.target.ptr, self.stream)
+            t = np.empty(self.n ** 3, np.float32)
+            self.target.download(t, self.stream)
+            out[k] = float(np.sqrt(np.mean(t.astype(np.float64) ** 2)))
+        return out
+
+    def train(self, data: Ensemble, steps: int, lr: float = 1e-3, batch: int = 2, log_every: int = 0,
+              lr_final: float = 0.05) -> list:
+        """Adam on mini-batches of `batch` augmented samples (gradients
+        accumulated), cosine learning-rate decay to lr * lr_final.  data.u
+        must hold the solutions.  Every sample is scaled to a unit-RMS
+        target (exact for the bias-free network).  Returns (step, loss)."""
+        assert data.n == self.n
+        s = self.scales(data)
+        rms = self.sample_norms(data)
+        g = np.where(rms > 0, 1.0 / np.maximum(rms, 1e-300), 0.0)
+        log = []
+        for step in range(steps):
+            for b in range(batch):
+                k = int(self.rng.integers(data.count))
+                sym = int(self.rng.integers(48))
+                sign = 1.0 if self.rng.integers(2) else -1.0
+                inv = sign * g[k] / s[k]
+                self._input(data, k, sym, inv)
+                lgdhip.call("lgdhip_poisson_unet_target", data.u.ptr, C.byref(data.probs[k]), inv, sym, self._Hp,
+                            self.target.ptr, self.stream)
+                self.net.forward()
+                read = log_every and (step % log_every == 0 or step == steps - 1) and b == 0
+                loss = self.net.loss(self.target, read=bool(read))
+                if read:
+                    log.append((step, loss))
+                self.net.backward(accumulate=1 if b else 0)
+            frac = step / max(1, steps - 1)
+            lr_t = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * frac)))
+            self.net.adam(lr=lr_t)
+        return log
+
+    def propose(self, data: Ensemble, k: int, s: float):
+        """Writes the network's guess for problem k into data.u."""
+        self._input(data, k, 0, 1.0 / s)
+        pred = self.net.forward()
+        lgdhip.call("lgdhip_poisson_unet_guess", pred.ptr, C.byref(data.probs[k]), s, self._Hp, data.u.ptr,
+                    self.stream)
+
+
+def _arm(data: Ensemble, name: str, exact: list, results, t_guess: float, t_solve: float, r_guess=None,
+         guess=None) -> dict:
+    sw = np.array([r.sweeps for r in results], dtype=np.float64)
+    out = {"arm": name, "converged": int(sum(r.converged for r in results)), "sweeps_mean": float(sw.mean()),
+           "sweeps_total": float(sw.sum()), "wall_guess_s": t_guess, "wall_solve_s": t_solve,
+           "wall_s": t_guess + t_solve,
+           "final_rel_residual_max": float(max(r.residual_norm / r.reference_norm for r in results))}
+    if guess is not None:
+        err = [float(np.linalg.norm(g - e) / max(np.linalg.norm(e), 1e-300)) for g, e in zip(guess, exact)]
+        out["guess_rel_error_median"] = float(np.median(err))
+        out["guess_rel_error_max"] = float(np.max(err))
+        out["guess_rel_residual_median"] = float(np.median(r_guess))
+    return out
+
+
+def evaluate(model: LearnedGuess, data: Ensemble, rel_tol: float = 1e-6, chunk: int = 64,
+             smooth_sweeps: int = 128) -> dict:
+    """Zero guess (plain Jacobi), and with the harmonic basis the boundary
+    response h alone and the learned guess h + s net(f/s), on the same
+    problems with the same solver, tolerance and reference norms (the zero
+    guess's residual).  Guessed arms start with `smooth_sweeps` weighted
+    (2/3) Jacobi sweeps, counted in their sweeps.  Wall times include
+    computing the guess; the harmonic basis is per cube size, computed once
+    and reported separately."""
+    data.reset()
+    _sync()
+    t0 = time.perf_counter()
+    res0 = data.solve(rel_tol, chunk)
+    _sync()
+    t_zero = time.perf_counter() - t0
+    exact = [data.interior(k) for k in range(data.count)]
+    arms = [_arm(data, "zero", exact, res0, 0.0, t_zero)]
+
+    def guessed(name, make):
+        data.reset()
+        _sync()
+        t0 = time.perf_counter()
+        make()
+        _sync()
+        t_g = time.perf_counter() - t0
+        guess = [data.interior(k) for k in range(data.count)]
+        r_guess = data.residual_norms() / data.reference
+        t0 = time.perf_counter()
+        res = data.solve(rel_tol, chunk, smooth_sweeps=smooth_sweeps)
+        _sync()
+        arms.append(_arm(data, name, exact, res, t_g, time.perf_counter() - t0, r_guess, guess))
+
+    if model.H is not None:
+        guessed("harmonic", lambda: [lgdhip.call("lgdhip_poisson_face_guess", model.H.ptr, C.byref(data.probs[k]),
+                                                 data.u.ptr, data.stream) for k in range(data.count)])
+    s = model.scales(data)
+    guessed("learned", lambda: [model.propose(data, k, s[k]) for k in range(data.count)])
+    best = arms[-1]
+    return {"problems": data.count, "n": data.n, "rel_tol": rel_tol, "smooth_sweeps": smooth_sweeps, "arms": arms,
+            "sweep_ratio": arms[0]["sweeps_total"] / max(best["sweeps_total"], 1.0),
+            "wall_speedup": arms[0]["wall_s"] / best["wall_s"]}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--train", type=int, default=256, help="solved training problems")
+    ap.add_argument("--eval", type=int, default=16, help="held-out problems")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=2)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--smooth", type=int, default=128, help="weighted-Jacobi sweeps before polishing a guess")
+    ap.add_argument("--no-harmonic", action="store_true", help="learn u from (f, b) without the harmonic split")
+    ap.add_argument("--seed", type=int, default=synthetic.DEFAULT_SEED)
+    ap.add_argument("--log-every", type=int, default=250)
+    a = ap.parse_args(argv)
+    lgdhip.load()
+    lgdhip.call("lgdhip_set_device", 0)
+    stream = lgdhip.Stream()
+    out = {"n": a.n, "train_problems": a.train, "eval_problems": a.eval, "steps": a.steps, "batch": a.batch,
+           "harmonic": not a.no_harmonic}
+
+    H = None
+    if not a.no_harmonic:
+        t0 = time.perf_counter()
+        H = harmonic_basis(a.n, stream=stream.h)
+        _sync()
+        out["harmonic_basis_s"] = time.perf_counter() - t0
+
+    t0 = time.perf_counter()
+    train = Ensemble(synthetic.poisson_ensemble(a.train, sizes=(a.n,), seed=a.seed), stream=stream.h)
+    res = train.solve(a.tol)
+    _sync()
+    out["generate_s"] = time.perf_counter() - t0
+    out["generate_sweeps_mean"] = float(np.mean([r.sweeps for r in res]))
+    out["generate_converged"] = int(sum(r.converged for r in res))
+    print(f"[learned] generated {a.train} solutions of {a.n}^3 in {out['generate_s']:.1f} s "
+          f"({out['generate_sweeps_mean']:.0f} sweeps each)", flush=True)
+
+    model = LearnedGuess(a.n, seed=a.seed % 1000, harmonic=H, stream=stream.h)
+    t0 = time.perf_counter()
+    log = model.train(train, a.steps, lr=a.lr, batch=a.batch, log_every=a.log_every)
+    _sync()
+    out["train_s"] = time.perf_counter() - t0
+    out["loss_log"] = log
+    print(f"[learned] trained {a.steps} steps in {out['train_s']:.1f} s, loss {log[0][1]:.3e} -> {log[-1][1]:.3e}",
+          flush=True)
+    del train
+
+    ev = Ensemble(synthetic.poisson_ensemble(a.eval, sizes=(a.n,), seed=a.seed + 7919), stream=stream.h)
+    out["eval"] = evaluate(model, ev, a.tol, smooth_sweeps=a.smooth)
+    for arm in out["eval"]["arms"]:
+        print(f"[learned] {arm['arm']:9s} sweeps {arm['sweeps_mean']:9.0f}  wall {arm['wall_s']:.3f} s  "
+              f"guess err {arm.get('guess_rel_error_median', 1.0):.2e}", flush=True)
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -53,7 +53,8 @@ class lgdhip_poisson_problem(C.Structure):
 class lgdhip_poisson_options(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("chunk", C.c_int32), ("max_sweeps", C.c_int32),
                 ("inner_reduction", C.c_double), ("use_graph", C.c_int32), ("reserved", C.c_int32),
-                ("reference_norms", C.POINTER(C.c_double))]
+                ("reference_norms", C.POINTER(C.c_double)), ("smooth_sweeps", C.c_int32),
+                ("smooth_omega", C.c_float)]
 
 
 class lgdhip_poisson_result(C.Structure):
@@ -145,6 +146,10 @@ SIGNATURES = {
     "lgdhip_mse_loss": (_I, [_P, _P, _I64, _P, _I, _P, _P]),
     "lgdhip_adam_step": (_I, [_P, _P, _P, _P, _I64, _I, _I, C.c_float, C.c_float, C.c_float, C.c_float, _I,
                               _P, _P]),
+    "lgdhip_poisson_unet_input": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _I, _P, _P, _I, _P]),
+    "lgdhip_poisson_face_guess": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _P, _P]),
+    "lgdhip_poisson_unet_target": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _I, _P, _P, _P]),
+    "lgdhip_poisson_unet_guess": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _P, _P, _P]),
 }
 
 # lgdhip_conv3d_kpad returns a value, not a status

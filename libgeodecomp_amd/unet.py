@@ -103,10 +103,15 @@ class Conv:
 class UNet3D:
     """Weights, activations and gradients of one U-Net on n^3 volumes."""
 
-    def __init__(self, n: int = 128, channels=(16, 32, 64, 128), in_ch: int = 2, seed: int = 0, stream=None):
+    def __init__(self, n: int = 128, channels=(16, 32, 64, 128), in_ch: int = 2, seed: int = 0, stream=None,
+                 bias: bool = True):
+        """bias=False drops every bias: the network is then positively
+        homogeneous, net(c x) = c net(x) for c > 0."""
         assert n % 8 == 0 and n >= 8, "three 2x poolings need n to be a multiple of 8"
+        assert 1 <= in_ch <= 8
         self.n, self.ch, self.in_ch = n, channels, in_ch
         self.stream = stream
+        self.use_bias = bias
         self.lib = lgdhip.load()
         c0, c1, c2, c3 = channels
         s = [n, n // 2, n // 4, n // 8]
@@ -139,7 +144,7 @@ class UNet3D:
             l.gb = lgdhip.DeviceBuffer(l.cout * 4)
             l.m = [lgdhip.DeviceBuffer(l.cout * l.kp * 4), lgdhip.DeviceBuffer(l.cout * 4)]
             l.v = [lgdhip.DeviceBuffer(l.cout * l.kp * 4), lgdhip.DeviceBuffer(l.cout * 4)]
-            for b in l.m + l.v:
+            for b in l.m + l.v + [l.gb]:
                 lgdhip.call("lgdhip_fill_u8", b.ptr, 0, b.nbytes, stream)
         self.t = 0
         # activations (forward) and their gradients
@@ -175,9 +180,9 @@ class UNet3D:
             l.wb = lgdhip.DeviceBuffer(wl.size * 2)
             l.wt = lgdhip.DeviceBuffer(l.cin * self.lib.lgdhip_conv3d_kpad(l.cout, l.ksize) * 2)
             l.bias = lgdhip.DeviceBuffer(l.cout * 4)
-        l.w32.upload(wl)
-        l.wb.upload(to_bf16_bits(wl))
-        l.bias.upload(np.asarray(b, dtype=np.float32))
+        l.w32.upload(wl, self.stream)
+        l.wb.upload(to_bf16_bits(wl), self.stream)
+        l.bias.upload(np.asarray(b, dtype=np.float32), self.stream)
         self._refresh_wt(l)
 
     def _refresh_wt(self, l: Conv):
@@ -186,10 +191,10 @@ class UNet3D:
     def get_weights(self, name: str):
         l = self.layers[name]
         wl = np.empty((l.cout, l.kp), np.float32)
-        l.w32.download(wl)
+        l.w32.download(wl, self.stream)
         taps = 27 if l.ksize == 3 else 1
         b = np.empty(l.cout, np.float32)
-        l.bias.download(b)
+        l.bias.download(b, self.stream)
         return wl[:, :taps * l.cin].reshape(l.cout, taps, l.cin), b
 
     def set_weights(self, name: str, w, b):
@@ -198,7 +203,8 @@ class UNet3D:
     # ------------------------------------------------------------------
     def _conv(self, name, x: Tensor, xoff, y: Tensor | None, yoff, y32=None, accumulate=0):
         l = self.layers[name]
-        lgdhip.call("lgdhip_conv3d_bf16_forward", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, l.wb.ptr, l.bias.ptr,
+        lgdhip.call("lgdhip_conv3d_bf16_forward", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, l.wb.ptr,
+                    l.bias.ptr if self.use_bias else None,
                     l.cout, y.ptr if y is not None else None, y.ld if y is not None else 1, yoff,
                     y32, 1, 1 if l.relu else 0, accumulate, self.stream)
 
@@ -234,13 +240,16 @@ class UNet3D:
         self._conv("out", self.b0d, 0, None, 0, y32=self.pred.ptr)
         return self.pred
 
-    def loss(self, target) -> float:
-        """MSE against an FP32 device buffer; sets the prediction gradient."""
+    def loss(self, target, read: bool = True) -> float:
+        """MSE against an FP32 device buffer; sets the prediction gradient.
+        read=False skips the host read (and its synchronisation): returns nan."""
         nv = self.n ** 3
         lgdhip.call("lgdhip_mse_loss", self.pred.ptr, target.ptr, nv, self.gpred.ptr, 1, self.loss_partial.ptr,
                     self.stream)
+        if not read:
+            return float("nan")
         parts = np.empty(self.lib.lgdhip_mse_partials(nv), np.float32)
-        self.loss_partial.download(parts)
+        self.loss_partial.download(parts, self.stream)
         return float(parts.astype(np.float64).sum() / nv)
 
     def _back(self, name, x: Tensor, xoff, g_out: Tensor, goff, y_act: Tensor | None, yoff, g_in: Tensor | None,
@@ -253,7 +262,8 @@ class UNet3D:
             lgdhip.call("lgdhip_relu_backward_bf16", g_out.ptr, g_out.ld, goff, y_act.ptr, y_act.ld, yoff, l.cout,
                         l.n ** 3, S)
         lgdhip.call("lgdhip_conv3d_bf16_wgrad", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, g_out.ptr, g_out.ld, goff,
-                    l.cout, l.gw.ptr, l.gb.ptr, accumulate_w, 4096, self.ws.ptr, self.ws_bytes, S)
+                    l.cout, l.gw.ptr, l.gb.ptr if self.use_bias else None, accumulate_w, 4096, self.ws.ptr,
+                    self.ws_bytes, S)
         if g_in is not None:
             lgdhip.call("lgdhip_conv3d_bf16_forward", g_out.ptr, l.n, g_out.ld, goff, l.cout, l.ksize, l.wt.ptr,
                         None, l.cin, g_in.ptr, g_in.ld, ginoff, None, 1, 0, accumulate_in, S)
@@ -293,9 +303,9 @@ class UNet3D:
         self._back("e0b", self.a0, 0, G["cat0"], c1, self.cat0, c1, G["a0"], 0, acc)
         self._back("e0a", self.x, 0, G["a0"], 0, self.a0, 0, None, 0, acc)
 
-    def adam(self, lr: float = 1e-3, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8, scale: float = 1.0):
-        """One Adam step on every layer (gradients divided by `scale`, the
-        number of accumulated samples, through the learning rate)."""
+    def adam(self, lr: float = 1e-3, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):
+        """One Adam step on every layer (Adam is invariant to the gradient's
+        scale, so accumulated mini-batch sums need no division)."""
         self.t += 1
         S = self.stream
         for name in self.order:
@@ -303,16 +313,17 @@ class UNet3D:
             taps = 27 if l.ksize == 3 else 1
             lgdhip.call("lgdhip_adam_step", l.w32.ptr, l.m[0].ptr, l.v[0].ptr, l.gw.ptr, l.cout * l.kp,
                         taps * l.cin, l.kp, lr, b1, b2, eps, self.t, l.wb.ptr, S)
-            lgdhip.call("lgdhip_adam_step", l.bias.ptr, l.m[1].ptr, l.v[1].ptr, l.gb.ptr, l.cout, 0, 0, lr, b1, b2,
-                        eps, self.t, None, S)
+            if self.use_bias:
+                lgdhip.call("lgdhip_adam_step", l.bias.ptr, l.m[1].ptr, l.v[1].ptr, l.gb.ptr, l.cout, 0, 0, lr, b1,
+                            b2, eps, self.t, None, S)
             self._refresh_wt(l)
 
     def grads(self, name: str):
         l = self.layers[name]
         g = np.empty((l.cout, l.kp), np.float32)
-        l.gw.download(g)
+        l.gw.download(g, self.stream)
         gb = np.empty(l.cout, np.float32)
-        l.gb.download(gb)
+        l.gb.download(gb, self.stream)
         taps = 27 if l.ksize == 3 else 1
         return g[:, :taps * l.cin].reshape(l.cout, taps, l.cin), gb
 
