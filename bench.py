@@ -7,8 +7,10 @@ constant dead edge (Cube topology), 1000 generations.  One benchmark step is
 one whole run: restore the pristine uint8 input (device copy), pack, 1000
 generations, unpack -- uint8 grid in HBM in, uint8 grid in HBM out.
 
-Other workloads (`--workload jacobi|lbm`) measure the Jacobi3D FP64 and the
-D3Q19 FP32 families on their configured shapes.
+Other workloads (`--workload jacobi|lbm|poisson`) measure the Jacobi3D FP64
+and D3Q19 FP32 families on their configured shapes and the batched Poisson
+ensemble (one step = the whole ensemble solved to 1e-6; run it with
+`--steps 1 --warmup 0`).
 
 Prints ONE JSON line (see the keys below).  GPU time comes from HIP events
 recorded on the stream the kernels run on; the dominant kernel's average
@@ -188,13 +190,21 @@ class Jacobi:
         lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps,
                     C.byref(self.tile), None, s)
 
+    # sweeps per launch of the temporal-blocking variants
+    TB_DEPTH = {4: 2, 5: 3, 6: 4, 7: 2, 8: 3}
+
     def dominant_kernel(self):
-        launches = 200
+        k = self.TB_DEPTH.get(self.tile.variant, 1)
+        sweeps = 240   # a multiple of every blocking depth: launches of one kernel only
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr,
-                                                    C.byref(self.g), launches, C.byref(self.tile), None,
+                                                    C.byref(self.g), sweeps, C.byref(self.tile), None,
                                                     self.stream.h))
-        name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
-        return name, ms / launches, self.bytes_per_lup * float(self.n) ** 3
+        if k > 1:
+            name = "jacobi3d_f64_tb"
+        else:
+            name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
+        # one launch advances k sweeps: algorithmic bytes are k sweeps' worth
+        return name, ms / (sweeps // k), k * self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -298,6 +308,104 @@ def equilibrium_f32(rho, jx, jy, jz):
     return f
 
 
+# ----------------------------------------------------------------------
+# Batched Poisson ensemble (FP32 sweeps, FP64 residuals)
+# ----------------------------------------------------------------------
+class Poisson:
+    """512 independent n^3 problems, n uniform in {32, 48, ..., 192}, 1-8
+    Gaussian sources, Dirichlet faces U[-1,1]; one benchmark step solves the
+    whole ensemble from the zero guess to a 1e-6 relative FP64 residual
+    (one batched solve, hipGraph chunks).  GLUPS counts sweep-cell updates
+    (sum over problems of n^3 x sweeps)."""
+    name = "poisson"
+    dtype = "f32"   # the sweeps; residuals and updates are FP64
+    bytes_per_lup = 12.0  # read e + read r + write e (FP32)
+
+    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, **_):
+        self.count, self.seed = n, seed   # --size overrides the problem count
+        self.stream = lgdhip.Stream()
+        s = self.stream.h
+        t0 = time.perf_counter()
+        problems = synthetic.poisson_ensemble(self.count, seed=seed)
+        log(f"generated {self.count} problems on the host in {time.perf_counter() - t0:.1f} s")
+        self.sizes = np.array([p[0] for p in problems], dtype=np.int64)
+        self.probs = (lgdhip.lgdhip_poisson_problem * self.count)()
+        for k, (m, faces, _) in enumerate(problems):
+            self.probs[k].n = m
+            for j in range(6):
+                self.probs[k].face[j] = faces[j]
+        lib = lgdhip.load()
+        self.elems = lib.lgdhip_poisson_layout(self.probs, self.count)
+        self.ws_bytes = lib.lgdhip_poisson_workspace_bytes(self.probs, self.count)
+        self.ws = lgdhip.DeviceBuffer(self.ws_bytes)
+        self.u = lgdhip.DeviceBuffer(self.elems * 8)
+        self.f = lgdhip.DeviceBuffer(self.elems * 4)
+        for k, (m, _, f) in enumerate(problems):
+            host = np.ascontiguousarray(f, dtype=np.float32).ravel()
+            dst = C.c_void_p(self.f.value + self.probs[k].offset * 4)
+            lgdhip.call("lgdhip_memcpy_h2d", dst, host.ctypes.data_as(C.c_void_p), host.nbytes, s)
+        self.sample = problems[int(np.argmin(np.abs(self.sizes - 64)))]
+        del problems
+        self.cells = float(np.sum(self.sizes.astype(np.float64) ** 3))
+        self.last_lups = 0.0
+        self.last = None
+
+    def config(self):
+        sw = [r.sweeps for r in self.last] if self.last is not None else []
+        return {"workload": f"batched Poisson ensemble, {self.count} problems n in {{32..192 step 16}}, 1-8 Gaussian "
+                            f"sources, Dirichlet faces U[-1,1], zero guess to 1e-6 relative FP64 residual",
+                "problems": self.count, "cells": int(self.cells), "seed": self.seed,
+                "sweeps_max": int(max(sw)) if sw else None,
+                "converged": int(sum(r.converged for r in self.last)) if self.last is not None else None,
+                "parallelism": "single GPU"}
+
+    def lups_per_step(self):
+        return self.last_lups
+
+    def step(self):
+        s = self.stream.h
+        lgdhip.call("lgdhip_fill_u8", self.u.ptr, 0, self.elems * 8, s)
+        lgdhip.call("lgdhip_poisson_set_faces", self.u.ptr, self.probs, self.count, self.ws.ptr, self.ws_bytes, s)
+        opt = lgdhip.lgdhip_poisson_options()
+        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = 1e-6, 64, 10_000_000, 1
+        res = (lgdhip.lgdhip_poisson_result * self.count)()
+        lgdhip.call("lgdhip_poisson_batch_solve", self.u.ptr, self.f.ptr, self.probs, self.count, C.byref(opt),
+                    self.ws.ptr, self.ws_bytes, res, s)
+        self.last = res
+        self.last_lups = float(sum(float(self.sizes[k]) ** 3 * res[k].sweeps for k in range(self.count)))
+        log(f"solved: max sweeps {max(r.sweeps for r in res)}, converged {sum(r.converged for r in res)}")
+
+    def dominant_kernel(self):
+        """poisson_sweep_f32 over every problem (the hot loop of the solve)."""
+        e0 = lgdhip.DeviceBuffer(self.elems * 4)
+        e1 = lgdhip.DeviceBuffer(self.elems * 4)
+        for b in (e0, e1):
+            lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.elems * 4, self.stream.h)
+        sweeps = 20
+        in_b = C.c_int(0)
+        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, self.f.ptr,
+                                                    self.probs, self.count, sweeps, self.ws.ptr, self.ws_bytes,
+                                                    C.byref(in_b), self.stream.h))
+        return "poisson_sweep_f32", ms / sweeps, self.bytes_per_lup * self.cells
+
+    def cpu_baseline(self, budget_s=15.0):
+        oracle = load_oracle()
+        m, faces, f = self.sample
+        u = np.zeros((m + 2,) * 3)
+        oracle.poisson_faces(u, faces)
+        sweeps = 20
+        t0 = time.perf_counter()
+        oracle.poisson_jacobi(u.copy(), f, 1e-300, max_sweeps=sweeps, check=sweeps)
+        dt = time.perf_counter() - t0
+        sweeps = max(20, min(200000, int(sweeps * budget_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.poisson_jacobi(u.copy(), f, 1e-300, max_sweeps=sweeps, check=sweeps)
+        dt = time.perf_counter() - t0
+        return {"value": 1e-9 * m ** 3 * sweeps / dt, "unit": "GLUPS", "cores": 1, "kind": "port",
+                "sample": f"C oracle plain FP64 Jacobi (1 thread), one {m}^3 problem of the ensemble, {sweeps} "
+                          f"sweeps, {dt:.1f}s"}
+
+
 class Copy:
     """Calibration: the plain 16-byte streaming copy kernel over 2 GiB, the
     measured HBM ceiling the stencil kernels are compared with (not a
@@ -331,7 +439,7 @@ class Copy:
         return None
 
 
-WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "copy": Copy}
+WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "poisson": Poisson, "copy": Copy}
 
 
 def main():

@@ -109,9 +109,126 @@ __global__ __launch_bounds__(kWaveX *BY) void jacobi3d_f64_zbatch(
     }
 }
 
+// value of lane-1 / lane+1 of a double (DPP wave_shr:1 / wave_shl:1 on both
+// halves; lanes at the wave's ends read 0 and only feed halo cells)
+__device__ __forceinline__ double lane_prev(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_next(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+
+// K sweeps per pass over HBM (temporal blocking).  A work-group is a 64 x TY
+// tile of (x, y) columns whose outer K lanes/rows are an overlap ring
+// recomputed by the neighbouring tiles, so it writes (64-2K) x (TY-2K)
+// columns.  It marches its z chunk as a pipeline: when level-0 plane p
+// arrives, level l (l = 1..K) is computed at plane p-l from level l-1 at
+// planes p-l-1, p-l, p-l+1 (a 3-plane register window per level); x
+// neighbours come from the adjacent lanes by DPP, y neighbours from the
+// level's plane p-l published in LDS in the previous step (double
+// buffered: one barrier per plane).  Cells outside the box hold the edge
+// value at every level, exactly as the halo does between single sweeps, and
+// every cell sums in the single sweep's order, so K passes of this kernel
+// equal K single sweeps bit for bit.  Level-0 planes are prefetched PF
+// planes ahead.
+template <int K, int TY, int PF>
+__global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restrict__ src, double *__restrict__ dst,
+                                                           int nx, int ny, int nz, int64_t sx, int64_t sxy,
+                                                           int64_t origin, int zchunk) {
+    constexpr int OX = 64 - 2 * K, OY = TY - 2 * K;
+    __shared__ double lds[2][K][TY][64];
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = (int)blockIdx.x * OX - K + tx;
+    const int y = (int)blockIdx.y * OY - K + ty;
+    const int z0 = (int)blockIdx.z * zchunk;
+    const int z1 = min(z0 + zchunk, nz);
+    const bool inxy = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny;
+    const bool writer = inxy && tx >= K && tx < 64 - K && ty >= K && ty < TY - K;
+    const double edge = src[origin - 1];  // the Cube edge value held by every halo cell
+    const int64_t col = origin + (int64_t)y * sx + x;
+    const int yd = ty > 0 ? ty - 1 : 0;
+    const int yu = ty < TY - 1 ? ty + 1 : TY - 1;
+
+    auto load0 = [&](int p) -> double {
+        return (inxy && (unsigned)p < (unsigned)nz) ? src[col + (int64_t)p * sxy] : edge;
+    };
+
+    double w[K][3];
+#pragma unroll
+    for (int l = 0; l < K; ++l) w[l][0] = w[l][1] = w[l][2] = edge;
+#pragma unroll
+    for (int l = 0; l < K; ++l) lds[0][l][ty][tx] = edge;
+    double pf[PF];
+    const int pbeg = z0 - K, pend = z1 + K;  // level-0 planes pbeg .. pend-1
+#pragma unroll
+    for (int d = 0; d < PF; ++d) pf[d] = load0(pbeg + d);
+    __syncthreads();
+
+    int buf = 0;
+    for (int p = pbeg; p < pend; ++p) {
+        const double v0 = pf[0];
+#pragma unroll
+        for (int d = 0; d < PF - 1; ++d) pf[d] = pf[d + 1];
+        pf[PF - 1] = load0(p + PF);
+        w[0][0] = w[0][1];
+        w[0][1] = w[0][2];
+        w[0][2] = v0;
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            // level l at plane q = p - l from level l-1 at planes q-1, q, q+1
+            const int q = p - l;
+            const double c = w[l - 1][1];
+            const double xm = lane_prev(c);
+            const double xp = lane_next(c);
+            const double ym = lds[buf][l - 1][yd][tx];
+            const double yp = lds[buf][l - 1][yu][tx];
+            double s = w[l - 1][0];  // ( 0,  0, -1)
+            s += ym;                 // ( 0, -1,  0)
+            s += xm;                 // (-1,  0,  0)
+            s += c;                  // ( 0,  0,  0)
+            s += xp;                 // ( 1,  0,  0)
+            s += yp;                 // ( 0,  1,  0)
+            s += w[l - 1][2];        // ( 0,  0,  1)
+            const double v = (inxy && (unsigned)q < (unsigned)nz) ? s * (1.0 / 7.0) : edge;
+            if (l < K) {
+                w[l][0] = w[l][1];
+                w[l][1] = w[l][2];
+                w[l][2] = v;
+            } else if (writer && q >= z0) {
+                __builtin_nontemporal_store(v, dst + col + (int64_t)q * sxy);
+            }
+        }
+        // publish level l at plane p - l for the next step's y neighbours
+#pragma unroll
+        for (int l = 0; l < K; ++l) lds[buf ^ 1][l][ty][tx] = w[l][2];
+        __syncthreads();
+        buf ^= 1;
+    }
+}
+
 // variant ids (lgdhip_tile::variant): 0 default, 1 single-plane march,
-// 2 four-plane batches, 3 eight-plane batches
+// 2 four-plane batches, 3 eight-plane batches; temporal blocking (K sweeps
+// per launch, tile 64 x TY): 4 K=2 TY=16, 5 K=3 TY=16, 6 K=4 TY=16,
+// 7 K=2 TY=8, 8 K=3 TY=8
 constexpr int kJacobiDefaultVariant = 2;
+
+struct TbShape {
+    int k, ty;
+};
+TbShape tb_shape(int variant) {
+    switch (variant) {
+    case 4: return {2, 16};
+    case 5: return {3, 16};
+    case 6: return {4, 16};
+    case 7: return {2, 8};
+    case 8: return {3, 8};
+    default: return {0, 0};
+    }
+}
 
 struct JacobiLaunch {
     dim3 grid, block;
@@ -128,7 +245,23 @@ JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     if (tile) {
         if (tile->by == 1 || tile->by == 2 || tile->by == 4 || tile->by == 8) L.by = tile->by;
         if (tile->bz > 0) L.zchunk = tile->bz;
-        if (tile->variant >= 1 && tile->variant <= 3) L.variant = tile->variant;
+        if (tile->variant >= 1 && tile->variant <= 8) L.variant = tile->variant;
+    }
+    const TbShape tb = tb_shape(L.variant);
+    if (tb.k > 0) {
+        // temporal blocking: tiles of (64-2K) x (TY-2K) output columns; z
+        // chunks of 64 planes (2K extra pipeline planes each), shrunk until
+        // the grid has >= 1024 work-groups
+        const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
+        const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty - 2 * tb.k);
+        if (!tile || tile->bz <= 0) {
+            L.zchunk = 64;
+            while (L.zchunk > 8 && (uint64_t)gx * gy * lgdhip::ceil_div(g.nz, L.zchunk) < 1024) L.zchunk /= 2;
+        }
+        L.by = tb.ty;
+        L.block = dim3(64, tb.ty, 1);
+        L.grid = dim3(gx, gy, lgdhip::ceil_div(g.nz, L.zchunk));
+        return L;
     }
     // aim for several thousand work-groups so all 256 CUs stay busy even on
     // small cubes: shrink the z chunk until the grid is large enough
@@ -152,6 +285,23 @@ void launch_batched(const double *src, double *dst, const lgdhip_grid &g, const 
     else
         jacobi3d_f64_zbatch<BY, 4><<<L.grid, L.block, 0, s>>>(src, dst, (int)g.nx, (int)g.ny, (int)g.nz, g.sx, sxy,
                                                              origin, L.zchunk);
+}
+
+// one launch of the temporal-blocked kernel: K sweeps src -> dst
+int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L, hipStream_t s) {
+    const int64_t sxy = g.sx * g.sy;
+    const int64_t origin = lgdhip::origin_index(g);
+    const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    switch (L.variant) {
+    case 4: jacobi3d_f64_tb<2, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk); break;
+    case 5: jacobi3d_f64_tb<3, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk); break;
+    case 6: jacobi3d_f64_tb<4, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk); break;
+    case 7: jacobi3d_f64_tb<2, 8, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk); break;
+    case 8: jacobi3d_f64_tb<3, 8, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk); break;
+    default: return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "not a temporal-blocking variant");
+    }
+    LGD_LAUNCHED("jacobi3d_f64_tb");
+    return 0;
 }
 
 int launch_sweep(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L,
@@ -201,8 +351,29 @@ extern "C" int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *
     const JacobiLaunch L = plan(*g, tile);
     hipStream_t s = lgdhip::as_stream(stream);
     double *cur = a, *nxt = b;
-    for (int t = 0; t < steps; ++t) {
-        rc = launch_sweep(cur, nxt, *g, L, s);
+    int t = 0;
+    const int k = tb_shape(L.variant).k;
+    if (k > 0) {
+        // the blocked kernel reads memory only inside the box (plus the edge
+        // value at the halo cell left of the origin)
+        for (; t + k <= steps; t += k) {
+            rc = launch_tb(cur, nxt, *g, L, s);
+            if (rc) return rc;
+            double *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+    }
+    JacobiLaunch single = L;
+    if (k > 0) {
+        lgdhip_tile plain = tile ? *tile : lgdhip_tile{};
+        plain.variant = kJacobiDefaultVariant;
+        plain.by = 0;
+        plain.bz = 0;
+        single = plan(*g, &plain);
+    }
+    for (; t < steps; ++t) {
+        rc = launch_sweep(cur, nxt, *g, single, s);
         if (rc) return rc;
         double *tmp = cur;
         cur = nxt;

@@ -156,7 +156,7 @@ class LearnedGuess:
         return out
 
     def train(self, data: Ensemble, steps: int, lr: float = 1e-3, batch: int = 2, log_every: int = 0,
-              lr_final: float = 0.05) -> list:
+              lr_final: float = 0.05, verbose: bool = False) -> list:
         """Adam on mini-batches of `batch` augmented samples (gradients
         accumulated), cosine learning-rate decay to lr * lr_final.  data.u
         must hold the solutions.  Every sample is scaled to a unit-RMS
@@ -180,6 +180,8 @@ class LearnedGuess:
                 loss = self.net.loss(self.target, read=bool(read))
                 if read:
                     log.append((step, loss))
+                    if verbose:
+                        print(f"[learned] step {step} loss {loss:.4e}", flush=True)
                 self.net.backward(accumulate=1 if b else 0)
             frac = step / max(1, steps - 1)
             lr_t = lr * (lr_final + (1.0 - lr_final) * 0.5 * (1.0 + math.cos(math.pi * frac)))
@@ -291,7 +293,7 @@ def main(argv=None):
 
     model = LearnedGuess(a.n, seed=a.seed % 1000, harmonic=H, stream=stream.h)
     t0 = time.perf_counter()
-    log = model.train(train, a.steps, lr=a.lr, batch=a.batch, log_every=a.log_every)
+    log = model.train(train, a.steps, lr=a.lr, batch=a.batch, log_every=a.log_every, verbose=True)
     _sync()
     out["train_s"] = time.perf_counter() - t0
     out["loss_log"] = log
