@@ -150,7 +150,8 @@ int lgdhip_grid_store(const void *grid, const lgdhip_grid *g, void *host,
 /* ------------------------------------------------------------------ */
 /* Runs `steps` sweeps ping-ponging between a and b (both carry the edge
  * value in their halo).  *result_in_b is set to 1 when the final state is
- * in b, 0 when it is in a. */
+ * in b, 0 when it is in a (tile->variant 4-8 run K = 2-4 sweeps per launch
+ * and swap once per launch, so this is not steps % 2 there). */
 int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
                             const lgdhip_tile *tile, int *result_in_b,
                             lgdhip_stream_t stream);
@@ -325,33 +326,39 @@ int lgdhip_adam_step(float *w, float *m, float *v, const float *g, int64_t count
                      float beta1, float beta2, float eps, int t, void *w_bf16, lgdhip_stream_t stream);
 
 /* ------------------------------------------------------------------ */
-/* Learned initial guess: Poisson pools <-> U-Net tensors             */
-/*   (no reference counterpart).  f and u are the pool base pointers; */
-/*   prob (host) gives n, the block offset and the faces.  symmetry   */
-/*   in [0,48) reads the problem through a cube symmetry: axis        */
-/*   permutation (symmetry % 6) and reflections (bits of symmetry/6). */
+/* Learned initial guess: Poisson pools <-> coarse grid <-> U-Net      */
+/*   tensors (no reference counterpart).  f and u are the pool base    */
+/*   pointers; prob (host) gives n, the block offset and the faces.    */
+/*   symmetry in [0,48) reads the problem through a cube symmetry:     */
+/*   axis permutation (symmetry % 6) and reflections (bits of          */
+/*   symmetry / 6).  A base field is dense n^3 FP32 (x fastest).       */
 /* ------------------------------------------------------------------ */
-/* The harmonic basis H (dense n^3 FP32, x fastest) is the solution for
- * f = 0, value 1 on the face x = -1 and 0 on the other faces; face j's basis
- * field is H read at (x,y,z), (n-1-x,y,z), (y,x,z), (n-1-y,x,z), (z,y,x),
- * (n-1-z,y,x) for j = 0..5, and h = sum_j face_j H_j solves f = 0 with the
- * problem's faces. */
 /* x[v][0] = bf16(f/s), x[v][1] = bf16(b/s) with b the face field (sum of the
- * Dirichlet values of the faces cell v touches), x[v][2] = bf16(h/s) when
- * harmonic != NULL, remaining channels up to ldx zero; inv_scale = 1/s */
+ * Dirichlet values of the faces cell v touches), x[v][2] = bf16(base/s) when
+ * base != NULL, remaining channels up to ldx zero; inv_scale = 1/s */
 int lgdhip_poisson_unet_input(const float *f, const lgdhip_poisson_problem *prob, double inv_scale, int symmetry,
-                              const float *harmonic, void *x, int ldx, lgdhip_stream_t stream);
-/* t[v] = (float)((u_interior[v] - h[v]) / s), h = 0 when harmonic == NULL:
- * the training target (with harmonic: the source response alone) */
+                              const float *base, void *x, int ldx, lgdhip_stream_t stream);
+/* t[v] = (float)((u_interior[v] - base[v]) / s) (base = 0 when NULL): the
+ * training target */
 int lgdhip_poisson_unet_target(const double *u, const lgdhip_poisson_problem *prob, double inv_scale, int symmetry,
-                               const float *harmonic, float *t, lgdhip_stream_t stream);
-/* u_interior[v] = pred[v] * s + h[v] (h = 0 when harmonic == NULL): the
+                               const float *base, float *t, lgdhip_stream_t stream);
+/* u_interior[v] = pred[v] * s + base[v] (either may be NULL, not both): the
  * initial guess; the halo is left alone */
-int lgdhip_poisson_unet_guess(const float *pred, const lgdhip_poisson_problem *prob, double scale,
-                              const float *harmonic, double *u, lgdhip_stream_t stream);
-/* u_interior = h = sum_j face_j H_j (the boundary part alone, no network) */
-int lgdhip_poisson_face_guess(const float *harmonic, const lgdhip_poisson_problem *prob, double *u,
-                              lgdhip_stream_t stream);
+int lgdhip_poisson_unet_guess(const float *pred, const lgdhip_poisson_problem *prob, double scale, const float *base,
+                              double *u, lgdhip_stream_t stream);
+/* Coarse grid of a problem: m^3 cells of spacing H = (n+1)/(m+1) at fine
+ * positions (j+1)H - 1, its halo on the fine halo planes (same faces); the
+ * coarse equation is 6U - sum U_nb = H^2 f_c. */
+double lgdhip_poisson_coarse_spacing(int n, int m);
+/* fc (coarse pool, block coarse->offset) = H^2 x hat-weighted average of f
+ * around each coarse cell (weights max(0, 1 - |i - X|/H) per axis) */
+int lgdhip_poisson_coarse_restrict(const float *f, const lgdhip_poisson_problem *prob, float *fc,
+                                   const lgdhip_poisson_problem *coarse, lgdhip_stream_t stream);
+/* out (dense n^3 FP32) = trilinear interpolation of the coarse solution uc
+ * (coarse pool, halo = faces; edge/corner halo cells read as the mean of the
+ * faces meeting there) at the fine cells */
+int lgdhip_poisson_coarse_prolong(const double *uc, const lgdhip_poisson_problem *coarse, int n, float *out,
+                                  lgdhip_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,25 +1,32 @@
 """Learned initial guess for the Poisson ensemble.
 
-The problem is linear, so a solution splits into u = h + u_f: h is the
-response to the Dirichlet faces with f = 0 and u_f the response to the source
-with zero faces.  h is exact and cheap: h = sum_j face_j H_j where the six
-basis fields are symmetric images of one field H (value 1 on the face
-x = -1), solved once per cube size by the batched Jacobi solver itself.  u_f
-is what the 3D U-Net (libgeodecomp_amd.unet) learns: it is trained on source
-responses of steady states that the batched Jacobi solver generates, reading
-every problem through one of the 48 symmetries of the cube and a random sign
-(exact augmentations), with bias-free layers (a positively homogeneous
-network) and per-sample normalised targets, so that every sample weighs by
-its relative error.  At inference it proposes u0 = h + s * net(f / s)
-(s = max |f|), which the same solver polishes -- a short weighted-Jacobi
-smoothing phase for the high-frequency part of the network's error, then
-plain Jacobi -- to the same relative residual as the plain solve from the
-zero guess (both relative to the zero guess's residual ||f + b||_2).
+Plain Jacobi spends its sweeps on the smoothest error modes (the iteration
+damps them by cos(pi/(n+1)) per sweep), and those are the global modes a
+4-level U-Net's receptive field does not span at 128^3.  So the guess is
+built in two parts:
+
+  * base: the problem restricted to a coarse grid (m^3, spacing
+    H = (n+1)/(m+1), same faces), solved there by the same batched Jacobi
+    solver (cheap: m^3 cells, ~m^2 sweeps), prolongated trilinearly -- it
+    carries the smooth modes to the coarse grid's discretisation error;
+  * learned correction: the 3D U-Net (libgeodecomp_amd.unet) maps
+    (f, b, base) / s to (u - base) / s, trained on steady states that the
+    batched Jacobi solver generates, every problem read through one of the 48
+    symmetries of the cube and a random sign (exact augmentations), with
+    bias-free layers (a positively homogeneous network) and per-sample
+    normalised targets (every sample weighs by its relative error).
+
+The guess u0 = base + s * net(...) is then polished by the same solver -- a
+short weighted-Jacobi smoothing phase for the high-frequency part of the
+network's error, then plain Jacobi -- to the same relative residual as the
+plain solve from the zero guess (both relative to the zero guess's residual
+||f + b||_2).  evaluate() reports the zero guess, the base alone and the
+learned guess, so the network's own contribution is visible.
 
 No reference counterpart.  Everything on the device is a liblgdhip kernel;
 this module only orchestrates (and times) them.
 
-    python -m libgeodecomp_amd.learned_guess --n 128 --train 256 --eval 16 --steps 3000
+    python -m libgeodecomp_amd.learned_guess --n 128 --train 384 --eval 16 --steps 3000
 """
 from __future__ import annotations
 
@@ -37,6 +44,11 @@ from .unet import UNet3D
 
 def _sync():
     lgdhip.call("lgdhip_device_synchronize")
+
+
+def default_coarse_size(n: int, spacing: float = 4.0) -> int:
+    """Coarse cells per edge for a coarse spacing of about `spacing` fine cells."""
+    return max(1, int(round((n + 1) / spacing)) - 1)
 
 
 class Ensemble:
@@ -67,9 +79,12 @@ class Ensemble:
             host = np.ascontiguousarray(f, dtype=np.float32).ravel()
             assert host.size == block
             dst = C.c_void_p(self.f.value + self.probs[k].offset * 4)
-            lgdhip.call("lgdhip_memcpy_h2d", dst, host.ctypes.data_as(C.c_void_p), host.nbytes, None)
+            lgdhip.call("lgdhip_memcpy_h2d", dst, host.ctypes.data_as(C.c_void_p), host.nbytes, self.stream)
         self.reset()
         self.reference = self.residual_norms()
+
+    def prob(self, k: int):
+        return C.byref(self.probs[k])
 
     def reset(self):
         """u = 0 in every interior, faces in every halo."""
@@ -106,64 +121,83 @@ class Ensemble:
         return host.reshape((n + 2,) * 3)[1:-1, 1:-1, 1:-1].copy()
 
 
-def harmonic_basis(n: int, rel_tol: float = 1e-9, stream=None) -> "lgdhip.DeviceBuffer":
-    """H: the solution for f = 0, value 1 on the face x = -1, 0 on the other
-    faces, by the batched Jacobi solver; dense n^3 FP32 on the device."""
-    f = np.zeros((n + 2,) * 3, np.float32)
-    ens = Ensemble([(n, np.array([1.0, 0, 0, 0, 0, 0]), f)], stream=stream)
-    res = ens.solve(rel_tol)
-    assert res[0].converged == 1, "harmonic basis did not converge"
-    H = np.ascontiguousarray(ens.interior(0).astype(np.float32))
-    buf = lgdhip.DeviceBuffer(H.nbytes)
-    buf.upload(H, stream)
-    return buf
+class CoarseBase:
+    """The base field of every problem of an ensemble: its coarse-grid
+    solution (m^3, same faces) prolongated to the fine cells (dense n^3 FP32
+    per problem)."""
+
+    def __init__(self, data: Ensemble, m: int, rel_tol: float = 1e-6):
+        self.data, self.m, self.rel_tol = data, m, rel_tol
+        n = data.n
+        zeros = np.zeros((m + 2,) * 3, np.float32)
+        faces = [np.array([data.probs[k].face[j] for j in range(6)]) for k in range(data.count)]
+        self.coarse = Ensemble([(m, faces[k], zeros) for k in range(data.count)], stream=data.stream)
+        self.field = lgdhip.DeviceBuffer(data.count * n ** 3 * 4)
+        self.results = None
+
+    def ptr(self, k: int) -> C.c_void_p:
+        return C.c_void_p(self.field.value + k * self.data.n ** 3 * 4)
+
+    def compute(self):
+        """Restrict, solve on the coarse grid, prolongate (all problems)."""
+        d, c = self.data, self.coarse
+        for k in range(d.count):
+            lgdhip.call("lgdhip_poisson_coarse_restrict", d.f.ptr, d.prob(k), c.f.ptr, c.prob(k), d.stream)
+        c.reset()
+        c.reference = c.residual_norms()
+        self.results = c.solve(self.rel_tol, chunk=32)
+        for k in range(d.count):
+            lgdhip.call("lgdhip_poisson_coarse_prolong", c.u.ptr, c.prob(k), d.n, self.ptr(k), d.stream)
 
 
 class LearnedGuess:
-    """U-Net proposing initial guesses for n^3 problems.  With the harmonic
-    basis H (default) it learns the source response (u - h)/s from f/s
-    (s = max |f|); without, u/s from (f/s, b/s) (s = max(|f|, |faces|))."""
+    """U-Net proposing initial guesses for n^3 problems: (u - base)/s from
+    (f, b, base)/s, s = max(|f|, |faces|); without a base, u/s from (f, b)/s."""
 
-    def __init__(self, n: int, channels=(16, 32, 64, 128), seed: int = 0, harmonic=None, stream=None):
+    def __init__(self, n: int, channels=(16, 32, 64, 128), seed: int = 0, use_base: bool = True, stream=None):
         self.n = n
-        self.H = harmonic
+        self.use_base = use_base
         self.stream = stream
-        self.in_ch = 1 if harmonic is not None else 2
-        self.net = UNet3D(n=n, channels=channels, in_ch=self.in_ch, seed=seed, stream=stream, bias=False)
+        self.net = UNet3D(n=n, channels=channels, in_ch=3 if use_base else 2, seed=seed, stream=stream, bias=False)
         self.target = lgdhip.DeviceBuffer(n ** 3 * 4)
         self.rng = np.random.default_rng(seed + 1)
-        self._Hp = harmonic.ptr if harmonic is not None else None
 
-    def scales(self, data: "Ensemble") -> np.ndarray:
-        s = data.fmax if self.H is not None else np.maximum(data.fmax, data.facemax)
-        return np.maximum(s, 1e-30)
+    @staticmethod
+    def scales(data: Ensemble) -> np.ndarray:
+        return np.maximum(np.maximum(data.fmax, data.facemax), 1e-30)
 
-    def _input(self, data: Ensemble, k: int, sym: int, inv: float):
-        lgdhip.call("lgdhip_poisson_unet_input", data.f.ptr, C.byref(data.probs[k]), inv, sym, None,
+    def _base(self, base: CoarseBase | None, k: int):
+        return base.ptr(k) if (self.use_base and base is not None) else None
+
+    def _input(self, data: Ensemble, base, k: int, sym: int, inv: float):
+        lgdhip.call("lgdhip_poisson_unet_input", data.f.ptr, data.prob(k), inv, sym, self._base(base, k),
                     self.net.x.ptr, self.net.x.ld, self.stream)
 
-    def sample_norms(self, data: Ensemble) -> np.ndarray:
+    def _target(self, data: Ensemble, base, k: int, sym: int, inv: float):
+        lgdhip.call("lgdhip_poisson_unet_target", data.u.ptr, data.prob(k), inv, sym, self._base(base, k),
+                    self.target.ptr, self.stream)
+
+    def sample_norms(self, data: Ensemble, base) -> np.ndarray:
         """RMS of every problem's target at unit input scale (data.u must
         hold the solutions)."""
         s = self.scales(data)
         out = np.zeros(data.count)
         for k in range(data.count):
-            lgdhip.call("lgdhip_poisson_unet_target", data.u.ptr, C.byref(data.probs[k]), 1.0 / s[k], 0, self._Hp,
-                        self.target.ptr, self.stream)
+            self._target(data, base, k, 0, 1.0 / s[k])
             t = np.empty(self.n ** 3, np.float32)
             self.target.download(t, self.stream)
             out[k] = float(np.sqrt(np.mean(t.astype(np.float64) ** 2)))
         return out
 
-    def train(self, data: Ensemble, steps: int, lr: float = 1e-3, batch: int = 2, log_every: int = 0,
-              lr_final: float = 0.05, verbose: bool = False) -> list:
+    def train(self, data: Ensemble, base: CoarseBase | None, steps: int, lr: float = 1e-3, batch: int = 2,
+              log_every: int = 0, lr_final: float = 0.05, verbose: bool = False) -> list:
         """Adam on mini-batches of `batch` augmented samples (gradients
         accumulated), cosine learning-rate decay to lr * lr_final.  data.u
         must hold the solutions.  Every sample is scaled to a unit-RMS
         target (exact for the bias-free network).  Returns (step, loss)."""
         assert data.n == self.n
         s = self.scales(data)
-        rms = self.sample_norms(data)
+        rms = self.sample_norms(data, base)
         g = np.where(rms > 0, 1.0 / np.maximum(rms, 1e-300), 0.0)
         log = []
         for step in range(steps):
@@ -172,9 +206,8 @@ class LearnedGuess:
                 sym = int(self.rng.integers(48))
                 sign = 1.0 if self.rng.integers(2) else -1.0
                 inv = sign * g[k] / s[k]
-                self._input(data, k, sym, inv)
-                lgdhip.call("lgdhip_poisson_unet_target", data.u.ptr, C.byref(data.probs[k]), inv, sym, self._Hp,
-                            self.target.ptr, self.stream)
+                self._input(data, base, k, sym, inv)
+                self._target(data, base, k, sym, inv)
                 self.net.forward()
                 read = log_every and (step % log_every == 0 or step == steps - 1) and b == 0
                 loss = self.net.loss(self.target, read=bool(read))
@@ -188,16 +221,15 @@ class LearnedGuess:
             self.net.adam(lr=lr_t)
         return log
 
-    def propose(self, data: Ensemble, k: int, s: float):
-        """Writes the network's guess for problem k into data.u."""
-        self._input(data, k, 0, 1.0 / s)
+    def propose(self, data: Ensemble, base: CoarseBase | None, k: int, s: float):
+        """Writes the guess for problem k into data.u."""
+        self._input(data, base, k, 0, 1.0 / s)
         pred = self.net.forward()
-        lgdhip.call("lgdhip_poisson_unet_guess", pred.ptr, C.byref(data.probs[k]), s, self._Hp, data.u.ptr,
+        lgdhip.call("lgdhip_poisson_unet_guess", pred.ptr, data.prob(k), s, self._base(base, k), data.u.ptr,
                     self.stream)
 
 
-def _arm(data: Ensemble, name: str, exact: list, results, t_guess: float, t_solve: float, r_guess=None,
-         guess=None) -> dict:
+def _arm(name: str, exact: list, results, t_guess: float, t_solve: float, r_guess=None, guess=None) -> dict:
     sw = np.array([r.sweeps for r in results], dtype=np.float64)
     out = {"arm": name, "converged": int(sum(r.converged for r in results)), "sweeps_mean": float(sw.mean()),
            "sweeps_total": float(sw.sum()), "wall_guess_s": t_guess, "wall_solve_s": t_solve,
@@ -211,15 +243,14 @@ def _arm(data: Ensemble, name: str, exact: list, results, t_guess: float, t_solv
     return out
 
 
-def evaluate(model: LearnedGuess, data: Ensemble, rel_tol: float = 1e-6, chunk: int = 64,
+def evaluate(model: LearnedGuess, data: Ensemble, coarse_m: int = 0, rel_tol: float = 1e-6, chunk: int = 64,
              smooth_sweeps: int = 128) -> dict:
-    """Zero guess (plain Jacobi), and with the harmonic basis the boundary
-    response h alone and the learned guess h + s net(f/s), on the same
-    problems with the same solver, tolerance and reference norms (the zero
-    guess's residual).  Guessed arms start with `smooth_sweeps` weighted
-    (2/3) Jacobi sweeps, counted in their sweeps.  Wall times include
-    computing the guess; the harmonic basis is per cube size, computed once
-    and reported separately."""
+    """Zero guess (plain Jacobi); with a coarse grid, the prolongated coarse
+    solution alone; and the learned guess -- on the same problems with the
+    same solver, tolerance and reference norms (the zero guess's residual).
+    Guessed arms start with `smooth_sweeps` weighted (2/3) Jacobi sweeps,
+    counted in their sweeps; their wall times include building the guess
+    (restriction, coarse solve, prolongation, network)."""
     data.reset()
     _sync()
     t0 = time.perf_counter()
@@ -227,7 +258,9 @@ def evaluate(model: LearnedGuess, data: Ensemble, rel_tol: float = 1e-6, chunk: 
     _sync()
     t_zero = time.perf_counter() - t0
     exact = [data.interior(k) for k in range(data.count)]
-    arms = [_arm(data, "zero", exact, res0, 0.0, t_zero)]
+    arms = [_arm("zero", exact, res0, 0.0, t_zero)]
+    base = CoarseBase(data, coarse_m) if coarse_m > 0 else None
+    s = model.scales(data)
 
     def guessed(name, make):
         data.reset()
@@ -241,15 +274,25 @@ def evaluate(model: LearnedGuess, data: Ensemble, rel_tol: float = 1e-6, chunk: 
         t0 = time.perf_counter()
         res = data.solve(rel_tol, chunk, smooth_sweeps=smooth_sweeps)
         _sync()
-        arms.append(_arm(data, name, exact, res, t_g, time.perf_counter() - t0, r_guess, guess))
+        arms.append(_arm(name, exact, res, t_g, time.perf_counter() - t0, r_guess, guess))
 
-    if model.H is not None:
-        guessed("harmonic", lambda: [lgdhip.call("lgdhip_poisson_face_guess", model.H.ptr, C.byref(data.probs[k]),
-                                                 data.u.ptr, data.stream) for k in range(data.count)])
-    s = model.scales(data)
-    guessed("learned", lambda: [model.propose(data, k, s[k]) for k in range(data.count)])
+    if base is not None:
+        def coarse_only():
+            base.compute()
+            for k in range(data.count):
+                lgdhip.call("lgdhip_poisson_unet_guess", None, data.prob(k), 1.0, base.ptr(k), data.u.ptr,
+                            data.stream)
+        guessed("coarse", coarse_only)
+
+    def learned():
+        if base is not None:
+            base.compute()
+        for k in range(data.count):
+            model.propose(data, base, k, s[k])
+    guessed("learned", learned)
     best = arms[-1]
-    return {"problems": data.count, "n": data.n, "rel_tol": rel_tol, "smooth_sweeps": smooth_sweeps, "arms": arms,
+    return {"problems": data.count, "n": data.n, "coarse_m": coarse_m, "rel_tol": rel_tol,
+            "smooth_sweeps": smooth_sweeps, "arms": arms,
             "sweep_ratio": arms[0]["sweeps_total"] / max(best["sweeps_total"], 1.0),
             "wall_speedup": arms[0]["wall_s"] / best["wall_s"]}
 
@@ -257,54 +300,52 @@ def evaluate(model: LearnedGuess, data: Ensemble, rel_tol: float = 1e-6, chunk: 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--train", type=int, default=256, help="solved training problems")
+    ap.add_argument("--train", type=int, default=384, help="solved training problems")
     ap.add_argument("--eval", type=int, default=16, help="held-out problems")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--coarse", type=int, default=-1, help="coarse cells per edge (-1: spacing ~4, 0: no base)")
     ap.add_argument("--smooth", type=int, default=128, help="weighted-Jacobi sweeps before polishing a guess")
-    ap.add_argument("--no-harmonic", action="store_true", help="learn u from (f, b) without the harmonic split")
     ap.add_argument("--seed", type=int, default=synthetic.DEFAULT_SEED)
     ap.add_argument("--log-every", type=int, default=250)
     a = ap.parse_args(argv)
     lgdhip.load()
     lgdhip.call("lgdhip_set_device", 0)
     stream = lgdhip.Stream()
-    out = {"n": a.n, "train_problems": a.train, "eval_problems": a.eval, "steps": a.steps, "batch": a.batch,
-           "harmonic": not a.no_harmonic}
-
-    H = None
-    if not a.no_harmonic:
-        t0 = time.perf_counter()
-        H = harmonic_basis(a.n, stream=stream.h)
-        _sync()
-        out["harmonic_basis_s"] = time.perf_counter() - t0
+    m = default_coarse_size(a.n) if a.coarse < 0 else a.coarse
+    out = {"n": a.n, "coarse_m": m, "train_problems": a.train, "eval_problems": a.eval, "steps": a.steps,
+           "batch": a.batch}
 
     t0 = time.perf_counter()
     train = Ensemble(synthetic.poisson_ensemble(a.train, sizes=(a.n,), seed=a.seed), stream=stream.h)
     res = train.solve(a.tol)
+    base = None
+    if m > 0:
+        base = CoarseBase(train, m)
+        base.compute()
     _sync()
     out["generate_s"] = time.perf_counter() - t0
     out["generate_sweeps_mean"] = float(np.mean([r.sweeps for r in res]))
     out["generate_converged"] = int(sum(r.converged for r in res))
-    print(f"[learned] generated {a.train} solutions of {a.n}^3 in {out['generate_s']:.1f} s "
+    print(f"[learned] generated {a.train} solutions of {a.n}^3 (+ {m}^3 coarse) in {out['generate_s']:.1f} s "
           f"({out['generate_sweeps_mean']:.0f} sweeps each)", flush=True)
 
-    model = LearnedGuess(a.n, seed=a.seed % 1000, harmonic=H, stream=stream.h)
+    model = LearnedGuess(a.n, seed=a.seed % 1000, use_base=m > 0, stream=stream.h)
     t0 = time.perf_counter()
-    log = model.train(train, a.steps, lr=a.lr, batch=a.batch, log_every=a.log_every, verbose=True)
+    log = model.train(train, base, a.steps, lr=a.lr, batch=a.batch, log_every=a.log_every, verbose=True)
     _sync()
     out["train_s"] = time.perf_counter() - t0
     out["loss_log"] = log
     print(f"[learned] trained {a.steps} steps in {out['train_s']:.1f} s, loss {log[0][1]:.3e} -> {log[-1][1]:.3e}",
           flush=True)
-    del train
+    del base, train
 
     ev = Ensemble(synthetic.poisson_ensemble(a.eval, sizes=(a.n,), seed=a.seed + 7919), stream=stream.h)
-    out["eval"] = evaluate(model, ev, a.tol, smooth_sweeps=a.smooth)
+    out["eval"] = evaluate(model, ev, m, a.tol, smooth_sweeps=a.smooth)
     for arm in out["eval"]["arms"]:
-        print(f"[learned] {arm['arm']:9s} sweeps {arm['sweeps_mean']:9.0f}  wall {arm['wall_s']:.3f} s  "
+        print(f"[learned] {arm['arm']:8s} sweeps {arm['sweeps_mean']:9.0f}  wall {arm['wall_s']:.3f} s  "
               f"guess err {arm.get('guess_rel_error_median', 1.0):.2e}", flush=True)
     print(json.dumps(out), flush=True)
     return out

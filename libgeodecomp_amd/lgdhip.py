@@ -147,9 +147,12 @@ SIGNATURES = {
     "lgdhip_adam_step": (_I, [_P, _P, _P, _P, _I64, _I, _I, C.c_float, C.c_float, C.c_float, C.c_float, _I,
                               _P, _P]),
     "lgdhip_poisson_unet_input": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _I, _P, _P, _I, _P]),
-    "lgdhip_poisson_face_guess": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _P, _P]),
     "lgdhip_poisson_unet_target": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _I, _P, _P, _P]),
     "lgdhip_poisson_unet_guess": (_I, [_P, C.POINTER(lgdhip_poisson_problem), C.c_double, _P, _P, _P]),
+    "lgdhip_poisson_coarse_spacing": (C.c_double, [_I, _I]),
+    "lgdhip_poisson_coarse_restrict": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _P,
+                                            C.POINTER(lgdhip_poisson_problem), _P]),
+    "lgdhip_poisson_coarse_prolong": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _I, _P, _P]),
 }
 
 # lgdhip_conv3d_kpad returns a value, not a status
