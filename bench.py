@@ -162,9 +162,9 @@ class Jacobi:
     dtype = "f64"
     bytes_per_lup = 16.0
 
-    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, variant=0, by=0, bz=0, **_):
+    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, variant=0, by=0, bz=0, flags=0, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
-        self.tile = lgdhip.tile(by=by, bz=bz, variant=variant)
+        self.tile = lgdhip.tile(by=by, bz=bz, variant=variant, flags=flags)
         self.stream = lgdhip.Stream()
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=8)
         host = synthetic.jacobi_initial(n, n, n, seed)
@@ -190,8 +190,9 @@ class Jacobi:
         lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps,
                     C.byref(self.tile), None, s)
 
-    # sweeps per launch of the temporal-blocking variants
-    TB_DEPTH = {4: 2, 5: 3, 6: 4, 7: 2, 8: 3}
+    # sweeps per launch of the temporal-blocking variants; variant 0 is the
+    # library default, 6 (4 sweeps per launch)
+    TB_DEPTH = {0: 4, 4: 2, 5: 3, 6: 4, 7: 2, 8: 3}
 
     def dominant_kernel(self):
         k = self.TB_DEPTH.get(self.tile.variant, 1)
@@ -229,8 +230,9 @@ class LBM:
     dtype = "f32"
     bytes_per_lup = 153.0  # 19 x 4 B read + 19 x 4 B write + 1 B wall flags
 
-    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, **_):
+    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
+        self.tile = lgdhip.tile(by=by, flags=flags, variant=variant)
         self.stream = lgdhip.Stream()
         s = self.stream.h
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19)
@@ -264,13 +266,13 @@ class LBM:
         s = self.stream.h
         lgdhip.call("lgdhip_memcpy_d2d", self.a.ptr, self.pristine.ptr, self.nbytes, s)
         lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr, self.flags.ptr, C.byref(self.g),
-                    C.byref(self.p), self.sweeps, None, None, s)
+                    C.byref(self.p), self.sweeps, C.byref(self.tile), None, s)
 
     def dominant_kernel(self):
         launches = 20
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
                                                     self.flags.ptr, C.byref(self.g), C.byref(self.p), launches,
-                                                    None, None, self.stream.h))
+                                                    C.byref(self.tile), None, self.stream.h))
         return "lbm_d3q19_pull", ms / launches, self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
@@ -454,6 +456,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant id (-1: tuned default, 0: library default)")
     ap.add_argument("--by", type=int, default=0, help="jacobi: rows per work-group (0: default)")
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
+    ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
+    ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -476,8 +480,12 @@ def main():
         kw["generations" if args.workload == "life" else "steps"] = args.iters
     if args.workload == "life":
         kw.update(per_launch=args.per_launch, variant=7 if args.variant < 0 else args.variant)
+    flags = (lgdhip.TILE_NO_XCD_SWIZZLE if args.no_xcd_swizzle else 0) | \
+        (lgdhip.TILE_XCD_SWIZZLE if args.xcd_swizzle else 0)
     if args.workload == "jacobi":
-        kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz)
+        kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
+    if args.workload == "lbm":
+        kw.update(by=args.by, flags=flags, variant=max(0, args.variant))
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):

@@ -71,8 +71,14 @@ typedef struct lgdhip_tile {
     int32_t bx, by, bz;        /* work-group shape (threads)           */
     int32_t steps_per_launch;  /* temporal blocking depth              */
     int32_t variant;           /* kernel variant id (family specific)  */
-    int32_t reserved;
+    int32_t flags;             /* LGDHIP_TILE_* bits                   */
 } lgdhip_tile;
+/* Work-group -> tile order: XCD_SWIZZLE gives each of the 8 XCDs a
+ * contiguous range of tiles (neighbouring tiles share one L2),
+ * NO_XCD_SWIZZLE maps them in launch order; neither bit: the family's
+ * default (Jacobi temporal blocking: swizzled, LBM: launch order). */
+#define LGDHIP_TILE_XCD_SWIZZLE    0x1
+#define LGDHIP_TILE_NO_XCD_SWIZZLE 0x2
 
 /* Parameters of the D3Q19 lid-driven cavity kernel family. */
 typedef struct lgdhip_lbm_params {
@@ -192,7 +198,8 @@ int lgdhip_life2d_run(uint8_t *grid, const lgdhip_grid *g, int edge_alive, int s
  * C N E W S T B NW SW NE SE TW BW TE BE TN BN TS BS.  `flags` is a uint8
  * grid made with lgdhip_grid_make(3, nx, ny, nz, halo, 1, 1) holding the
  * LGDHIP_LBM_* wall bits of every cell.  Ping-pongs a/b like the other
- * families. */
+ * families.  tile (optional): by = rows per 64-wide work-group (1, 2, 4, 8),
+ * variant 1 = nontemporal population loads, flags (LGDHIP_TILE_*). */
 #define LGDHIP_LBM_WALL_XLO 0x01
 #define LGDHIP_LBM_WALL_XHI 0x02
 #define LGDHIP_LBM_WALL_YLO 0x04

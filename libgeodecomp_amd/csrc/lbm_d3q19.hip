@@ -22,6 +22,8 @@
 // FP32 mass drift of the unshifted form (about one ulp of rho per step).
 #include "lgdhip_internal.h"
 
+#include <type_traits>
+
 namespace {
 
 // e_i, opposite direction, weight class (0: rest, 1: axis, 2: diagonal)
@@ -64,15 +66,22 @@ __device__ __forceinline__ bool blocked(const Dir &d, uint32_t fl) {
 
 // IDX is int32_t when every element offset of the 19 planes fits 31 bits
 // (then addresses are a uniform base plus a 32-bit lane offset), else int64_t.
-template <int BY, typename IDX>
+// A work-group is a 64 x BY tile of one z plane; tiles are numbered x
+// fastest, then y, then z, and work-groups map to them through the XCD
+// swizzle (each XCD a contiguous range), so the cache lines that x-shifted
+// pulls straddle between neighbouring tiles are fetched into one L2.
+template <int BY, typename IDX, bool NTL>
 __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restrict__ src, float *__restrict__ dst,
                                                           const uint8_t *__restrict__ flags, int nx, int ny,
                                                           int nz, IDX sx, IDX sxy, IDX plane, IDX origin,
                                                           IDX fsx, IDX fsxy, IDX forigin, float omega, float lux,
-                                                          float luy, float luz) {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int y = blockIdx.y * BY + threadIdx.y;
-    const int z = blockIdx.z;
+                                                          float luy, float luz, unsigned gx, unsigned gy,
+                                                          unsigned ntiles, unsigned per, int swizzle) {
+    const unsigned t = lgdhip::xcd_tile(blockIdx.x, per, swizzle != 0);
+    if (t >= ntiles) return;
+    const int x = (int)(t % gx) * 64 + threadIdx.x;
+    const int y = (int)((t / gx) % gy) * BY + threadIdx.y;
+    const int z = (int)(t / (gx * gy));
     if (x >= nx || y >= ny || z >= nz) return;
     const IDX idx = origin + (IDX)z * sxy + (IDX)y * sx + x;
     const uint32_t fl = flags[forigin + (IDX)z * fsxy + (IDX)y * fsx + x];
@@ -89,7 +98,7 @@ __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restric
     }
     float f[19];
 #pragma unroll
-    for (int i = 0; i < 19; ++i) f[i] = src[at[i]];
+    for (int i = 0; i < 19; ++i) f[i] = NTL ? __builtin_nontemporal_load(src + at[i]) : src[at[i]];
     // moving lid: populations reflected from the +y face gain 6 w_i e_i.u
     const bool lid = (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI);
 #pragma unroll
@@ -177,13 +186,17 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     LGD_REQUIRE(a && b && a != b && flags && p, "NULL or aliased argument");
     LGD_REQUIRE(steps >= 0, "steps must be >= 0");
     LGD_REQUIRE(g->nz <= 65535, "nz must be <= 65535");
-    (void)tile;
     lgdhip_grid fg;
     rc = flags_grid(*g, &fg);
     if (rc) return rc;
-    constexpr int BY = 4;
-    dim3 grid(lgdhip::ceil_div(g->nx, 64), lgdhip::ceil_div(g->ny, BY), (unsigned)g->nz);
-    dim3 block(64, BY);
+    const int BY = (tile && (tile->by == 1 || tile->by == 2 || tile->by == 8)) ? tile->by : 4;
+    const int swizzle = lgdhip::xcd_swizzle(tile, false);
+    const bool ntl = tile && tile->variant == 1;
+    const unsigned gx = lgdhip::ceil_div(g->nx, 64), gy = lgdhip::ceil_div(g->ny, BY);
+    const unsigned ntiles = gx * gy * (unsigned)g->nz;
+    const unsigned per = lgdhip::xcd_per(ntiles);
+    const dim3 grid(swizzle ? lgdhip::xcd_groups(ntiles) : ntiles);
+    const dim3 block(64, BY);
     const int64_t sxy = g->sx * g->sy;
     const int64_t plane = lgdhip::plane_elems(*g);
     const int64_t origin = lgdhip::origin_index(*g);
@@ -192,16 +205,33 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     const bool small_index = 19 * plane < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
     hipStream_t s = lgdhip::as_stream(stream);
     float *cur = a, *nxt = b;
-    for (int t = 0; t < steps; ++t) {
+    auto launch_nt = [&](auto by_tag, auto nt_tag, const float *src, float *dst) {
+        constexpr int B = decltype(by_tag)::value;
+        constexpr bool NT = decltype(nt_tag)::value;
         if (small_index) {
-            lbm_d3q19_pull<BY, int32_t><<<grid, block, 0, s>>>(
-                cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz, (int32_t)g->sx, (int32_t)sxy, (int32_t)plane,
+            lbm_d3q19_pull<B, int32_t, NT><<<grid, block, 0, s>>>(
+                src, dst, flags, (int)g->nx, (int)g->ny, (int)g->nz, (int32_t)g->sx, (int32_t)sxy, (int32_t)plane,
                 (int32_t)origin, (int32_t)fg.sx, (int32_t)(fg.sx * fg.sy), (int32_t)forigin, p->omega, p->lid_ux,
-                p->lid_uy, p->lid_uz);
+                p->lid_uy, p->lid_uz, gx, gy, ntiles, per, swizzle);
         } else {
-            lbm_d3q19_pull<BY, int64_t><<<grid, block, 0, s>>>(cur, nxt, flags, (int)g->nx, (int)g->ny, (int)g->nz,
-                                                               g->sx, sxy, plane, origin, fg.sx, fg.sx * fg.sy,
-                                                               forigin, p->omega, p->lid_ux, p->lid_uy, p->lid_uz);
+            lbm_d3q19_pull<B, int64_t, NT><<<grid, block, 0, s>>>(src, dst, flags, (int)g->nx, (int)g->ny, (int)g->nz,
+                                                              g->sx, sxy, plane, origin, fg.sx, fg.sx * fg.sy,
+                                                              forigin, p->omega, p->lid_ux, p->lid_uy, p->lid_uz, gx,
+                                                              gy, ntiles, per, swizzle);
+        }
+    };
+    auto launch = [&](auto by_tag, const float *src, float *dst) {
+        if (ntl)
+            launch_nt(by_tag, std::true_type{}, src, dst);
+        else
+            launch_nt(by_tag, std::false_type{}, src, dst);
+    };
+    for (int t = 0; t < steps; ++t) {
+        switch (BY) {
+        case 1: launch(std::integral_constant<int, 1>{}, cur, nxt); break;
+        case 2: launch(std::integral_constant<int, 2>{}, cur, nxt); break;
+        case 8: launch(std::integral_constant<int, 8>{}, cur, nxt); break;
+        default: launch(std::integral_constant<int, 4>{}, cur, nxt); break;
         }
         LGD_LAUNCHED("lbm_d3q19_pull");
         float *tmp = cur;

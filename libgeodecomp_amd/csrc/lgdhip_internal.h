@@ -48,4 +48,23 @@ inline int64_t origin_index(const lgdhip_grid &g) {
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+// XCD-aware work distribution.  The dispatcher places work-group w on XCD
+// w % 8 (8 XCDs, each with its own L2); tiles that share cache lines
+// (overlap rings, rows straddled by shifted reads) should run on the same
+// XCD at about the same time.  A launch of xcd_groups(T) work-groups gives
+// XCD c the contiguous tile range [c*per, (c+1)*per), per = ceil(T/8), in
+// order; ids mapped past T exit (before any barrier).
+constexpr unsigned kXcds = 8;
+inline unsigned xcd_per(unsigned tiles) { return (tiles + kXcds - 1) / kXcds; }
+inline unsigned xcd_groups(unsigned tiles) { return kXcds * xcd_per(tiles); }
+__device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned per, bool swizzle) {
+    return swizzle ? (w % kXcds) * per + w / kXcds : w;
+}
+// the swizzle choice of a tile's flags, else the family default
+inline int xcd_swizzle(const lgdhip_tile *tile, bool family_default) {
+    if (tile && (tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) return 0;
+    if (tile && (tile->flags & LGDHIP_TILE_XCD_SWIZZLE)) return 1;
+    return family_default ? 1 : 0;
+}
+
 } // namespace lgdhip

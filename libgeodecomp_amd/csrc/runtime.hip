@@ -332,14 +332,14 @@ int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz
     g->nx = nx;
     g->ny = ny;
     g->nz = nz;
-    // the first logical cell of every row starts on a 64-byte boundary and
+    // the first logical cell of every row starts on a 128-byte boundary and
     // the row pitch is a multiple of 128 bytes (one gfx950 cache line), so
-    // wave-wide row accesses are aligned; at least 64 bytes of padding
+    // wave-wide row accesses cover whole lines; at least 64 bytes of padding
     // follow the last cell so vector kernels may read whole groups past nx
     const int64_t per64 = 64 / elem_bytes;
     const int64_t per128 = 128 / elem_bytes;
     const int64_t right = halo > per64 ? halo : per64;
-    g->ox = ((halo + per64 - 1) / per64) * per64;
+    g->ox = ((halo + per128 - 1) / per128) * per128;
     g->sx = ((g->ox + nx + right + per128 - 1) / per128) * per128;
     g->oy = halo;
     g->sy = ny + 2 * halo;

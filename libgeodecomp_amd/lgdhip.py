@@ -38,7 +38,7 @@ class lgdhip_grid(C.Structure):
 class lgdhip_tile(C.Structure):
     _fields_ = [
         ("bx", C.c_int32), ("by", C.c_int32), ("bz", C.c_int32),
-        ("steps_per_launch", C.c_int32), ("variant", C.c_int32), ("reserved", C.c_int32),
+        ("steps_per_launch", C.c_int32), ("variant", C.c_int32), ("flags", C.c_int32),
     ]
 
 
@@ -338,5 +338,9 @@ class Event:
             pass
 
 
-def tile(bx=0, by=0, bz=0, steps_per_launch=0, variant=0) -> lgdhip_tile:
-    return lgdhip_tile(bx, by, bz, steps_per_launch, variant, 0)
+TILE_XCD_SWIZZLE = 0x1
+TILE_NO_XCD_SWIZZLE = 0x2
+
+
+def tile(bx=0, by=0, bz=0, steps_per_launch=0, variant=0, flags=0) -> lgdhip_tile:
+    return lgdhip_tile(bx, by, bz, steps_per_launch, variant, flags)

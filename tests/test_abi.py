@@ -34,7 +34,7 @@ def test_binding_covers_every_declared_symbol():
 def test_grid_descriptor_without_gpu():
     # pure host logic of the C-ABI: descriptor layout and error reporting
     g = lgdhip.make_grid(3, 130, 7, 5, halo=1, elem_bytes=8)
-    assert (g.ox * 8) % 64 == 0 and (g.sx * 8) % 128 == 0
+    assert (g.ox * 8) % 128 == 0 and (g.sx * 8) % 128 == 0
     assert g.sx >= g.ox + g.nx + 1 and g.sy == 9 and g.sz == 7
     assert lgdhip.grid_bytes(g) == g.sx * g.sy * g.sz * 8
     g2 = lgdhip.make_grid(2, 33, 5, 1, halo=1, elem_bytes=1)

@@ -76,7 +76,7 @@ def restrict_np(f, m):
     """f [n,n,n] interior -> f_c [m,m,m] = H^2 x hat-weighted average."""
     n = f.shape[0]
     W, H = hat_weights(n, m)
-    num = np.einsum("Zz,Yy,Xx,zyx->ZYX", W, W, W, f.astype(np.float64))
+    num = np.einsum("Zz,Yy,Xx,zyx->ZYX", W, W, W, f.astype(np.float64), optimize=True)
     den = np.einsum("Z,Y,X->ZYX", W.sum(1), W.sum(1), W.sum(1))
     return H * H * num / den
 
@@ -112,7 +112,7 @@ def prolong_np(P, n):
         t = xi - k
         I[x, k] += 1 - t
         I[x, k + 1] += t
-    return np.einsum("zc,yb,xa,cba->zyx", I, I, I, P)
+    return np.einsum("zc,yb,xa,cba->zyx", I, I, I, P, optimize=True)
 
 
 def coarse_solve_np(fc, faces, H):
