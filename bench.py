@@ -52,16 +52,19 @@ def timed(stream, fn, reps=1):
     return e0.elapsed_ms(e1)
 
 
-def load_traffic(name: str):
-    """HBM bytes per launch of the dominant kernel from the committed rocprof
-    PMC summary (profiles/<name>.json), or None."""
-    p = REPO / "profiles" / f"{name}.json"
+def load_traffic(key: str):
+    """HBM bytes per launch of a dominant kernel on one workload shape
+    ("kernel@shape") from the committed rocprof PMC summaries
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE scaled by the copy
+    calibration), or None when that shape was not profiled."""
+    p = REPO / "profiles" / "traffic.json"
     if not p.exists():
         return None
     try:
-        return json.loads(p.read_text()).get("hbm_bytes_per_launch")
-    except Exception:
+        entry = json.loads(p.read_text()).get(key)
+    except ValueError:
         return None
+    return entry.get("hbm_bytes_per_launch") if entry else None
 
 
 def cpu_threads() -> int:
@@ -133,6 +136,7 @@ class Life:
         per_launch_bytes = self.bytes_per_lup * self.n * self.n * self.per_launch
         # variant 0 is the library default (a stacked-wave kernel)
         name = "life_packed_tb" if 1 <= self.tile.variant <= 4 else "life_packed_stacked"
+        self.shape = f"{self.n}x{self.n},{self.per_launch}gen"
         return name, ms / launches, per_launch_bytes
 
     def cpu_baseline(self, budget_s=15.0):
@@ -205,6 +209,7 @@ class Jacobi:
         else:
             name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
         # one launch advances k sweeps: algorithmic bytes are k sweeps' worth
+        self.shape = f"{self.n}^3,K{k}" if k > 1 else f"{self.n}^3"
         return name, ms / (sweeps // k), k * self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
@@ -273,6 +278,7 @@ class LBM:
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
                                                     self.flags.ptr, C.byref(self.g), C.byref(self.p), launches,
                                                     C.byref(self.tile), None, self.stream.h))
+        self.shape = f"{self.n}^3"
         return "lbm_d3q19_pull", ms / launches, self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
@@ -388,6 +394,7 @@ class Poisson:
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, self.f.ptr,
                                                     self.probs, self.count, sweeps, self.ws.ptr, self.ws_bytes,
                                                     C.byref(in_b), self.stream.h))
+        self.shape = f"{self.count}problems"
         return "poisson_sweep_f32", ms / sweeps, self.bytes_per_lup * self.cells
 
     def cpu_baseline(self, budget_s=15.0):
@@ -435,6 +442,7 @@ class Copy:
     def dominant_kernel(self):
         launches = 10
         ms = timed(self.stream, self.step, reps=launches)
+        self.shape = f"{self.n}B"
         return "lgd_copy16_kernel", ms / launches, 2.0 * self.n
 
     def cpu_baseline(self, budget_s=0.0):
@@ -506,7 +514,8 @@ def main():
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
     log(f"dominant kernel {kname}: {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
     roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": load_traffic(kname),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic(f"{kname}@{getattr(wl, 'shape', '')}"),
                 "avg_launch_us": round(k_ms * 1e3, 3), "algorithmic_bytes_per_launch": k_bytes}
 
     cpu = None

@@ -34,8 +34,9 @@ def gpu_jacobi(hip, host, steps, edge=0.0, tile=None):
     hip.call("lgdhip_jacobi3d_f64_run", a.ptr, b.ptr, C.byref(g), steps, t, C.byref(in_b), None)
     hip.call("lgdhip_device_synchronize")
     # one launch swaps the buffers once: K sweeps per launch for the
-    # temporal-blocking variants, so the result's buffer follows the launches
-    if tile is None or tile.variant < 4:
+    # temporal-blocking variants (and the default, which blocks 4 sweeps),
+    # so the result's buffer follows the launches
+    if tile is not None and 1 <= tile.variant <= 3:
         assert in_b.value == steps % 2
     assert in_b.value in (0, 1)
     return (b if in_b.value else a).store()
