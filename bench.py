@@ -7,10 +7,12 @@ constant dead edge (Cube topology), 1000 generations.  One benchmark step is
 one whole run: restore the pristine uint8 input (device copy), pack, 1000
 generations, unpack -- uint8 grid in HBM in, uint8 grid in HBM out.
 
-Other workloads (`--workload jacobi|lbm|poisson`) measure the Jacobi3D FP64
-and D3Q19 FP32 families on their configured shapes and the batched Poisson
-ensemble (one step = the whole ensemble solved to 1e-6; run it with
-`--steps 1 --warmup 0`).
+Other workloads (`--workload jacobi|lbm|poisson|learned`) measure the
+Jacobi3D FP64 and D3Q19 FP32 families on their configured shapes, the
+batched Poisson ensemble (one step = the whole ensemble solved to 1e-6) and
+the learned initial guess (set-up = data generation and training; one step =
+held-out problems solved from the zero and from the learned guess; value =
+the wall-time ratio).  Run those two with `--steps 1 --warmup 0`.
 
 Prints ONE JSON line (see the keys below).  GPU time comes from HIP events
 recorded on the stream the kernels run on; the dominant kernel's average
@@ -329,8 +331,9 @@ class Poisson:
     dtype = "f32"   # the sweeps; residuals and updates are FP64
     bytes_per_lup = 12.0  # read e + read r + write e (FP32)
 
-    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, **_):
+    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, use_graph=1, **_):
         self.count, self.seed = n, seed   # --size overrides the problem count
+        self.use_graph = use_graph
         self.stream = lgdhip.Stream()
         s = self.stream.h
         t0 = time.perf_counter()
@@ -359,29 +362,48 @@ class Poisson:
         self.last = None
 
     def config(self):
-        sw = [r.sweeps for r in self.last] if self.last is not None else []
+        sw = getattr(self, "total_sweeps", None)
         return {"workload": f"batched Poisson ensemble, {self.count} problems n in {{32..192 step 16}}, 1-8 Gaussian "
                             f"sources, Dirichlet faces U[-1,1], zero guess to 1e-6 relative FP64 residual",
                 "problems": self.count, "cells": int(self.cells), "seed": self.seed,
-                "sweeps_max": int(max(sw)) if sw else None,
+                "sweeps_max": int(sw.max()) if sw is not None else None,
+                "sweep_lups": self.last_lups,
                 "converged": int(sum(r.converged for r in self.last)) if self.last is not None else None,
-                "parallelism": "single GPU"}
+                "hip_graph": bool(self.use_graph), "parallelism": "single GPU"}
 
     def lups_per_step(self):
         return self.last_lups
 
+    SWEEPS_PER_CALL = 10000   # bounded solver calls, so progress is reported between them
+
     def step(self):
+        """Zero guess -> 1e-6 for the whole ensemble.  The solve is issued as
+        calls of at most SWEEPS_PER_CALL sweeps per problem, each continuing
+        from the FP64 state the previous one left, all against the zero
+        guess's residual norms (the same iteration as one unbounded call,
+        with one extra FP64 residual per call)."""
         s = self.stream.h
         lgdhip.call("lgdhip_fill_u8", self.u.ptr, 0, self.elems * 8, s)
         lgdhip.call("lgdhip_poisson_set_faces", self.u.ptr, self.probs, self.count, self.ws.ptr, self.ws_bytes, s)
+        ref = (C.c_double * self.count)()
+        lgdhip.call("lgdhip_poisson_residual_norms", self.u.ptr, self.f.ptr, self.probs, self.count, self.ws.ptr,
+                    self.ws_bytes, ref, s)
         opt = lgdhip.lgdhip_poisson_options()
-        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = 1e-6, 64, 10_000_000, 1
+        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = 1e-6, 64, self.SWEEPS_PER_CALL, self.use_graph
+        opt.reference_norms = ref
+        total = np.zeros(self.count, dtype=np.int64)
         res = (lgdhip.lgdhip_poisson_result * self.count)()
-        lgdhip.call("lgdhip_poisson_batch_solve", self.u.ptr, self.f.ptr, self.probs, self.count, C.byref(opt),
-                    self.ws.ptr, self.ws_bytes, res, s)
+        for call in range(1000):
+            lgdhip.call("lgdhip_poisson_batch_solve", self.u.ptr, self.f.ptr, self.probs, self.count, C.byref(opt),
+                        self.ws.ptr, self.ws_bytes, res, s)
+            total += np.array([r.sweeps for r in res], dtype=np.int64)
+            done = sum(r.converged for r in res)
+            log(f"solver call {call}: {done}/{self.count} converged, max sweeps so far {total.max()}")
+            if done == self.count:
+                break
         self.last = res
-        self.last_lups = float(sum(float(self.sizes[k]) ** 3 * res[k].sweeps for k in range(self.count)))
-        log(f"solved: max sweeps {max(r.sweeps for r in res)}, converged {sum(r.converged for r in res)}")
+        self.total_sweeps = total
+        self.last_lups = float(np.sum(self.sizes.astype(np.float64) ** 3 * total))
 
     def dominant_kernel(self):
         """poisson_sweep_f32 over every problem (the hot loop of the solve)."""
@@ -413,6 +435,101 @@ class Poisson:
         return {"value": 1e-9 * m ** 3 * sweeps / dt, "unit": "GLUPS", "cores": 1, "kind": "port",
                 "sample": f"C oracle plain FP64 Jacobi (1 thread), one {m}^3 problem of the ensemble, {sweeps} "
                           f"sweeps, {dt:.1f}s"}
+
+
+# ----------------------------------------------------------------------
+# Learned initial guess (U-Net on a coarse-grid base) vs the zero guess
+# ----------------------------------------------------------------------
+class Learned:
+    """Set-up (untimed, reported in config): solve `train` 128^3 problems with
+    the batched Jacobi solver, solve their 31^3 coarse problems, train the
+    U-Net for `steps` Adam steps.  One benchmark step: the held-out problems
+    solved to 1e-6 from the zero guess and from the learned guess (coarse
+    solve + prolongation + network + polishing), same solver and reference
+    norms.  value = zero-guess wall time / learned wall time."""
+    name = "learned"
+    dtype = "bf16"   # the network (BF16 operands, FP32 accumulation); polishing is FP32/FP64
+    bytes_per_lup = 12.0
+
+    def __init__(self, n=128, steps=3000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, **_):
+        from libgeodecomp_amd import learned_guess as lgm
+        self.lgm = lgm
+        self.n, self.train_steps, self.seed, self.ntrain, self.neval = n, steps, seed, train, eval
+        self.m = lgm.default_coarse_size(n)
+        self.stream = lgdhip.Stream()
+        t0 = time.perf_counter()
+        data = lgm.Ensemble(synthetic.poisson_ensemble(train, sizes=(n,), seed=seed), stream=self.stream.h)
+        res = data.solve(1e-6)
+        base = lgm.CoarseBase(data, self.m)
+        base.compute()
+        lgdhip.call("lgdhip_device_synchronize")
+        self.generate_s = time.perf_counter() - t0
+        self.gen_sweeps = float(np.mean([r.sweeps for r in res]))
+        log(f"generated {train} solutions of {n}^3 in {self.generate_s:.1f} s")
+        self.model = lgm.LearnedGuess(n, seed=seed % 1000, stream=self.stream.h)
+        t0 = time.perf_counter()
+        self.log = self.model.train(data, base, steps, log_every=250, verbose=True)
+        lgdhip.call("lgdhip_device_synchronize")
+        self.train_s = time.perf_counter() - t0
+        del base, data
+        self.eval = lgm.Ensemble(synthetic.poisson_ensemble(eval, sizes=(n,), seed=seed + 7919), stream=self.stream.h)
+        self.result = None
+
+    def config(self):
+        r = self.result or {}
+        return {"workload": f"learned initial guess vs zero guess, {self.neval} held-out Poisson problems {self.n}^3 "
+                            f"(Gaussian sources, Dirichlet faces), Jacobi polishing to 1e-6 relative FP64 residual",
+                "network": "3D U-Net 16-32-64-128, 3x3x3 BF16 convs on MFMA, FP32 accumulation, bias-free",
+                "coarse_grid": self.m, "train_problems": self.ntrain, "train_steps": self.train_steps,
+                "generate_s": round(self.generate_s, 1), "train_s": round(self.train_s, 1),
+                "final_loss": self.log[-1][1] if self.log else None, "arms": r.get("arms"),
+                "sweep_ratio": r.get("sweep_ratio"), "seed": self.seed, "parallelism": "single GPU"}
+
+    def step(self):
+        self.result = self.lgm.evaluate(self.model, self.eval, self.m)
+        for a in self.result["arms"]:
+            log(f"{a['arm']:8s} sweeps {a['sweeps_mean']:.0f} wall {a['wall_s']:.3f} s")
+
+    def value(self):
+        """(value, unit, ms per step): wall-time speedup of the learned guess."""
+        arms = {a["arm"]: a for a in self.result["arms"]}
+        return self.result["wall_speedup"], "x (zero-guess wall time / learned-guess wall time)", \
+            1e3 * arms["learned"]["wall_s"]
+
+    def lups_per_step(self):
+        return 0.0
+
+    def dominant_kernel(self):
+        """The polishing sweep over the held-out set (poisson_sweep_f32)."""
+        e0 = lgdhip.DeviceBuffer(self.eval.elems * 4)
+        e1 = lgdhip.DeviceBuffer(self.eval.elems * 4)
+        for b in (e0, e1):
+            lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.eval.elems * 4, self.stream.h)
+        sweeps = 50
+        in_b = C.c_int(0)
+        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, self.eval.f.ptr,
+                                                    self.eval.probs, self.eval.count, sweeps, self.eval.ws.ptr,
+                                                    self.eval.ws_bytes, C.byref(in_b), self.stream.h))
+        self.shape = f"{self.neval}x{self.n}^3"
+        return "poisson_sweep_f32", ms / sweeps, self.bytes_per_lup * self.neval * float(self.n) ** 3
+
+    def cpu_baseline(self, budget_s=15.0):
+        """The oracle's plain FP64 Jacobi on one held-out problem (sweep rate)."""
+        oracle = load_oracle()
+        u = np.zeros((self.n + 2,) * 3)
+        (_, faces, f), = synthetic.poisson_ensemble(1, sizes=(self.n,), seed=self.seed + 7919)
+        oracle.poisson_faces(u, faces)
+        sweeps = 4
+        t0 = time.perf_counter()
+        oracle.poisson_jacobi(u.copy(), f, 1e-300, max_sweeps=sweeps, check=sweeps)
+        dt = time.perf_counter() - t0
+        sweeps = max(4, min(100000, int(sweeps * budget_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        oracle.poisson_jacobi(u.copy(), f, 1e-300, max_sweeps=sweeps, check=sweeps)
+        dt = time.perf_counter() - t0
+        return {"value": 1e-9 * self.n ** 3 * sweeps / dt, "unit": "GLUPS", "cores": 1, "kind": "port",
+                "sample": f"C oracle plain FP64 Jacobi (1 thread), one held-out {self.n}^3 problem, {sweeps} sweeps, "
+                          f"{dt:.1f}s (sweep rate; the zero-guess solve needs ~3e4 sweeps per problem)"}
 
 
 class Copy:
@@ -449,7 +566,7 @@ class Copy:
         return None
 
 
-WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "poisson": Poisson, "copy": Copy}
+WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "poisson": Poisson, "learned": Learned, "copy": Copy}
 
 
 def main():
@@ -466,6 +583,7 @@ def main():
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
     ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
+    ap.add_argument("--no-graph", action="store_true", help="poisson: plain launches instead of hipGraph chunks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -494,6 +612,8 @@ def main():
         kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
     if args.workload == "lbm":
         kw.update(by=args.by, flags=flags, variant=max(0, args.variant))
+    if args.workload == "poisson":
+        kw.update(use_graph=0 if args.no_graph else 1)
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):
@@ -507,8 +627,12 @@ def main():
     lgdhip.call("lgdhip_device_synchronize")
     wall = time.perf_counter() - t0
     ms_per_step = ms / args.steps
-    value = 1e-9 * wl.lups_per_step() / (ms_per_step * 1e-3)
-    log(f"timed {args.steps} steps: {ms:.3f} ms (wall {wall * 1e3:.1f} ms), {value:.1f} GLUPS")
+    unit = "GLUPS"
+    if hasattr(wl, "value"):
+        value, unit, ms_per_step = wl.value()
+    else:
+        value = 1e-9 * wl.lups_per_step() / (ms_per_step * 1e-3)
+    log(f"timed {args.steps} steps: {ms:.3f} ms (wall {wall * 1e3:.1f} ms), {value:.3f} {unit}")
 
     kname, k_ms, k_bytes = wl.dominant_kernel()
     achieved = k_bytes / (k_ms * 1e-3) / 1e9
@@ -526,7 +650,7 @@ def main():
     line = {
         "metric": METRIC,
         "value": round(value, 3),
-        "unit": "GLUPS",
+        "unit": unit,
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
