@@ -318,6 +318,18 @@ def equilibrium_f32(rho, jx, jy, jz):
     return f
 
 
+def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, wl, shape):
+    """(name, ms per launch, algorithmic bytes per launch) of the Poisson
+    solver's default sweep kernel: temporal blocking, 4 sweeps per launch
+    (12 B per sweep-LUP, so 4 x 12 B x cells per launch)."""
+    k, sweeps = 4, 48
+    in_b = C.c_int(0)
+    ms = timed(stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, f.ptr, probs, count, sweeps,
+                                           None, ws.ptr, ws_bytes, C.byref(in_b), stream.h))
+    wl.shape = shape
+    return "poisson_sweep_tb", ms / (sweeps // k), k * 12.0 * cells
+
+
 # ----------------------------------------------------------------------
 # Batched Poisson ensemble (FP32 sweeps, FP64 residuals)
 # ----------------------------------------------------------------------
@@ -406,18 +418,14 @@ class Poisson:
         self.last_lups = float(np.sum(self.sizes.astype(np.float64) ** 3 * total))
 
     def dominant_kernel(self):
-        """poisson_sweep_f32 over every problem (the hot loop of the solve)."""
+        """The solver's sweep kernel over every problem (the library default:
+        poisson_sweep_tb, 4 sweeps per launch)."""
         e0 = lgdhip.DeviceBuffer(self.elems * 4)
         e1 = lgdhip.DeviceBuffer(self.elems * 4)
         for b in (e0, e1):
             lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.elems * 4, self.stream.h)
-        sweeps = 20
-        in_b = C.c_int(0)
-        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, self.f.ptr,
-                                                    self.probs, self.count, sweeps, self.ws.ptr, self.ws_bytes,
-                                                    C.byref(in_b), self.stream.h))
-        self.shape = f"{self.count}problems"
-        return "poisson_sweep_f32", ms / sweeps, self.bytes_per_lup * self.cells
+        return poisson_sweep_kernel(self.stream, e0, e1, self.f, self.probs, self.count, self.ws, self.ws_bytes,
+                                    self.cells, self, f"{self.count}problems")
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -505,13 +513,9 @@ class Learned:
         e1 = lgdhip.DeviceBuffer(self.eval.elems * 4)
         for b in (e0, e1):
             lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.eval.elems * 4, self.stream.h)
-        sweeps = 50
-        in_b = C.c_int(0)
-        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, self.eval.f.ptr,
-                                                    self.eval.probs, self.eval.count, sweeps, self.eval.ws.ptr,
-                                                    self.eval.ws_bytes, C.byref(in_b), self.stream.h))
-        self.shape = f"{self.neval}x{self.n}^3"
-        return "poisson_sweep_f32", ms / sweeps, self.bytes_per_lup * self.neval * float(self.n) ** 3
+        return poisson_sweep_kernel(self.stream, e0, e1, self.eval.f, self.eval.probs, self.eval.count, self.eval.ws,
+                                    self.eval.ws_bytes, self.neval * float(self.n) ** 3, self,
+                                    f"{self.neval}x{self.n}^3")
 
     def cpu_baseline(self, budget_s=15.0):
         """The oracle's plain FP64 Jacobi on one held-out problem (sweep rate)."""
@@ -583,6 +587,7 @@ def main():
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
     ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
+    ap.add_argument("--xcd-rows", action="store_true", help="lbm: x rows of tiles round-robin over the XCDs")
     ap.add_argument("--no-graph", action="store_true", help="poisson: plain launches instead of hipGraph chunks")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -607,7 +612,7 @@ def main():
     if args.workload == "life":
         kw.update(per_launch=args.per_launch, variant=7 if args.variant < 0 else args.variant)
     flags = (lgdhip.TILE_NO_XCD_SWIZZLE if args.no_xcd_swizzle else 0) | \
-        (lgdhip.TILE_XCD_SWIZZLE if args.xcd_swizzle else 0)
+        (lgdhip.TILE_XCD_SWIZZLE if args.xcd_swizzle else 0) | (lgdhip.TILE_XCD_ROWS if args.xcd_rows else 0)
     if args.workload == "jacobi":
         kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
     if args.workload == "lbm":

@@ -79,6 +79,8 @@ typedef struct lgdhip_tile {
  * default (Jacobi temporal blocking: swizzled, LBM: launch order). */
 #define LGDHIP_TILE_XCD_SWIZZLE    0x1
 #define LGDHIP_TILE_NO_XCD_SWIZZLE 0x2
+/* LBM: each x row of tiles on one XCD, rows round-robin over the XCDs */
+#define LGDHIP_TILE_XCD_ROWS       0x4
 
 /* Parameters of the D3Q19 lid-driven cavity kernel family. */
 typedef struct lgdhip_lbm_params {
@@ -241,7 +243,8 @@ typedef struct lgdhip_poisson_options {
     double inner_reduction;      /* FP64 refresh once the correction's residual
                                     fell by this factor (0: 1e-2)                 */
     int32_t use_graph;           /* replay chunks as a captured hipGraph           */
-    int32_t reserved;
+    int32_t sweep_variant;       /* 0 default (3), 1 plain sweeps, 2 / 3 temporal
+                                    blocking with 2 / 4 sweeps per launch         */
     const double *reference_norms; /* host array, one per problem, or NULL: the
                                       residual norm of the initial guess          */
     int32_t smooth_sweeps;       /* weighted-Jacobi sweeps before the plain ones
@@ -269,10 +272,13 @@ int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_
                                   int count, void *workspace, size_t workspace_bytes, double *norms,
                                   lgdhip_stream_t stream);
 /* The hot sweep alone: `sweeps` FP32 Jacobi sweeps on A e = r for every
- * problem (e pools with zero halos, ping-pong ea/eb). */
+ * problem (e pools with zero halos, ping-pong ea/eb).  tile->variant as
+ * lgdhip_poisson_options::sweep_variant (NULL: default); blocked launches
+ * swap the buffers once per 2 or 4 sweeps, so *result_in_b is not sweeps % 2
+ * there. */
 int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count,
-                              int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
-                              lgdhip_stream_t stream);
+                              int sweeps, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes,
+                              int *result_in_b, lgdhip_stream_t stream);
 /* Solves every problem from the guess in u (interior) to opt->rel_tol:
  * FP64 residual and update, FP32 Jacobi sweeps on the correction, all
  * active problems swept by one launch per sweep. */

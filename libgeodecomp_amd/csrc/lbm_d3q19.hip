@@ -77,7 +77,10 @@ __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restric
                                                           IDX fsx, IDX fsxy, IDX forigin, float omega, float lux,
                                                           float luy, float luz, unsigned gx, unsigned gy,
                                                           unsigned ntiles, unsigned per, int swizzle) {
-    const unsigned t = lgdhip::xcd_tile(blockIdx.x, per, swizzle != 0);
+    // swizzle: 0 launch order, 1 one contiguous tile range per XCD, 2 rows
+    // of gx tiles (one x row of the grid) per XCD, rows round-robin
+    const unsigned t = swizzle == 2 ? lgdhip::xcd_group_tile(blockIdx.x, gx)
+                                    : lgdhip::xcd_tile(blockIdx.x, per, swizzle == 1);
     if (t >= ntiles) return;
     const int x = (int)(t % gx) * 64 + threadIdx.x;
     const int y = (int)((t / gx) % gy) * BY + threadIdx.y;
@@ -190,12 +193,17 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     rc = flags_grid(*g, &fg);
     if (rc) return rc;
     const int BY = (tile && (tile->by == 1 || tile->by == 2 || tile->by == 8)) ? tile->by : 4;
-    const int swizzle = lgdhip::xcd_swizzle(tile, false);
+    // tile order: LGDHIP_TILE_XCD_SWIZZLE one range per XCD,
+    // LGDHIP_TILE_XCD_ROWS x rows of tiles round-robin over the XCDs,
+    // default (or LGDHIP_TILE_NO_XCD_SWIZZLE) launch order
+    int swizzle = lgdhip::xcd_swizzle(tile, false);
+    if (tile && (tile->flags & LGDHIP_TILE_XCD_ROWS) && !(tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) swizzle = 2;
     const bool ntl = tile && tile->variant == 1;
     const unsigned gx = lgdhip::ceil_div(g->nx, 64), gy = lgdhip::ceil_div(g->ny, BY);
     const unsigned ntiles = gx * gy * (unsigned)g->nz;
     const unsigned per = lgdhip::xcd_per(ntiles);
-    const dim3 grid(swizzle ? lgdhip::xcd_groups(ntiles) : ntiles);
+    const dim3 grid(swizzle == 2 ? lgdhip::xcd_group_groups(ntiles, gx)
+                                 : (swizzle ? lgdhip::xcd_groups(ntiles) : ntiles));
     const dim3 block(64, BY);
     const int64_t sxy = g->sx * g->sy;
     const int64_t plane = lgdhip::plane_elems(*g);

@@ -60,6 +60,18 @@ inline unsigned xcd_groups(unsigned tiles) { return kXcds * xcd_per(tiles); }
 __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned per, bool swizzle) {
     return swizzle ? (w % kXcds) * per + w / kXcds : w;
 }
+// Grouped order: groups of G consecutive tiles run on one XCD and
+// consecutive groups on consecutive XCDs, so tiles sharing lines within a
+// group share an L2 while the whole GPU still streams one contiguous front.
+// Launch xcd_group_groups(T, G) work-groups; ids mapped past T exit.
+inline unsigned xcd_group_groups(unsigned tiles, unsigned g) {
+    const unsigned span = kXcds * g;
+    return (tiles + span - 1) / span * span;
+}
+__device__ __forceinline__ unsigned xcd_group_tile(unsigned w, unsigned g) {
+    const unsigned span = kXcds * g;
+    return (w / span) * span + (w % kXcds) * g + (w / kXcds) % g;
+}
 // the swizzle choice of a tile's flags, else the family default
 inline int xcd_swizzle(const lgdhip_tile *tile, bool family_default) {
     if (tile && (tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) return 0;

@@ -147,6 +147,122 @@ __global__ __launch_bounds__(kThreads) void poisson_sweep_f32(const float *__res
     }
 }
 
+// value of lane-1 / lane+1 (DPP wave_shr:1 / wave_shl:1; the wave's end
+// lanes read 0 and only feed overlap cells)
+__device__ __forceinline__ float lane_prev(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xF, 0xF, true));
+}
+__device__ __forceinline__ float lane_next(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xF, 0xF, true));
+}
+
+// K FP32 Jacobi sweeps on A e = r per launch (temporal blocking).  A
+// work-group is a 64 x TY tile of (x, y) columns of one problem with a K-cell
+// overlap ring (it writes (64-2K) x (TY-2K) columns; tile.x0/y0 is the
+// origin of that written region) and marches a z chunk as a pipeline: when
+// plane p of the input arrives, level l is computed at plane p-l from level
+// l-1 (3-plane register window per level, x neighbours by DPP, y neighbours
+// through a double-buffered LDS plane per level, r from a register ring).
+// Cells outside the problem hold 0 at every level (the correction's zero
+// Dirichlet halo) and every cell sums in poisson_sweep_f32's order, so one
+// launch equals K launches of that kernel bit for bit.  With `partial`, the
+// last level adds the FP64 square of the residual of its input iterate over
+// the written cells into one partial per tile.
+template <int K, int TY>
+__global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restrict__ e, float *__restrict__ e2,
+                                                            const float *__restrict__ r,
+                                                            const Tile *__restrict__ tiles,
+                                                            const ProblemDev *__restrict__ probs,
+                                                            double *__restrict__ partial, int zchunk) {
+    __shared__ float lds[2][K][TY][64];
+    __shared__ double red[TY];
+    const Tile t = tiles[blockIdx.x];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = t.x0 - K + tx, y = t.y0 - K + ty;
+    const int z0 = t.z0, z1 = min(t.z0 + zchunk, n);
+    const bool inxy = (unsigned)x < (unsigned)n && (unsigned)y < (unsigned)n;
+    const bool writer = inxy && tx >= K && tx < 64 - K && ty >= K && ty < TY - K;
+    const int64_t sx = n + 2, sxy = sx * sx;
+    const int64_t col = P.base + (int64_t)(y + 1) * sx + (x + 1);  // padded column, plane index = z + 1
+    const int yd = ty > 0 ? ty - 1 : 0;
+    const int yu = ty < TY - 1 ? ty + 1 : TY - 1;
+    auto in_plane = [&](int p) { return inxy && (unsigned)p < (unsigned)n; };
+
+    float w[K][3];
+    float rw[K + 1];  // rw[j] = r at plane p - j
+#pragma unroll
+    for (int l = 0; l < K; ++l) w[l][0] = w[l][1] = w[l][2] = 0.0f;
+#pragma unroll
+    for (int j = 0; j <= K; ++j) rw[j] = 0.0f;
+#pragma unroll
+    for (int l = 0; l < K; ++l) lds[0][l][ty][tx] = 0.0f;
+    const int pbeg = z0 - K, pend = z1 + K;
+    float ne = in_plane(pbeg) ? e[col + (int64_t)(pbeg + 1) * sxy] : 0.0f;
+    float nr = in_plane(pbeg) ? r[col + (int64_t)(pbeg + 1) * sxy] : 0.0f;
+    __syncthreads();
+
+    double acc = 0.0;
+    int buf = 0;
+    for (int p = pbeg; p < pend; ++p) {
+        const float v0 = ne, r0 = nr;
+        if (in_plane(p + 1)) {  // next plane's loads in flight during this step
+            ne = e[col + (int64_t)(p + 2) * sxy];
+            nr = r[col + (int64_t)(p + 2) * sxy];
+        } else {
+            ne = 0.0f;
+            nr = 0.0f;
+        }
+        w[0][0] = w[0][1];
+        w[0][1] = w[0][2];
+        w[0][2] = v0;
+#pragma unroll
+        for (int j = K; j > 0; --j) rw[j] = rw[j - 1];
+        rw[0] = r0;
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            const int q = p - l;
+            const float c = w[l - 1][1];
+            float sum = w[l - 1][0];          // ( 0,  0, -1)
+            sum += lds[buf][l - 1][yd][tx];   // ( 0, -1,  0)
+            sum += lane_prev(c);              // (-1,  0,  0)
+            sum += lane_next(c);              // ( 1,  0,  0)
+            sum += lds[buf][l - 1][yu][tx];   // ( 0,  1,  0)
+            sum += w[l - 1][2];               // ( 0,  0,  1)
+            sum += rw[l];
+            const bool live = in_plane(q);
+            const float v = live ? sum * (1.0f / 6.0f) : 0.0f;
+            if (l < K) {
+                w[l][0] = w[l][1];
+                w[l][1] = w[l][2];
+                w[l][2] = v;
+            } else if (writer && q >= z0) {
+                e2[col + (int64_t)(q + 1) * sxy] = v;
+                if (partial) {
+                    const double rr = (double)sum - 6.0 * (double)c;
+                    acc += rr * rr;
+                }
+            }
+        }
+#pragma unroll
+        for (int l = 0; l < K; ++l) lds[buf ^ 1][l][ty][tx] = w[l][2];
+        __syncthreads();
+        buf ^= 1;
+    }
+    if (partial) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (tx == 0) red[ty] = acc;
+        __syncthreads();
+        if (tx == 0 && ty == 0) {
+            double sum = 0.0;
+            for (int i = 0; i < TY; ++i) sum += red[i];
+            partial[blockIdx.x] = sum;
+        }
+    }
+}
+
 // u += e over the interior and reset both correction buffers to zero
 __global__ __launch_bounds__(kThreads) void poisson_update_f64(double *__restrict__ u, float *e, float *eother,
                                                                const Tile *__restrict__ tiles,
@@ -216,6 +332,33 @@ void append_tiles(std::vector<Tile> *out, int p, int n) {
             for (int x = 0; x < n; x += kTX) out->push_back(Tile{p, x, y, z});
 }
 
+// temporal blocking shapes: variant -> (K, TY); tiles write (64-2K) x (TY-2K)
+// columns and march z chunks of kTbZ planes
+struct TbShape {
+    int k, ty;
+};
+constexpr int kTbZ = 64;
+constexpr int kDefaultSweepVariant = 3;
+TbShape tb_shape(int variant) {
+    switch (variant) {
+    case 2: return {2, 16};
+    case 3: return {4, 16};
+    default: return {0, 0};  // 1: plain sweeps
+    }
+}
+
+int64_t tb_tiles_of(int n, TbShape sh) {
+    const int ox = 64 - 2 * sh.k, oy = sh.ty - 2 * sh.k;
+    return (int64_t)((n + ox - 1) / ox) * ((n + oy - 1) / oy) * ((n + kTbZ - 1) / kTbZ);
+}
+
+void append_tb_tiles(std::vector<Tile> *out, int p, int n, TbShape sh) {
+    const int ox = 64 - 2 * sh.k, oy = sh.ty - 2 * sh.k;
+    for (int z = 0; z < n; z += kTbZ)
+        for (int y = 0; y < n; y += oy)
+            for (int x = 0; x < n; x += ox) out->push_back(Tile{p, x, y, z});
+}
+
 int check_problems(const lgdhip_poisson_problem *probs, int count) {
     LGD_REQUIRE(probs && count > 0, "no problems");
     for (int k = 0; k < count; ++k) {
@@ -234,12 +377,14 @@ struct Workspace {
     ProblemDev *probs;
     int *begin, *end;
     double *faces;
+    Tile *tb_tiles;  // temporal-blocking sweep tiles of the active set
+    int *tb_begin, *tb_end;
     size_t bytes;
 };
 
 size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
-Workspace carve(void *base, int64_t pool, int64_t ntiles, int count) {
+Workspace carve(void *base, int64_t pool, int64_t ntiles, int64_t ntb, int count) {
     Workspace w;
     char *p = (char *)base;
     size_t off = 0;
@@ -251,13 +396,16 @@ Workspace carve(void *base, int64_t pool, int64_t ntiles, int count) {
     w.e0 = (float *)take(pool * 4);
     w.e1 = (float *)take(pool * 4);
     w.r = (float *)take(pool * 4);
-    w.partial = (double *)take(ntiles * 8);
+    w.partial = (double *)take(std::max(ntiles, ntb) * 8);
     w.norms = (double *)take((size_t)count * 8);
     w.tiles = (Tile *)take(ntiles * sizeof(Tile));
     w.probs = (ProblemDev *)take((size_t)count * sizeof(ProblemDev));
     w.begin = (int *)take((size_t)count * 4);
     w.end = (int *)take((size_t)count * 4);
     w.faces = (double *)take((size_t)count * 6 * 8);
+    w.tb_tiles = (Tile *)take(ntb * sizeof(Tile));
+    w.tb_begin = (int *)take((size_t)count * 4);
+    w.tb_end = (int *)take((size_t)count * 4);
     w.bytes = off;
     return w;
 }
@@ -272,6 +420,47 @@ int64_t total_tiles(const lgdhip_poisson_problem *probs, int count) {
     int64_t t = 0;
     for (int k = 0; k < count; ++k) t += tiles_of(probs[k].n);
     return t;
+}
+
+// tile capacity for every temporal-blocking shape
+int64_t total_tb_tiles(const lgdhip_poisson_problem *probs, int count) {
+    int64_t t = 0;
+    for (int k = 0; k < count; ++k)
+        t += std::max(tb_tiles_of(probs[k].n, tb_shape(2)), tb_tiles_of(probs[k].n, tb_shape(3)));
+    return t;
+}
+
+Workspace carve_for(void *base, const lgdhip_poisson_problem *probs, int count) {
+    return carve(base, pool_extent(probs, count), total_tiles(probs, count), total_tb_tiles(probs, count), count);
+}
+
+// Uploads the temporal-blocking tile list of the problems in `set`;
+// tb_begin/tb_end index the tiles of set[k].  Returns the tile count.
+int upload_tb_set(const Workspace &w, const lgdhip_poisson_problem *probs, const std::vector<int> &set, TbShape sh,
+                  int64_t *ntiles, hipStream_t s) {
+    std::vector<Tile> tiles;
+    std::vector<int> b(set.size()), e(set.size());
+    for (size_t k = 0; k < set.size(); ++k) {
+        b[k] = (int)tiles.size();
+        append_tb_tiles(&tiles, set[k], probs[set[k]].n, sh);
+        e[k] = (int)tiles.size();
+    }
+    *ntiles = (int64_t)tiles.size();
+    if (tiles.empty()) return 0;
+    LGD_HIP(hipMemcpyAsync(w.tb_tiles, tiles.data(), tiles.size() * sizeof(Tile), hipMemcpyHostToDevice, s));
+    LGD_HIP(hipMemcpyAsync(w.tb_begin, b.data(), b.size() * 4, hipMemcpyHostToDevice, s));
+    LGD_HIP(hipMemcpyAsync(w.tb_end, e.data(), e.size() * 4, hipMemcpyHostToDevice, s));
+    LGD_HIP(hipStreamSynchronize(s));
+    return 0;
+}
+
+// one launch of K blocked sweeps a -> b over the uploaded tile list
+void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
+               double *partial, hipStream_t s) {
+    if (sh.k == 2)
+        poisson_sweep_tb<2, 16><<<(unsigned)ntiles, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ);
+    else
+        poisson_sweep_tb<4, 16><<<(unsigned)ntiles, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ);
 }
 
 // Uploads problem descriptors and the tile list of the problems in `set`;
@@ -324,7 +513,7 @@ int64_t lgdhip_poisson_layout(lgdhip_poisson_problem *probs, int count) {
 
 size_t lgdhip_poisson_workspace_bytes(const lgdhip_poisson_problem *probs, int count) {
     if (check_problems(probs, count)) return 0;
-    return carve(nullptr, pool_extent(probs, count), total_tiles(probs, count), count).bytes;
+    return carve_for(nullptr, probs, count).bytes;
 }
 
 int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int count, void *workspace,
@@ -332,7 +521,7 @@ int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int
     int rc = check_problems(probs, count);
     if (rc) return rc;
     LGD_REQUIRE(u && workspace, "NULL pointer");
-    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    const Workspace w = carve_for(workspace, probs, count);
     if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
     hipStream_t s = lgdhip::as_stream(stream);
     std::vector<ProblemDev> pd(count);
@@ -354,12 +543,12 @@ int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int
 }
 
 int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count,
-                              int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
-                              lgdhip_stream_t stream) {
+                              int sweeps, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes,
+                              int *result_in_b, lgdhip_stream_t stream) {
     int rc = check_problems(probs, count);
     if (rc) return rc;
     LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
-    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    const Workspace w = carve_for(workspace, probs, count);
     if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
     hipStream_t s = lgdhip::as_stream(stream);
     std::vector<ProblemDev> pd(count);
@@ -370,8 +559,21 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
     std::vector<Tile> tiles;
     rc = upload_set(w, probs, all, &tiles, s);
     if (rc) return rc;
+    const int variant = (tile && tile->variant >= 1 && tile->variant <= 3) ? tile->variant : kDefaultSweepVariant;
+    const TbShape sh = tb_shape(variant);
     float *cur = ea, *nxt = eb;
-    for (int t = 0; t < sweeps; ++t) {
+    int t = 0;
+    if (sh.k > 0 && sweeps >= sh.k) {
+        int64_t ntb = 0;
+        rc = upload_tb_set(w, probs, all, sh, &ntb, s);
+        if (rc) return rc;
+        for (; t + sh.k <= sweeps; t += sh.k) {
+            launch_tb(sh, cur, nxt, r, w, ntb, nullptr, s);
+            LGD_LAUNCHED("poisson_sweep_tb");
+            std::swap(cur, nxt);
+        }
+    }
+    for (; t < sweeps; ++t) {
         poisson_sweep_f32<false>
             <<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr, 1.0f);
         LGD_LAUNCHED("poisson_sweep_f32");
@@ -387,7 +589,7 @@ int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_
     int rc = check_problems(probs, count);
     if (rc) return rc;
     LGD_REQUIRE(u && f && workspace && norms, "NULL pointer");
-    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    const Workspace w = carve_for(workspace, probs, count);
     if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
     hipStream_t s = lgdhip::as_stream(stream);
     std::vector<ProblemDev> pd(count);
@@ -415,11 +617,18 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
     LGD_REQUIRE(opt->smooth_sweeps >= 0 && (opt->smooth_sweeps == 0 || (opt->smooth_omega > 0.0f &&
                                                                         opt->smooth_omega <= 1.0f)),
                 "smooth_omega must be in (0, 1]");
-    const Workspace w = carve(workspace, pool_extent(probs, count), total_tiles(probs, count), count);
+    const Workspace w = carve_for(workspace, probs, count);
     if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
     hipStream_t s = lgdhip::as_stream(stream);
     const bool graphs = opt->use_graph != 0 && s != nullptr;
     const double theta = opt->inner_reduction > 0 ? opt->inner_reduction : 1e-2;
+    const int variant =
+        (opt->sweep_variant >= 1 && opt->sweep_variant <= 3) ? opt->sweep_variant : kDefaultSweepVariant;
+    const TbShape sh = tb_shape(variant);
+    // sweeps per chunk: even (plain) or a multiple of 2K (blocked), so every
+    // chunk leaves the correction in the same buffer
+    const int unit = sh.k > 0 ? 2 * sh.k : 2;
+    const int C = std::max(unit, (opt->chunk + unit - 1) / unit * unit);
 
     std::vector<ProblemDev> pd(count);
     for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
@@ -490,16 +699,28 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
         if (active.empty()) break;
         rc = upload_set(w, probs, active, &tiles, s);
         if (rc) return rc;
+        int64_t ntb = 0;
+        if (sh.k > 0) {
+            rc = upload_tb_set(w, probs, active, sh, &ntb, s);
+            if (rc) return rc;
+        }
         graph_valid = false;
 
         bool set_changed = false;
         while (!set_changed) {
-            // one chunk: chunk-1 plain sweeps, then a sweep that also
-            // reports the inner residual of its input iterate; an even
-            // chunk leaves the correction in ecur, so the captured graph
-            // stays valid from chunk to chunk
-            const int C = opt->chunk + (opt->chunk & 1);
+            // one chunk of C sweeps whose last sweep also reports the inner
+            // residual of its input iterate; the chunk leaves the correction
+            // in ecur, so the captured graph stays valid from chunk to chunk
             auto enqueue = [&](float *a, float *b) -> int {
+                if (sh.k > 0) {
+                    for (int t = 0; t < C; t += sh.k) {
+                        launch_tb(sh, a, b, w.r, w, ntb, t + sh.k == C ? w.partial : nullptr, s);
+                        std::swap(a, b);
+                    }
+                    poisson_reduce<<<(unsigned)active.size(), 256, 0, s>>>(w.partial, w.tb_begin, w.tb_end,
+                                                                           w.norms);
+                    return 0;
+                }
                 for (int t = 0; t < C; ++t) {
                     poisson_sweep_f32<false><<<(unsigned)tiles.size(), kThreads, 0, s>>>(
                         a, b, w.r, w.tiles, w.probs, t == C - 1 ? w.partial : nullptr, 1.0f);

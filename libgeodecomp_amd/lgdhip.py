@@ -52,7 +52,7 @@ class lgdhip_poisson_problem(C.Structure):
 
 class lgdhip_poisson_options(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("chunk", C.c_int32), ("max_sweeps", C.c_int32),
-                ("inner_reduction", C.c_double), ("use_graph", C.c_int32), ("reserved", C.c_int32),
+                ("inner_reduction", C.c_double), ("use_graph", C.c_int32), ("sweep_variant", C.c_int32),
                 ("reference_norms", C.POINTER(C.c_double)), ("smooth_sweeps", C.c_int32),
                 ("smooth_omega", C.c_float)]
 
@@ -127,7 +127,8 @@ SIGNATURES = {
     "lgdhip_poisson_set_faces": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ, _P]),
     "lgdhip_poisson_residual_norms": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ,
                                            C.POINTER(C.c_double), _P]),
-    "lgdhip_poisson_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _P, _SZ, _IP, _P]),
+    "lgdhip_poisson_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _TP, _P, _SZ, _IP,
+                                       _P]),
     "lgdhip_poisson_batch_solve": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I,
                                         C.POINTER(lgdhip_poisson_options), _P, _SZ,
                                         C.POINTER(lgdhip_poisson_result), _P]),
@@ -340,6 +341,7 @@ class Event:
 
 TILE_XCD_SWIZZLE = 0x1
 TILE_NO_XCD_SWIZZLE = 0x2
+TILE_XCD_ROWS = 0x4
 
 
 def tile(bx=0, by=0, bz=0, steps_per_launch=0, variant=0, flags=0) -> lgdhip_tile:

@@ -153,9 +153,13 @@ class Batch:
 
 
 @pytest.mark.gpu
-def test_gpu_sweeps_bit_exact(hip, oracle):
-    rng = np.random.default_rng(3)
-    sizes = [1, 5, 8, 17, 32, 33, 40]
+@pytest.mark.parametrize("variant,sweeps", [(1, 5), (2, 5), (3, 9), (3, 8), (0, 4)])
+def test_gpu_sweeps_bit_exact(hip, oracle, variant, sweeps):
+    """Plain sweeps (variant 1) and temporal blocking with 2 (variant 2) or 4
+    (variant 3, the default) sweeps per launch, sweep counts with remainders,
+    sizes from 1 to several tiles per axis: bit-identical to the oracle."""
+    rng = np.random.default_rng(3 + variant)
+    sizes = [1, 5, 8, 17, 32, 33, 40, 70, 113]
     problems = [(n, np.zeros(6), np.zeros((n + 2,) * 3, np.float32)) for n in sizes]
     b = Batch(hip, problems)
     e_host = np.zeros(b.elems, np.float32)
@@ -174,11 +178,11 @@ def test_gpu_sweeps_bit_exact(hip, oracle):
     ea.upload(e_host)
     eb.upload(np.zeros(b.elems, np.float32))
     r.upload(r_host)
-    sweeps = 5
     in_b = C.c_int(-1)
-    print("[case] poisson sweeps", sizes, flush=True)
-    hip.call("lgdhip_poisson_sweeps_f32", ea.ptr, eb.ptr, r.ptr, b.probs, b.count, sweeps, b.ws.ptr, b.ws_bytes,
-             C.byref(in_b), None)
+    print("[case] poisson sweeps", sizes, variant, sweeps, flush=True)
+    tile = hip.tile(variant=variant)
+    hip.call("lgdhip_poisson_sweeps_f32", ea.ptr, eb.ptr, r.ptr, b.probs, b.count, sweeps, C.byref(tile), b.ws.ptr,
+             b.ws_bytes, C.byref(in_b), None)
     out = np.empty(b.elems, np.float32)
     (eb if in_b.value else ea).download(out)
     for k, (e, rr) in enumerate(blocks):
