@@ -237,9 +237,9 @@ class LBM:
     dtype = "f32"
     bytes_per_lup = 153.0  # 19 x 4 B read + 19 x 4 B write + 1 B wall flags
 
-    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, **_):
+    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, bx=0, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
-        self.tile = lgdhip.tile(by=by, flags=flags, variant=variant)
+        self.tile = lgdhip.tile(bx=bx, by=by, flags=flags, variant=variant)
         self.stream = lgdhip.Stream()
         s = self.stream.h
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19)
@@ -343,9 +343,10 @@ class Poisson:
     dtype = "f32"   # the sweeps; residuals and updates are FP64
     bytes_per_lup = 12.0  # read e + read r + write e (FP32)
 
-    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, use_graph=1, **_):
+    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, use_graph=1, tol=1e-6, **_):
         self.count, self.seed = n, seed   # --size overrides the problem count
         self.use_graph = use_graph
+        self.tol = tol
         self.stream = lgdhip.Stream()
         s = self.stream.h
         t0 = time.perf_counter()
@@ -376,7 +377,7 @@ class Poisson:
     def config(self):
         sw = getattr(self, "total_sweeps", None)
         return {"workload": f"batched Poisson ensemble, {self.count} problems n in {{32..192 step 16}}, 1-8 Gaussian "
-                            f"sources, Dirichlet faces U[-1,1], zero guess to 1e-6 relative FP64 residual",
+                            f"sources, Dirichlet faces U[-1,1], zero guess to {self.tol:g} relative FP64 residual",
                 "problems": self.count, "cells": int(self.cells), "seed": self.seed,
                 "sweeps_max": int(sw.max()) if sw is not None else None,
                 "sweep_lups": self.last_lups,
@@ -401,7 +402,7 @@ class Poisson:
         lgdhip.call("lgdhip_poisson_residual_norms", self.u.ptr, self.f.ptr, self.probs, self.count, self.ws.ptr,
                     self.ws_bytes, ref, s)
         opt = lgdhip.lgdhip_poisson_options()
-        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = 1e-6, 64, self.SWEEPS_PER_CALL, self.use_graph
+        opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = self.tol, 64, self.SWEEPS_PER_CALL, self.use_graph
         opt.reference_norms = ref
         total = np.zeros(self.count, dtype=np.int64)
         res = (lgdhip.lgdhip_poisson_result * self.count)()
@@ -583,12 +584,14 @@ def main():
     ap.add_argument("--iters", type=int, default=0, help="override generations / steps per run")
     ap.add_argument("--per-launch", type=int, default=31, help="life: generations per launch")
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant id (-1: tuned default, 0: library default)")
-    ap.add_argument("--by", type=int, default=0, help="jacobi: rows per work-group (0: default)")
+    ap.add_argument("--by", type=int, default=0, help="jacobi/lbm: rows per work-group (0: default)")
+    ap.add_argument("--bx", type=int, default=0, help="lbm: work-group width 64 or 256 (0: default)")
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
     ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
     ap.add_argument("--xcd-rows", action="store_true", help="lbm: x rows of tiles round-robin over the XCDs")
     ap.add_argument("--no-graph", action="store_true", help="poisson: plain launches instead of hipGraph chunks")
+    ap.add_argument("--tol", type=float, default=1e-6, help="poisson: relative residual to solve to")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
 
@@ -616,9 +619,9 @@ def main():
     if args.workload == "jacobi":
         kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
     if args.workload == "lbm":
-        kw.update(by=args.by, flags=flags, variant=max(0, args.variant))
+        kw.update(by=args.by, flags=flags, variant=max(0, args.variant), bx=args.bx)
     if args.workload == "poisson":
-        kw.update(use_graph=0 if args.no_graph else 1)
+        kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol)
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):

@@ -201,7 +201,8 @@ int lgdhip_life2d_run(uint8_t *grid, const lgdhip_grid *g, int edge_alive, int s
  * grid made with lgdhip_grid_make(3, nx, ny, nz, halo, 1, 1) holding the
  * LGDHIP_LBM_* wall bits of every cell.  Ping-pongs a/b like the other
  * families.  tile (optional): by = rows per 64-wide work-group (1, 2, 4, 8),
- * variant 1 = nontemporal population loads, flags (LGDHIP_TILE_*). */
+ * or bx = 256 for 256 x 1 work-groups; variant 1 = nontemporal population
+ * loads; flags (LGDHIP_TILE_*). */
 #define LGDHIP_LBM_WALL_XLO 0x01
 #define LGDHIP_LBM_WALL_XHI 0x02
 #define LGDHIP_LBM_WALL_YLO 0x04

@@ -70,8 +70,8 @@ __device__ __forceinline__ bool blocked(const Dir &d, uint32_t fl) {
 // fastest, then y, then z, and work-groups map to them through the XCD
 // swizzle (each XCD a contiguous range), so the cache lines that x-shifted
 // pulls straddle between neighbouring tiles are fetched into one L2.
-template <int BY, typename IDX, bool NTL>
-__global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restrict__ src, float *__restrict__ dst,
+template <int BX, int BY, typename IDX, bool NTL>
+__global__ __launch_bounds__(BX * BY) void lbm_d3q19_pull(const float *__restrict__ src, float *__restrict__ dst,
                                                           const uint8_t *__restrict__ flags, int nx, int ny,
                                                           int nz, IDX sx, IDX sxy, IDX plane, IDX origin,
                                                           IDX fsx, IDX fsxy, IDX forigin, float omega, float lux,
@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull(const float *__restric
     const unsigned t = swizzle == 2 ? lgdhip::xcd_group_tile(blockIdx.x, gx)
                                     : lgdhip::xcd_tile(blockIdx.x, per, swizzle == 1);
     if (t >= ntiles) return;
-    const int x = (int)(t % gx) * 64 + threadIdx.x;
+    const int x = (int)(t % gx) * BX + threadIdx.x;
     const int y = (int)((t / gx) % gy) * BY + threadIdx.y;
     const int z = (int)(t / (gx * gy));
     if (x >= nx || y >= ny || z >= nz) return;
@@ -192,19 +192,23 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     lgdhip_grid fg;
     rc = flags_grid(*g, &fg);
     if (rc) return rc;
-    const int BY = (tile && (tile->by == 1 || tile->by == 2 || tile->by == 8)) ? tile->by : 4;
+    // work-group: 64 x BY cells, or (bx = 256) 256 x 1: a whole 256-cell row
+    // segment in one work-group, so lines straddled by the x-shifted pulls
+    // of its four waves are fetched once into that CU's caches
+    const int BX = (tile && tile->bx == 256) ? 256 : 64;
+    const int BY = BX == 256 ? 1 : ((tile && (tile->by == 1 || tile->by == 2 || tile->by == 8)) ? tile->by : 4);
     // tile order: LGDHIP_TILE_XCD_SWIZZLE one range per XCD,
     // LGDHIP_TILE_XCD_ROWS x rows of tiles round-robin over the XCDs,
     // default (or LGDHIP_TILE_NO_XCD_SWIZZLE) launch order
     int swizzle = lgdhip::xcd_swizzle(tile, false);
     if (tile && (tile->flags & LGDHIP_TILE_XCD_ROWS) && !(tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) swizzle = 2;
     const bool ntl = tile && tile->variant == 1;
-    const unsigned gx = lgdhip::ceil_div(g->nx, 64), gy = lgdhip::ceil_div(g->ny, BY);
+    const unsigned gx = lgdhip::ceil_div(g->nx, BX), gy = lgdhip::ceil_div(g->ny, BY);
     const unsigned ntiles = gx * gy * (unsigned)g->nz;
     const unsigned per = lgdhip::xcd_per(ntiles);
     const dim3 grid(swizzle == 2 ? lgdhip::xcd_group_groups(ntiles, gx)
                                  : (swizzle ? lgdhip::xcd_groups(ntiles) : ntiles));
-    const dim3 block(64, BY);
+    const dim3 block(BX, BY);
     const int64_t sxy = g->sx * g->sy;
     const int64_t plane = lgdhip::plane_elems(*g);
     const int64_t origin = lgdhip::origin_index(*g);
@@ -213,33 +217,39 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     const bool small_index = 19 * plane < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
     hipStream_t s = lgdhip::as_stream(stream);
     float *cur = a, *nxt = b;
-    auto launch_nt = [&](auto by_tag, auto nt_tag, const float *src, float *dst) {
+    auto launch_nt = [&](auto bx_tag, auto by_tag, auto nt_tag, const float *src, float *dst) {
+        constexpr int X = decltype(bx_tag)::value;
         constexpr int B = decltype(by_tag)::value;
         constexpr bool NT = decltype(nt_tag)::value;
         if (small_index) {
-            lbm_d3q19_pull<B, int32_t, NT><<<grid, block, 0, s>>>(
+            lbm_d3q19_pull<X, B, int32_t, NT><<<grid, block, 0, s>>>(
                 src, dst, flags, (int)g->nx, (int)g->ny, (int)g->nz, (int32_t)g->sx, (int32_t)sxy, (int32_t)plane,
                 (int32_t)origin, (int32_t)fg.sx, (int32_t)(fg.sx * fg.sy), (int32_t)forigin, p->omega, p->lid_ux,
                 p->lid_uy, p->lid_uz, gx, gy, ntiles, per, swizzle);
         } else {
-            lbm_d3q19_pull<B, int64_t, NT><<<grid, block, 0, s>>>(src, dst, flags, (int)g->nx, (int)g->ny, (int)g->nz,
+            lbm_d3q19_pull<X, B, int64_t, NT><<<grid, block, 0, s>>>(src, dst, flags, (int)g->nx, (int)g->ny, (int)g->nz,
                                                               g->sx, sxy, plane, origin, fg.sx, fg.sx * fg.sy,
                                                               forigin, p->omega, p->lid_ux, p->lid_uy, p->lid_uz, gx,
                                                               gy, ntiles, per, swizzle);
         }
     };
-    auto launch = [&](auto by_tag, const float *src, float *dst) {
+    using I64 = std::integral_constant<int, 64>;
+    auto launch = [&](auto bx_tag, auto by_tag, const float *src, float *dst) {
         if (ntl)
-            launch_nt(by_tag, std::true_type{}, src, dst);
+            launch_nt(bx_tag, by_tag, std::true_type{}, src, dst);
         else
-            launch_nt(by_tag, std::false_type{}, src, dst);
+            launch_nt(bx_tag, by_tag, std::false_type{}, src, dst);
     };
     for (int t = 0; t < steps; ++t) {
-        switch (BY) {
-        case 1: launch(std::integral_constant<int, 1>{}, cur, nxt); break;
-        case 2: launch(std::integral_constant<int, 2>{}, cur, nxt); break;
-        case 8: launch(std::integral_constant<int, 8>{}, cur, nxt); break;
-        default: launch(std::integral_constant<int, 4>{}, cur, nxt); break;
+        if (BX == 256) {
+            launch(std::integral_constant<int, 256>{}, std::integral_constant<int, 1>{}, cur, nxt);
+        } else {
+            switch (BY) {
+            case 1: launch(I64{}, std::integral_constant<int, 1>{}, cur, nxt); break;
+            case 2: launch(I64{}, std::integral_constant<int, 2>{}, cur, nxt); break;
+            case 8: launch(I64{}, std::integral_constant<int, 8>{}, cur, nxt); break;
+            default: launch(I64{}, std::integral_constant<int, 4>{}, cur, nxt); break;
+            }
         }
         LGD_LAUNCHED("lbm_d3q19_pull");
         float *tmp = cur;

@@ -84,8 +84,10 @@ __device__ __forceinline__ Frag load_a(const u16 *__restrict__ X, int n, int ld,
 
 constexpr int kMaxNTiles = 12;  // cout up to 192 (the dgrad of a 192-channel concat input)
 
-// One wave: 16 consecutive voxels (linear order, any n), all cout (NT tiles of 16).
-template <int NT>
+// One wave: MT x 16 consecutive voxels (linear order, any n), all cout (NT
+// tiles of 16).  Each k step loads MT A fragments and NT B fragments for
+// MT x NT MFMAs: the weight fragments are shared by the MT voxel tiles.
+template <int NT, int MT>
 __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
                                                        int ksize, const u16 *__restrict__ W, int Kp,
                                                        const float *__restrict__ bias, int cout, u16 *Y, int ldy,
@@ -93,54 +95,75 @@ __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X
     const int lane = threadIdx.x & 63;
     const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
     const int64_t nvox = (int64_t)n * n * n;
-    const int64_t v0 = wave * 16;
+    const int64_t v0 = wave * 16 * MT;
     if (v0 >= nvox) return;
     const int i = lane & 15;   // voxel of the A rows / channel of the B columns
     const int kq = lane >> 4;  // which 8 of the 32 k
     const int K = (ksize == 1 ? 1 : 27) * cin;
-    // this lane's A voxel
-    const int64_t va = v0 + i;
-    const bool a_ok = va < nvox;
-    const int ax = (int)(va % n), ay = (int)((va / n) % n), az = (int)(va / ((int64_t)n * n));
-
-    f32x4 acc[NT];
+    // this lane's A voxel in each of the MT tiles
+    bool a_ok[MT];
+    int ax[MT], ay[MT], az[MT];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int m = 0; m < MT; ++m) {
+        const int64_t va = v0 + m * 16 + i;
+        a_ok[m] = va < nvox;
+        ax[m] = (int)(va % n);
+        ay[m] = (int)((va / n) % n);
+        az[m] = (int)(va / ((int64_t)n * n));
+    }
+
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     for (int kc = 0; kc < Kp; kc += 32) {
         const int k0 = kc + 8 * kq;
-        Frag a;
-        a.q = make_uint4(0, 0, 0, 0);
-        if (a_ok) a = load_a(X, n, ldx, offx, cin, ksize, az, ay, ax, k0, K);
+        Frag a[MT], b[NT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            a[m].q = make_uint4(0, 0, 0, 0);
+            if (a_ok[m]) a[m] = load_a(X, n, ldx, offx, cin, ksize, az[m], ay[m], ax[m], k0, K);
+        }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            Frag b;
-            b.q = make_uint4(0, 0, 0, 0);
-            if (t * 16 + i < cout) b.q = *reinterpret_cast<const uint4 *>(W + (int64_t)(t * 16 + i) * Kp + k0);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[t], 0, 0, 0);
+            b[t].q = make_uint4(0, 0, 0, 0);
+            if (t * 16 + i < cout) b[t].q = *reinterpret_cast<const uint4 *>(W + (int64_t)(t * 16 + i) * Kp + k0);
         }
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, b[t].v, acc[m][t], 0, 0, 0);
     }
-    // D[r][c] in lane c + 16*(r/4), element r%4: voxel v0 + 4*(lane/16) + e, channel t*16 + lane%16
+    // D[r][c] in lane c + 16*(r/4), element r%4: voxel v0 + 16m + 4*(lane/16) + e, channel t*16 + lane%16
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + i;
         if (co >= cout) continue;
-        const float b = bias ? bias[co] : 0.0f;
+        const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int64_t v = v0 + 4 * kq + e;
-            if (v >= nvox) continue;
-            float r = acc[t][e] + b;
-            if (relu) r = r > 0.0f ? r : 0.0f;
-            if (Y) {
-                u16 *p = Y + v * ldy + offy + co;
-                if (accumulate) r += bf2f(*p);
-                *p = f2bf(r);
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t v = v0 + m * 16 + 4 * kq + e;
+                if (v >= nvox) continue;
+                float r = acc[m][t][e] + bv;
+                if (relu) r = r > 0.0f ? r : 0.0f;
+                if (Y) {
+                    u16 *p = Y + v * ldy + offy + co;
+                    if (accumulate) r += bf2f(*p);
+                    *p = f2bf(r);
+                }
+                if (Y32) Y32[v * ld32 + co] = r;
             }
-            if (Y32) Y32[v * ld32 + co] = r;
-        }
     }
 }
+
+// voxel tiles per wave of the forward kernel: weight-fragment reuse bounded
+// by the accumulator registers (MT x NT x 4)
+constexpr int fwd_mt(int nt) { return nt <= 4 ? 4 : (nt <= 8 ? 2 : 1); }
 
 // dW partial over a chunk of voxels: P[chunk][co][k] = sum_v dY[v][co] X[v+tap(k)][ci(k)]
 // one wave per (chunk, k tile of 16), all co tiles
@@ -408,17 +431,19 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
     LGD_REQUIRE(((uintptr_t)w & 15) == 0, "w must be 16-byte aligned");
     const int Kp = lgdhip_conv3d_kpad(cin, ksize);
     const int nt = (cout + 15) / 16;
-    const int64_t waves = ((int64_t)n * n * n + 15) / 16;
-    const unsigned grid = (unsigned)((waves + 3) / 4);
     hipStream_t s = lgdhip::as_stream(stream);
     const u16 *X = (const u16 *)x;
     const u16 *Wb = (const u16 *)w;
     u16 *Y = (u16 *)y;
+    const int64_t nvox = (int64_t)n * n * n;
 #define LGD_CONV_CASE(T)                                                                                    \
-    case T:                                                                                                 \
-        conv3d_fwd_mfma<T><<<grid, 256, 0, s>>>(X, n, ldx, offx, cin, ksize, Wb, Kp, bias, cout, Y, ldy,    \
-                                                offy, y32, ld32, relu, accumulate);                         \
-        break;
+    case T: {                                                                                               \
+        constexpr int MT = fwd_mt(T);                                                                       \
+        const int64_t waves = (nvox + 16 * MT - 1) / (16 * MT);                                             \
+        conv3d_fwd_mfma<T, MT><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(                                 \
+            X, n, ldx, offx, cin, ksize, Wb, Kp, bias, cout, Y, ldy, offy, y32, ld32, relu, accumulate);    \
+        break;                                                                                              \
+    }
     switch (nt) {
         LGD_CONV_CASE(1)
         LGD_CONV_CASE(2)
