@@ -1,9 +1,10 @@
 // CoordBox<DIM>: axis-aligned box (origin, dimensions) with an x-fastest
-// iterator (src/libgeodecomp/geometry/coordbox.h).
+// iterator and a row (streak) iterator (src/libgeodecomp/geometry/coordbox.h).
 #ifndef LIBGEODECOMP_GEOMETRY_COORDBOX_H
 #define LIBGEODECOMP_GEOMETRY_COORDBOX_H
 
 #include <libgeodecomp/geometry/coord.h>
+#include <libgeodecomp/geometry/streak.h>
 
 namespace LibGeoDecomp {
 
@@ -80,6 +81,52 @@ public:
             e[DIM - 1] = origin[DIM - 1] + dimensions[DIM - 1];
         }
         return Iterator(origin, e, dimensions);
+    }
+
+    // the box's rows along x, y fastest then z: one Streak per row
+    class StreakIterator
+    {
+    public:
+        StreakIterator(const CoordBox& box, const Coord<DIM>& start) : box(box), cursor(start) {}
+
+        Streak<DIM> operator*() const { return Streak<DIM>(cursor, cursor.x() + box.dimensions.x()); }
+        bool operator==(const StreakIterator& o) const { return cursor == o.cursor; }
+        bool operator!=(const StreakIterator& o) const { return !(cursor == o.cursor); }
+
+        StreakIterator& operator++()
+        {
+            for (int d = 1; d < DIM; ++d) {
+                if (++cursor[d] < box.origin[d] + box.dimensions[d]) {
+                    return *this;
+                }
+                if (d < DIM - 1) {
+                    cursor[d] = box.origin[d];
+                }
+            }
+            if (DIM == 1) {
+                cursor[0] = box.origin[0] + box.dimensions[0];
+            }
+            return *this;
+        }
+
+    private:
+        CoordBox box;
+        Coord<DIM> cursor;
+    };
+
+    StreakIterator beginStreak() const { return size() == 0 ? endStreak() : StreakIterator(*this, origin); }
+
+    StreakIterator endStreak() const
+    {
+        Coord<DIM> e = origin;
+        if (size() != 0) {
+            if (DIM == 1) {
+                e[0] = origin[0] + dimensions[0];
+            } else {
+                e[DIM - 1] = origin[DIM - 1] + dimensions[DIM - 1];
+            }
+        }
+        return StreakIterator(*this, e);
     }
 };
 

@@ -8,10 +8,14 @@
 #include "cells.h"
 #include "harness.h"
 
+#include <libgeodecomp/io/serialbovwriter.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
 
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
 #include <memory>
 
 template<typename CELL>
@@ -200,6 +204,44 @@ LGD_TEST(jacobi_bit_exact)
     CHECK(eh == er);
     CHECK_EQ(sh.grids.size(), (std::size_t)(1 + 4 + 1));
     CHECK(sh.grids == sr.grids);
+}
+
+// the BOV snapshots of a HipSimulator run equal the oracle simulator's byte
+// for byte (headers up to the file-name prefix)
+static std::string slurp(const std::string& name)
+{
+    std::ifstream in(name.c_str(), std::ios::binary);
+    std::stringstream buf;
+    buf << in.rdbuf();
+    return buf.str();
+}
+
+LGD_TEST(jacobi_bov_snapshots)
+{
+    {
+        HipSimulator<JacobiCell> hip(makeJacobi());
+        hip.addWriter(new SerialBOVWriter<JacobiCell>(Selector<JacobiCell>(&JacobiCell::temp, "temp"), "hipbov", 7));
+        hip.run();
+        SerialSimulator<JacobiCell> ref(makeJacobi());
+        ref.addWriter(new SerialBOVWriter<JacobiCell>(Selector<JacobiCell>(&JacobiCell::temp, "temp"), "refbov", 7));
+        ref.run();
+    }
+    for (unsigned step : {0u, 7u, 14u, 21u, 28u, 30u}) {
+        char a[64], b[64];
+        std::snprintf(a, sizeof(a), "hipbov.%05u", step);
+        std::snprintf(b, sizeof(b), "refbov.%05u", step);
+        const std::string da = slurp(std::string(a) + ".data"), db = slurp(std::string(b) + ".data");
+        CHECK(!da.empty());
+        CHECK(da == db);
+        std::string ha = slurp(std::string(a) + ".bov"), hb = slurp(std::string(b) + ".bov");
+        CHECK(!ha.empty());
+        ha.replace(ha.find("hipbov"), 6, "refbov");
+        CHECK(ha == hb);
+        for (const char *stem : {a, b}) {
+            std::remove((std::string(stem) + ".bov").c_str());
+            std::remove((std::string(stem) + ".data").c_str());
+        }
+    }
 }
 
 static void heatCorner(GridBase<JacobiCell, 3> *grid, unsigned, Steerer<JacobiCell>::SteererFeedback *, void *)

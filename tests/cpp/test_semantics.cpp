@@ -5,6 +5,12 @@
 #include "harness.h"
 
 #include <libgeodecomp/geometry/region.h>
+#include <libgeodecomp/io/serialbovwriter.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
 #include <serialsimulator.h>
 
 // A cell that counts its updates and checks that every neighbour inside the
@@ -230,6 +236,85 @@ LGD_TEST(serial_life_glider)
     for (const auto& c : glider) {
         CHECK(sim.getGrid()->get(Coord<2>(c[0] + 10, c[1] + 10)).alive);
     }
+}
+
+// ---------------------------------------------------------------------
+// SerialBOVWriter (io/test/unit/serialbovwritertest.h testBasic: period 4
+// over 21 steps gives snapshots 0, 4, ..., 20, 21 with this header)
+// ---------------------------------------------------------------------
+static std::string slurp(const std::string& name)
+{
+    std::ifstream in(name.c_str(), std::ios::binary);
+    std::stringstream buf;
+    buf << in.rdbuf();
+    return buf.str();
+}
+
+LGD_TEST(bov_writer_files)
+{
+    const Coord<3> dim(13, 12, 11);
+    SerialSimulator<JacobiCell> sim(new JacobiInitializer(dim, 21, 5, 0.0));
+    sim.addWriter(new SerialBOVWriter<JacobiCell>(Selector<JacobiCell>(&JacobiCell::temp, "val"), "lgdbovtest", 4));
+    Grid<JacobiCell, Topologies::Cube<3>::Topology> initial(dim);
+    JacobiInitializer(dim, 21, 5, 0.0).grid(&initial);
+    sim.run();
+    const unsigned steps[] = {0, 4, 8, 12, 16, 20, 21};
+    for (unsigned step : steps) {
+        char stem[64];
+        std::snprintf(stem, sizeof(stem), "lgdbovtest.%05u", step);
+        std::ostringstream exp;
+        exp << "TIME: " << step << "\n"
+            << "DATA_FILE: " << stem << ".data\n"
+            << "DATA_SIZE: 13 12 11\n"
+            << "DATA_FORMAT: DOUBLE\n"
+            << "VARIABLE: val\n"
+            << "DATA_ENDIAN: LITTLE\n"
+            << "BRICK_ORIGIN: 0 0 0\n"
+            << "BRICK_SIZE: 13 12 11\n"
+            << "DIVIDE_BRICK: true\n"
+            << "DATA_BRICKLETS: 13 12 11\n"
+            << "DATA_COMPONENTS: 1\n";
+        CHECK_EQ(slurp(std::string(stem) + ".bov"), exp.str());
+        const std::string data = slurp(std::string(stem) + ".data");
+        CHECK_EQ(data.size(), (std::size_t)dim.prod() * sizeof(double));
+        if (step == 0 || step == 21) {
+            const GridBase<JacobiCell, 3>& g =
+                step == 0 ? static_cast<const GridBase<JacobiCell, 3>&>(initial) : *sim.getGrid();
+            bool same = true;
+            CoordBox<3> box(Coord<3>(), dim);
+            for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+                double v;
+                std::memcpy(&v, data.data() + i->toIndex(dim) * sizeof(double), sizeof(double));
+                same = same && v == g.get(*i).temp;
+            }
+            CHECK(same);
+        }
+        std::remove((std::string(stem) + ".bov").c_str());
+        std::remove((std::string(stem) + ".data").c_str());
+    }
+    // no snapshot off the period
+    CHECK(slurp("lgdbovtest.00005.bov").empty());
+}
+
+LGD_TEST(bov_writer_2d_byte_and_type_names)
+{
+    SerialSimulator<ConwayCell> sim(new ExampleLifeInitializer(3));
+    sim.addWriter(new SerialBOVWriter<ConwayCell>(&ConwayCell::alive, "lgdbovlife", 3));
+    sim.run();
+    const std::string h = slurp("lgdbovlife.00003.bov");
+    CHECK(h.find("DATA_SIZE: 160 90 1\n") != std::string::npos);
+    CHECK(h.find("DATA_FORMAT: BYTE\n") != std::string::npos);
+    CHECK(h.find("VARIABLE: lgdbovlife\n") != std::string::npos);
+    CHECK_EQ(slurp("lgdbovlife.00003.data").size(), (std::size_t)(160 * 90));
+    for (unsigned step : {0u, 3u}) {
+        char stem[64];
+        std::snprintf(stem, sizeof(stem), "lgdbovlife.%05u", step);
+        std::remove((std::string(stem) + ".bov").c_str());
+        std::remove((std::string(stem) + ".data").c_str());
+    }
+    struct Odd { unsigned short u; };
+    CHECK_THROWS(Selector<Odd>(&Odd::u, "u").typeName(), std::invalid_argument);
+    CHECK_EQ(Selector<Odd>(&Odd::u, "u").sizeOfExternal(), sizeof(unsigned short));
 }
 
 int main(int argc, char **argv)

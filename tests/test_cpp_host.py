@@ -13,19 +13,21 @@ import pytest
 BIN = Path(__file__).resolve().parent / "cpp" / "bin"
 
 
-def run(name, timeout):
+def run(name, timeout, cwd):
+    """Runs one program in `cwd` (the writer tests create files there)."""
     exe = BIN / name
     assert exe.exists(), f"{exe} missing: build with __graft_entry__.build()"
-    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout)
+    r = subprocess.run([str(exe)], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=timeout,
+                       cwd=str(cwd))
     print(r.stdout)
     assert r.returncode == 0, r.stdout[-4000:]
     assert "0 failures" in r.stdout
 
 
-def test_cpp_semantics():
-    run("test_semantics", 300)
+def test_cpp_semantics(tmp_path):
+    run("test_semantics", 300, tmp_path)
 
 
 @pytest.mark.gpu
-def test_cpp_hipsimulator():
-    run("test_hipsimulator", 600)
+def test_cpp_hipsimulator(tmp_path):
+    run("test_hipsimulator", 600, tmp_path)
