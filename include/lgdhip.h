@@ -144,6 +144,14 @@ int64_t lgdhip_grid_bytes(const lgdhip_grid *g);
  * cudasimulator.h:508-513 and cudasoagrid.h:16-57. */
 int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge,
                           lgdhip_stream_t stream);
+/* Refresh every halo layer for a (partly) periodic topology: a halo cell
+ * takes the interior value at its coordinate wrapped along the axes in
+ * wrap_mask (1 x, 2 y, 4 z) or, if it is still outside the box along an
+ * unwrapped axis, `edge` (members*elem_bytes bytes) -- the neighbourhood
+ * rule of the Torus topologies (geometry/topologies.h normalize /
+ * isOutOfBounds).  Call before each single sweep of a periodic grid. */
+int lgdhip_grid_fill_halo(void *grid, const lgdhip_grid *g, const void *edge, int wrap_mask,
+                          lgdhip_stream_t stream);
 /* Copy the logical box between a dense host array (member-major, x
  * fastest, nx*ny*nz elements per member) and a device grid. */
 int lgdhip_grid_load(void *grid, const lgdhip_grid *g, const void *host,

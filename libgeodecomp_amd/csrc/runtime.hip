@@ -95,6 +95,31 @@ __global__ void lgd_fill_edge_kernel(T *__restrict__ plane, T edge, int64_t sx, 
     }
 }
 
+// Refreshes the halo layers of one member plane for a (partly) periodic
+// topology: a halo cell takes the interior value at its coordinate wrapped
+// along the axes set in `wrap` (bit 0 x, 1 y, 2 z) or, when it is still
+// outside the box along an unwrapped axis, the edge value -- the rule of
+// Topology::normalize + isOutOfBounds.  Reads only interior cells, writes
+// only halo cells.
+template <typename T>
+__global__ void lgd_fill_halo_kernel(T *__restrict__ plane, T edge, int64_t sx, int64_t sy, int64_t ox, int64_t oy,
+                                     int64_t oz, int64_t nx, int64_t ny, int64_t nz, int h, int dim, int wrap) {
+    const int64_t ex = nx + 2 * h, ey = ny + 2 * h, ez = dim == 3 ? nz + 2 * h : 1;
+    const int64_t total = ex * ey * ez;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t x = i % ex - h, y = (i / ex) % ey - h, z = dim == 3 ? i / (ex * ey) - h : 0;
+        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) continue;
+        int64_t wx = x, wy = y, wz = z;
+        if (wrap & 1) wx = ((x % nx) + nx) % nx;
+        if (wrap & 2) wy = ((y % ny) + ny) % ny;
+        if ((wrap & 4) && dim == 3) wz = ((z % nz) + nz) % nz;
+        const bool inside = wx >= 0 && wx < nx && wy >= 0 && wy < ny && wz >= 0 && wz < nz;
+        const T v = inside ? plane[((wz + oz) * sy + (wy + oy)) * sx + (wx + ox)] : edge;
+        plane[((z + oz) * sy + (y + oy)) * sx + (x + ox)] = v;
+    }
+}
+
 static unsigned fill_blocks(size_t n) {
     size_t b = (n + 255) / 256;
     if (b > 65536) b = 65536;
@@ -357,6 +382,41 @@ int64_t lgdhip_grid_plane_elems(const lgdhip_grid *g) { return g ? plane_elems(*
 
 int64_t lgdhip_grid_bytes(const lgdhip_grid *g) {
     return g ? plane_elems(*g) * g->members * g->elem_bytes : 0;
+}
+
+int lgdhip_grid_fill_halo(void *grid, const lgdhip_grid *g, const void *edge, int wrap_mask,
+                          lgdhip_stream_t stream) {
+    int rc = check_grid(g, 0, 0, 0);
+    if (rc) return rc;
+    LGD_REQUIRE(grid && edge, "NULL pointer");
+    LGD_REQUIRE(wrap_mask >= 0 && wrap_mask <= 7, "wrap_mask must be a combination of bits 1, 2, 4");
+    const int64_t n = plane_elems(*g);
+    const int h = g->halo;
+    const int64_t shell = (g->nx + 2 * h) * (g->ny + 2 * h) * (g->dim == 3 ? g->nz + 2 * h : 1);
+    const unsigned blocks = fill_blocks((size_t)shell);
+    hipStream_t s = as_stream(stream);
+    for (int m = 0; m < g->members; ++m) {
+        const uint8_t *ev = (const uint8_t *)edge + (size_t)m * g->elem_bytes;
+        char *plane = (char *)grid + (size_t)m * n * g->elem_bytes;
+#define LGD_HALO_CASE(BYTES, T)                                                                                   \
+    case BYTES: {                                                                                                 \
+        T v;                                                                                                      \
+        std::memcpy(&v, ev, BYTES);                                                                               \
+        lgd_fill_halo_kernel<T><<<blocks, 256, 0, s>>>((T *)plane, v, g->sx, g->sy, g->ox, g->oy, g->oz, g->nx,  \
+                                                       g->ny, g->nz, h, g->dim, wrap_mask);                       \
+        break;                                                                                                    \
+    }
+        switch (g->elem_bytes) {
+            LGD_HALO_CASE(1, uint8_t)
+            LGD_HALO_CASE(2, uint16_t)
+            LGD_HALO_CASE(4, uint32_t)
+            LGD_HALO_CASE(8, uint64_t)
+        default: return set_error(LGDHIP_ERR_UNSUPPORTED, "element size must be 1, 2, 4 or 8 bytes");
+        }
+#undef LGD_HALO_CASE
+        LGD_LAUNCHED("lgdhip_grid_fill_halo");
+    }
+    return 0;
 }
 
 int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge, lgdhip_stream_t stream) {

@@ -109,6 +109,7 @@ SIGNATURES = {
     "lgdhip_grid_plane_elems": (_I64, [_GP]),
     "lgdhip_grid_bytes": (_I64, [_GP]),
     "lgdhip_grid_fill_edge": (_I, [_P, _GP, _P, _P]),
+    "lgdhip_grid_fill_halo": (_I, [_P, _GP, _P, _I, _P]),
     "lgdhip_grid_load": (_I, [_P, _GP, _P, _P]),
     "lgdhip_grid_store": (_I, [_P, _GP, _P, _P]),
     "lgdhip_jacobi3d_f64_run": (_I, [_P, _P, _GP, _I, _TP, _IP, _P]),

@@ -93,6 +93,26 @@ struct Tuning
     const lgdhip_tile *get() const { return useTile ? &tile : 0; }
 };
 
+// wrapped axes of the cell's topology as a mask (bit d = axis d)
+template<typename CELL>
+int wrapMask()
+{
+    typedef typename APITraits::SelectTopology<CELL>::Value Topology;
+    int mask = 0;
+    for (int d = 0; d < Topology::DIM; ++d) {
+        if (Topology::wrapsAxis(d)) {
+            mask |= 1 << d;
+        }
+    }
+    return mask;
+}
+
+template<typename CELL>
+void requireSingleNanoStep()
+{
+    static_assert(APITraits::SelectNanoSteps<CELL>::VALUE == 1, "kernel family supports one nano step per step");
+}
+
 template<typename CELL>
 void requireCubeSingleNanoStep()
 {
@@ -117,10 +137,14 @@ public:
     static const int DIM = 3;
     static_assert(Topology::DIM == 3, "Jacobi3DF64 is a 3D family");
 
+    // Cube: the tuned (temporally blocked) sweeps.  Wrapped axes (Torus
+    // topologies): single sweeps, each after a periodic halo refresh.
     Jacobi3DF64(const Coord<3>& dim, const Tuning& tuning) :
-        tuning(tuning)
+        tuning(tuning),
+        wrap(wrapMask<CELL>()),
+        edge(0.0)
     {
-        requireCubeSingleNanoStep<CELL>();
+        requireSingleNanoStep<CELL>();
         check(lgdhip_grid_make(&g, 3, dim.x(), dim.y(), dim.z(), 1, 8, 1), "lgdhip_grid_make");
         const std::size_t n = (std::size_t)lgdhip_grid_bytes(&g);
         a.reset(new DeviceMemory(n));
@@ -133,7 +157,7 @@ public:
         for (std::size_t i = 0; i < dense.size(); ++i) {
             dense[i] = host.data()[i].*MEMBER;
         }
-        const double edge = host.getEdge().*MEMBER;
+        edge = host.getEdge().*MEMBER;
         check(lgdhip_grid_fill_edge(a->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_fill_edge(b->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_load(a->get(), &g, dense.data(), stream.get()), "lgdhip_grid_load");
@@ -150,6 +174,20 @@ public:
 
     void run(unsigned nanoSteps)
     {
+        if (wrap) {
+            lgdhip_tile single;
+            std::memset(&single, 0, sizeof(single));
+            single.variant = 2;  // a single sweep that reads its halo from memory
+            for (unsigned t = 0; t < nanoSteps; ++t) {
+                int inB = 0;
+                check(lgdhip_grid_fill_halo(a->get(), &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
+                check(lgdhip_jacobi3d_f64_run((double *)a->get(), (double *)b->get(), &g, 1, &single, &inB,
+                                              stream.get()),
+                      "lgdhip_jacobi3d_f64_run");
+                a.swap(b);
+            }
+            return;
+        }
         int inB = 0;
         check(lgdhip_jacobi3d_f64_run((double *)a->get(), (double *)b->get(), &g, (int)nanoSteps, tuning.get(), &inB,
                                       stream.get()),
@@ -167,6 +205,8 @@ private:
     Stream stream;
     lgdhip_grid g;
     std::unique_ptr<DeviceMemory> a, b;
+    int wrap;
+    double edge;
 };
 
 // ---------------------------------------------------------------------
@@ -181,13 +221,20 @@ public:
     static const int DIM = 2;
     static_assert(Topology::DIM == 2, "Life2D is a 2D family");
 
+    // Cube: bit-packed temporally blocked sweeps.  Wrapped axes (Torus
+    // topologies): byte-grid single generations, each after a periodic halo
+    // refresh (the state then stays in the byte grids).
     Life2D(const Coord<2>& dim, const Tuning& tuning) :
         tuning(tuning),
-        edgeAlive(0)
+        edgeAlive(0),
+        wrap(wrapMask<CELL>())
     {
-        requireCubeSingleNanoStep<CELL>();
+        requireSingleNanoStep<CELL>();
         check(lgdhip_grid_make(&g, 2, dim.x(), dim.y(), 1, 1, 1, 1), "lgdhip_grid_make");
         grid.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));
+        if (wrap) {
+            grid2.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));
+        }
         const std::size_t ws = lgdhip_life2d_workspace_bytes(&g);
         packed.reset(new DeviceMemory(ws));
         pa = (uint32_t *)packed->get();
@@ -204,12 +251,18 @@ public:
         const uint8_t edge = (uint8_t)edgeAlive;
         check(lgdhip_grid_fill_edge(grid->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_load(grid->get(), &g, dense.data(), stream.get()), "lgdhip_grid_load");
+        if (wrap) {
+            check(lgdhip_grid_fill_edge(grid2->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
+            return;
+        }
         check(lgdhip_life2d_pack((const uint8_t *)grid->get(), &g, pa, stream.get()), "lgdhip_life2d_pack");
     }
 
     void download(Grid<CELL, Topology> *host)
     {
-        check(lgdhip_life2d_unpack(pa, &g, (uint8_t *)grid->get(), stream.get()), "lgdhip_life2d_unpack");
+        if (!wrap) {
+            check(lgdhip_life2d_unpack(pa, &g, (uint8_t *)grid->get(), stream.get()), "lgdhip_life2d_unpack");
+        }
         std::vector<uint8_t> dense(host->size());
         check(lgdhip_grid_store(grid->get(), &g, dense.data(), stream.get()), "lgdhip_grid_store");
         for (std::size_t i = 0; i < dense.size(); ++i) {
@@ -219,6 +272,18 @@ public:
 
     void run(unsigned nanoSteps)
     {
+        if (wrap) {
+            const uint8_t edge = (uint8_t)edgeAlive;
+            for (unsigned t = 0; t < nanoSteps; ++t) {
+                int inB = 0;
+                check(lgdhip_grid_fill_halo(grid->get(), &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
+                check(lgdhip_life2d_u8_run((uint8_t *)grid->get(), (uint8_t *)grid2->get(), &g, 1, &inB,
+                                           stream.get()),
+                      "lgdhip_life2d_u8_run");
+                grid.swap(grid2);
+            }
+            return;
+        }
         int inB = 0;
         check(lgdhip_life2d_packed_run(pa, pb, &g, edgeAlive, (int)nanoSteps, tuning.get(), &inB, stream.get()),
               "lgdhip_life2d_packed_run");
@@ -233,9 +298,10 @@ private:
     Tuning tuning;
     Stream stream;
     lgdhip_grid g;
-    std::unique_ptr<DeviceMemory> grid, packed;
+    std::unique_ptr<DeviceMemory> grid, grid2, packed;
     uint32_t *pa, *pb;
     int edgeAlive;
+    int wrap;
 };
 
 // ---------------------------------------------------------------------

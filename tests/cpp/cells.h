@@ -189,6 +189,103 @@ private:
 };
 
 // ---------------------------------------------------------------------
+// Periodic variants: the same updates on Torus topologies and on a
+// topology that wraps only some axes (reference topologies.h: wrapped axes
+// normalise, the others fall back to the edge cell)
+// ---------------------------------------------------------------------
+template<typename TOPOLOGY_API>
+class LifeCellOn
+{
+public:
+    class API : public TOPOLOGY_API
+    {};
+
+    explicit LifeCellOn(bool alive = false) : alive(alive) {}
+
+    template<typename COORD_MAP>
+    void update(const COORD_MAP& hood, unsigned)
+    {
+        int living = 0;
+        for (int y = -1; y < 2; ++y) {
+            for (int x = -1; x < 2; ++x) {
+                living += hood[Coord<2>(x, y)].alive;
+            }
+        }
+        living -= hood[Coord<2>(0, 0)].alive;
+        alive = hood[Coord<2>(0, 0)].alive ? (living == 2 || living == 3) : (living == 3);
+    }
+
+    bool operator==(const LifeCellOn& o) const { return alive == o.alive; }
+
+    bool alive;
+};
+
+template<typename TOPOLOGY_API>
+class JacobiCellOn
+{
+public:
+    class API : public APITraits::HasFixedCoordsOnlyUpdate,
+                public APITraits::HasStencil<Stencils::VonNeumann<3, 1> >,
+                public TOPOLOGY_API
+    {};
+
+    explicit JacobiCellOn(double t = 0) : temp(t) {}
+
+    template<typename NEIGHBORHOOD>
+    void update(const NEIGHBORHOOD& hood, int)
+    {
+        temp = (hood[FixedCoord<0, 0, -1>()].temp + hood[FixedCoord<0, -1, 0>()].temp +
+                hood[FixedCoord<-1, 0, 0>()].temp + hood[FixedCoord<0, 0, 0>()].temp +
+                hood[FixedCoord<1, 0, 0>()].temp + hood[FixedCoord<0, 1, 0>()].temp +
+                hood[FixedCoord<0, 0, 1>()].temp) * (1.0 / 7.0);
+    }
+
+    bool operator==(const JacobiCellOn& o) const { return temp == o.temp; }
+
+    double temp;
+};
+
+typedef LifeCellOn<APITraits::HasTorusTopology<2> > TorusLifeCell;
+typedef LifeCellOn<APITraits::HasTopology<TopologiesHelpers::Topology<2, true, false, false> > > CylinderLifeCell;
+typedef JacobiCellOn<APITraits::HasTorusTopology<3> > TorusJacobiCell;
+typedef JacobiCellOn<APITraits::HasTopology<TopologiesHelpers::Topology<3, true, false, true> > > SlabJacobiCell;
+
+#ifdef LGD_WITH_HIP
+LIBGEODECOMP_REGISTER_HIP_KERNEL(TorusLifeCell, HipKernels::Life2D<TorusLifeCell, bool, &TorusLifeCell::alive>)
+LIBGEODECOMP_REGISTER_HIP_KERNEL(CylinderLifeCell,
+                                 HipKernels::Life2D<CylinderLifeCell, bool, &CylinderLifeCell::alive>)
+LIBGEODECOMP_REGISTER_HIP_KERNEL(TorusJacobiCell,
+                                 HipKernels::Jacobi3DF64<TorusJacobiCell, double, &TorusJacobiCell::temp>)
+LIBGEODECOMP_REGISTER_HIP_KERNEL(SlabJacobiCell,
+                                 HipKernels::Jacobi3DF64<SlabJacobiCell, double, &SlabJacobiCell::temp>)
+#endif
+
+// random cells of a one-member cell type (value from a SplitMix64 stream)
+template<typename CELL, int DIM>
+class RandomCellInitializer : public SimpleInitializer<CELL>
+{
+public:
+    typedef CELL (*Make)(double u);
+    RandomCellInitializer(Coord<DIM> dim, unsigned steps, uint64_t seed, Make make, CELL edge) :
+        SimpleInitializer<CELL>(dim, steps), seed(seed), make(make), edge(edge)
+    {}
+
+    virtual void grid(GridBase<CELL, DIM> *ret)
+    {
+        ret->setEdge(edge);
+        CoordBox<DIM> box = ret->boundingBox();
+        for (typename CoordBox<DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+            ret->set(*i, make(uniform01(seed, i->toIndex(this->dimensions))));
+        }
+    }
+
+private:
+    uint64_t seed;
+    Make make;
+    CELL edge;
+};
+
+// ---------------------------------------------------------------------
 // D3Q19 BGK lid-driven cavity cell (structure of
 // src/examples/latticeboltzmann/main.cpp:16-237; walls as per-link
 // bounce-back from the cell's wall flags, +y lid moving with lidVelocity)

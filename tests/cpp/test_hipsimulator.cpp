@@ -206,6 +206,69 @@ LGD_TEST(jacobi_bit_exact)
     CHECK(sh.grids == sr.grids);
 }
 
+// ---------------------------------------------------------------------
+// periodic topologies (Torus, and axes wrapped selectively): single sweeps
+// after a periodic halo refresh, identical to the oracle at every writer
+// callback
+// ---------------------------------------------------------------------
+template<typename CELL> CELL makeLife(double u) { return CELL(u < 0.3); }
+template<typename CELL> CELL makeTemp(double u) { return CELL(u); }
+
+static RandomCellInitializer<TorusLifeCell, 2> *makeTorusLife()
+{
+    return new RandomCellInitializer<TorusLifeCell, 2>(Coord<2>(131, 77), 40, 11, &makeLife<TorusLifeCell>,
+                                                       TorusLifeCell(true));
+}
+static RandomCellInitializer<CylinderLifeCell, 2> *makeCylinderLife()
+{
+    return new RandomCellInitializer<CylinderLifeCell, 2>(Coord<2>(96, 45), 33, 12, &makeLife<CylinderLifeCell>,
+                                                          CylinderLifeCell(true));
+}
+static RandomCellInitializer<TorusJacobiCell, 3> *makeTorusJacobi()
+{
+    return new RandomCellInitializer<TorusJacobiCell, 3>(Coord<3>(37, 22, 15), 12, 13, &makeTemp<TorusJacobiCell>,
+                                                         TorusJacobiCell(0.5));
+}
+static RandomCellInitializer<SlabJacobiCell, 3> *makeSlabJacobi()
+{
+    return new RandomCellInitializer<SlabJacobiCell, 3>(Coord<3>(30, 17, 21), 9, 14, &makeTemp<SlabJacobiCell>,
+                                                        SlabJacobiCell(0.25));
+}
+
+LGD_TEST(life_torus)
+{
+    std::vector<Event> eh, er;
+    Snapshots<TorusLifeCell> sh, sr;
+    runBoth<TorusLifeCell>(&makeTorusLife, 6, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
+LGD_TEST(life_wrapped_x_only)
+{
+    std::vector<Event> eh, er;
+    Snapshots<CylinderLifeCell> sh, sr;
+    runBoth<CylinderLifeCell>(&makeCylinderLife, 5, &eh, &er, &sh, &sr);
+    CHECK(sh.grids == sr.grids);
+}
+
+LGD_TEST(jacobi_torus)
+{
+    std::vector<Event> eh, er;
+    Snapshots<TorusJacobiCell> sh, sr;
+    runBoth<TorusJacobiCell>(&makeTorusJacobi, 4, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
+LGD_TEST(jacobi_wrapped_x_and_z)
+{
+    std::vector<Event> eh, er;
+    Snapshots<SlabJacobiCell> sh, sr;
+    runBoth<SlabJacobiCell>(&makeSlabJacobi, 3, &eh, &er, &sh, &sr);
+    CHECK(sh.grids == sr.grids);
+}
+
 // the BOV snapshots of a HipSimulator run equal the oracle simulator's byte
 // for byte (headers up to the file-name prefix)
 static std::string slurp(const std::string& name)

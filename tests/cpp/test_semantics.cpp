@@ -317,6 +317,43 @@ LGD_TEST(bov_writer_2d_byte_and_type_names)
     CHECK_EQ(Selector<Odd>(&Odd::u, "u").sizeOfExternal(), sizeof(unsigned short));
 }
 
+// Torus topology in the oracle: a glider moves (1, 1) every 4 generations,
+// so on a 16 x 16 torus it is back in place after 64 (and has crossed both
+// seams); on the Cube it would have died against the edge
+LGD_TEST(oracle_torus_glider_returns)
+{
+    class GliderInit : public SimpleInitializer<TorusLifeCell>
+    {
+    public:
+        explicit GliderInit(unsigned steps) : SimpleInitializer<TorusLifeCell>(Coord<2>(16, 16), steps) {}
+        virtual void grid(GridBase<TorusLifeCell, 2> *ret)
+        {
+            const int g[5][2] = {{13, 12}, {14, 13}, {12, 14}, {13, 14}, {14, 14}};
+            for (auto& c : g) {
+                ret->set(Coord<2>(c[0], c[1]), TorusLifeCell(true));
+            }
+        }
+    };
+    SerialSimulator<TorusLifeCell> at64(new GliderInit(64)), at0(new GliderInit(0)), at32(new GliderInit(32));
+    at64.run();
+    at0.run();
+    at32.run();
+    CHECK(*at64.getGrid() == *at0.getGrid());
+    // halfway: moved by (8, 8), still five cells
+    int alive = 0;
+    bool moved = true;
+    const int g[5][2] = {{13, 12}, {14, 13}, {12, 14}, {13, 14}, {14, 14}};
+    for (auto& c : g) {
+        moved = moved && at32.getGrid()->get(Coord<2>((c[0] + 8) % 16, (c[1] + 8) % 16)).alive;
+    }
+    CoordBox<2> box(Coord<2>(), Coord<2>(16, 16));
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        alive += at32.getGrid()->get(*i).alive;
+    }
+    CHECK(moved);
+    CHECK_EQ(alive, 5);
+}
+
 int main(int argc, char **argv)
 {
     return lgdtest::runAll(argc, argv);
