@@ -460,7 +460,7 @@ class Learned:
     dtype = "bf16"   # the network (BF16 operands, FP32 accumulation); polishing is FP32/FP64
     bytes_per_lup = 12.0
 
-    def __init__(self, n=128, steps=3000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, **_):
+    def __init__(self, n=128, steps=9000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, **_):
         from libgeodecomp_amd import learned_guess as lgm
         self.lgm = lgm
         self.n, self.train_steps, self.seed, self.ntrain, self.neval = n, steps, seed, train, eval

@@ -26,7 +26,7 @@ learned guess, so the network's own contribution is visible.
 No reference counterpart.  Everything on the device is a liblgdhip kernel;
 this module only orchestrates (and times) them.
 
-    python -m libgeodecomp_amd.learned_guess --n 128 --train 384 --eval 16 --steps 3000
+    python -m libgeodecomp_amd.learned_guess --n 128 --train 512 --eval 16 --steps 9000
 """
 from __future__ import annotations
 
@@ -300,9 +300,9 @@ def evaluate(model: LearnedGuess, data: Ensemble, coarse_m: int = 0, rel_tol: fl
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--train", type=int, default=384, help="solved training problems")
+    ap.add_argument("--train", type=int, default=512, help="solved training problems")
     ap.add_argument("--eval", type=int, default=16, help="held-out problems")
-    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps", type=int, default=9000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--tol", type=float, default=1e-6)

@@ -3,6 +3,8 @@
 #ifndef LIBGEODECOMP_GEOMETRY_FIXEDCOORD_H
 #define LIBGEODECOMP_GEOMETRY_FIXEDCOORD_H
 
+#include <libgeodecomp/misc/hostdevice.h>
+
 namespace LibGeoDecomp {
 
 template<int DIM_X = 0, int DIM_Y = 0, int DIM_Z = 0>
@@ -13,7 +15,7 @@ public:
     static const int Y = DIM_Y;
     static const int Z = DIM_Z;
 
-    inline FixedCoord() {}
+    __host__ __device__ inline FixedCoord() {}
 };
 
 }

@@ -25,14 +25,14 @@ public:
         static const bool VALUE = (D == 0) ? W0 : ((D == 1) ? W1 : W2);
     };
 
-    static bool wrapsAxis(int d)
+    __host__ __device__ static bool wrapsAxis(int d)
     {
         return d == 0 ? W0 : (d == 1 ? W1 : W2);
     }
 
     // coordinates on wrapping axes are folded into [0, dim); a coordinate
     // outside a non-wrapping axis is out of bounds
-    static bool isOutOfBounds(const Coord<DIM>& c, const Coord<DIM>& dim)
+    __host__ __device__ static bool isOutOfBounds(const Coord<DIM>& c, const Coord<DIM>& dim)
     {
         for (int d = 0; d < DIM; ++d) {
             if (!wrapsAxis(d) && (c[d] < 0 || c[d] >= dim[d])) {
@@ -42,7 +42,7 @@ public:
         return false;
     }
 
-    static Coord<DIM> normalize(const Coord<DIM>& c, const Coord<DIM>& dim)
+    __host__ __device__ static Coord<DIM> normalize(const Coord<DIM>& c, const Coord<DIM>& dim)
     {
         Coord<DIM> r = c;
         for (int d = 0; d < DIM; ++d) {
