@@ -152,6 +152,39 @@ int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge,
  * isOutOfBounds).  Call before each single sweep of a periodic grid. */
 int lgdhip_grid_fill_halo(void *grid, const lgdhip_grid *g, const void *edge, int wrap_mask,
                           lgdhip_stream_t stream);
+/* ------------------------------------------------------------------ */
+/* cell grids (device array-of-structs storage of whole user cells)    */
+/* ------------------------------------------------------------------ */
+/* The storage of the generic kernel family, which runs any user cell whose
+ * update() is compiled by hipcc.  Cells of cell_bytes bytes, x fastest,
+ * padded by `halo` cells on both sides of each of the grid's dim axes:
+ *   px = nx + 2h,  py = dim >= 2 ? ny + 2h : 1,  pz = dim == 3 ? nz + 2h : 1,
+ *   cell (x, y, z) at index ((z + hz) * py + (y + hy)) * px + (x + h),
+ *   hy = dim >= 2 ? h : 0, hz = dim == 3 ? h : 0.
+ * Replaces the padded AoS device buffers of CUDASimulator
+ * (cudasimulator.h:314-429, :502-534). */
+typedef struct lgdhip_cell_grid {
+    int32_t dim, halo;
+    int64_t cell_bytes;
+    int64_t nx, ny, nz;
+    int64_t px, py, pz;
+} lgdhip_cell_grid;
+int lgdhip_cell_grid_make(lgdhip_cell_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz, int halo,
+                          int64_t cell_bytes);
+int64_t lgdhip_cell_grid_bytes(const lgdhip_cell_grid *g);
+/* Write every halo cell: the interior cell at its coordinate wrapped along
+ * the axes in wrap_mask (1 x, 2 y, 4 z), or, where the coordinate is still
+ * outside an unwrapped axis, the cell at `edge` (DEVICE memory, cell_bytes)
+ * -- the neighbourhood rule of the reference's topologies
+ * (geometry/topologies.h:185-234, replacing the per-thread wrap offsets of
+ * cudasimulator.h:52-64).  wrap_mask 0 writes the edge cell everywhere. */
+int lgdhip_cell_grid_fill_halo(void *cells, const lgdhip_cell_grid *g, const void *edge, int wrap_mask,
+                               lgdhip_stream_t stream);
+/* Copy the logical box between a dense host array (x fastest, nx*ny*nz
+ * cells) and a device cell grid; both return after the copy completed. */
+int lgdhip_cell_grid_load(void *cells, const lgdhip_cell_grid *g, const void *host, lgdhip_stream_t stream);
+int lgdhip_cell_grid_store(const void *cells, const lgdhip_cell_grid *g, void *host, lgdhip_stream_t stream);
+
 /* Copy the logical box between a dense host array (member-major, x
  * fastest, nx*ny*nz elements per member) and a device grid. */
 int lgdhip_grid_load(void *grid, const lgdhip_grid *g, const void *host,

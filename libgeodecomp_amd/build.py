@@ -34,8 +34,8 @@ HIP_FLAGS = [
     f"-I{REPO / 'include'}",
 ]
 
-SOURCES = ["runtime.hip", "jacobi3d.hip", "life2d.hip", "lbm_d3q19.hip", "poisson_batch.hip", "unet3d.hip",
-           "learned_guess.hip"]
+SOURCES = ["runtime.hip", "cellgrid.hip", "jacobi3d.hip", "life2d.hip", "lbm_d3q19.hip", "poisson_batch.hip",
+           "unet3d.hip", "learned_guess.hip"]
 
 
 def _run(cmd, cwd=None):

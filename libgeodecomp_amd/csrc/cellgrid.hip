@@ -90,7 +90,9 @@ __global__ void __launch_bounds__(256) cell_fill_halo_kernel(W *__restrict__ cel
         if (wrap_mask & 4) wz = wrap(z, s.nz);
         const bool inside = wx >= 0 && wx < s.nx && wy >= 0 && wy < s.ny && wz >= 0 && wz < s.nz;
         const int64_t dst = ((z + s.hz) * s.py + (y + s.hy)) * s.px + (x + s.h);
-        const int64_t src = ((wz + s.hz) * s.py + (wy + s.hy)) * s.px + (wx + s.h);
+        // outside: point the (unused) source at the destination, so the
+        // index is in the buffer whatever the compiler does with the select
+        const int64_t src = inside ? ((wz + s.hz) * s.py + (wy + s.hy)) * s.px + (wx + s.h) : dst;
         cells[dst * words + w] = inside ? cells[src * words + w] : edge[w];
     }
 }

@@ -35,6 +35,14 @@ class lgdhip_grid(C.Structure):
     ]
 
 
+class lgdhip_cell_grid(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("halo", C.c_int32), ("cell_bytes", C.c_int64),
+        ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
+        ("px", C.c_int64), ("py", C.c_int64), ("pz", C.c_int64),
+    ]
+
+
 class lgdhip_tile(C.Structure):
     _fields_ = [
         ("bx", C.c_int32), ("by", C.c_int32), ("bz", C.c_int32),
@@ -72,6 +80,7 @@ _I = C.c_int
 _I64 = C.c_int64
 _SZ = C.c_size_t
 _GP = C.POINTER(lgdhip_grid)
+_CGP = C.POINTER(lgdhip_cell_grid)
 _TP = C.POINTER(lgdhip_tile)
 _IP = C.POINTER(C.c_int)
 
@@ -110,6 +119,11 @@ SIGNATURES = {
     "lgdhip_grid_bytes": (_I64, [_GP]),
     "lgdhip_grid_fill_edge": (_I, [_P, _GP, _P, _P]),
     "lgdhip_grid_fill_halo": (_I, [_P, _GP, _P, _I, _P]),
+    "lgdhip_cell_grid_make": (_I, [_CGP, _I, _I64, _I64, _I64, _I, _I64]),
+    "lgdhip_cell_grid_bytes": (_I64, [_CGP]),
+    "lgdhip_cell_grid_fill_halo": (_I, [_P, _CGP, _P, _I, _P]),
+    "lgdhip_cell_grid_load": (_I, [_P, _CGP, _P, _P]),
+    "lgdhip_cell_grid_store": (_I, [_P, _CGP, _P, _P]),
     "lgdhip_grid_load": (_I, [_P, _GP, _P, _P]),
     "lgdhip_grid_store": (_I, [_P, _GP, _P, _P]),
     "lgdhip_jacobi3d_f64_run": (_I, [_P, _P, _GP, _I, _TP, _IP, _P]),

@@ -8,7 +8,10 @@
 // Added for the HIP path: HasHipKernel<FAMILY> / SelectHipKernel<CELL>,
 // naming the hand-written kernel family that implements the cell's update
 // (see libgeodecomp/hip/kernelfamilies.h).  It can also be attached after
-// the class with LIBGEODECOMP_REGISTER_HIP_KERNEL.
+// the class with LIBGEODECOMP_REGISTER_HIP_KERNEL.  Cells without a family
+// run through the generic family (libgeodecomp/hip/genericfamily.h), which
+// calls update(), or updateHIP() / updateCUDA() under
+// HasSeparateHIPUpdate / HasSeparateCUDAUpdate.
 #ifndef LIBGEODECOMP_MISC_APITRAITS_H
 #define LIBGEODECOMP_MISC_APITRAITS_H
 
@@ -88,6 +91,13 @@ public:
     class SelectSeparateCUDAUpdate<CELL, typename CELL::API::SupportsSeparateCUDAUpdate>
     { public: typedef TrueType Value; };
     class HasSeparateCUDAUpdate { public: typedef void SupportsSeparateCUDAUpdate; };
+    // the HIP spelling: the generic kernel family calls updateHIP()
+    template<typename CELL, typename HAS = void>
+    class SelectSeparateHIPUpdate { public: typedef FalseType Value; };
+    template<typename CELL>
+    class SelectSeparateHIPUpdate<CELL, typename CELL::API::SupportsSeparateHIPUpdate>
+    { public: typedef TrueType Value; };
+    class HasSeparateHIPUpdate { public: typedef void SupportsSeparateHIPUpdate; };
 
     // ---- static data block (default char)
     template<typename CELL, typename HAS = void>

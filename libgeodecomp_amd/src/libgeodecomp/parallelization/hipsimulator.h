@@ -14,7 +14,9 @@
 //
 // The grid lives on the device between plugin callbacks.  The sweeps are
 // the hand-written kernels of the cell's registered kernel family
-// (libgeodecomp/hip/kernelfamilies.h), reached through the lgdhip C-ABI; the
+// (libgeodecomp/hip/kernelfamilies.h), reached through the lgdhip C-ABI, or,
+// for a cell without one, the cell's own update() compiled for the GPU by
+// the generic family (libgeodecomp/hip/genericfamily.h, needs hipcc); the
 // host only holds the Grid that Writers and Steerers see.  Differences from
 // CUDASimulator, all in the user's favour:
 //  * steerer edits are uploaded before the next sweep (the reference leaves
@@ -35,20 +37,36 @@
 
 namespace LibGeoDecomp {
 
+namespace HipKernels {
+
+// defined in libgeodecomp/hip/genericfamily.h (hipcc translation units)
+template<typename CELL>
+class Generic;
+
+template<typename T, typename = void>
+class IsComplete : public std::false_type {};
+template<typename T>
+class IsComplete<T, decltype(void(sizeof(T)))> : public std::true_type {};
+
+}
+
 template<typename CELL_TYPE>
 class HipSimulator : public MonolithicSimulator<CELL_TYPE>
 {
 public:
     typedef typename MonolithicSimulator<CELL_TYPE>::Topology Topology;
-    typedef typename APITraits::SelectHipKernel<CELL_TYPE>::Value Family;
+    typedef typename APITraits::SelectHipKernel<CELL_TYPE>::Value RegisteredFamily;
+    typedef typename std::conditional<std::is_void<RegisteredFamily>::value, HipKernels::Generic<CELL_TYPE>,
+                                      RegisteredFamily>::type Family;
     typedef Grid<CELL_TYPE, Topology> GridType;
     typedef typename Steerer<CELL_TYPE>::SteererFeedback SteererFeedback;
     static const int DIM = Topology::DIM;
     static const unsigned NANO_STEPS = APITraits::SelectNanoSteps<CELL_TYPE>::VALUE;
 
-    static_assert(!std::is_void<Family>::value,
-                  "HipSimulator needs a HIP kernel family for this cell: derive its API from "
-                  "APITraits::HasHipKernel<...> or use LIBGEODECOMP_REGISTER_HIP_KERNEL");
+    static_assert(HipKernels::IsComplete<Family>::value,
+                  "no hand-written HIP kernel family is registered for this cell: include "
+                  "<libgeodecomp/hip/genericfamily.h> and compile with hipcc to run the cell's own update() on "
+                  "the GPU, or name a family with APITraits::HasHipKernel<...> / LIBGEODECOMP_REGISTER_HIP_KERNEL");
 
     using MonolithicSimulator<CELL_TYPE>::chronometer;
     using MonolithicSimulator<CELL_TYPE>::initializer;

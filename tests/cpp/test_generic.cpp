@@ -1,0 +1,297 @@
+// GPU tests of the generic kernel family: user cells without a hand-written
+// family, their update() compiled by hipcc, run through HipSimulator and
+// through the oracle SerialSimulator.  Grids at every writer callback and
+// the plugin event sequences must agree exactly.  The ProbeCell cases mirror
+// src/libgeodecomp/parallelization/test/unit/cudasimulatortest.h:69-209
+// (1D 777, 2D 121 x 151, 3D 50 x 20 x 10; Cube and Torus; nano steps) with
+// mixed topologies, radius-2 stencils and tiny tori added.
+//
+// Compiled with hipcc (tests/cpp/Makefile): the cells' update() runs on the
+// GPU.
+#include "devicecells.h"
+#include "harness.h"
+
+#include <libgeodecomp/hip/genericfamily.h>
+#include <libgeodecomp/parallelization/hipsimulator.h>
+#include <serialsimulator.h>
+
+#include <cstdio>
+#include <memory>
+
+template<typename CELL>
+struct Snapshots
+{
+    typedef typename Writer<CELL>::GridType GridType;
+    std::vector<std::vector<CELL> > grids;
+
+    static void hook(const GridType& g, unsigned, void *ctx)
+    {
+        Snapshots *s = static_cast<Snapshots *>(ctx);
+        std::vector<CELL> cells;
+        CoordBox<GridType::DIM> box = g.boundingBox();
+        for (typename CoordBox<GridType::DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+            cells.push_back(g.get(*i));
+        }
+        s->grids.push_back(cells);
+    }
+};
+
+// runs INIT's simulation through both simulators with an event-recording
+// writer of the given period; returns whether grids and events agree
+template<typename CELL>
+bool runBoth(std::function<Initializer<CELL> *()> make, unsigned writerPeriod, Coord<3> block,
+             Snapshots<CELL> *snapHip, Snapshots<CELL> *snapRef, unsigned steererPeriod = 0,
+             typename RecordingSteerer<CELL>::Hook steer = 0)
+{
+    std::vector<Event> evHip, evRef;
+    HipSimulator<CELL> hip(make(), block);
+    SerialSimulator<CELL> ref(make());
+    hip.addWriter(new RecordingWriter<CELL>(&evHip, writerPeriod, &Snapshots<CELL>::hook, snapHip));
+    ref.addWriter(new RecordingWriter<CELL>(&evRef, writerPeriod, &Snapshots<CELL>::hook, snapRef));
+    if (steererPeriod) {
+        hip.addSteerer(new RecordingSteerer<CELL>(&evHip, steererPeriod, steer));
+        ref.addSteerer(new RecordingSteerer<CELL>(&evRef, steererPeriod, steer));
+    }
+    hip.run();
+    ref.run();
+    CHECK_EQ(hip.getStep(), ref.getStep());
+    CHECK(evHip == evRef);
+    CHECK_EQ(snapHip->grids.size(), snapRef->grids.size());
+    const bool same = snapHip->grids == snapRef->grids;
+    CHECK(same);
+    return same;
+}
+
+// every cell of the last snapshot valid, with cycle == steps * nano steps
+template<typename CELL>
+void checkProbes(const Snapshots<CELL>& s, unsigned steps, unsigned nano)
+{
+    CHECK(!s.grids.empty());
+    if (s.grids.empty()) {
+        return;
+    }
+    int invalid = 0, badBits = 0, wrongCycle = 0;
+    for (const CELL& c : s.grids.back()) {
+        invalid += !c.valid;
+        badBits |= c.bad;
+        wrongCycle += c.cycle != steps * nano;
+    }
+    if (invalid || wrongCycle) {
+        std::printf("  %d invalid cells (failure bits 0x%x), %d with a wrong cycle count\n", invalid, badBits,
+                    wrongCycle);
+    }
+    CHECK_EQ(invalid, 0);
+    CHECK_EQ(wrongCycle, 0);
+}
+
+template<typename CELL>
+void probeCase(const Coord<CELL::API::Topology::DIM>& dim, unsigned steps, unsigned writerPeriod,
+               Coord<3> block = Coord<3>())
+{
+    typedef ProbeInitializer<CELL> Init;
+    Snapshots<CELL> sh, sr;
+    runBoth<CELL>([&]() { return new Init(dim, steps, 20261202); }, writerPeriod, block, &sh, &sr);
+    checkProbes(sh, steps, APITraits::SelectNanoSteps<CELL>::VALUE);
+    checkProbes(sr, steps, APITraits::SelectNanoSteps<CELL>::VALUE);
+}
+
+typedef Topologies::Cube<1>::Topology Line;
+typedef Topologies::Torus<1>::Topology Ring;
+typedef Topologies::Cube<2>::Topology Square;
+typedef Topologies::Torus<2>::Topology Torus2;
+typedef TopologiesHelpers::Topology<2, false, true, false> Cylinder2;
+typedef Topologies::Cube<3>::Topology Cube3;
+typedef Topologies::Torus<3>::Topology Torus3;
+typedef TopologiesHelpers::Topology<3, true, false, true> Slab3;
+
+// ---------------------------------------------------------------------
+// the reference simulator test's shapes, Cube and Torus, nano steps
+// ---------------------------------------------------------------------
+LGD_TEST(probe_1d_cube_777)
+{
+    probeCase<ProbeCell<1, Line, 1, Stencils::Moore<1, 1> > >(Coord<1>(777), 13, 3);
+}
+
+LGD_TEST(probe_1d_torus_777_three_nano_steps)
+{
+    probeCase<ProbeCell<1, Ring, 3, Stencils::Moore<1, 1> > >(Coord<1>(777), 13, 4);
+}
+
+LGD_TEST(probe_2d_cube_121x151)
+{
+    probeCase<ProbeCell<2, Square, 1, Stencils::Moore<2, 1> > >(Coord<2>(121, 151), 11, 5);
+}
+
+LGD_TEST(probe_2d_torus_121x151_two_nano_steps)
+{
+    probeCase<ProbeCell<2, Torus2, 2, Stencils::Moore<2, 1> > >(Coord<2>(121, 151), 11, 5);
+}
+
+LGD_TEST(probe_2d_cylinder_von_neumann_radius_2)
+{
+    probeCase<ProbeCell<2, Cylinder2, 1, Stencils::VonNeumann<2, 2> > >(Coord<2>(67, 45), 9, 3);
+}
+
+LGD_TEST(probe_3d_cube_50x20x10)
+{
+    probeCase<ProbeCell<3, Cube3, 1, Stencils::Moore<3, 1> > >(Coord<3>(50, 20, 10), 7, 2);
+}
+
+LGD_TEST(probe_3d_torus_50x20x10_two_nano_steps)
+{
+    probeCase<ProbeCell<3, Torus3, 2, Stencils::Moore<3, 1> > >(Coord<3>(50, 20, 10), 7, 2);
+}
+
+LGD_TEST(probe_3d_slab_von_neumann_radius_2)
+{
+    probeCase<ProbeCell<3, Slab3, 1, Stencils::VonNeumann<3, 2> > >(Coord<3>(23, 17, 9), 5, 5);
+}
+
+// tori smaller than the stencil: the halo wraps around more than once
+LGD_TEST(probe_tiny_tori)
+{
+    probeCase<ProbeCell<1, Ring, 1, Stencils::Moore<1, 3> > >(Coord<1>(2), 6, 2);
+    probeCase<ProbeCell<2, Torus2, 1, Stencils::Moore<2, 2> > >(Coord<2>(3, 1), 6, 2);
+    probeCase<ProbeCell<3, Torus3, 2, Stencils::Moore<3, 1> > >(Coord<3>(1, 2, 1), 4, 1);
+}
+
+// work-group shapes (HipSimulator's blockSize): same grids
+LGD_TEST(probe_block_shapes)
+{
+    typedef ProbeCell<3, Torus3, 1, Stencils::Moore<3, 1> > Cell;
+    probeCase<Cell>(Coord<3>(70, 9, 5), 4, 2, Coord<3>(128, 2, 1));
+    probeCase<Cell>(Coord<3>(70, 9, 5), 4, 2, Coord<3>(64, 16, 1));
+    probeCase<Cell>(Coord<3>(70, 9, 5), 4, 2, Coord<3>(256, 1, 1));
+    typedef ProbeCell<1, Line, 1, Stencils::Moore<1, 2> > Cell1;
+    probeCase<Cell1>(Coord<1>(1000), 5, 5, Coord<3>(64, 1, 1));
+}
+
+LGD_TEST(block_shape_rejected)
+{
+    typedef ProbeCell<2, Square, 1, Stencils::Moore<2, 1> > Cell;
+    CHECK_THROWS(HipSimulator<Cell>(new ProbeInitializer<Cell>(Coord<2>(8, 8), 1, 1), Coord<3>(48, 1, 1)),
+                 std::invalid_argument);
+    CHECK_THROWS(HipSimulator<Cell>(new ProbeInitializer<Cell>(Coord<2>(8, 8), 1, 1), Coord<3>(512, 4, 1)),
+                 std::invalid_argument);
+}
+
+// ---------------------------------------------------------------------
+// steerer edits reach the device (the reference CUDASimulator drops them,
+// cudasimulator.h:450-465)
+// ---------------------------------------------------------------------
+typedef ProbeCell<2, Torus2, 1, Stencils::Moore<2, 1> > SteeredCell;
+
+static void nudge(GridBase<SteeredCell, 2> *grid, unsigned step, Steerer<SteeredCell>::SteererFeedback *, void *)
+{
+    const Coord<2> c((int)(step * 13) % 40, (int)(step * 7) % 30);
+    SteeredCell cell = grid->get(c);
+    cell.value += 1.0f;
+    grid->set(c, cell);
+}
+
+LGD_TEST(probe_steerer_edits_uploaded)
+{
+    Snapshots<SteeredCell> sh, sr;
+    const bool same = runBoth<SteeredCell>([]() { return new ProbeInitializer<SteeredCell>(Coord<2>(40, 30), 20, 7); },
+                                           4, Coord<3>(), &sh, &sr, 3, &nudge);
+    CHECK(same);
+    checkProbes(sh, 20, 1);
+}
+
+// ---------------------------------------------------------------------
+// the generic path against the hand-written families on the same update
+// ---------------------------------------------------------------------
+LGD_TEST(generic_life_equals_oracle_and_life2d_family)
+{
+    const Coord<2> dim(301, 217);
+    const unsigned steps = 60;
+    const uint64_t seed = 99;
+    Snapshots<DeviceConwayCell> sh, sr;
+    runBoth<DeviceConwayCell>(
+        [&]() {
+            return new RandomCellInitializer<DeviceConwayCell, 2>(
+                dim, steps, seed, [](double u) { return DeviceConwayCell(u < 0.3); }, DeviceConwayCell(false));
+        },
+        10, Coord<3>(), &sh, &sr);
+    // the registered Life2D family (bit-packed multi-generation kernels) on
+    // the same start
+    HipSimulator<ConwayCell> family(new RandomLifeInitializer(dim, steps, seed, 0.3));
+    family.run();
+    const GridBase<ConwayCell, 2> *g = family.getGrid();
+    int differ = 0;
+    std::size_t k = 0;
+    CoordBox<2> box(Coord<2>(), dim);
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i, ++k) {
+        differ += g->get(*i).alive != sh.grids.back()[k].alive;
+    }
+    CHECK_EQ(differ, 0);
+}
+
+LGD_TEST(generic_jacobi_equals_oracle_and_jacobi_family)
+{
+    const Coord<3> dim(37, 29, 23);
+    const unsigned steps = 12;
+    const uint64_t seed = 5;
+    Snapshots<DeviceJacobiCell> sh, sr;
+    runBoth<DeviceJacobiCell>(
+        [&]() {
+            return new RandomCellInitializer<DeviceJacobiCell, 3>(
+                dim, steps, seed, [](double u) { return DeviceJacobiCell(u); }, DeviceJacobiCell(0.25));
+        },
+        4, Coord<3>(), &sh, &sr);
+    HipSimulator<JacobiCell> family(new RandomCellInitializer<JacobiCell, 3>(
+        dim, steps, seed, [](double u) { return JacobiCell(u); }, JacobiCell(0.25)));
+    family.run();
+    const GridBase<JacobiCell, 3> *g = family.getGrid();
+    int differ = 0;
+    std::size_t k = 0;
+    CoordBox<3> box(Coord<3>(), dim);
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i, ++k) {
+        differ += g->get(*i).temp != sh.grids.back()[k].temp;
+    }
+    CHECK_EQ(differ, 0);
+}
+
+// ---------------------------------------------------------------------
+// updateHIP() / updateCUDA() replace update() on the GPU
+// ---------------------------------------------------------------------
+template<typename CELL>
+void separateUpdateCase(int gpuPath)
+{
+    const Coord<2> dim(45, 31);
+    const unsigned steps = 9;
+    auto make = [&]() {
+        return new RandomCellInitializer<CELL, 2>(
+            dim, steps, 3, [](double u) { return CELL((int)(u * 1000)); }, CELL(0));
+    };
+    HipSimulator<CELL> hip(make());
+    SerialSimulator<CELL> ref(make());
+    hip.run();
+    ref.run();
+    int valueDiffers = 0, hipPath = 0, refPath = 0;
+    CoordBox<2> box(Coord<2>(), dim);
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const CELL a = hip.getGrid()->get(*i), b = ref.getGrid()->get(*i);
+        valueDiffers += a.v != b.v;
+        hipPath += a.path == gpuPath;
+        refPath += b.path == 1;
+    }
+    CHECK_EQ(valueDiffers, 0);
+    CHECK_EQ(hipPath, (int)box.size());
+    CHECK_EQ(refPath, (int)box.size());
+}
+
+LGD_TEST(update_hip_runs_on_gpu)
+{
+    separateUpdateCase<HipUpdateCell>(2);
+}
+
+LGD_TEST(update_cuda_runs_on_gpu)
+{
+    separateUpdateCase<CudaUpdateCell>(3);
+}
+
+int main(int argc, char **argv)
+{
+    return lgdtest::runAll(argc, argv);
+}

@@ -2,6 +2,7 @@
 // SerialSimulator's plugin semantics, restating the expectations of
 // src/libgeodecomp/parallelization/test/unit/serialsimulatortest.h.
 #include "cells.h"
+#include "devicecells.h"
 #include "harness.h"
 
 #include <libgeodecomp/geometry/region.h>
@@ -352,6 +353,87 @@ LGD_TEST(oracle_torus_glider_returns)
     }
     CHECK(moved);
     CHECK_EQ(alive, 5);
+}
+
+// The probe cell of the generic-kernel GPU tests catches the mistakes it is
+// there for (planted here, on the CPU): a wrong topology, a neighbour from
+// another cycle, at the wrong place, or invalid, and a wrong nano step.
+typedef ProbeCell<2, Topologies::Cube<2>::Topology, 1, Stencils::Moore<2, 1> > CubeProbe;
+typedef ProbeCell<2, Topologies::Cube<2>::Topology, 1, Stencils::Moore<2, 1>, Topologies::Torus<2>::Topology>
+    CubeProbeCheckedAsTorus;
+
+template<typename CELL>
+struct ProbeHood
+{
+    std::vector<CELL> cells;   // 3 x 3, x fastest
+    const CELL& operator[](const Coord<2>& rel) const { return cells[(rel.y() + 1) * 3 + rel.x() + 1]; }
+    template<int X, int Y, int Z>
+    const CELL& operator[](FixedCoord<X, Y, Z>) const { return cells[(Y + 1) * 3 + X + 1]; }
+};
+
+static ProbeHood<CubeProbe> probeHood(const Coord<2>& center, const Coord<2>& dim)
+{
+    ProbeHood<CubeProbe> h;
+    for (int y = -1; y < 2; ++y) {
+        for (int x = -1; x < 2; ++x) {
+            h.cells.push_back(CubeProbe(center + Coord<2>(x, y), dim, 0.5f));
+        }
+    }
+    return h;
+}
+
+LGD_TEST(probe_cell_detects_planted_mistakes)
+{
+    const Coord<2> dim(10, 8);
+    // consistent neighbourhood: valid, cycle advanced, value averaged
+    {
+        ProbeHood<CubeProbe> h = probeHood(Coord<2>(4, 4), dim);
+        CubeProbe c;
+        c.update(h, 0);
+        CHECK(c.valid && c.bad == 0 && c.cycle == 1u && c.value == 0.5f);
+    }
+    // neighbour one cycle ahead / misplaced / invalid; wrong nano step
+    const struct { int which; int bit; } cases[] = {
+        {0, PROBE_CYCLE}, {1, PROBE_POSITION}, {2, PROBE_INVALID_NEIGHBOR}, {3, PROBE_NANO_STEP}};
+    for (auto& k : cases) {
+        ProbeHood<CubeProbe> h = probeHood(Coord<2>(4, 4), dim);
+        unsigned nano = 0;
+        if (k.which == 0) h.cells[2].cycle = 1;
+        if (k.which == 1) h.cells[5].pos = Coord<2>(6, 4);
+        if (k.which == 2) h.cells[7].valid = 0;
+        if (k.which == 3) nano = 1;
+        CubeProbe c;
+        c.update(h, nano);
+        CHECK(!c.valid);
+        CHECK_EQ(c.bad, k.bit);
+    }
+    // the edge cell where the topology has none
+    {
+        ProbeHood<CubeProbe> h = probeHood(Coord<2>(0, 4), dim);
+        h.cells[3] = CubeProbe::makeEdge(dim);
+        h.cells[0] = CubeProbe::makeEdge(dim);
+        h.cells[6] = CubeProbe::makeEdge(dim);
+        CubeProbe c;
+        c.update(h, 0);
+        CHECK(c.valid);                                  // correct for the Cube
+        h.cells[3] = CubeProbe(Coord<2>(-1, 4), dim, 0.5f);
+        c = CubeProbe();
+        c.update(h, 0);
+        CHECK(!c.valid && (c.bad & PROBE_BAD_EDGE));     // a cell where the edge belongs
+    }
+    // a whole run whose checks assume the wrong topology: exactly the
+    // boundary cells fail in the first step
+    SerialSimulator<CubeProbeCheckedAsTorus> sim(new ProbeInitializer<CubeProbeCheckedAsTorus>(dim, 1, 3));
+    sim.run();
+    int invalid = 0, edgeBits = 0;
+    CoordBox<2> box(Coord<2>(), dim);
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const CubeProbeCheckedAsTorus c = sim.getGrid()->get(*i);
+        invalid += !c.valid;
+        edgeBits += (c.bad & PROBE_BAD_EDGE) != 0;
+    }
+    CHECK_EQ(invalid, 2 * (10 + 8) - 4);
+    CHECK_EQ(edgeBits, invalid);
 }
 
 int main(int argc, char **argv)

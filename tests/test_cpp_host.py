@@ -4,6 +4,8 @@ test_semantics (CPU): the reference-compatible API layer and the oracle
 SerialSimulator's plugin event semantics.
 test_hipsimulator (GPU): HipSimulator against the oracle SerialSimulator on
 the same user cells and plugins.
+test_generic (GPU): cells without a hand-written kernel family, their own
+update() compiled by hipcc (the generic kernel family), against the oracle.
 """
 import subprocess
 from pathlib import Path
@@ -31,3 +33,8 @@ def test_cpp_semantics(tmp_path):
 @pytest.mark.gpu
 def test_cpp_hipsimulator(tmp_path):
     run("test_hipsimulator", 600, tmp_path)
+
+
+@pytest.mark.gpu
+def test_cpp_generic(tmp_path):
+    run("test_generic", 600, tmp_path)
