@@ -237,12 +237,13 @@ class LBM:
     dtype = "f32"
     bytes_per_lup = 153.0  # 19 x 4 B read + 19 x 4 B write + 1 B wall flags
 
-    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, bx=0, **_):
+    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, bx=0,
+                 layout=lgdhip.LAYOUT_Z_SLABS, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
         self.tile = lgdhip.tile(bx=bx, by=by, flags=flags, variant=variant)
         self.stream = lgdhip.Stream()
         s = self.stream.h
-        self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19)
+        self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19, layout=layout)
         self.fg = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=1, members=1)
         sys.path.insert(0, str(REPO / "tests"))
         ux, uy, uz = synthetic.lbm_initial_velocity(n, n, n, 1e-3, seed)
@@ -264,6 +265,7 @@ class LBM:
     def config(self):
         return {"workload": f"LBM D3Q19 BGK lid-driven cavity FP32 {self.n}^3, omega 1.7, lid 0.05, "
                             f"{self.sweeps} steps", "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed,
+                "layout": "z slabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "member planes",
                 "parallelism": "single GPU"}
 
     def lups_per_step(self):
@@ -280,7 +282,7 @@ class LBM:
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
                                                     self.flags.ptr, C.byref(self.g), C.byref(self.p), launches,
                                                     C.byref(self.tile), None, self.stream.h))
-        self.shape = f"{self.n}^3"
+        self.shape = f"{self.n}^3" + (",zslabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "")
         return "lbm_d3q19_pull", ms / launches, self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
@@ -586,6 +588,8 @@ def main():
     ap.add_argument("--variant", type=int, default=-1, help="kernel variant id (-1: tuned default, 0: library default)")
     ap.add_argument("--by", type=int, default=0, help="jacobi/lbm: rows per work-group (0: default)")
     ap.add_argument("--bx", type=int, default=0, help="lbm: work-group width 64 or 256 (0: default)")
+    ap.add_argument("--layout", choices=("planes", "zslabs"), default=None,
+                    help="lbm: member layout of the populations (default: the tuned one)")
     ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
     ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
     ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
@@ -620,6 +624,8 @@ def main():
         kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
     if args.workload == "lbm":
         kw.update(by=args.by, flags=flags, variant=max(0, args.variant), bx=args.bx)
+        if args.layout:
+            kw.update(layout=lgdhip.LAYOUT_Z_SLABS if args.layout == "zslabs" else lgdhip.LAYOUT_MEMBER_PLANES)
     if args.workload == "poisson":
         kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol)
     wl = WORKLOADS[args.workload](**kw)

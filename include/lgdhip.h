@@ -56,11 +56,26 @@ typedef void *lgdhip_event_t;   /* hipEvent_t  */
  * reference adds on non-wrapping axes (cudasimulator.h:328-335,
  * soagrid.h:578-584).  For 2D grids nz == 1, oz == 0, sz == 1.
  */
+/* Member layouts.  Element (x, y, z) of member m (padded coordinates
+ * X = x + ox, Y = y + oy, Z = z + oz) is at element index
+ *   m * MS + Z * ZS + Y * sx + X
+ * LGDHIP_LAYOUT_MEMBER_PLANES: MS = sx*sy*sz, ZS = sx*sy (one whole padded
+ *   plane per member, the member-major SoA of libflatarray);
+ * LGDHIP_LAYOUT_Z_SLABS: MS = sx*sy, ZS = members*sx*sy (the members of
+ *   one z slice stored together: a sweep over a few z slices touches one
+ *   compact address range instead of one range per member, which keeps
+ *   HBM streams fewer; the D3Q19 sweep runs ~4% faster).
+ * Both layouts take members*sx*sy*sz elements. */
+#define LGDHIP_LAYOUT_MEMBER_PLANES 0
+#define LGDHIP_LAYOUT_Z_SLABS       1
+
 typedef struct lgdhip_grid {
     int32_t dim;         /* 2 or 3                                  */
     int32_t halo;        /* padding layers on every side (>= 1)     */
     int32_t elem_bytes;  /* bytes per element of one member         */
     int32_t members;     /* number of SoA member planes             */
+    int32_t layout;      /* LGDHIP_LAYOUT_*                         */
+    int32_t reserved;
     int64_t nx, ny, nz;  /* logical extents                         */
     int64_t ox, oy, oz;  /* element offset of logical (0,0,0)       */
     int64_t sx, sy, sz;  /* allocated extents (row pitch, rows, planes) */
@@ -136,6 +151,9 @@ int lgdhip_graph_destroy(lgdhip_graph_t graph);
 /* Fill a descriptor with this library's padding/alignment convention. */
 int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz,
                      int halo, int elem_bytes, int members);
+/* Switch a descriptor made by lgdhip_grid_make to another member layout
+ * (LGDHIP_LAYOUT_*); extents and pitches are unchanged. */
+int lgdhip_grid_set_layout(lgdhip_grid *g, int layout);
 /* Elements per member plane (sx*sy*sz) and total bytes of a grid. */
 int64_t lgdhip_grid_plane_elems(const lgdhip_grid *g);
 int64_t lgdhip_grid_bytes(const lgdhip_grid *g);

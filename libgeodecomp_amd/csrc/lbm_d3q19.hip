@@ -11,8 +11,11 @@
 //  * the +y face may be a moving lid: a population coming back from it gets
 //    6 w_i (e_i . u_lid), the moving-wall bounce-back, where the reference
 //    injects a fixed momentum on two links (:206-220).
-// Layout: 19 member planes of the padded SoA grid (one per population), a
-// separate uint8 flag grid with the same logical extents.
+// Layout: 19 members of the padded SoA grid (one per population), in either
+// member layout of the grid (whole member planes, or z slabs with the 19
+// members of each z slice together: the kernels take the member stride
+// `plane` and the z-slice stride `sxy` from the descriptor), and a separate
+// uint8 flag grid with the same logical extents.
 //
 // Storage is shifted: the device holds g_i = f_i - w_i (deviation from the
 // rest state).  Bounce-back and streaming are unchanged (w_i = w_opp(i)),
@@ -133,6 +136,96 @@ __global__ __launch_bounds__(BX * BY) void lbm_d3q19_pull(const float *__restric
     }
 }
 
+// Four cells per thread (x0 .. x0+3, x0 a multiple of 4): every population
+// is pulled with one 16-byte load (x-shifted pulls are 4-byte aligned,
+// which gfx950 serves as one dwordx4 access) and written with one 16-byte
+// nontemporal store; a wave covers 256 consecutive cells of a row.  Links
+// blocked by a wall are fixed up afterwards, only in lanes whose four cells
+// carry wall flags.  Same arithmetic per cell, in the same order, as
+// lbm_d3q19_pull (bit-identical results).
+typedef float float4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef float float4a __attribute__((ext_vector_type(4)));
+
+template <int BY, typename IDX>
+__global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull_x4(const float *__restrict__ src, float *__restrict__ dst,
+                                                             const uint8_t *__restrict__ flags, int nx, int ny,
+                                                             int nz, IDX sx, IDX sxy, IDX plane, IDX origin,
+                                                             IDX fsx, IDX fsxy, IDX forigin, float omega, float lux,
+                                                             float luy, float luz, unsigned gx, unsigned gy,
+                                                             unsigned ntiles, unsigned per, int swizzle) {
+    const unsigned t = swizzle == 2 ? lgdhip::xcd_group_tile(blockIdx.x, gx)
+                                    : lgdhip::xcd_tile(blockIdx.x, per, swizzle == 1);
+    if (t >= ntiles) return;
+    const int x0 = ((int)(t % gx) * 64 + (int)threadIdx.x) * 4;
+    const int y = (int)((t / gx) % gy) * BY + threadIdx.y;
+    const int z = (int)(t / (gx * gy));
+    if (x0 >= nx || y >= ny || z >= nz) return;
+    const IDX idx = origin + (IDX)z * sxy + (IDX)y * sx + x0;
+    const uint32_t fl4 = *(const uint32_t *)(flags + forigin + (IDX)z * fsxy + (IDX)y * fsx + x0);
+
+    float f[19][4];
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const Dir d = kDirs[i];
+        const float4u v = *(const float4u *)(src + ((IDX)i * plane + idx - (d.ex + d.ey * sx + d.ez * sxy)));
+        f[i][0] = v.x;
+        f[i][1] = v.y;
+        f[i][2] = v.z;
+        f[i][3] = v.w;
+    }
+    if (fl4) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t fl = (fl4 >> (8 * c)) & 0xffu;
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const Dir d = kDirs[i];
+                if (blocked(d, fl)) f[i][c] = src[(IDX)d.opp * plane + idx + c];
+            }
+            const bool lid = (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI);
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const Dir d = kDirs[i];
+                if (d.ey == -1) {
+                    const float term =
+                        6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
+                    f[i][c] += lid ? term : 0.0f;
+                }
+            }
+        }
+    }
+    float out[19][4];
+    const float keep = 1.0f - omega;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            drho += f[i][c];
+            if (kDirs[i].ex) jx += (float)kDirs[i].ex * f[i][c];
+            if (kDirs[i].ey) jy += (float)kDirs[i].ey * f[i][c];
+            if (kDirs[i].ez) jz += (float)kDirs[i].ez * f[i][c];
+        }
+        const float dterm = drho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
+#pragma unroll
+        for (int i = 0; i < 19; ++i) {
+            const Dir d = kDirs[i];
+            const float eu = (float)d.ex * jx + (float)d.ey * jy + (float)d.ez * jz;
+            const float w3 = 3.0f * weight(d.wc) * omega;
+            out[i][c] = keep * f[i][c] + w3 * (dterm + eu + 1.5f * eu * eu);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        float4a v;
+        v.x = out[i][0];
+        v.y = out[i][1];
+        v.z = out[i][2];
+        v.w = out[i][3];
+        __builtin_nontemporal_store(v, (float4a *)(dst + ((IDX)i * plane + idx)));
+    }
+}
+
 __global__ void lbm_cavity_flags_kernel(uint8_t *__restrict__ flags, int nx, int ny, int nz, int64_t sx,
                                         int64_t sxy, int64_t origin) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -177,6 +270,52 @@ int flags_grid(const lgdhip_grid &g, lgdhip_grid *fg) {
     return lgdhip_grid_make(fg, 3, g.nx, g.ny, g.nz, g.halo, 1, 1);
 }
 
+int run_x4(float *a, float *b, const uint8_t *flags, const lgdhip_grid &g, const lgdhip_grid &fg,
+           const lgdhip_lbm_params &p, int steps, const lgdhip_tile *tile, int swizzle, int *result_in_b,
+           hipStream_t s) {
+    const int BY = (tile->by == 2 || tile->by == 4) ? tile->by : 1;
+    const unsigned gx = lgdhip::ceil_div(g.nx, 256), gy = lgdhip::ceil_div(g.ny, BY);
+    const unsigned ntiles = gx * gy * (unsigned)g.nz;
+    const unsigned per = lgdhip::xcd_per(ntiles);
+    const dim3 grid(swizzle == 2 ? lgdhip::xcd_group_groups(ntiles, gx)
+                                 : (swizzle ? lgdhip::xcd_groups(ntiles) : ntiles));
+    const dim3 block(64, BY);
+    // z-slice and member strides of the grid's layout
+    const int64_t sxy = lgdhip::z_stride(g);
+    const int64_t plane = lgdhip::member_stride(g);
+    const int64_t origin = lgdhip::origin_index(g);
+    const int64_t forigin = lgdhip::origin_index(fg);
+    const bool small_index =
+        19 * lgdhip::plane_elems(g) < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
+    float *cur = a, *nxt = b;
+    auto launch = [&](auto by_tag) {
+        constexpr int B = decltype(by_tag)::value;
+        if (small_index) {
+            lbm_d3q19_pull_x4<B, int32_t><<<grid, block, 0, s>>>(
+                cur, nxt, flags, (int)g.nx, (int)g.ny, (int)g.nz, (int32_t)g.sx, (int32_t)sxy, (int32_t)plane,
+                (int32_t)origin, (int32_t)fg.sx, (int32_t)(fg.sx * fg.sy), (int32_t)forigin, p.omega, p.lid_ux,
+                p.lid_uy, p.lid_uz, gx, gy, ntiles, per, swizzle);
+        } else {
+            lbm_d3q19_pull_x4<B, int64_t><<<grid, block, 0, s>>>(
+                cur, nxt, flags, (int)g.nx, (int)g.ny, (int)g.nz, g.sx, sxy, plane, origin, fg.sx, fg.sx * fg.sy,
+                forigin, p.omega, p.lid_ux, p.lid_uy, p.lid_uz, gx, gy, ntiles, per, swizzle);
+        }
+    };
+    for (int t = 0; t < steps; ++t) {
+        switch (BY) {
+        case 2: launch(std::integral_constant<int, 2>{}); break;
+        case 4: launch(std::integral_constant<int, 4>{}); break;
+        default: launch(std::integral_constant<int, 1>{}); break;
+        }
+        LGD_LAUNCHED("lbm_d3q19_pull_x4");
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    if (result_in_b) *result_in_b = (cur == b) ? 1 : 0;
+    return 0;
+}
+
 } // namespace
 
 extern "C" {
@@ -203,18 +342,26 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     int swizzle = lgdhip::xcd_swizzle(tile, false);
     if (tile && (tile->flags & LGDHIP_TILE_XCD_ROWS) && !(tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) swizzle = 2;
     const bool ntl = tile && tile->variant == 1;
+    // variant 2: four cells per thread, 64 x BY threads = 256 x BY cells
+    const bool x4 = tile && tile->variant == 2;
+    if (x4) {
+        if (g->nx % 4 != 0) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "variant 2 needs nx % 4 == 0");
+        return run_x4(a, b, flags, *g, fg, *p, steps, tile, swizzle, result_in_b, lgdhip::as_stream(stream));
+    }
     const unsigned gx = lgdhip::ceil_div(g->nx, BX), gy = lgdhip::ceil_div(g->ny, BY);
     const unsigned ntiles = gx * gy * (unsigned)g->nz;
     const unsigned per = lgdhip::xcd_per(ntiles);
     const dim3 grid(swizzle == 2 ? lgdhip::xcd_group_groups(ntiles, gx)
                                  : (swizzle ? lgdhip::xcd_groups(ntiles) : ntiles));
     const dim3 block(BX, BY);
-    const int64_t sxy = g->sx * g->sy;
-    const int64_t plane = lgdhip::plane_elems(*g);
+    // z-slice and member strides of the grid's layout
+    const int64_t sxy = lgdhip::z_stride(*g);
+    const int64_t plane = lgdhip::member_stride(*g);
     const int64_t origin = lgdhip::origin_index(*g);
     const int64_t forigin = lgdhip::origin_index(fg);
-    // 32-bit element offsets when all 19 planes (and the flag grid) fit 31 bits
-    const bool small_index = 19 * plane < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
+    // 32-bit element offsets when all 19 members (and the flag grid) fit 31 bits
+    const bool small_index =
+        19 * lgdhip::plane_elems(*g) < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
     hipStream_t s = lgdhip::as_stream(stream);
     float *cur = a, *nxt = b;
     auto launch_nt = [&](auto bx_tag, auto by_tag, auto nt_tag, const float *src, float *dst) {
@@ -280,7 +427,7 @@ int lgdhip_lbm_d3q19_f32_moments(const float *f, const lgdhip_grid *g, float *rh
     LGD_REQUIRE(g->ny <= 65535 && g->nz <= 65535, "ny, nz must be <= 65535");
     dim3 grid(lgdhip::ceil_div(g->nx, 256), (unsigned)g->ny, (unsigned)g->nz);
     lbm_moments_kernel<<<grid, 256, 0, lgdhip::as_stream(stream)>>>(
-        f, (int)g->nx, (int)g->ny, (int)g->nz, g->sx, g->sx * g->sy, lgdhip::plane_elems(*g),
+        f, (int)g->nx, (int)g->ny, (int)g->nz, g->sx, lgdhip::z_stride(*g), lgdhip::member_stride(*g),
         lgdhip::origin_index(*g), rho, ux, uy, uz);
     LGD_LAUNCHED("lbm_moments");
     return 0;

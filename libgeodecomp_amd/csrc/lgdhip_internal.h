@@ -42,9 +42,15 @@ inline hipStream_t as_stream(lgdhip_stream_t s) { return reinterpret_cast<hipStr
 int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members);
 
 inline int64_t plane_elems(const lgdhip_grid &g) { return g.sx * g.sy * g.sz; }
-inline int64_t origin_index(const lgdhip_grid &g) {
-    return (g.oz * g.sy + g.oy) * g.sx + g.ox;
+// element strides between members and between z slices (LGDHIP_LAYOUT_*)
+inline int64_t member_stride(const lgdhip_grid &g) {
+    return g.layout == LGDHIP_LAYOUT_Z_SLABS ? g.sx * g.sy : g.sx * g.sy * g.sz;
 }
+inline int64_t z_stride(const lgdhip_grid &g) {
+    return g.layout == LGDHIP_LAYOUT_Z_SLABS ? (int64_t)g.members * g.sx * g.sy : g.sx * g.sy;
+}
+// element index of logical (0, 0, 0) of member 0
+inline int64_t origin_index(const lgdhip_grid &g) { return g.oz * z_stride(g) + g.oy * g.sx + g.ox; }
 
 inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 

@@ -32,6 +32,8 @@ int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members) {
     if (members && g->members != members)
         return set_error(LGDHIP_ERR_INVALID_ARG, "grid has the wrong number of members");
     if (g->dim != 2 && g->dim != 3) return set_error(LGDHIP_ERR_INVALID_ARG, "grid dim must be 2 or 3");
+    if (g->layout != LGDHIP_LAYOUT_MEMBER_PLANES && g->layout != LGDHIP_LAYOUT_Z_SLABS)
+        return set_error(LGDHIP_ERR_INVALID_ARG, "unknown grid layout");
     if (g->nx < 1 || g->ny < 1 || g->nz < 1) return set_error(LGDHIP_ERR_INVALID_ARG, "empty grid");
     if (g->halo < 1) return set_error(LGDHIP_ERR_INVALID_ARG, "grid halo must be >= 1");
     const int64_t h = g->halo;
@@ -78,11 +80,12 @@ __global__ __launch_bounds__(256) void lgd_copy16_kernel(uint4 *__restrict__ dst
     if (i < n) dst[i] = src[i];
 }
 
-// Writes the edge value into every halo element of one member plane.
-// One thread per allocated element; interior elements are left alone.
+// Writes the edge value into every halo element of one member (z slices
+// `zs` elements apart).  One thread per allocated element of the member;
+// interior elements are left alone.
 template <typename T>
 __global__ void lgd_fill_edge_kernel(T *__restrict__ plane, T edge, int64_t sx, int64_t sy, int64_t sz,
-                                     int64_t ox, int64_t oy, int64_t oz, int64_t nx, int64_t ny,
+                                     int64_t zs, int64_t ox, int64_t oy, int64_t oz, int64_t nx, int64_t ny,
                                      int64_t nz) {
     const int64_t n = sx * sy * sz;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
@@ -91,7 +94,7 @@ __global__ void lgd_fill_edge_kernel(T *__restrict__ plane, T edge, int64_t sx, 
         const int64_t y = (i / sx) % sy;
         const int64_t z = i / (sx * sy);
         const bool inside = x >= ox && x < ox + nx && y >= oy && y < oy + ny && z >= oz && z < oz + nz;
-        if (!inside) plane[i] = edge;
+        if (!inside) plane[z * zs + y * sx + x] = edge;
     }
 }
 
@@ -102,7 +105,7 @@ __global__ void lgd_fill_edge_kernel(T *__restrict__ plane, T edge, int64_t sx, 
 // Topology::normalize + isOutOfBounds.  Reads only interior cells, writes
 // only halo cells.
 template <typename T>
-__global__ void lgd_fill_halo_kernel(T *__restrict__ plane, T edge, int64_t sx, int64_t sy, int64_t ox, int64_t oy,
+__global__ void lgd_fill_halo_kernel(T *__restrict__ plane, T edge, int64_t sx, int64_t zs, int64_t ox, int64_t oy,
                                      int64_t oz, int64_t nx, int64_t ny, int64_t nz, int h, int dim, int wrap) {
     const int64_t ex = nx + 2 * h, ey = ny + 2 * h, ez = dim == 3 ? nz + 2 * h : 1;
     const int64_t total = ex * ey * ez;
@@ -115,8 +118,8 @@ __global__ void lgd_fill_halo_kernel(T *__restrict__ plane, T edge, int64_t sx, 
         if (wrap & 2) wy = ((y % ny) + ny) % ny;
         if ((wrap & 4) && dim == 3) wz = ((z % nz) + nz) % nz;
         const bool inside = wx >= 0 && wx < nx && wy >= 0 && wy < ny && wz >= 0 && wz < nz;
-        const T v = inside ? plane[((wz + oz) * sy + (wy + oy)) * sx + (wx + ox)] : edge;
-        plane[((z + oz) * sy + (y + oy)) * sx + (x + ox)] = v;
+        const T v = inside ? plane[(wz + oz) * zs + (wy + oy) * sx + (wx + ox)] : edge;
+        plane[(z + oz) * zs + (y + oy) * sx + (x + ox)] = v;
     }
 }
 
@@ -378,6 +381,13 @@ int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz
     return check_grid(g, dim, elem_bytes, members);
 }
 
+int lgdhip_grid_set_layout(lgdhip_grid *g, int layout) {
+    LGD_REQUIRE(g, "grid descriptor is NULL");
+    LGD_REQUIRE(layout == LGDHIP_LAYOUT_MEMBER_PLANES || layout == LGDHIP_LAYOUT_Z_SLABS, "unknown grid layout");
+    g->layout = layout;
+    return check_grid(g, 0, 0, 0);
+}
+
 int64_t lgdhip_grid_plane_elems(const lgdhip_grid *g) { return g ? plane_elems(*g) : 0; }
 
 int64_t lgdhip_grid_bytes(const lgdhip_grid *g) {
@@ -390,11 +400,12 @@ int lgdhip_grid_fill_halo(void *grid, const lgdhip_grid *g, const void *edge, in
     if (rc) return rc;
     LGD_REQUIRE(grid && edge, "NULL pointer");
     LGD_REQUIRE(wrap_mask >= 0 && wrap_mask <= 7, "wrap_mask must be a combination of bits 1, 2, 4");
-    const int64_t n = plane_elems(*g);
+    const int64_t n = member_stride(*g);
     const int h = g->halo;
     const int64_t shell = (g->nx + 2 * h) * (g->ny + 2 * h) * (g->dim == 3 ? g->nz + 2 * h : 1);
     const unsigned blocks = fill_blocks((size_t)shell);
     hipStream_t s = as_stream(stream);
+    const int64_t zs = z_stride(*g);
     for (int m = 0; m < g->members; ++m) {
         const uint8_t *ev = (const uint8_t *)edge + (size_t)m * g->elem_bytes;
         char *plane = (char *)grid + (size_t)m * n * g->elem_bytes;
@@ -402,7 +413,7 @@ int lgdhip_grid_fill_halo(void *grid, const lgdhip_grid *g, const void *edge, in
     case BYTES: {                                                                                                 \
         T v;                                                                                                      \
         std::memcpy(&v, ev, BYTES);                                                                               \
-        lgd_fill_halo_kernel<T><<<blocks, 256, 0, s>>>((T *)plane, v, g->sx, g->sy, g->ox, g->oy, g->oz, g->nx,  \
+        lgd_fill_halo_kernel<T><<<blocks, 256, 0, s>>>((T *)plane, v, g->sx, zs, g->ox, g->oy, g->oz, g->nx,     \
                                                        g->ny, g->nz, h, g->dim, wrap_mask);                       \
         break;                                                                                                    \
     }
@@ -423,35 +434,35 @@ int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge, lg
     int rc = check_grid(g, 0, 0, 0);
     if (rc) return rc;
     LGD_REQUIRE(grid && edge, "NULL pointer");
-    const int64_t n = plane_elems(*g);
-    const unsigned blocks = fill_blocks((size_t)n);
+    const unsigned blocks = fill_blocks((size_t)plane_elems(*g));
+    const int64_t ms = member_stride(*g), zs = z_stride(*g);
     for (int m = 0; m < g->members; ++m) {
         const uint8_t *ev = (const uint8_t *)edge + (size_t)m * g->elem_bytes;
-        char *plane = (char *)grid + (size_t)m * n * g->elem_bytes;
+        char *plane = (char *)grid + (size_t)m * ms * g->elem_bytes;
         switch (g->elem_bytes) {
         case 1:
             lgd_fill_edge_kernel<uint8_t><<<blocks, 256, 0, as_stream(stream)>>>(
-                (uint8_t *)plane, *ev, g->sx, g->sy, g->sz, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
+                (uint8_t *)plane, *ev, g->sx, g->sy, g->sz, zs, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
             break;
         case 2: {
             uint16_t v;
             std::memcpy(&v, ev, 2);
             lgd_fill_edge_kernel<uint16_t><<<blocks, 256, 0, as_stream(stream)>>>(
-                (uint16_t *)plane, v, g->sx, g->sy, g->sz, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
+                (uint16_t *)plane, v, g->sx, g->sy, g->sz, zs, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
             break;
         }
         case 4: {
             uint32_t v;
             std::memcpy(&v, ev, 4);
             lgd_fill_edge_kernel<uint32_t><<<blocks, 256, 0, as_stream(stream)>>>(
-                (uint32_t *)plane, v, g->sx, g->sy, g->sz, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
+                (uint32_t *)plane, v, g->sx, g->sy, g->sz, zs, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
             break;
         }
         default: {
             uint64_t v;
             std::memcpy(&v, ev, 8);
             lgd_fill_edge_kernel<uint64_t><<<blocks, 256, 0, as_stream(stream)>>>(
-                (uint64_t *)plane, v, g->sx, g->sy, g->sz, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
+                (uint64_t *)plane, v, g->sx, g->sy, g->sz, zs, g->ox, g->oy, g->oz, g->nx, g->ny, g->nz);
             break;
         }
         }
@@ -465,14 +476,16 @@ static int grid_copy(void *dev, const lgdhip_grid *g, void *host, bool to_device
     if (rc) return rc;
     LGD_REQUIRE(dev && host, "NULL pointer");
     const size_t eb = (size_t)g->elem_bytes;
-    const int64_t n = plane_elems(*g);
+    const int64_t n = member_stride(*g);
+    // rows per z slice of the device side: sy, or members * sy for z slabs
+    const size_t slice_rows = (size_t)(z_stride(*g) / g->sx);
     const size_t host_plane = (size_t)(g->nx * g->ny * g->nz) * eb;
     for (int m = 0; m < g->members; ++m) {
         char *dplane = (char *)dev + (size_t)m * n * eb;
         char *hplane = (char *)host + (size_t)m * host_plane;
         hipMemcpy3DParms p;
         std::memset(&p, 0, sizeof(p));
-        hipPitchedPtr dptr = make_hipPitchedPtr(dplane, (size_t)g->sx * eb, (size_t)g->sx * eb, (size_t)g->sy);
+        hipPitchedPtr dptr = make_hipPitchedPtr(dplane, (size_t)g->sx * eb, (size_t)g->sx * eb, slice_rows);
         hipPitchedPtr hptr = make_hipPitchedPtr(hplane, (size_t)g->nx * eb, (size_t)g->nx * eb, (size_t)g->ny);
         hipPos dpos = make_hipPos((size_t)g->ox * eb, (size_t)g->oy, (size_t)g->oz);
         hipPos hpos = make_hipPos(0, 0, 0);

@@ -26,9 +26,13 @@ class LgdHipError(RuntimeError):
         self.code = code
 
 
+LAYOUT_MEMBER_PLANES, LAYOUT_Z_SLABS = 0, 1
+
+
 class lgdhip_grid(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("halo", C.c_int32), ("elem_bytes", C.c_int32), ("members", C.c_int32),
+        ("layout", C.c_int32), ("reserved", C.c_int32),
         ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
         ("ox", C.c_int64), ("oy", C.c_int64), ("oz", C.c_int64),
         ("sx", C.c_int64), ("sy", C.c_int64), ("sz", C.c_int64),
@@ -115,6 +119,7 @@ SIGNATURES = {
     "lgdhip_graph_launch": (_I, [_P, _P]),
     "lgdhip_graph_destroy": (_I, [_P]),
     "lgdhip_grid_make": (_I, [_GP, _I, _I64, _I64, _I64, _I, _I, _I]),
+    "lgdhip_grid_set_layout": (_I, [_GP, _I]),
     "lgdhip_grid_plane_elems": (_I64, [_GP]),
     "lgdhip_grid_bytes": (_I64, [_GP]),
     "lgdhip_grid_fill_edge": (_I, [_P, _GP, _P, _P]),
@@ -268,9 +273,11 @@ class DeviceBuffer:
 
 
 def make_grid(dim: int, nx: int, ny: int, nz: int = 1, halo: int = 1, elem_bytes: int = 8,
-              members: int = 1) -> lgdhip_grid:
+              members: int = 1, layout: int = LAYOUT_MEMBER_PLANES) -> lgdhip_grid:
     g = lgdhip_grid()
     call("lgdhip_grid_make", C.byref(g), dim, nx, ny, nz, halo, elem_bytes, members)
+    if layout != LAYOUT_MEMBER_PLANES:
+        call("lgdhip_grid_set_layout", C.byref(g), layout)
     return g
 
 

@@ -320,6 +320,9 @@ public:
     {
         requireCubeSingleNanoStep<CELL>();
         check(lgdhip_grid_make(&g, 3, dim.x(), dim.y(), dim.z(), 1, 4, 19), "lgdhip_grid_make");
+        // the 19 populations of each z slice stored together: fewer
+        // concurrent HBM streams per sweep than whole member planes
+        check(lgdhip_grid_set_layout(&g, LGDHIP_LAYOUT_Z_SLABS), "lgdhip_grid_set_layout");
         check(lgdhip_grid_make(&fg, 3, dim.x(), dim.y(), dim.z(), 1, 1, 1), "lgdhip_grid_make");
         a.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));
         b.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));

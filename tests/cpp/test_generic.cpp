@@ -15,6 +15,7 @@
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
 
+#include <chrono>
 #include <cstdio>
 #include <memory>
 
@@ -289,6 +290,61 @@ LGD_TEST(update_hip_runs_on_gpu)
 LGD_TEST(update_cuda_runs_on_gpu)
 {
     separateUpdateCase<CudaUpdateCell>(3);
+}
+
+// ---------------------------------------------------------------------
+// throughput of the generic path next to the hand-written families on the
+// same update (printed, not asserted).  run() includes the initializer and
+// the transfers, so the rate is taken from the difference of two runs of
+// different lengths: cells x (s2 - s1) / (t2 - t1).
+// ---------------------------------------------------------------------
+template<typename CELL>
+double runSeconds(Initializer<CELL> *init)
+{
+    HipSimulator<CELL> sim(init);
+    sim.run();   // warm-up: module load, allocation, first upload
+    const auto t0 = std::chrono::steady_clock::now();
+    sim.run();
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+template<typename CELL>
+double glups(std::function<Initializer<CELL> *(unsigned)> make, double cells, unsigned s1, unsigned s2)
+{
+    const double t1 = runSeconds<CELL>(make(s1));
+    const double t2 = runSeconds<CELL>(make(s2));
+    return cells * (double)(s2 - s1) / (t2 - t1) * 1e-9;
+}
+
+LGD_TEST(generic_vs_family_throughput)
+{
+    const Coord<3> j(128, 128, 128);
+    const double jc = 128.0 * 128 * 128;
+    const double jg = glups<DeviceJacobiCell>(
+        [&](unsigned s) {
+            return new RandomCellInitializer<DeviceJacobiCell, 3>(
+                j, s, 1, [](double u) { return DeviceJacobiCell(u); }, DeviceJacobiCell(0.0));
+        },
+        jc, 100, 1100);
+    const double jf = glups<JacobiCell>(
+        [&](unsigned s) {
+            return new RandomCellInitializer<JacobiCell, 3>(
+                j, s, 1, [](double u) { return JacobiCell(u); }, JacobiCell(0.0));
+        },
+        jc, 100, 1100);
+    std::printf("  Jacobi3D FP64 128^3: generic %.1f GLUPS, Jacobi3DF64 family %.1f GLUPS\n", jg, jf);
+    const Coord<2> l(4096, 4096);
+    const double lc = 4096.0 * 4096;
+    const double lg = glups<DeviceConwayCell>(
+        [&](unsigned s) {
+            return new RandomCellInitializer<DeviceConwayCell, 2>(
+                l, s, 2, [](double u) { return DeviceConwayCell(u < 0.3); }, DeviceConwayCell(false));
+        },
+        lc, 200, 2200);
+    const double lf = glups<ConwayCell>([&](unsigned s) { return new RandomLifeInitializer(l, s, 2, 0.3); }, lc, 200,
+                                        2200);
+    std::printf("  Life 4096^2: generic %.1f GLUPS, Life2D family %.1f GLUPS\n", lg, lf);
+    CHECK(jg > 0 && jf > 0 && lg > 0 && lf > 0);
 }
 
 int main(int argc, char **argv)

@@ -52,6 +52,20 @@ def test_grid_descriptor_without_gpu():
 
 
 def test_struct_layouts_match_header():
-    assert C.sizeof(lgdhip.lgdhip_grid) == 4 * 4 + 9 * 8
+    assert C.sizeof(lgdhip.lgdhip_grid) == 6 * 4 + 9 * 8
+    assert lgdhip.lgdhip_grid.nx.offset == 24
+    assert C.sizeof(lgdhip.lgdhip_cell_grid) == 2 * 4 + 7 * 8
     assert C.sizeof(lgdhip.lgdhip_tile) == 6 * 4
     assert C.sizeof(lgdhip.lgdhip_lbm_params) == 4 * 4
+
+
+def test_lbm_x4_variant_rejects_ragged_rows_without_gpu():
+    # four cells per thread needs nx % 4 == 0; the check comes before any
+    # device work, so it runs on the host
+    lib = lgdhip.load()
+    g = lgdhip.make_grid(3, 30, 8, 8, halo=1, elem_bytes=4, members=19)
+    p = lgdhip.lgdhip_lbm_params(1.7, 0.05, 0.0, 0.0)
+    t = lgdhip.tile(variant=2)
+    fake = C.c_void_p(0x1000), C.c_void_p(0x2000), C.c_void_p(0x3000)
+    rc = lib.lgdhip_lbm_d3q19_f32_run(fake[0], fake[1], fake[2], C.byref(g), C.byref(p), 1, C.byref(t), None, None)
+    assert rc == -2 and "nx % 4" in lib.lgdhip_last_error().decode()

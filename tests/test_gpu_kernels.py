@@ -166,10 +166,10 @@ def test_life_packed_rejects_bad_tile(hip):
 # ----------------------------------------------------------------------
 # D3Q19 LBM
 # ----------------------------------------------------------------------
-def gpu_lbm(hip, f_host, steps, omega=1.7, lid=(0.05, 0.0, 0.0), tile=None):
+def gpu_lbm(hip, f_host, steps, omega=1.7, lid=(0.05, 0.0, 0.0), tile=None, layout=0):
     import ctypes as C
     _, nz, ny, nx = f_host.shape
-    g = hip.make_grid(3, nx, ny, nz, halo=1, elem_bytes=4, members=19)
+    g = hip.make_grid(3, nx, ny, nz, halo=1, elem_bytes=4, members=19, layout=layout)
     fg = hip.make_grid(3, nx, ny, nz, halo=1, elem_bytes=1, members=1)
     a, b = hip.DeviceGrid(g, np.float32), hip.DeviceGrid(g, np.float32)
     fl = hip.DeviceGrid(fg, np.uint8)
@@ -225,18 +225,34 @@ def test_lbm_vs_oracle(hip, oracle, shape, steps):
 
 
 def test_lbm_tile_variants_identical(hip, oracle):
-    """Work-group shapes (64 x 1/2/4/8, 256 x 1), nontemporal loads and the
-    three tile orders change the schedule, not the per-cell arithmetic: all
-    results equal the default launch's bit for bit, on a grid ragged in
-    every direction."""
+    """Work-group shapes (64 x 1/2/4/8, 256 x 1), nontemporal loads, four
+    cells per thread (variant 2) and the three tile orders change the
+    schedule, not the per-cell arithmetic: all results equal the default
+    launch's bit for bit, on a grid ragged in every direction (nx = 300: the
+    last 256-cell segment of a row is partly used)."""
     nx, ny, nz = 300, 37, 11
     ux, uy, uz = synthetic.lbm_initial_velocity(nx, ny, nz)
     g0 = shifted(oracle.lbm_equilibrium(np.ones((nz, ny, nx)), ux, uy, uz))
     ref, _ = gpu_lbm(hip, g0, 7)
     for kw in (dict(by=1), dict(by=2), dict(by=8), dict(bx=256), dict(variant=1), dict(flags=hip.TILE_XCD_SWIZZLE),
-               dict(flags=hip.TILE_XCD_ROWS), dict(bx=256, flags=hip.TILE_XCD_SWIZZLE)):
+               dict(flags=hip.TILE_XCD_ROWS), dict(bx=256, flags=hip.TILE_XCD_SWIZZLE), dict(variant=2),
+               dict(variant=2, by=2), dict(variant=2, by=4, flags=hip.TILE_XCD_SWIZZLE),
+               dict(variant=2, flags=hip.TILE_XCD_ROWS)):
         say("lbm tile", kw)
         out, _ = gpu_lbm(hip, g0, 7, tile=hip.tile(**kw))
+        assert np.array_equal(out, ref), kw
+
+
+def test_lbm_z_slab_layout_identical(hip, oracle):
+    """The z-slab member layout changes addresses only: every launch shape
+    gives the member-plane layout's result bit for bit."""
+    nx, ny, nz = 300, 37, 11
+    ux, uy, uz = synthetic.lbm_initial_velocity(nx, ny, nz)
+    g0 = shifted(oracle.lbm_equilibrium(np.ones((nz, ny, nx)), ux, uy, uz))
+    ref, _ = gpu_lbm(hip, g0, 7)
+    for kw in (dict(), dict(by=2), dict(bx=256), dict(variant=2), dict(flags=hip.TILE_XCD_SWIZZLE)):
+        say("lbm z slabs", kw)
+        out, _ = gpu_lbm(hip, g0, 7, tile=hip.tile(**kw) if kw else None, layout=hip.LAYOUT_Z_SLABS)
         assert np.array_equal(out, ref), kw
 
 
