@@ -12,6 +12,8 @@
 #include "harness.h"
 
 #include <libgeodecomp/hip/genericfamily.h>
+#include <libgeodecomp/io/mpiioinitializer.h>
+#include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
 
@@ -290,6 +292,31 @@ LGD_TEST(update_hip_runs_on_gpu)
 LGD_TEST(update_cuda_runs_on_gpu)
 {
     separateUpdateCase<CudaUpdateCell>(3);
+}
+
+// ---------------------------------------------------------------------
+// restart from an MPI-IO snapshot through the generic path: cycle counters
+// and nano-step phase survive, grids end equal to the uninterrupted run's
+// ---------------------------------------------------------------------
+LGD_TEST(probe_restart_from_snapshot)
+{
+    typedef ProbeCell<3, Torus3, 2, Stencils::Moore<3, 1> > Cell;
+    const Coord<3> dim(21, 13, 8);
+    HipSimulator<Cell> full(new ProbeInitializer<Cell>(dim, 12, 4));
+    full.addWriter(new MPIIOWriter<Cell>("probe", 5, 12));
+    full.run();
+    HipSimulator<Cell> resumed(new MPIIOInitializer<Cell>("probe00005.mpiio"));
+    CHECK_EQ(resumed.getStep(), 5u);
+    resumed.run();
+    int differ = 0, invalid = 0;
+    CoordBox<3> box(Coord<3>(), dim);
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const Cell a = full.getGrid()->get(*i), b = resumed.getGrid()->get(*i);
+        differ += !(a == b);
+        invalid += !b.valid || b.cycle != 24u;
+    }
+    CHECK_EQ(differ, 0);
+    CHECK_EQ(invalid, 0);
 }
 
 // ---------------------------------------------------------------------

@@ -8,6 +8,8 @@
 #include "cells.h"
 #include "harness.h"
 
+#include <libgeodecomp/io/mpiioinitializer.h>
+#include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
@@ -377,6 +379,65 @@ LGD_TEST(lbm_cavity)
     CHECK(errRho < 2e-6 && errRho <= errRho32);
     CHECK(errJ < 1e-6);
     CHECK(std::fabs(mass - mass0) < 1e-7 * mass0 + 19 * dim.prod() * std::ldexp(1.0, -25));
+}
+
+// ---------------------------------------------------------------------
+// restart from an MPI-IO snapshot: the resumed HipSimulator run ends with
+// the uninterrupted run's grid (and the oracle's)
+// ---------------------------------------------------------------------
+template<typename CELL>
+void restartCase(std::function<Initializer<CELL> *()> make, unsigned at, const std::string& prefix,
+                 std::function<bool(const CELL&, const CELL&)> same)
+{
+    typedef typename HipSimulator<CELL>::Topology Topology;
+    static const int DIM = Topology::DIM;
+    Initializer<CELL> *first = make();
+    const unsigned total = first->maxSteps();
+    HipSimulator<CELL> full(first);
+    full.addWriter(new MPIIOWriter<CELL>(prefix, at, total));
+    full.run();
+    char name[256];
+    std::snprintf(name, sizeof(name), "%s%05u.mpiio", prefix.c_str(), at);
+    HipSimulator<CELL> resumed(new MPIIOInitializer<CELL>(name));
+    resumed.run();
+    SerialSimulator<CELL> ref(make());
+    ref.run();
+    CHECK_EQ(resumed.getStep(), total);
+    int differFull = 0, differRef = 0;
+    CoordBox<DIM> box = full.getGrid()->boundingBox();
+    for (typename CoordBox<DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const CELL a = full.getGrid()->get(*i), b = resumed.getGrid()->get(*i), c = ref.getGrid()->get(*i);
+        differFull += !same(a, b);
+        differRef += !same(b, c);
+    }
+    CHECK_EQ(differFull, 0);
+    CHECK_EQ(differRef, 0);
+}
+
+LGD_TEST(restart_from_snapshot_families)
+{
+    restartCase<ConwayCell>([]() { return new RandomLifeInitializer(Coord<2>(517, 301), 90, 42, 0.3); }, 40,
+                            "life", [](const ConwayCell& a, const ConwayCell& b) { return a == b; });
+    restartCase<TorusLifeCell>(
+        []() {
+            return new RandomCellInitializer<TorusLifeCell, 2>(
+                Coord<2>(131, 77), 50, 3, [](double u) { return TorusLifeCell(u < 0.3); }, TorusLifeCell(false));
+        },
+        20, "torus", [](const TorusLifeCell& a, const TorusLifeCell& b) { return a.alive == b.alive; });
+    restartCase<JacobiCell>([]() { return new JacobiInitializer(Coord<3>(40, 33, 29), 26, 9, 0.5); }, 13,
+                            "jacobi", [](const JacobiCell& a, const JacobiCell& b) { return a == b; });
+    // LBM: the host cell holds f, the device g = f - w; a snapshot of f
+    // re-uploaded rounds once more, so agreement is to FP32 rounding
+    restartCase<LBMCavityCell<float> >(
+        []() { return new CavityInitializer<float>(Coord<3>(24, 20, 16), 30, 9); }, 10, "lbm",
+        [](const LBMCavityCell<float>& a, const LBMCavityCell<float>& b) {
+            for (int q = 0; q < 19; ++q) {
+                if (std::fabs(a.f[q] - b.f[q]) > 2e-6f) {
+                    return false;
+                }
+            }
+            return a.flags == b.flags;
+        });
 }
 
 int main(int argc, char **argv)

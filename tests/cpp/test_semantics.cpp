@@ -6,11 +6,14 @@
 #include "harness.h"
 
 #include <libgeodecomp/geometry/region.h>
+#include <libgeodecomp/io/mpiioinitializer.h>
+#include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
 
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 #include <serialsimulator.h>
 
@@ -434,6 +437,138 @@ LGD_TEST(probe_cell_detects_planted_mistakes)
     }
     CHECK_EQ(invalid, 2 * (10 + 8) - 4);
     CHECK_EQ(edgeBits, invalid);
+}
+
+// ---------------------------------------------------------------------
+// MPI-IO snapshots (io/mpiio.h:26-233 format, written by one process)
+// ---------------------------------------------------------------------
+static std::vector<char> fileBytes(const std::string& name)
+{
+    std::ifstream f(name.c_str(), std::ios::binary);
+    return std::vector<char>((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+}
+
+LGD_TEST(mpiio_writer_file_layout)
+{
+    const Coord<3> dim(7, 5, 3);
+    SerialSimulator<JacobiCell> sim(new RandomCellInitializer<JacobiCell, 3>(
+        dim, 7, 11, [](double u) { return JacobiCell(u); }, JacobiCell(-2.5)));
+    sim.addWriter(new MPIIOWriter<JacobiCell>("snap", 3, 7));
+    sim.run();
+    // initial step, every 3rd step, last step
+    for (unsigned step : {0u, 3u, 6u, 7u}) {
+        char name[32];
+        std::snprintf(name, sizeof(name), "snap%05u.mpiio", step);
+        const std::vector<char> b = fileBytes(name);
+        CHECK_EQ(b.size(), (std::size_t)(3 * 4 + 2 * 4 + 8 + 7 * 5 * 3 * 8));
+        if (b.size() != (std::size_t)(3 * 4 + 2 * 4 + 8 + 7 * 5 * 3 * 8)) {
+            continue;
+        }
+        int32_t dims[3];
+        uint32_t st, mx;
+        double edge;
+        std::memcpy(dims, &b[0], 12);
+        std::memcpy(&st, &b[12], 4);
+        std::memcpy(&mx, &b[16], 4);
+        std::memcpy(&edge, &b[20], 8);
+        CHECK(dims[0] == 7 && dims[1] == 5 && dims[2] == 3);
+        CHECK_EQ(st, step);
+        CHECK_EQ(mx, 7u);
+        CHECK_EQ(edge, -2.5);
+    }
+    // the last file's body is the final grid, x fastest
+    const std::vector<char> b = fileBytes("snap00007.mpiio");
+    const GridBase<JacobiCell, 3> *g = sim.getGrid();
+    CoordBox<3> box(Coord<3>(), dim);
+    int differ = 0;
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+        double v;
+        std::memcpy(&v, &b[28 + 8 * i->toIndex(dim)], 8);
+        differ += v != g->get(*i).temp;
+    }
+    CHECK_EQ(differ, 0);
+}
+
+LGD_TEST(mpiio_restart_matches_uninterrupted_run)
+{
+    const Coord<2> dim(61, 37);
+    auto make = [&]() { return new RandomLifeInitializer(dim, 40, 5, 0.35); };
+    std::vector<Event> evFull, evResumed;
+    SerialSimulator<ConwayCell> full(make());
+    full.addWriter(new MPIIOWriter<ConwayCell>("life", 10, 40));
+    full.addWriter(new RecordingWriter<ConwayCell>(&evFull, 4));
+    full.run();
+    MPIIOInitializer<ConwayCell> *init = new MPIIOInitializer<ConwayCell>("life00020.mpiio");
+    CHECK_EQ(init->startStep(), 20u);
+    CHECK_EQ(init->maxSteps(), 40u);
+    CHECK(init->gridDimensions() == dim);
+    SerialSimulator<ConwayCell> resumed(init);
+    resumed.addWriter(new RecordingWriter<ConwayCell>(&evResumed, 4));
+    resumed.run();
+    CHECK_EQ(resumed.getStep(), 40u);
+    int differ = 0;
+    CoordBox<2> box(Coord<2>(), dim);
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        differ += full.getGrid()->get(*i).alive != resumed.getGrid()->get(*i).alive;
+    }
+    CHECK_EQ(differ, 0);
+    // the resumed writer keeps the phase: initialised at 20, then 24, 28, ...
+    CHECK(!evResumed.empty() && evResumed.front().step == 20u && evResumed.front().event == WRITER_INITIALIZED);
+    std::vector<Event> tail(evFull.end() - (long)(evResumed.size() - 1), evFull.end());
+    std::vector<Event> resumedTail(evResumed.begin() + 1, evResumed.end());
+    CHECK(tail == resumedTail);
+}
+
+LGD_TEST(mpiio_regions_and_torus_normalisation)
+{
+    typedef Topologies::Torus<2>::Topology Torus;
+    const Coord<2> dim(9, 4);
+    Grid<JacobiCell, Torus> grid(dim);
+    grid.setEdge(JacobiCell(7.0));
+    CoordBox<2> box(Coord<2>(), dim);
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        grid.set(*i, JacobiCell(i->x() + 100.0 * i->y()));
+    }
+    MPIIO<JacobiCell, Torus> io;
+    Region<2> all;
+    all << box;
+    io.writeRegion(grid, dim, 3, 9, "torus.mpiio", all);
+    // rewrite one row through a streak left of the box: on the torus it
+    // lands on row 2, columns 7 and 8 (normalised origin), other bytes stay
+    Grid<JacobiCell, Torus> changed = grid;
+    changed.set(Coord<2>(7, 2), JacobiCell(-1.0));
+    changed.set(Coord<2>(8, 2), JacobiCell(-2.0));
+    Region<2> part;
+    part << Streak<2>(Coord<2>(-2, 2), 0);
+    io.writeRegion(changed, dim, 3, 9, "torus.mpiio", part);
+    Grid<JacobiCell, Torus> back(dim);
+    io.readRegion(&back, "torus.mpiio", all);
+    CHECK_EQ(back.getEdge().temp, 7.0);
+    int differ = 0;
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        differ += back.get(*i).temp != changed.get(*i).temp;
+    }
+    CHECK_EQ(differ, 0);
+    CHECK_EQ(back.get(Coord<2>(7, 2)).temp, -1.0);
+    Coord<2> d;
+    unsigned step, maxSteps;
+    io.readMetadata(&d, &step, &maxSteps, "torus.mpiio");
+    CHECK(d == dim && step == 3u && maxSteps == 9u);
+}
+
+LGD_TEST(mpiio_errors)
+{
+    MPIIO<JacobiCell, Topologies::Cube<3>::Topology> io;
+    Coord<3> d;
+    unsigned s, m;
+    CHECK_THROWS(io.readMetadata(&d, &s, &m, "no_such_file.mpiio"), std::runtime_error);
+    {
+        std::ofstream f("short.mpiio", std::ios::binary);
+        const int32_t dims[3] = {4, 4, 4};
+        f.write(reinterpret_cast<const char *>(dims), sizeof(dims));
+    }
+    CHECK_THROWS(io.readMetadata(&d, &s, &m, "short.mpiio"), std::runtime_error);
+    CHECK_THROWS(MPIIOInitializer<JacobiCell>("short.mpiio"), std::runtime_error);
 }
 
 int main(int argc, char **argv)
