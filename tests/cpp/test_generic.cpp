@@ -14,6 +14,8 @@
 #include <libgeodecomp/hip/genericfamily.h>
 #include <libgeodecomp/io/mpiioinitializer.h>
 #include <libgeodecomp/io/mpiiowriter.h>
+#include <libgeodecomp/misc/patternoptimizer.h>
+#include <libgeodecomp/parallelization/autotuningsimulator.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
 
@@ -316,6 +318,22 @@ LGD_TEST(probe_restart_from_snapshot)
         invalid += !b.valid || b.cycle != 24u;
     }
     CHECK_EQ(differ, 0);
+    CHECK_EQ(invalid, 0);
+}
+
+// the autotuner over the generic family's work-group shapes
+LGD_TEST(autotuning_generic_cells)
+{
+    typedef ProbeCell<2, Torus2, 2, Stencils::Moore<2, 1> > Cell;
+    AutoTuningSimulator<Cell, PatternOptimizer> tuned(new ProbeInitializer<Cell>(Coord<2>(300, 200), 30, 8), 4);
+    tuned.run();
+    std::printf("  probe 300x200: best %s\n", tuned.bestParameters().toString().c_str());
+    int invalid = 0;
+    CoordBox<2> box(Coord<2>(), Coord<2>(300, 200));
+    for (CoordBox<2>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const Cell c = tuned.getGrid()->get(*i);
+        invalid += !c.valid || c.cycle != 60u;
+    }
     CHECK_EQ(invalid, 0);
 }
 

@@ -11,6 +11,8 @@
 #include <libgeodecomp/io/mpiioinitializer.h>
 #include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
+#include <libgeodecomp/misc/patternoptimizer.h>
+#include <libgeodecomp/parallelization/autotuningsimulator.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
 #include <serialsimulator.h>
 
@@ -438,6 +440,87 @@ LGD_TEST(restart_from_snapshot_families)
             }
             return a.flags == b.flags;
         });
+}
+
+// ---------------------------------------------------------------------
+// the autotuner over HipSimulator (misc/cudasimulationfactory.h and
+// parallelization/autotuningsimulator.h counterparts)
+// ---------------------------------------------------------------------
+template<typename CELL, int DIM>
+bool sameGrid(const GridBase<CELL, DIM> *a, const GridBase<CELL, DIM> *b)
+{
+    CoordBox<DIM> box = a->boundingBox();
+    for (typename CoordBox<DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+        if (!(a->get(*i) == b->get(*i))) {
+            return false;
+        }
+    }
+    return true;
+}
+
+LGD_TEST(hip_simulation_factory_every_jacobi_variant)
+{
+    const Coord<3> dim(45, 38, 31);
+    std::shared_ptr<VarStepInitializerProxy<JacobiCell> > proxy(
+        new VarStepInitializerProxy<JacobiCell>(new JacobiInitializer(dim, 19, 4, 0.25)));
+    HipSimulationFactory<JacobiCell> factory(proxy);
+    CHECK(factory.parameters().contains("Variant"));
+    SerialSimulator<JacobiCell> ref(proxy->clone());
+    ref.run();
+    SimulationParameters p = factory.parameters();
+    for (int k = 0; k < 8; ++k) {
+        p["Variant"].setValue(k);
+        const double fit = factory(p);
+        CHECK(fit < 0 && fit > -60);
+        factory.parameters()["Variant"].setValue(k);
+        std::unique_ptr<Simulator<JacobiCell> > sim(factory());
+        sim->run();
+        CHECK(sameGrid(dynamic_cast<MonolithicSimulator<JacobiCell> *>(sim.get())->getGrid(), ref.getGrid()));
+    }
+}
+
+LGD_TEST(hip_simulation_factory_rejected_configuration_rates_worst)
+{
+    std::shared_ptr<VarStepInitializerProxy<ConwayCell> > proxy(
+        new VarStepInitializerProxy<ConwayCell>(new RandomLifeInitializer(Coord<2>(300, 200), 40, 8, 0.3)));
+    HipSimulationFactory<ConwayCell> factory(proxy);
+    SimulationParameters p = factory.parameters();
+    CHECK(factory(p) < 0 && factory(p) > -60);
+    // 32 generations per launch need more than the 64 register rows of
+    // variant 3: the family rejects it (before any launch) with a runtime
+    // error, which rates the worst fitness
+    p["StepsPerLaunch"].setValue(31);
+    p["Variant"].setValue(2);
+    CHECK_EQ((int)p["StepsPerLaunch"], 32);
+    CHECK_EQ((int)p["Variant"], 3);
+    CHECK_EQ(factory(p), std::numeric_limits<double>::lowest());
+}
+
+LGD_TEST(autotuning_simulator_jacobi_and_life)
+{
+    std::vector<Event> evTuned, evRef;
+    Snapshots<JacobiCell> sTuned, sRef;
+    AutoTuningSimulator<JacobiCell, PatternOptimizer> tuned(new JacobiInitializer(Coord<3>(64, 64, 64), 40, 2, 0.5),
+                                                           6);
+    tuned.addWriter(new RecordingWriter<JacobiCell>(&evTuned, 10, &Snapshots<JacobiCell>::hook, &sTuned));
+    tuned.run();
+    SerialSimulator<JacobiCell> ref(new JacobiInitializer(Coord<3>(64, 64, 64), 40, 2, 0.5));
+    ref.addWriter(new RecordingWriter<JacobiCell>(&evRef, 10, &Snapshots<JacobiCell>::hook, &sRef));
+    ref.run();
+    std::printf("  Jacobi 64^3: best %s after tuning runs of %u steps (fitness %g)\n",
+                tuned.bestParameters().toString().c_str(), tuned.stepsPerTuningRun(), tuned.bestTuningFitness());
+    CHECK(evTuned == evRef);
+    CHECK(sTuned.grids == sRef.grids);
+    CHECK_EQ(tuned.getStep(), 40u);
+    CHECK(tuned.stepsPerTuningRun() >= 1u && tuned.stepsPerTuningRun() <= 40u);
+
+    AutoTuningSimulator<ConwayCell, PatternOptimizer> life(new RandomLifeInitializer(Coord<2>(1024, 768), 300, 6),
+                                                           5);
+    life.run();
+    SerialSimulator<ConwayCell> lref(new RandomLifeInitializer(Coord<2>(1024, 768), 300, 6));
+    lref.run();
+    std::printf("  Life 1024x768: best %s\n", life.bestParameters().toString().c_str());
+    CHECK(sameGrid(life.getGrid(), lref.getGrid()));
 }
 
 int main(int argc, char **argv)

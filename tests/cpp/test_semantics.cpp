@@ -9,6 +9,9 @@
 #include <libgeodecomp/io/mpiioinitializer.h>
 #include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
+#include <libgeodecomp/io/varstepinitializerproxy.h>
+#include <libgeodecomp/misc/patternoptimizer.h>
+#include <libgeodecomp/misc/simulationfactory.h>
 
 #include <cstdio>
 #include <cstring>
@@ -569,6 +572,141 @@ LGD_TEST(mpiio_errors)
     }
     CHECK_THROWS(io.readMetadata(&d, &s, &m, "short.mpiio"), std::runtime_error);
     CHECK_THROWS(MPIIOInitializer<JacobiCell>("short.mpiio"), std::runtime_error);
+}
+
+// ---------------------------------------------------------------------
+// autotuner building blocks (misc/simulationparameters.h, optimizer.h,
+// patternoptimizer.cpp, simulationfactory.h, io/varstepinitializerproxy.h)
+// ---------------------------------------------------------------------
+LGD_TEST(simulation_parameters_semantics)
+{
+    SimulationParameters p;
+    p.addParameter("BlockDimX", 1, 128);
+    p.addParameter("Coarse", 0, 64, 4.0);
+    p.addParameter("Width", std::vector<int>{64, 128, 256});
+    CHECK_EQ(p.size(), (std::size_t)3);
+    CHECK_EQ((int)p["BlockDimX"], 1);
+    p["BlockDimX"].setValue(10.7);                 // index rounds down to 10
+    CHECK_EQ((int)p["BlockDimX"], 11);
+    CHECK_EQ(p["BlockDimX"].getValue(), 10.0);
+    p["BlockDimX"].setValue(500);                  // the interval is [1, 128)
+    CHECK_EQ((int)p["BlockDimX"], 127);
+    p["BlockDimX"].setValue(-3);
+    CHECK_EQ((int)p["BlockDimX"], 1);
+    p["BlockDimX"] += 5;
+    CHECK_EQ((int)p["BlockDimX"], 6);
+    p["Coarse"].setValue(10);                      // multiples of the granularity
+    CHECK_EQ((int)p["Coarse"], 8);
+    p["Coarse"] += 3;
+    CHECK_EQ((int)p["Coarse"], 8);
+    p["Coarse"] += 100;
+    CHECK_EQ((int)p["Coarse"], 60);
+    CHECK_EQ((int)p["Width"], 64);
+    p["Width"].setValue(1.9);
+    CHECK(p["Width"] == 128);
+    p["Width"] += 10;
+    CHECK_EQ((int)p["Width"], 256);
+    CHECK_EQ(p["Width"].getMax(), 3.0);
+    CHECK_THROWS((void)(double)p["Width"], std::logic_error);
+    CHECK_THROWS(p["NoSuch"], std::invalid_argument);
+    SimulationParameters copy(p);                  // deep copy
+    copy["Width"].setValue(0);
+    CHECK_EQ((int)p["Width"], 256);
+    CHECK_EQ((int)copy["Width"], 64);
+    p.replaceParameter("Width", std::vector<int>{1, 2});
+    CHECK_EQ((int)p["Width"], 1);
+}
+
+class Paraboloid : public Optimizer::Evaluator
+{
+public:
+    int calls = 0;
+    double operator()(const SimulationParameters& p)
+    {
+        ++calls;
+        const double x = (int)p["x"], y = (int)p["y"], z = (int)p["z"];
+        return -((x - 37) * (x - 37) + 3 * (y - 5) * (y - 5) + 10 * (z - 128) * (z - 128) / 4096.0);
+    }
+};
+
+LGD_TEST(pattern_optimizer_finds_maximum)
+{
+    SimulationParameters p;
+    p.addParameter("x", 0, 100);
+    p.addParameter("y", 0, 16);
+    p.addParameter("z", std::vector<int>{16, 32, 64, 128, 256, 512});
+    PatternOptimizer opt(p);
+    Paraboloid f;
+    SimulationParameters best = opt(200, f);
+    CHECK_EQ((int)best["x"], 37);
+    CHECK_EQ((int)best["y"], 5);
+    CHECK_EQ((int)best["z"], 128);
+    CHECK_EQ(opt.getFitness(), 0.0);
+    CHECK(f.calls < 100 * 16 * 6 / 20);           // far fewer than the 9600 points
+    // explicit step widths
+    PatternOptimizer small(p, std::vector<double>{1, 1, 1});
+    Paraboloid g;
+    best = small(500, g);
+    CHECK(best["x"] == 37 && best["y"] == 5 && best["z"] == 128);
+    CHECK_THROWS(PatternOptimizer(p, std::vector<double>{1, 1}), std::invalid_argument);
+}
+
+LGD_TEST(var_step_initializer_proxy)
+{
+    VarStepInitializerProxy<ConwayCell> proxy(new RandomLifeInitializer(Coord<2>(20, 10), 50, 1, 0.3, 7));
+    CHECK_EQ(proxy.startStep(), 7u);
+    CHECK_EQ(proxy.maxSteps(), 50u);
+    proxy.setMaxSteps(3);
+    CHECK_EQ(proxy.maxSteps(), 10u);
+    std::unique_ptr<ClonableInitializer<ConwayCell> > clone(proxy.clone());
+    CHECK_EQ(clone->maxSteps(), 10u);
+    proxy.resetMaxSteps();
+    CHECK_EQ(proxy.maxSteps(), 50u);
+    Grid<ConwayCell> a(Coord<2>(20, 10)), b(Coord<2>(20, 10));
+    proxy.grid(&a);
+    clone->grid(&b);
+    CHECK(a == b);
+}
+
+// a test-only factory over the oracle simulator: the factory mechanics
+// (fitness is minus the run time, failures rate worst, plugins cloned in)
+class OracleLifeFactory : public SimulationFactory<ConwayCell>
+{
+public:
+    explicit OracleLifeFactory(InitPtr init) : SimulationFactory<ConwayCell>(init)
+    {
+        parameterSet.addParameter("Fail", std::vector<int>{0, 1});
+    }
+    std::string name() const { return "oracle"; }
+
+protected:
+    Simulator<ConwayCell> *buildSimulator(InitPtr init, const SimulationParameters& p) const
+    {
+        if ((int)p["Fail"]) {
+            throw std::runtime_error("configuration rejected");
+        }
+        SerialSimulator<ConwayCell> *sim = new SerialSimulator<ConwayCell>(init->clone());
+        addPlugins(sim);
+        return sim;
+    }
+};
+
+LGD_TEST(simulation_factory_mechanics)
+{
+    std::shared_ptr<VarStepInitializerProxy<ConwayCell> > proxy(
+        new VarStepInitializerProxy<ConwayCell>(new RandomLifeInitializer(Coord<2>(40, 30), 12, 3)));
+    OracleLifeFactory factory(proxy);
+    std::vector<Event> events;
+    factory.addWriter(RecordingWriter<ConwayCell>(&events, 4));
+    SimulationParameters p = factory.parameters();
+    const double ok = factory(p);
+    CHECK(ok < 0 && ok > -10);
+    CHECK_EQ(events.size(), (std::size_t)4);        // 0, 4, 8, 12 through the cloned writer
+    p["Fail"].setValue(1);
+    CHECK_EQ(factory(p), std::numeric_limits<double>::lowest());
+    std::unique_ptr<Simulator<ConwayCell> > sim(factory());
+    sim->run();
+    CHECK_EQ(sim->getStep(), 12u);
 }
 
 int main(int argc, char **argv)
