@@ -261,14 +261,25 @@ JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     }
     const TbShape tb = tb_shape(L.variant);
     if (tb.k > 0) {
-        // temporal blocking: tiles of (64-2K) x (TY-2K) output columns; z
-        // chunks of 64 planes (2K extra pipeline planes each), shrunk until
-        // the grid has >= 1024 work-groups
+        // temporal blocking: tiles of (64-2K) x (TY-2K) output columns; a z
+        // chunk of C planes costs its work-group C + 2K pipeline planes.
+        // One 64 x TY work-group fits a CU at a time (register windows), so
+        // the launch takes about ceil(work-groups / CUs) rounds of C + 2K
+        // planes: pick C in {64, 32, 16, 8} minimising that (measured at
+        // 128^3: C = 32 gives 281 GLUPS where 8 gave 250; at 512^3 it picks
+        // 64, the measured best)
         const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
         const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty - 2 * tb.k);
         if (!tile || tile->bz <= 0) {
-            L.zchunk = 64;
-            while (L.zchunk > 8 && (uint64_t)gx * gy * lgdhip::ceil_div(g.nz, L.zchunk) < 1024) L.zchunk /= 2;
+            uint64_t best = ~(uint64_t)0;
+            for (int c = 64; c >= 8; c /= 2) {
+                const uint64_t groups = (uint64_t)gx * gy * lgdhip::ceil_div(g.nz, c);
+                const uint64_t cost = (groups + lgdhip::kCUs - 1) / lgdhip::kCUs * (uint64_t)(c + 2 * tb.k);
+                if (cost < best) {
+                    best = cost;
+                    L.zchunk = c;
+                }
+            }
         }
         L.by = tb.ty;
         L.block = dim3(64, tb.ty, 1);

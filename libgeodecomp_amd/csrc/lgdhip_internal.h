@@ -61,6 +61,8 @@ inline unsigned ceil_div(int64_t a, int64_t b) { return (unsigned)((a + b - 1) /
 // XCD c the contiguous tile range [c*per, (c+1)*per), per = ceil(T/8), in
 // order; ids mapped past T exit (before any barrier).
 constexpr unsigned kXcds = 8;
+// compute units of an MI355X (32 per XCD), for launch-size models
+constexpr unsigned kCUs = 256;
 inline unsigned xcd_per(unsigned tiles) { return (tiles + kXcds - 1) / kXcds; }
 inline unsigned xcd_groups(unsigned tiles) { return kXcds * xcd_per(tiles); }
 __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned per, bool swizzle) {
