@@ -208,6 +208,22 @@ public:
         check(lgdhip_cell_grid_fill_halo(b->get(), &g, edge->get(), wrap, stream.get()), "lgdhip_cell_grid_fill_halo");
     }
 
+    void uploadRegion(const Grid<CELL, Topology>& host, const Region<DIM>& region)
+    {
+        requireBox(host);
+        const std::vector<Streak<DIM> > streaks = relativeStreaks(region, host.boundingBox());
+        const long hy = DIM >= 2 ? RADIUS : 0, hz = DIM == 3 ? RADIUS : 0;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<DIM>& st = streaks[k];
+            const long y = coordOf(st.origin, 1), z = coordOf(st.origin, 2);
+            const long i = ((z + hz) * g.py + (y + hy)) * g.px + (st.origin.x() + RADIUS);
+            const CELL *src = host.data() + st.origin.toIndex(host.boundingBox().dimensions);
+            check(lgdhip_memcpy_h2d(static_cast<CELL *>(a->get()) + i, src,
+                                    sizeof(CELL) * (std::size_t)(st.endX - st.origin.x()), stream.get()),
+                  "lgdhip_memcpy_h2d");
+        }
+    }
+
     void download(Grid<CELL, Topology> *host)
     {
         requireBox(*host);

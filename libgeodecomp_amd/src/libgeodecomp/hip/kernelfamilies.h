@@ -3,9 +3,11 @@
 //
 // A family owns the device-resident state of one simulation (padded SoA
 // grid, ping-pong buffers, workspaces, a HIP stream) and offers
-//     upload(hostGrid)   host cells -> device state
-//     download(&grid)    device state -> host cells
-//     run(nanoSteps)     advance the device state
+//     upload(hostGrid)                  host cells -> device state
+//     uploadRegion(hostGrid, region)    only the region's cells (optional;
+//                                       HipSimulator uses it after steerers)
+//     download(&grid)                   device state -> host cells
+//     run(nanoSteps)                    advance the device state
 // A cell opts in with APITraits::HasHipKernel<FAMILY> or, after its
 // definition, LIBGEODECOMP_REGISTER_HIP_KERNEL(CELL, FAMILY).  The family's
 // kernels compute exactly the update() of the cell they stand for; the
@@ -22,6 +24,7 @@
 #ifndef LIBGEODECOMP_HIP_KERNELFAMILIES_H
 #define LIBGEODECOMP_HIP_KERNELFAMILIES_H
 
+#include <libgeodecomp/geometry/region.h>
 #include <libgeodecomp/geometry/stencils.h>
 #include <libgeodecomp/misc/apitraits.h>
 #include <libgeodecomp/storage/grid.h>
@@ -93,6 +96,34 @@ struct Tuning
     const lgdhip_tile *get() const { return useTile ? &tile : 0; }
 };
 
+// element index of (x, y, z) of member m in a padded grid, in either member
+// layout (the address formula documented with lgdhip_grid)
+inline int64_t elementIndex(const lgdhip_grid& g, int m, int64_t x, int64_t y, int64_t z)
+{
+    const bool slabs = g.layout == LGDHIP_LAYOUT_Z_SLABS;
+    const int64_t ms = slabs ? g.sx * g.sy : g.sx * g.sy * g.sz;
+    const int64_t zs = slabs ? (int64_t)g.members * g.sx * g.sy : g.sx * g.sy;
+    return m * ms + (z + g.oz) * zs + (y + g.oy) * g.sx + (x + g.ox);
+}
+
+// the region's streaks relative to the host grid's origin
+template<int DIM>
+std::vector<Streak<DIM> > relativeStreaks(const Region<DIM>& region, const CoordBox<DIM>& box)
+{
+    std::vector<Streak<DIM> > out = region.streaks();
+    for (std::size_t i = 0; i < out.size(); ++i) {
+        out[i].endX -= box.origin.x();
+        out[i].origin = out[i].origin - box.origin;
+    }
+    return out;
+}
+
+template<int DIM>
+int64_t coordOf(const Coord<DIM>& c, int d)
+{
+    return d < DIM ? c[d] : 0;
+}
+
 // wrapped axes of the cell's topology as a mask (bit d = axis d)
 template<typename CELL>
 int wrapMask()
@@ -161,6 +192,21 @@ public:
         check(lgdhip_grid_fill_edge(a->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_fill_edge(b->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_load(a->get(), &g, dense.data(), stream.get()), "lgdhip_grid_load");
+    }
+
+    void uploadRegion(const Grid<CELL, Topology>& host, const Region<3>& region)
+    {
+        std::vector<double> line;
+        const std::vector<Streak<3> > streaks = relativeStreaks(region, host.boundingBox());
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<3>& st = streaks[k];
+            line.resize(st.endX - st.origin.x());
+            for (std::size_t i = 0; i < line.size(); ++i) {
+                line[i] = host.data()[(st.origin + Coord<3>((int)i, 0, 0)).toIndex(host.boundingBox().dimensions)].*MEMBER;
+            }
+            double *dst = (double *)a->get() + elementIndex(g, 0, st.origin.x(), st.origin.y(), st.origin.z());
+            check(lgdhip_memcpy_h2d(dst, line.data(), line.size() * sizeof(double), stream.get()), "lgdhip_memcpy_h2d");
+        }
     }
 
     void download(Grid<CELL, Topology> *host)
@@ -256,6 +302,40 @@ public:
             return;
         }
         check(lgdhip_life2d_pack((const uint8_t *)grid->get(), &g, pa, stream.get()), "lgdhip_life2d_pack");
+    }
+
+    // wrapped axes: bytes into the byte grid; otherwise the packed words
+    // covering each streak, packed from the (current) host row
+    void uploadRegion(const Grid<CELL, Topology>& host, const Region<2>& region)
+    {
+        const Coord<2> dim = host.boundingBox().dimensions;
+        const std::vector<Streak<2> > streaks = relativeStreaks(region, host.boundingBox());
+        const int64_t nw = (g.nx + 31) / 32;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<2>& st = streaks[k];
+            const int y = st.origin.y();
+            const CELL *row = host.data() + (std::size_t)y * dim.x();
+            if (wrap) {
+                std::vector<uint8_t> bytes(st.endX - st.origin.x());
+                for (std::size_t i = 0; i < bytes.size(); ++i) {
+                    bytes[i] = (row[st.origin.x() + i].*MEMBER) ? 1 : 0;
+                }
+                uint8_t *dst = (uint8_t *)grid->get() + elementIndex(g, 0, st.origin.x(), y, 0);
+                check(lgdhip_memcpy_h2d(dst, bytes.data(), bytes.size(), stream.get()), "lgdhip_memcpy_h2d");
+                continue;
+            }
+            const int64_t w0 = st.origin.x() / 32, w1 = (st.endX - 1) / 32;
+            std::vector<uint32_t> words(w1 - w0 + 1, 0u);
+            for (int64_t w = w0; w <= w1; ++w) {
+                for (int i = 0; i < 32 && 32 * w + i < g.nx; ++i) {
+                    if (row[32 * w + i].*MEMBER) {
+                        words[w - w0] |= 1u << i;
+                    }
+                }
+            }
+            check(lgdhip_memcpy_h2d(pa + y * nw + w0, words.data(), words.size() * 4, stream.get()),
+                  "lgdhip_memcpy_h2d");
+        }
     }
 
     void download(Grid<CELL, Topology> *host)
@@ -355,6 +435,33 @@ public:
         check(lgdhip_grid_fill_edge(flags->get(), &fg, &edgeFlags, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_load(a->get(), &g, dense.data(), stream.get()), "lgdhip_grid_load");
         check(lgdhip_grid_load(flags->get(), &fg, fl.data(), stream.get()), "lgdhip_grid_load");
+    }
+
+    void uploadRegion(const Grid<CELL, Topology>& host, const Region<3>& region)
+    {
+        const Coord<3> dim = host.boundingBox().dimensions;
+        const std::vector<Streak<3> > streaks = relativeStreaks(region, host.boundingBox());
+        std::vector<float> line;
+        std::vector<uint8_t> fl;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<3>& st = streaks[k];
+            const std::size_t n = st.endX - st.origin.x();
+            const CELL *cells = host.data() + st.origin.toIndex(dim);
+            line.resize(n);
+            for (int q = 0; q < 19; ++q) {
+                for (std::size_t i = 0; i < n; ++i) {
+                    line[i] = (float)((double)cells[i].f[q] - weight(q));
+                }
+                float *dst = (float *)a->get() + elementIndex(g, q, st.origin.x(), st.origin.y(), st.origin.z());
+                check(lgdhip_memcpy_h2d(dst, line.data(), n * sizeof(float), stream.get()), "lgdhip_memcpy_h2d");
+            }
+            fl.resize(n);
+            for (std::size_t i = 0; i < n; ++i) {
+                fl[i] = cells[i].flags;
+            }
+            uint8_t *fdst = (uint8_t *)flags->get() + elementIndex(fg, 0, st.origin.x(), st.origin.y(), st.origin.z());
+            check(lgdhip_memcpy_h2d(fdst, fl.data(), n, stream.get()), "lgdhip_memcpy_h2d");
+        }
     }
 
     void download(Grid<CELL, Topology> *host)

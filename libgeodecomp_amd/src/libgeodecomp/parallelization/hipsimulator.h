@@ -24,13 +24,18 @@
 //  * between two plugin callbacks run() advances all steps in one call into
 //    the family (temporal blocking / multi-step launches), instead of one
 //    launch per nano-step;
-//  * the device state is downloaded only when a plugin looks at it.
+//  * the device state is downloaded only when a plugin looks at it;
+//  * after steerers, only the cells they changed are uploaded (the host grid
+//    records every write through GridBase, see storage/trackinggrid.h),
+//    when the family supports region uploads and the change is small; a
+//    changed edge cell or a large change uploads the whole grid.
 #ifndef LIBGEODECOMP_PARALLELIZATION_HIPSIMULATOR_H
 #define LIBGEODECOMP_PARALLELIZATION_HIPSIMULATOR_H
 
 #include <libgeodecomp/hip/kernelfamilies.h>
 #include <libgeodecomp/parallelization/monolithicsimulator.h>
 #include <libgeodecomp/storage/grid.h>
+#include <libgeodecomp/storage/trackinggrid.h>
 
 #include <memory>
 #include <type_traits>
@@ -48,6 +53,15 @@ class IsComplete : public std::false_type {};
 template<typename T>
 class IsComplete<T, decltype(void(sizeof(T)))> : public std::true_type {};
 
+// does FAMILY offer uploadRegion(grid, region)?
+template<typename FAMILY, typename GRID, typename REGION, typename = void>
+class HasUploadRegion : public std::false_type {};
+template<typename FAMILY, typename GRID, typename REGION>
+class HasUploadRegion<FAMILY, GRID, REGION,
+                      decltype(std::declval<FAMILY&>().uploadRegion(std::declval<const GRID&>(),
+                                                                    std::declval<const REGION&>()))>
+    : public std::true_type {};
+
 }
 
 template<typename CELL_TYPE>
@@ -58,7 +72,7 @@ public:
     typedef typename APITraits::SelectHipKernel<CELL_TYPE>::Value RegisteredFamily;
     typedef typename std::conditional<std::is_void<RegisteredFamily>::value, HipKernels::Generic<CELL_TYPE>,
                                       RegisteredFamily>::type Family;
-    typedef Grid<CELL_TYPE, Topology> GridType;
+    typedef TrackingGrid<CELL_TYPE, Topology> GridType;
     typedef typename Steerer<CELL_TYPE>::SteererFeedback SteererFeedback;
     static const int DIM = Topology::DIM;
     static const unsigned NANO_STEPS = APITraits::SelectNanoSteps<CELL_TYPE>::VALUE;
@@ -154,12 +168,23 @@ public:
     // number of device sweeps issued so far (for tests and tracing)
     unsigned long long sweepsIssued() const { return sweeps; }
 
+    // host -> device transfers so far: whole-grid uploads, region uploads
+    // after steerers, and the cells those region uploads carried
+    unsigned long long fullUploads() const { return nFull; }
+    unsigned long long regionUploads() const { return nRegion; }
+    unsigned long long regionCellsUploaded() const { return regionCells; }
+
+    // a region upload is used when it carries at most this share of the grid
+    static constexpr double REGION_UPLOAD_MAX_SHARE = 0.125;
+
 protected:
     GridType hostGrid;
     std::unique_ptr<Family> device;
     bool hostCurrent;
     bool deviceCurrent;
     unsigned long long sweeps = 0;
+    Region<DIM> pendingRegion;   // host edits not yet on the device
+    unsigned long long nFull = 0, nRegion = 0, regionCells = 0;
 
     void syncHost()
     {
@@ -174,6 +199,48 @@ protected:
         if (!deviceCurrent) {
             device->upload(hostGrid);
             deviceCurrent = true;
+            pendingRegion = Region<DIM>();
+            ++nFull;
+            return;
+        }
+        if (!pendingRegion.empty()) {
+            uploadPending(HipKernels::HasUploadRegion<Family, Grid<CELL_TYPE, Topology>, Region<DIM> >());
+        }
+    }
+
+    void uploadPending(std::true_type)
+    {
+        device->uploadRegion(hostGrid, pendingRegion);
+        ++nRegion;
+        regionCells += pendingRegion.size();
+        pendingRegion = Region<DIM>();
+    }
+
+    void uploadPending(std::false_type)
+    {
+        device->upload(hostGrid);
+        ++nFull;
+        pendingRegion = Region<DIM>();
+    }
+
+    // after steerers: remember what they changed for the next upload
+    void noteSteererEdits()
+    {
+        if (!deviceCurrent) {
+            return;   // a whole-grid upload is due anyway
+        }
+        const Region<DIM>& changed = hostGrid.changedRegion();
+        if (hostGrid.edgeWasChanged()) {
+            deviceCurrent = false;
+            return;
+        }
+        const std::vector<Streak<DIM> > streaks = changed.streaks();
+        for (std::size_t i = 0; i < streaks.size(); ++i) {
+            pendingRegion << streaks[i];
+        }
+        const bool region = HipKernels::HasUploadRegion<Family, Grid<CELL_TYPE, Topology>, Region<DIM> >::value;
+        if (!region || (double)pendingRegion.size() > REGION_UPLOAD_MAX_SHARE * (double)hostGrid.size()) {
+            deviceCurrent = false;
         }
     }
 
@@ -234,13 +301,22 @@ protected:
         TimeInput t(&chronometer);
         Region<DIM> simArea;
         simArea << hostGrid.boundingBox();
+        bool any = false;
         for (std::size_t i = 0; i < steerers.size(); ++i) {
             if ((event != STEERER_NEXT_STEP) || (stepNum % steerers[i]->getPeriod() == 0)) {
                 syncHost();
+                if (!any) {
+                    hostGrid.startTracking();
+                    any = true;
+                }
                 steerers[i]->nextStep(&hostGrid, simArea, gridDim, stepNum, event, 0, true, feedback);
-                // the steerer may have changed cells: upload before the next sweep
-                deviceCurrent = false;
             }
+        }
+        if (any) {
+            // the steerers may have changed cells: upload them before the
+            // next sweep
+            hostGrid.stopTracking();
+            noteSteererEdits();
         }
     }
 

@@ -523,6 +523,110 @@ LGD_TEST(autotuning_simulator_jacobi_and_life)
     CHECK(sameGrid(life.getGrid(), lref.getGrid()));
 }
 
+// ---------------------------------------------------------------------
+// steerer edits travel as region uploads (only the changed streaks)
+// ---------------------------------------------------------------------
+template<typename CELL>
+bool exactlyEqual(const CELL& a, const CELL& b)
+{
+    return a == b;
+}
+
+template<typename CELL, typename EDIT>
+void regionUploadCase(Initializer<CELL> *(*make)(), unsigned steererPeriod, EDIT edit, bool expectRegion,
+                      bool (*same)(const CELL&, const CELL&) = &exactlyEqual<CELL>)
+{
+    struct EditSteerer : public Steerer<CELL>
+    {
+        EDIT edit;
+        EditSteerer(unsigned period, EDIT edit) : Steerer<CELL>(period), edit(edit) {}
+        void nextStep(typename Steerer<CELL>::GridType *grid, const Region<Steerer<CELL>::Topology::DIM>&,
+                      const Coord<Steerer<CELL>::Topology::DIM>&, unsigned step, SteererEvent event, std::size_t, bool,
+                      typename Steerer<CELL>::SteererFeedback *)
+        {
+            if (event == STEERER_NEXT_STEP) {
+                edit(grid, step);
+            }
+        }
+    };
+    HipSimulator<CELL> hip(make());
+    SerialSimulator<CELL> ref(make());
+    hip.addSteerer(new EditSteerer(steererPeriod, edit));
+    ref.addSteerer(new EditSteerer(steererPeriod, edit));
+    hip.run();
+    ref.run();
+    typedef typename HipSimulator<CELL>::Topology Topology;
+    CoordBox<Topology::DIM> box = hip.getGrid()->boundingBox();
+    int differ = 0;
+    for (typename CoordBox<Topology::DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+        differ += !same(hip.getGrid()->get(*i), ref.getGrid()->get(*i));
+    }
+    std::printf("  full uploads %llu, region uploads %llu (%llu cells)\n", hip.fullUploads(), hip.regionUploads(),
+                hip.regionCellsUploaded());
+    CHECK_EQ(differ, 0);
+    if (expectRegion) {
+        CHECK_EQ(hip.fullUploads(), 1ull);
+        CHECK(hip.regionUploads() > 0);
+    } else {
+        CHECK(hip.fullUploads() > 1);
+    }
+}
+
+static Initializer<ConwayCell> *makeSteeredLife() { return new RandomLifeInitializer(Coord<2>(517, 301), 60, 42); }
+static Initializer<TorusLifeCell> *makeSteeredTorus()
+{
+    return new RandomCellInitializer<TorusLifeCell, 2>(Coord<2>(131, 77), 30, 3,
+                                                       [](double u) { return TorusLifeCell(u < 0.3); },
+                                                       TorusLifeCell(false));
+}
+static Initializer<JacobiCell> *makeSteeredJacobi() { return new JacobiInitializer(Coord<3>(40, 33, 29), 24, 9, 0.5); }
+static Initializer<LBMCavityCell<float> > *makeSteeredLBM() { return new CavityInitializer<float>(Coord<3>(24, 20, 16), 20, 9); }
+
+LGD_TEST(steerer_region_uploads)
+{
+    // Life (bit-packed device state): a glider planted every 5 steps,
+    // including streaks that cross 32-cell word boundaries
+    regionUploadCase<ConwayCell>(&makeSteeredLife, 5, [](GridBase<ConwayCell, 2> *g, unsigned step) {
+        plantGlider(g, step, 0, 0);
+        Streak<2> s(Coord<2>(29, 100 + (int)step % 50), 70);
+        std::vector<ConwayCell> row(41, ConwayCell(true));
+        g->set(s, row.data());
+    }, true);
+    // Torus Life (byte device state), writes across the wrapped seam
+    regionUploadCase<TorusLifeCell>(&makeSteeredTorus, 4, [](GridBase<TorusLifeCell, 2> *g, unsigned step) {
+        for (int x = -2; x < 3; ++x) {
+            g->set(Coord<2>(x, (int)step % 77), TorusLifeCell(true));
+        }
+    }, true);
+    // Jacobi: a hot streak every 3 steps
+    regionUploadCase<JacobiCell>(&makeSteeredJacobi, 3, [](GridBase<JacobiCell, 3> *g, unsigned step) {
+        std::vector<JacobiCell> row(17, JacobiCell(2.0 + step));
+        g->set(Streak<3>(Coord<3>(5, (int)step % 33, 7), 22), row.data());
+    }, true);
+    // LBM: populations and flags of one cell every 4 steps
+    regionUploadCase<LBMCavityCell<float> >(&makeSteeredLBM, 4,
+        [](GridBase<LBMCavityCell<float>, 3> *g, unsigned step) {
+            const Coord<3> c(3 + (int)step % 15, 10, 8);
+            LBMCavityCell<float> cell = g->get(c);
+            cell.f[2] += 1e-3f;
+            cell.f[3] -= 1e-3f;
+            g->set(c, cell);
+        }, true,
+        [](const LBMCavityCell<float>& a, const LBMCavityCell<float>& b) {
+            // FP32 populations against the FP32 oracle: within rounding
+            for (int q = 0; q < 19; ++q) {
+                if (std::fabs(a.f[q] - b.f[q]) > 1e-5f) {
+                    return false;
+                }
+            }
+            return a.flags == b.flags;
+        });
+    // an edge change needs a whole-grid upload
+    regionUploadCase<JacobiCell>(&makeSteeredJacobi, 6, [](GridBase<JacobiCell, 3> *g, unsigned step) {
+        g->setEdge(JacobiCell(0.1 * step));
+    }, false);
+}
+
 int main(int argc, char **argv)
 {
     int n = 0;
