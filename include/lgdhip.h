@@ -68,6 +68,9 @@ typedef void *lgdhip_event_t;   /* hipEvent_t  */
  * Both layouts take members*sx*sy*sz elements. */
 #define LGDHIP_LAYOUT_MEMBER_PLANES 0
 #define LGDHIP_LAYOUT_Z_SLABS       1
+/* A view: a sub-box of another grid's logical box sharing its memory (see
+ * lgdhip_grid_view). */
+#define LGDHIP_GRID_VIEW            0x1
 
 typedef struct lgdhip_grid {
     int32_t dim;         /* 2 or 3                                  */
@@ -75,7 +78,7 @@ typedef struct lgdhip_grid {
     int32_t elem_bytes;  /* bytes per element of one member         */
     int32_t members;     /* number of SoA member planes             */
     int32_t layout;      /* LGDHIP_LAYOUT_*                         */
-    int32_t reserved;
+    int32_t flags;       /* LGDHIP_GRID_* (0 for grids from _make)  */
     int64_t nx, ny, nz;  /* logical extents                         */
     int64_t ox, oy, oz;  /* element offset of logical (0,0,0)       */
     int64_t sx, sy, sz;  /* allocated extents (row pitch, rows, planes) */
@@ -151,6 +154,18 @@ int lgdhip_graph_destroy(lgdhip_graph_t graph);
 /* Fill a descriptor with this library's padding/alignment convention. */
 int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz,
                      int halo, int elem_bytes, int members);
+/* A view of the sub-box [x0, x0+nx) x [y0, y0+ny) x [z0, z0+nz) of g's
+ * logical box: same memory, pitches and layout, origin moved to (x0, y0,
+ * z0).  Neighbours of the view's cells outside it are g's cells, so a
+ * single sweep over a view updates exactly its cells as the full sweep
+ * would -- the inner-set / rim split of a region stepper
+ * (nesting/cudastepper.h:255-590 updates the rim, exchanges it, then the
+ * inner set).  Accepted by lgdhip_grid_load / _store and by the Jacobi
+ * single sweeps (variants 1-3; the default picks one); operations that
+ * treat the halo as the edge (fills, temporal blocking, Life, LBM) reject
+ * views with LGDHIP_ERR_UNSUPPORTED. */
+int lgdhip_grid_view(const lgdhip_grid *g, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
+                     int64_t nz, lgdhip_grid *view);
 /* Switch a descriptor made by lgdhip_grid_make to another member layout
  * (LGDHIP_LAYOUT_*); extents and pitches are unchanged. */
 int lgdhip_grid_set_layout(lgdhip_grid *g, int layout);

@@ -387,10 +387,22 @@ int launch_sweep(const double *src, double *dst, const lgdhip_grid &g, const Jac
 
 extern "C" int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
                                        const lgdhip_tile *tile, int *result_in_b, lgdhip_stream_t stream) {
-    int rc = lgdhip::check_grid(g, 3, 8, 1);
+    int rc = lgdhip::check_grid(g, 3, 8, 1, true);
     if (rc) return rc;
     LGD_REQUIRE(a && b && a != b, "need two distinct device buffers");
     LGD_REQUIRE(steps >= 0, "steps must be >= 0");
+    // a view's surroundings are real cells, not the edge: only the single
+    // sweeps (which read neighbours from memory) apply; the default picks one
+    lgdhip_tile view_tile;
+    if (g->flags & LGDHIP_GRID_VIEW) {
+        if (tile && tb_shape(tile->variant).k > 0)
+            return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "temporal blocking needs a whole grid, not a view");
+        if (!tile || tile->variant == 0) {
+            view_tile = tile ? *tile : lgdhip_tile{};
+            view_tile.variant = kSingleSweepVariant;
+            tile = &view_tile;
+        }
+    }
     const JacobiLaunch L = plan(*g, tile);
     hipStream_t s = lgdhip::as_stream(stream);
     double *cur = a, *nxt = b;

@@ -39,7 +39,8 @@ inline hipStream_t as_stream(lgdhip_stream_t s) { return reinterpret_cast<hipStr
 
 // Validates the internal consistency of a descriptor (extents, offsets and
 // pitches leave room for the halo).  Returns 0 or an error code.
-int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members);
+// Views (LGDHIP_GRID_VIEW) are rejected unless allow_view.
+int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members, bool allow_view = false);
 
 inline int64_t plane_elems(const lgdhip_grid &g) { return g.sx * g.sy * g.sz; }
 // element strides between members and between z slices (LGDHIP_LAYOUT_*)

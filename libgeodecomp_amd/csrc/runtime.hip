@@ -24,8 +24,11 @@ int hip_fail(hipError_t e, const char *what) {
     return set_error(hip_code(e), m);
 }
 
-int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members) {
+int check_grid(const lgdhip_grid *g, int dim, int elem_bytes, int members, bool allow_view) {
     if (!g) return set_error(LGDHIP_ERR_INVALID_ARG, "grid descriptor is NULL");
+    if (g->flags & ~LGDHIP_GRID_VIEW) return set_error(LGDHIP_ERR_INVALID_ARG, "unknown grid flags");
+    if ((g->flags & LGDHIP_GRID_VIEW) && !allow_view)
+        return set_error(LGDHIP_ERR_UNSUPPORTED, "this operation needs a whole grid, not a view");
     if (dim && g->dim != dim) return set_error(LGDHIP_ERR_INVALID_ARG, "grid has the wrong dimensionality");
     if (elem_bytes && g->elem_bytes != elem_bytes)
         return set_error(LGDHIP_ERR_INVALID_ARG, "grid has the wrong element size");
@@ -381,6 +384,28 @@ int lgdhip_grid_make(lgdhip_grid *g, int dim, int64_t nx, int64_t ny, int64_t nz
     return check_grid(g, dim, elem_bytes, members);
 }
 
+int lgdhip_grid_view(const lgdhip_grid *g, int64_t x0, int64_t y0, int64_t z0, int64_t nx, int64_t ny,
+                     int64_t nz, lgdhip_grid *view) {
+    int rc = check_grid(g, 0, 0, 0, true);
+    if (rc) return rc;
+    LGD_REQUIRE(view, "view descriptor is NULL");
+    if (g->dim == 2) {
+        LGD_REQUIRE(z0 == 0 && nz == 1, "a view of a 2D grid has z0 == 0 and nz == 1");
+    }
+    LGD_REQUIRE(x0 >= 0 && y0 >= 0 && z0 >= 0 && nx >= 1 && ny >= 1 && nz >= 1, "empty or negative view box");
+    LGD_REQUIRE(x0 + nx <= g->nx && y0 + ny <= g->ny && z0 + nz <= g->nz, "view box outside the grid");
+    lgdhip_grid v = *g;
+    v.ox += x0;
+    v.oy += y0;
+    v.oz += z0;
+    v.nx = nx;
+    v.ny = ny;
+    v.nz = nz;
+    v.flags |= LGDHIP_GRID_VIEW;
+    *view = v;
+    return check_grid(view, 0, 0, 0, true);
+}
+
 int lgdhip_grid_set_layout(lgdhip_grid *g, int layout) {
     LGD_REQUIRE(g, "grid descriptor is NULL");
     LGD_REQUIRE(layout == LGDHIP_LAYOUT_MEMBER_PLANES || layout == LGDHIP_LAYOUT_Z_SLABS, "unknown grid layout");
@@ -472,7 +497,7 @@ int lgdhip_grid_fill_edge(void *grid, const lgdhip_grid *g, const void *edge, lg
 }
 
 static int grid_copy(void *dev, const lgdhip_grid *g, void *host, bool to_device, lgdhip_stream_t stream) {
-    int rc = check_grid(g, 0, 0, 0);
+    int rc = check_grid(g, 0, 0, 0, true);
     if (rc) return rc;
     LGD_REQUIRE(dev && host, "NULL pointer");
     const size_t eb = (size_t)g->elem_bytes;

@@ -27,12 +27,13 @@ class LgdHipError(RuntimeError):
 
 
 LAYOUT_MEMBER_PLANES, LAYOUT_Z_SLABS = 0, 1
+GRID_VIEW = 0x1
 
 
 class lgdhip_grid(C.Structure):
     _fields_ = [
         ("dim", C.c_int32), ("halo", C.c_int32), ("elem_bytes", C.c_int32), ("members", C.c_int32),
-        ("layout", C.c_int32), ("reserved", C.c_int32),
+        ("layout", C.c_int32), ("flags", C.c_int32),
         ("nx", C.c_int64), ("ny", C.c_int64), ("nz", C.c_int64),
         ("ox", C.c_int64), ("oy", C.c_int64), ("oz", C.c_int64),
         ("sx", C.c_int64), ("sy", C.c_int64), ("sz", C.c_int64),
@@ -120,6 +121,7 @@ SIGNATURES = {
     "lgdhip_graph_destroy": (_I, [_P]),
     "lgdhip_grid_make": (_I, [_GP, _I, _I64, _I64, _I64, _I, _I, _I]),
     "lgdhip_grid_set_layout": (_I, [_GP, _I]),
+    "lgdhip_grid_view": (_I, [_GP, _I64, _I64, _I64, _I64, _I64, _I64, _GP]),
     "lgdhip_grid_plane_elems": (_I64, [_GP]),
     "lgdhip_grid_bytes": (_I64, [_GP]),
     "lgdhip_grid_fill_edge": (_I, [_P, _GP, _P, _P]),
