@@ -232,29 +232,196 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__
     }
 }
 
-// dW[co][k] = sum over chunks of P (fixed order); db[co] = sum_v dY[v][co]
-__global__ void conv3d_wgrad_reduce(const float *__restrict__ P, int chunks, int cout, int Kp, float *__restrict__ dW,
-                                    int accumulate) {
+// channels-last (ld, off, c) -> channels-first [c][nvox] (c, ld, off
+// multiples of 8): one thread per (voxel, group of 8 channels), a 16-byte
+// read and 8 writes that are contiguous across the wave
+__global__ __launch_bounds__(256) void to_channels_first(const u16 *__restrict__ src, int64_t nvox, int ld, int off,
+                                                         u16 *__restrict__ dst) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int g = blockIdx.y;
+    if (v >= nvox) return;
+    Frag f;
+    f.q = *(const uint4 *)(src + v * ld + off + 8 * g);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) dst[(int64_t)(8 * g + e) * nvox + v] = f.h[e];
+}
+
+// 8 BF16 of a channels-first row shifted by one voxel: dx = -1 gives
+// element e = c[e-1] with element 0 from the previous 8 (prev[7]); dx = +1
+// gives c[e+1] with element 7 from the next 8 (next[0]); little-endian
+// halves, so each 32-bit word is an alignbit of two neighbours
+__device__ __forceinline__ uint4 shift_from_prev(uint4 c, uint4 prev) {
+    return make_uint4(__builtin_amdgcn_alignbit(c.x, prev.w, 16), __builtin_amdgcn_alignbit(c.y, c.x, 16),
+                      __builtin_amdgcn_alignbit(c.z, c.y, 16), __builtin_amdgcn_alignbit(c.w, c.z, 16));
+}
+__device__ __forceinline__ uint4 shift_from_next(uint4 c, uint4 next) {
+    return make_uint4(__builtin_amdgcn_alignbit(c.y, c.x, 16), __builtin_amdgcn_alignbit(c.z, c.y, 16),
+                      __builtin_amdgcn_alignbit(c.w, c.z, 16), __builtin_amdgcn_alignbit(next.x, c.w, 16));
+}
+
+// dW partial from channels-first copies: P[chunk][co][tap*cin + ci] =
+// sum over the chunk's voxels v of G[co][v] X[ci][v + tap].  One wave per
+// (chunk, taps, group of CT co tiles, group of IT ci tiles): with DXT = 3 a
+// wave takes the three x-shifted taps of one (dz, dy) row, whose B
+// fragments all come from the same three aligned loads (previous, current,
+// next 8 voxels), and reuses each A fragment for all of them; with DXT = 1
+// one tap.  Per 32 voxels (4 groups of 8 consecutive voxels of one x row,
+// n % 8 == 0) every fragment is one 16-byte load, and the CT x IT x DXT
+// MFMAs share them.  Needs ksize 3, cin and cout multiples of 16, chunk_vox
+// a multiple of 32.
+template <int CT, int IT, int DXT>
+__global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ Xc, const u16 *__restrict__ Gc, int n,
+                                                       int cin, int cout, int Kp, int chunk_vox, int cotg, int citg,
+                                                       float *__restrict__ P) {
+    constexpr int kGroups = 27 / DXT;  // (dz, dy) rows, or single taps
+    const int lane = threadIdx.x & 63;
+    const int64_t nvox = (int64_t)n * n * n;
+    const int64_t chunks = (nvox + chunk_vox - 1) / chunk_vox;
+    const int64_t tasks = (int64_t)kGroups * cotg * citg;
+    const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    if (wave >= chunks * tasks) return;
+    const int64_t chunk = wave / tasks;
+    int64_t rem = wave % tasks;
+    const int grp = (int)(rem % kGroups);  // taps fastest: neighbouring waves share the G fragments
+    rem /= kGroups;
+    const int cig = (int)(rem % citg), cog = (int)(rem / citg);
+    const int tap0 = grp * DXT;            // DXT = 3: taps (dz, dy, -1), (dz, dy, 0), (dz, dy, +1)
+    int dz, dy, dx0;
+    tap_offset(tap0, 3, dz, dy, dx0);
+    const int i = lane & 15, q = lane >> 4;
+    f32x4 acc[DXT][CT][IT];
+#pragma unroll
+    for (int d = 0; d < DXT; ++d)
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+#pragma unroll
+            for (int u = 0; u < IT; ++u) acc[d][t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t v_begin = chunk * chunk_vox;
+    const int64_t v_end = v_begin + chunk_vox < nvox ? v_begin + chunk_vox : nvox;
+    for (int64_t vb = v_begin; vb < v_end; vb += 32) {
+        const int64_t v = vb + 8 * q;
+        const int x = (int)(v % n);
+        const int64_t r = v / n;
+        const int y = (int)(r % n), z = (int)(r / n);
+        Frag a[CT];
+#pragma unroll
+        for (int t = 0; t < CT; ++t) {
+            const int co = (cog * CT + t) * 16 + i;
+            a[t].q = *(const uint4 *)(Gc + (int64_t)co * nvox + v);
+        }
+        const int yy = y + dy, zz = z + dz;
+        const bool rowok = (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n;
+        const int64_t base = ((int64_t)zz * n + yy) * n + x;
+        Frag b[DXT][IT];
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
+            const int ci = (cig * IT + u) * 16 + i;
+            const u16 *row = Xc + (int64_t)ci * nvox;
+            const uint4 zero = make_uint4(0, 0, 0, 0);
+            uint4 c = zero, prev = zero, next = zero;
+            if (rowok) {
+                c = *(const uint4 *)(row + base);
+                if ((DXT == 3 || dx0 < 0) && x > 0) prev = *(const uint4 *)(row + base - 8);
+                if ((DXT == 3 || dx0 > 0) && x + 8 < n) next = *(const uint4 *)(row + base + 8);
+            }
+            if (DXT == 3) {
+                b[0][u].q = shift_from_prev(c, prev);
+                b[1 % DXT][u].q = c;
+                b[2 % DXT][u].q = shift_from_next(c, next);
+            } else {
+                b[0][u].q = dx0 == 0 ? c : (dx0 < 0 ? shift_from_prev(c, prev) : shift_from_next(c, next));
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DXT; ++d)
+#pragma unroll
+            for (int t = 0; t < CT; ++t)
+#pragma unroll
+                for (int u = 0; u < IT; ++u)
+                    acc[d][t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t].v, b[d][u].v, acc[d][t][u], 0, 0, 0);
+    }
+    // D lane layout: row (co) 4*(lane/16) + e, column (ci) lane % 16
+#pragma unroll
+    for (int d = 0; d < DXT; ++d)
+#pragma unroll
+        for (int t = 0; t < CT; ++t)
+#pragma unroll
+            for (int u = 0; u < IT; ++u)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int co = (cog * CT + t) * 16 + 4 * q + e;
+                    const int ci = (cig * IT + u) * 16 + i;
+                    P[((int64_t)chunk * cout + co) * Kp + (tap0 + d) * cin + ci] = acc[d][t][u][e];
+                }
+}
+
+// dW[co][k] = sum over chunks of P (fixed order) for k < K; the padding
+// columns k >= K stay zero
+__global__ void conv3d_wgrad_reduce(const float *__restrict__ P, int chunks, int cout, int Kp, int K,
+                                    float *__restrict__ dW, int accumulate) {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (int64_t)cout * Kp) return;
+    if (idx % Kp >= K) {
+        dW[idx] = accumulate ? dW[idx] : 0.0f;
+        return;
+    }
     float s = 0.0f;
     for (int c = 0; c < chunks; ++c) s += P[(int64_t)c * cout * Kp + idx];
     dW[idx] = accumulate ? dW[idx] + s : s;
 }
 
-__global__ void conv3d_bias_grad(const u16 *__restrict__ dY, int nvox, int ldy, int offy, int cout,
-                                 float *__restrict__ db, int accumulate) {
-    __shared__ float lds[256];
-    const int co = blockIdx.x;
+// db from the channels-first gradient: partial sums of each row over
+// chunks of 8192 voxels (coalesced), then a fixed-order sum of the chunks
+constexpr int kBiasChunk = 8192;
+__global__ __launch_bounds__(256) void row_sum_partial(const u16 *__restrict__ Gc, int64_t nvox, float *__restrict__ Pb,
+                                                       int bchunks) {
+    __shared__ float red[256];
+    const int co = blockIdx.x, c = blockIdx.y;
+    const int64_t v0 = (int64_t)c * kBiasChunk;
+    const int64_t v1 = v0 + kBiasChunk < nvox ? v0 + kBiasChunk : nvox;
     float s = 0.0f;
-    for (int v = threadIdx.x; v < nvox; v += 256) s += bf2f(dY[(int64_t)v * ldy + offy + co]);
-    lds[threadIdx.x] = s;
+    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) s += bf2f(Gc[(int64_t)co * nvox + v]);
+    red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) db[co] = accumulate ? db[co] + lds[0] : lds[0];
+    if (threadIdx.x == 0) Pb[co * bchunks + c] = red[0];
+}
+__global__ void row_sum_final(const float *__restrict__ Pb, int bchunks, int cout, float *__restrict__ db,
+                              int accumulate) {
+    const int co = blockIdx.x * blockDim.x + threadIdx.x;
+    if (co >= cout) return;
+    float s = 0.0f;
+    for (int c = 0; c < bchunks; ++c) s += Pb[co * bchunks + c];
+    db[co] = accumulate ? db[co] + s : s;
+}
+
+// db[co] partial sums over chunks of 8192 voxels, channels-last input:
+// thread t of a block handles channel t % cout of every (256 / cout)-th
+// voxel, so a wave reads whole voxel rows (coalesced); fixed-order sums
+__global__ __launch_bounds__(256) void bias_grad_partial(const u16 *__restrict__ dY, int64_t nvox, int ldy, int offy,
+                                                         int cout, float *__restrict__ Pb, int bchunks) {
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    const int lanes = 256 / cout;  // voxel lanes
+    const int t = threadIdx.x;
+    float s = 0.0f;
+    if (t < lanes * cout) {
+        const int co = t % cout, vl = t / cout;
+        const int64_t v0 = (int64_t)c * kBiasChunk;
+        const int64_t v1 = v0 + kBiasChunk < nvox ? v0 + kBiasChunk : nvox;
+        for (int64_t v = v0 + vl; v < v1; v += lanes) s += bf2f(dY[v * ldy + offy + co]);
+    }
+    red[t] = s;
+    __syncthreads();
+    if (t < cout) {
+        float sum = 0.0f;
+        for (int vl = 0; vl < lanes; ++vl) sum += red[vl * cout + t];
+        Pb[t * bchunks + c] = sum;
+    }
 }
 
 // dgrad weights: Wt[ci][tap'*cout + co] = W[co][tap*cin + ci], tap' = 26 - tap
@@ -397,6 +564,11 @@ __global__ void adam_step(float *__restrict__ w, float *__restrict__ m, float *_
 
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
 
+// shapes the channels-first weight gradient handles
+bool wgrad_cf_shape(int n, int cin, int ksize, int cout) {
+    return ksize == 3 && cin % 16 == 0 && cout % 16 == 0 && n % 8 == 0;
+}
+
 int check_tensor(int n, int ld, int off, int c) {
     if (n < 1 || ld < 1 || off < 0 || c < 1 || off + c > ld)
         return lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, "bad tensor shape (n, ld, off, c)");
@@ -467,7 +639,11 @@ size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, 
     if (chunk_vox <= 0) chunk_vox = 4096;
     const int64_t nvox = (int64_t)n * n * n;
     const int64_t chunks = (nvox + chunk_vox - 1) / chunk_vox;
-    return (size_t)(chunks * cout * lgdhip_conv3d_kpad(cin, ksize) * 4);
+    // [P: chunk partials][bias partials][x and dy channels-first copies]
+    size_t bytes = (size_t)(chunks * cout * lgdhip_conv3d_kpad(cin, ksize) * 4);
+    bytes += (size_t)cout * (size_t)((nvox + kBiasChunk - 1) / kBiasChunk) * 4;
+    if (wgrad_cf_shape(n, cin, ksize, cout)) bytes += 256 + (size_t)(cin + cout) * (size_t)nvox * 2;
+    return bytes;
 }
 
 int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *dy, int ldy,
@@ -485,12 +661,59 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
     if (workspace_bytes < lgdhip_conv3d_wgrad_workspace_bytes(n, cin, ksize, cout, chunk_vox))
         return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "wgrad workspace too small");
     const int Kp = lgdhip_conv3d_kpad(cin, ksize);
+    const int K = (ksize == 1 ? 1 : 27) * cin;
     const int64_t nvox = (int64_t)n * n * n;
     const int chunks = (int)((nvox + chunk_vox - 1) / chunk_vox);
-    const int64_t waves = (int64_t)chunks * (Kp / 16);
-    const unsigned grid = (unsigned)((waves + 3) / 4);
     hipStream_t s = lgdhip::as_stream(stream);
     float *P = (float *)workspace;
+    const bool cf = wgrad_cf_shape(n, cin, ksize, cout) && ldx % 8 == 0 && offx % 8 == 0 && ldy % 8 == 0 &&
+                    offy % 8 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)dy % 16 == 0;
+    if (cf) {
+        // channels-first copies: every MFMA fragment becomes one 16-byte load
+        const int bchunks0 = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
+        float *Pb = P + (size_t)chunks * cout * Kp;
+        char *base = (char *)(Pb + (size_t)cout * bchunks0);
+        base += (256 - (uintptr_t)base % 256) % 256;
+        u16 *Xc = (u16 *)base;
+        u16 *Gc = Xc + (size_t)cin * nvox;
+        to_channels_first<<<dim3(blocks_for(nvox), cin / 8), 256, 0, s>>>((const u16 *)x, nvox, ldx, offx, Xc);
+        to_channels_first<<<dim3(blocks_for(nvox), cout / 8), 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, Gc);
+        LGD_LAUNCHED("to_channels_first");
+        const int cot = cout / 16, cit = cin / 16;
+        const int CT = cot % 4 == 0 ? 4 : (cot % 2 == 0 ? 2 : 1);
+        const int IT = cit % 4 == 0 ? 4 : (cit % 2 == 0 ? 2 : 1);
+        const int cotg = cot / CT, citg = cit / IT;
+        // small tile groups take the three x taps of a row at once (the
+        // accumulators stay within 48 registers)
+        const int DXT = CT * IT <= 4 ? 3 : 1;
+        const int64_t waves = (int64_t)chunks * (27 / DXT) * cotg * citg;
+        const unsigned grid = (unsigned)((waves + 3) / 4);
+#define LGD_CF_CASE(A, B, D)                                                                                 \
+    if (CT == A && IT == B && DXT == D)                                                                      \
+        conv3d_wgrad_cf<A, B, D><<<grid, 256, 0, s>>>(Xc, Gc, n, cin, cout, Kp, chunk_vox, cotg, citg, P);
+        LGD_CF_CASE(1, 1, 3)
+        LGD_CF_CASE(1, 2, 3)
+        LGD_CF_CASE(1, 4, 3)
+        LGD_CF_CASE(2, 1, 3)
+        LGD_CF_CASE(2, 2, 3)
+        LGD_CF_CASE(2, 4, 1)
+        LGD_CF_CASE(4, 1, 3)
+        LGD_CF_CASE(4, 2, 1)
+        LGD_CF_CASE(4, 4, 1)
+#undef LGD_CF_CASE
+        LGD_LAUNCHED("conv3d_wgrad_cf");
+        conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, K, dw, accumulate);
+        LGD_LAUNCHED("conv3d_wgrad_reduce");
+        if (db) {
+            const int bchunks = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
+            row_sum_partial<<<dim3(cout, bchunks), 256, 0, s>>>(Gc, nvox, Pb, bchunks);
+            row_sum_final<<<blocks_for(cout), 256, 0, s>>>(Pb, bchunks, cout, db, accumulate);
+            LGD_LAUNCHED("row_sum");
+        }
+        return 0;
+    }
+    const int64_t waves = (int64_t)chunks * (Kp / 16);
+    const unsigned grid = (unsigned)((waves + 3) / 4);
     const int nt = (cout + 15) / 16;
 #define LGD_WG_CASE(T)                                                                                       \
     case T:                                                                                                  \
@@ -513,11 +736,14 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
     }
 #undef LGD_WG_CASE
     LGD_LAUNCHED("conv3d_wgrad_mfma");
-    conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, dw, accumulate);
+    conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, K, dw, accumulate);
     LGD_LAUNCHED("conv3d_wgrad_reduce");
     if (db) {
-        conv3d_bias_grad<<<cout, 256, 0, s>>>((const u16 *)dy, (int)nvox, ldy, offy, cout, db, accumulate);
-        LGD_LAUNCHED("conv3d_bias_grad");
+        const int bchunks = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
+        float *Pb = P + (size_t)chunks * cout * Kp;
+        bias_grad_partial<<<bchunks, 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, cout, Pb, bchunks);
+        row_sum_final<<<blocks_for(cout), 256, 0, s>>>(Pb, bchunks, cout, db, accumulate);
+        LGD_LAUNCHED("bias_grad");
     }
     return 0;
 }

@@ -155,8 +155,14 @@ def test_conv_dgrad_vs_oracle(hip, oracle, n, cin, cout):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,cin,cout,ksize", [(8, 16, 16, 3), (12, 2, 16, 3), (4, 192, 64, 3), (16, 16, 1, 1)])
+@pytest.mark.parametrize("n,cin,cout,ksize", [(8, 16, 16, 3), (12, 2, 16, 3), (4, 192, 64, 3), (16, 16, 1, 1),
+                                              (16, 32, 64, 3), (8, 64, 32, 3), (16, 48, 16, 3), (24, 16, 48, 3),
+                                              (8, 192, 128, 3), (8, 32, 16, 3), (8, 64, 16, 3),
+                                              (8, 16, 32, 3), (8, 32, 32, 3), (8, 16, 64, 3)])
 def test_conv_wgrad_vs_oracle(hip, oracle, n, cin, cout, ksize):
+    """Element-gather kernel (odd channel counts, 1x1x1, n % 8 != 0) and the
+    channels-first kernel (3x3x3, channels multiples of 16, n % 8 == 0, every
+    tile grouping CT x IT in {1, 2, 4}^2 reached by some case)."""
     print("[case] conv wgrad", n, cin, cout, ksize, flush=True)
     rng = np.random.default_rng(7 * cin + cout)
     taps = 27 if ksize == 3 else 1
@@ -181,6 +187,38 @@ def test_conv_wgrad_vs_oracle(hip, oracle, n, cin, cout, ksize):
     got = out[:, :taps * cin].reshape(ref.shape)
     assert np.all(out[:, taps * cin:] == 0)
     assert np.all(np.abs(got - ref) <= (n ** 3 + 64) * 2.0 ** -24 * mag + 1e-30)
+    assert np.allclose(outb, refb, rtol=1e-5, atol=1e-3)
+
+
+@pytest.mark.gpu
+def test_conv_wgrad_strided_tensors(hip, oracle):
+    """x and dy as channel ranges of wider (concatenated) buffers: ld > c,
+    off > 0, as the U-Net's skip concatenations pass them."""
+    n, cin, cout, ldx, offx, ldy, offy = 16, 16, 32, 48, 32, 40, 8
+    rng = np.random.default_rng(3)
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    g = bf16_round(rng.standard_normal((n, n, n, cout)).astype(np.float32))
+    xs = np.zeros((n, n, n, ldx), np.float32)
+    xs[..., offx:offx + cin] = x
+    gs = np.zeros((n, n, n, ldy), np.float32)
+    gs[..., offy:offy + cout] = g
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, 3)
+    wsb = lib.lgdhip_conv3d_wgrad_workspace_bytes(n, cin, 3, cout, 512)
+    ws = hip.DeviceBuffer(wsb)
+    xb, gb = dev(hip, to_bf16_bits(xs)), dev(hip, to_bf16_bits(gs))
+    dw = hip.DeviceBuffer(cout * kp * 4)
+    db = hip.DeviceBuffer(cout * 4)
+    hip.call("lgdhip_conv3d_bf16_wgrad", xb.ptr, n, ldx, offx, cin, 3, gb.ptr, ldy, offy, cout, dw.ptr, db.ptr, 0,
+             512, ws.ptr, wsb, None)
+    out = np.empty((cout, kp), np.float32)
+    dw.download(out)
+    outb = np.empty(cout, np.float32)
+    db.download(outb)
+    ref, refb = oracle.conv3d_wgrad(x, g, 3)
+    mag, _ = oracle.conv3d_wgrad(np.abs(x), np.abs(g), 3)
+    assert np.all(np.abs(out[:, :27 * cin].reshape(ref.shape) - ref) <= (n ** 3 + 64) * 2.0 ** -24 * mag + 1e-30)
+    assert np.all(out[:, 27 * cin:] == 0)
     assert np.allclose(outb, refb, rtol=1e-5, atol=1e-3)
 
 
