@@ -462,10 +462,11 @@ class Learned:
     dtype = "bf16"   # the network (BF16 operands, FP32 accumulation); polishing is FP32/FP64
     bytes_per_lup = 12.0
 
-    def __init__(self, n=128, steps=9000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, **_):
+    def __init__(self, n=128, steps=20000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, lr=5e-4, **_):
         from libgeodecomp_amd import learned_guess as lgm
         self.lgm = lgm
         self.n, self.train_steps, self.seed, self.ntrain, self.neval = n, steps, seed, train, eval
+        self.lr = lr
         self.m = lgm.default_coarse_size(n)
         self.stream = lgdhip.Stream()
         t0 = time.perf_counter()
@@ -479,7 +480,7 @@ class Learned:
         log(f"generated {train} solutions of {n}^3 in {self.generate_s:.1f} s")
         self.model = lgm.LearnedGuess(n, seed=seed % 1000, stream=self.stream.h)
         t0 = time.perf_counter()
-        self.log = self.model.train(data, base, steps, log_every=250, verbose=True)
+        self.log = self.model.train(data, base, steps, lr=lr, log_every=1000, verbose=True)
         lgdhip.call("lgdhip_device_synchronize")
         self.train_s = time.perf_counter() - t0
         del base, data
@@ -492,6 +493,7 @@ class Learned:
                             f"(Gaussian sources, Dirichlet faces), Jacobi polishing to 1e-6 relative FP64 residual",
                 "network": "3D U-Net 16-32-64-128, 3x3x3 BF16 convs on MFMA, FP32 accumulation, bias-free",
                 "coarse_grid": self.m, "train_problems": self.ntrain, "train_steps": self.train_steps,
+                "peak_lr": self.lr,
                 "generate_s": round(self.generate_s, 1), "train_s": round(self.train_s, 1),
                 "final_loss": self.log[-1][1] if self.log else None, "arms": r.get("arms"),
                 "sweep_ratio": r.get("sweep_ratio"), "seed": self.seed, "parallelism": "single GPU"}
