@@ -602,8 +602,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py runs on one GPU only: the library has no multi-GPU path; the kernels shard as "
+              "z slabs with K-plane halos exchanged every K steps (Poisson problems: no exchange at all)",
+              file=sys.stderr)
         sys.exit(2)
 
     lgdhip.load()

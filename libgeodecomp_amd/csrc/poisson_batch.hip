@@ -4,8 +4,8 @@
 //     6 u_c - sum_{6 neighbours} u_nb = f_c
 // (h = 1), constant Dirichlet value per face outside the cube.  Jacobi:
 //     u_c <- (sum_nb u_nb + f_c) / 6.
-// The reference has no counterpart (no source term, residual or batching,
-// SURVEY row A19); the sweep is the Jacobi cell of
+// The reference has no counterpart (no source term, residual or
+// batching); the sweep is the Jacobi cell of
 // src/testbed/performancetests/main.cpp:1194-1205 with a source term.
 //
 // Precision scheme: FP32 storage of u cannot reach a 1e-6 relative
