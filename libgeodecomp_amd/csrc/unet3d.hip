@@ -1,7 +1,7 @@
 // 3D U-Net building blocks on the MFMA cores (BF16 operands, FP32 accumulate).
 //
-// No reference counterpart (the learned initial guess is new to this
-// build).  Conventions:
+// No reference counterpart (LibGeoDecomp has no learned initial guess).
+// Conventions:
 //  * activations are channels-last BF16 cubes: element (z,y,x,c) of an n^3
 //    tensor with channel stride `ld` and channel offset `off` lives at
 //    ((z*n + y)*n + x)*ld + off + c; concatenation is two producers writing
