@@ -237,10 +237,10 @@ class LBM:
     dtype = "f32"
     bytes_per_lup = 153.0  # 19 x 4 B read + 19 x 4 B write + 1 B wall flags
 
-    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, bx=0,
+    def __init__(self, n=256, steps=2000, seed=synthetic.DEFAULT_SEED, by=0, flags=0, variant=0, bx=0, bz=0,
                  layout=lgdhip.LAYOUT_Z_SLABS, **_):
         self.n, self.sweeps, self.seed = n, steps, seed
-        self.tile = lgdhip.tile(bx=bx, by=by, flags=flags, variant=variant)
+        self.tile = lgdhip.tile(bx=bx, by=by, bz=bz, flags=flags, variant=variant)
         self.stream = lgdhip.Stream()
         s = self.stream.h
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=4, members=19, layout=layout)
@@ -278,12 +278,16 @@ class LBM:
                     C.byref(self.p), self.sweeps, C.byref(self.tile), None, s)
 
     def dominant_kernel(self):
+        # variant 3 advances two steps per launch (credited two steps' bytes)
+        per = 2 if self.tile.variant == 3 else 1
         launches = 20
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
-                                                    self.flags.ptr, C.byref(self.g), C.byref(self.p), launches,
-                                                    C.byref(self.tile), None, self.stream.h))
-        self.shape = f"{self.n}^3" + (",zslabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "")
-        return "lbm_d3q19_pull", ms / launches, self.bytes_per_lup * float(self.n) ** 3
+                                                    self.flags.ptr, C.byref(self.g), C.byref(self.p),
+                                                    launches * per, C.byref(self.tile), None, self.stream.h))
+        self.shape = f"{self.n}^3" + (",zslabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "") + \
+            (",2steps" if per == 2 else "")
+        name = "lbm_d3q19_pull2" if per == 2 else "lbm_d3q19_pull"
+        return name, ms / launches, per * self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -592,7 +596,7 @@ def main():
     ap.add_argument("--bx", type=int, default=0, help="lbm: work-group width 64 or 256 (0: default)")
     ap.add_argument("--layout", choices=("planes", "zslabs"), default=None,
                     help="lbm: member layout of the populations (default: the tuned one)")
-    ap.add_argument("--bz", type=int, default=0, help="jacobi: z planes per work-group (0: default)")
+    ap.add_argument("--bz", type=int, default=0, help="jacobi, lbm variant 3: z planes per work-group (0: default)")
     ap.add_argument("--no-xcd-swizzle", action="store_true", help="jacobi/lbm: map work-groups to tiles in launch order")
     ap.add_argument("--xcd-swizzle", action="store_true", help="jacobi/lbm: one contiguous tile range per XCD")
     ap.add_argument("--xcd-rows", action="store_true", help="lbm: x rows of tiles round-robin over the XCDs")
@@ -626,7 +630,7 @@ def main():
     if args.workload == "jacobi":
         kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
     if args.workload == "lbm":
-        kw.update(by=args.by, flags=flags, variant=max(0, args.variant), bx=args.bx)
+        kw.update(by=args.by, flags=flags, variant=max(0, args.variant), bx=args.bx, bz=args.bz)
         if args.layout:
             kw.update(layout=lgdhip.LAYOUT_Z_SLABS if args.layout == "zslabs" else lgdhip.LAYOUT_MEMBER_PLANES)
     if args.workload == "poisson":

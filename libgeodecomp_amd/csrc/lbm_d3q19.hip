@@ -25,6 +25,8 @@
 // FP32 mass drift of the unshifted form (about one ulp of rho per step).
 #include "lgdhip_internal.h"
 
+#include <cstring>
+
 #include <type_traits>
 
 namespace {
@@ -67,6 +69,41 @@ __device__ __forceinline__ bool blocked(const Dir &d, uint32_t fl) {
            (d.ez == 1 && (fl & LGDHIP_LBM_WALL_ZLO)) || (d.ez == -1 && (fl & LGDHIP_LBM_WALL_ZHI));
 }
 
+// moving lid: populations reflected from the +y face gain 6 w_i e_i.u
+__device__ __forceinline__ void lid_term(float (&f)[19], uint32_t fl, float lux, float luy, float luz) {
+    const bool lid = (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI);
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const Dir d = kDirs[i];
+        if (d.ey == -1) {
+            const float term = 6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
+            f[i] += lid ? term : 0.0f;
+        }
+    }
+}
+
+// BGK collision of the streamed (shifted) populations f into out, in the
+// one order every LBM kernel here uses (bit-identical results)
+__device__ __forceinline__ void collide(const float (&f)[19], float omega, float (&out)[19]) {
+    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        drho += f[i];
+        if (kDirs[i].ex) jx += (float)kDirs[i].ex * f[i];
+        if (kDirs[i].ey) jy += (float)kDirs[i].ey * f[i];
+        if (kDirs[i].ez) jz += (float)kDirs[i].ez * f[i];
+    }
+    const float dterm = drho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
+    const float keep = 1.0f - omega;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+        const Dir d = kDirs[i];
+        const float eu = (float)d.ex * jx + (float)d.ey * jy + (float)d.ez * jz;
+        const float w3 = 3.0f * weight(d.wc) * omega;
+        out[i] = keep * f[i] + w3 * (dterm + eu + 1.5f * eu * eu);
+    }
+}
+
 // IDX is int32_t when every element offset of the 19 planes fits 31 bits
 // (then addresses are a uniform base plus a 32-bit lane offset), else int64_t.
 // A work-group is a 64 x BY tile of one z plane; tiles are numbered x
@@ -105,35 +142,12 @@ __global__ __launch_bounds__(BX * BY) void lbm_d3q19_pull(const float *__restric
     float f[19];
 #pragma unroll
     for (int i = 0; i < 19; ++i) f[i] = NTL ? __builtin_nontemporal_load(src + at[i]) : src[at[i]];
-    // moving lid: populations reflected from the +y face gain 6 w_i e_i.u
-    const bool lid = (fl & LGDHIP_LBM_WALL_YHI) && (fl & LGDHIP_LBM_LID_YHI);
+    lid_term(f, fl, lux, luy, luz);
+    float out[19];
+    collide(f, omega, out);
+    // streamed once per sweep: keep it out of the caches' way
 #pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const Dir d = kDirs[i];
-        if (d.ey == -1) {
-            const float term = 6.0f * weight(d.wc) * ((float)d.ex * lux + (float)d.ey * luy + (float)d.ez * luz);
-            f[i] += lid ? term : 0.0f;
-        }
-    }
-
-    float drho = 0.0f, jx = 0.0f, jy = 0.0f, jz = 0.0f;
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        drho += f[i];
-        if (kDirs[i].ex) jx += (float)kDirs[i].ex * f[i];
-        if (kDirs[i].ey) jy += (float)kDirs[i].ey * f[i];
-        if (kDirs[i].ez) jz += (float)kDirs[i].ez * f[i];
-    }
-    const float dterm = drho * (1.0f / 3.0f) - 0.5f * (jx * jx + jy * jy + jz * jz);
-    const float keep = 1.0f - omega;
-#pragma unroll
-    for (int i = 0; i < 19; ++i) {
-        const Dir d = kDirs[i];
-        const float eu = (float)d.ex * jx + (float)d.ey * jy + (float)d.ez * jz;
-        const float w3 = 3.0f * weight(d.wc) * omega;
-        // streamed once per sweep: keep it out of the caches' way
-        __builtin_nontemporal_store(keep * f[i] + w3 * (dterm + eu + 1.5f * eu * eu), &dst[(IDX)i * plane + idx]);
-    }
+    for (int i = 0; i < 19; ++i) __builtin_nontemporal_store(out[i], &dst[(IDX)i * plane + idx]);
 }
 
 // Four cells per thread (x0 .. x0+3, x0 a multiple of 4): every population
@@ -226,6 +240,119 @@ __global__ __launch_bounds__(64 * BY) void lbm_d3q19_pull_x4(const float *__rest
     }
 }
 
+// Two steps per pass (temporal blocking).  A work-group owns a tile of
+// 62 x BY columns and marches a z chunk; its 64 x (BY + 2) threads cover the
+// tile plus a one-cell ring.  For every plane p of the chunk (and one ring
+// plane on each side) the threads compute step 1 at (x, y, p) -- pull from
+// the source, bounce-back, lid, collision -- into a 3-plane LDS ring holding
+// all 19 populations; then the tile's inner threads compute step 2 at
+// (x, y, p - 1) by pulling from that ring (planes p - 2, p - 1, p), and
+// write it.  Step-1 cells outside the box hold the halo value (the edge,
+// never updated, as between single steps); cells more than one outside are
+// never read.  Same per-cell arithmetic as lbm_d3q19_pull (lid_term,
+// collide): two passes of this kernel equal four single steps bit for bit.
+constexpr int kPull2X = 62;  // output columns per tile (64 lanes minus the ring)
+
+template <int BY, typename IDX>
+__global__ __launch_bounds__(64 * (BY + 2)) void lbm_d3q19_pull2(
+    const float *__restrict__ src, float *__restrict__ dst, const uint8_t *__restrict__ flags, int nx, int ny, int nz,
+    IDX sx, IDX zs, IDX ms, IDX origin, IDX fsx, IDX fzs, IDX forigin, float omega, float lux, float luy, float luz,
+    unsigned gx, unsigned gy, unsigned ntiles, int zchunk) {
+    constexpr int RY = BY + 2;
+    __shared__ float ring[3][19][RY][64];
+    const unsigned t = blockIdx.x;
+    if (t >= ntiles) return;  // whole work-group, before any barrier
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = (int)(t % gx) * kPull2X - 1 + tx;
+    const int y = (int)((t / gx) % gy) * BY - 1 + ty;
+    const int z0 = (int)(t / (gx * gy)) * zchunk;
+    const int z1 = min(z0 + zchunk, nz);
+    const bool inxy = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny;
+    // one cell outside the box in x and/or y (and not farther): halo
+    const bool nearxy = x >= -1 && x <= nx && y >= -1 && y <= ny;
+    const bool writer = inxy && tx >= 1 && tx <= kPull2X && ty >= 1 && ty <= BY;
+    const IDX col = origin + (IDX)y * sx + x;           // (x, y, 0), valid for nearxy
+    const IDX fcol = forigin + (IDX)y * fsx + x;
+    // step-1 inputs of plane pp: the 19 pulled (or bounced) values of a box
+    // cell, the halo values of a cell one outside, zeros farther out
+    auto flag_at = [&](int pp) -> uint32_t {
+        return (inxy && (unsigned)pp < (unsigned)nz) ? (uint32_t)flags[fcol + (IDX)pp * fzs] : 0u;
+    };
+    auto load_plane = [&](int pp, uint32_t fl, float (&v)[19]) {
+        const IDX idx = col + (IDX)pp * zs;
+        if (inxy && (unsigned)pp < (unsigned)nz) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const Dir d = kDirs[i];
+                const IDX pulled = (IDX)i * ms + idx - (d.ex + d.ey * sx + d.ez * zs);
+                const IDX bounced = (IDX)d.opp * ms + idx;
+                v[i] = src[blocked(d, fl) ? bounced : pulled];
+            }
+        } else if (nearxy && pp >= -1 && pp <= nz) {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) v[i] = src[(IDX)i * ms + idx];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) v[i] = 0.0f;
+        }
+    };
+    // software pipeline: while plane p is computed, the values of plane
+    // p + 1 and the flags of plane p + 2 are in flight
+    uint32_t fl_cur = flag_at(z0 - 1), fl_next = flag_at(z0);
+    float pv[19];
+    load_plane(z0 - 1, fl_cur, pv);
+    uint32_t fl_prev = 0;
+    int slot = 0;                                        // ring slot of plane p
+    for (int p = z0 - 1; p <= z1; ++p) {
+        float f[19];
+#pragma unroll
+        for (int i = 0; i < 19; ++i) f[i] = pv[i];
+        const uint32_t fl = fl_cur;
+        if (p + 1 <= z1) load_plane(p + 1, fl_next, pv);
+        const uint32_t fl_after = p + 2 <= z1 ? flag_at(p + 2) : 0u;
+        // ---- step 1 at (x, y, p)
+        if (inxy && (unsigned)p < (unsigned)nz) {
+            lid_term(f, fl, lux, luy, luz);
+            float out[19];
+            collide(f, omega, out);
+#pragma unroll
+            for (int i = 0; i < 19; ++i) ring[slot][i][ty][tx] = out[i];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 19; ++i) ring[slot][i][ty][tx] = f[i];
+        }
+        __syncthreads();
+        // ---- step 2 at (x, y, q = p - 1)
+        const int q = p - 1;
+        if (writer && q >= z0 && q < z1) {
+            const int sm = slot == 0 ? 2 : slot - 1;    // plane q
+            const int sl = sm == 0 ? 2 : sm - 1;        // plane q - 1
+            float f[19];
+#pragma unroll
+            for (int i = 0; i < 19; ++i) {
+                const Dir d = kDirs[i];
+                if (blocked(d, fl_prev)) {
+                    f[i] = ring[sm][d.opp][ty][tx];
+                } else {
+                    const int s = d.ez > 0 ? sl : (d.ez < 0 ? slot : sm);
+                    f[i] = ring[s][i][ty - d.ey][tx - d.ex];
+                }
+            }
+            lid_term(f, fl_prev, lux, luy, luz);
+            float out[19];
+            collide(f, omega, out);
+            const IDX idx = col + (IDX)q * zs;
+#pragma unroll
+            for (int i = 0; i < 19; ++i) __builtin_nontemporal_store(out[i], &dst[(IDX)i * ms + idx]);
+        }
+        fl_prev = fl;
+        fl_cur = fl_next;
+        fl_next = fl_after;
+        __syncthreads();  // the next step 1 overwrites plane q - 1's slot
+        slot = slot == 2 ? 0 : slot + 1;
+    }
+}
+
 __global__ void lbm_cavity_flags_kernel(uint8_t *__restrict__ flags, int nx, int ny, int nz, int64_t sx,
                                         int64_t sxy, int64_t origin) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,6 +395,52 @@ __global__ void lbm_moments_kernel(const float *__restrict__ f, int nx, int ny, 
 
 int flags_grid(const lgdhip_grid &g, lgdhip_grid *fg) {
     return lgdhip_grid_make(fg, 3, g.nx, g.ny, g.nz, g.halo, 1, 1);
+}
+
+int run_pull2(float *a, float *b, const uint8_t *flags, const lgdhip_grid &g, const lgdhip_grid &fg,
+              const lgdhip_lbm_params &p, int steps, const lgdhip_tile *tile, int *result_in_b, hipStream_t s) {
+    const int BY = tile->by == 2 ? 2 : 4;  // 3 x 19 x (BY + 2) x 64 floats of LDS: 58 KB or 88 KB
+    const int zchunk = tile->bz > 0 ? tile->bz : 32;
+    const unsigned gx = lgdhip::ceil_div(g.nx, kPull2X), gy = lgdhip::ceil_div(g.ny, BY);
+    const unsigned ntiles = gx * gy * lgdhip::ceil_div(g.nz, zchunk);
+    const int64_t zs = lgdhip::z_stride(g), ms = lgdhip::member_stride(g);
+    const int64_t origin = lgdhip::origin_index(g), forigin = lgdhip::origin_index(fg);
+    const int64_t fzs = lgdhip::z_stride(fg);
+    const bool small_index =
+        19 * lgdhip::plane_elems(g) < ((int64_t)1 << 31) && lgdhip::plane_elems(fg) < ((int64_t)1 << 31);
+    float *cur = a, *nxt = b;
+    int t = 0;
+    for (; t + 2 <= steps; t += 2) {
+        // two steps: cur -> nxt (the intermediate step lives in LDS only)
+#define LGD_PULL2(B, I)                                                                                          \
+    lbm_d3q19_pull2<B, I><<<ntiles, dim3(64, B + 2), 0, s>>>(                                                    \
+        cur, nxt, flags, (int)g.nx, (int)g.ny, (int)g.nz, (I)g.sx, (I)zs, (I)ms, (I)origin, (I)fg.sx, (I)fzs,    \
+        (I)forigin, p.omega, p.lid_ux, p.lid_uy, p.lid_uz, gx, gy, ntiles, zchunk)
+        if (small_index) {
+            if (BY == 2) LGD_PULL2(2, int32_t); else LGD_PULL2(4, int32_t);
+        } else {
+            if (BY == 2) LGD_PULL2(2, int64_t); else LGD_PULL2(4, int64_t);
+        }
+#undef LGD_PULL2
+        LGD_LAUNCHED("lbm_d3q19_pull2");
+        float *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    if (t < steps) {
+        int in_b = 0;
+        lgdhip_tile single;
+        std::memset(&single, 0, sizeof(single));
+        const int rc = lgdhip_lbm_d3q19_f32_run(cur, nxt, flags, &g, &p, 1, &single, &in_b, s);
+        if (rc) return rc;
+        if (in_b) {
+            float *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+    }
+    if (result_in_b) *result_in_b = (cur == b) ? 1 : 0;
+    return 0;
 }
 
 int run_x4(float *a, float *b, const uint8_t *flags, const lgdhip_grid &g, const lgdhip_grid &fg,
@@ -342,6 +515,10 @@ int lgdhip_lbm_d3q19_f32_run(float *a, float *b, const uint8_t *flags, const lgd
     int swizzle = lgdhip::xcd_swizzle(tile, false);
     if (tile && (tile->flags & LGDHIP_TILE_XCD_ROWS) && !(tile->flags & LGDHIP_TILE_NO_XCD_SWIZZLE)) swizzle = 2;
     const bool ntl = tile && tile->variant == 1;
+    // variant 3: two steps per pass (pairs), an odd remainder single-stepped
+    if (tile && tile->variant == 3) {
+        return run_pull2(a, b, flags, *g, fg, *p, steps, tile, result_in_b, lgdhip::as_stream(stream));
+    }
     // variant 2: four cells per thread, 64 x BY threads = 256 x BY cells
     const bool x4 = tile && tile->variant == 2;
     if (x4) {

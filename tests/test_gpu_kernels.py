@@ -226,8 +226,9 @@ def test_lbm_vs_oracle(hip, oracle, shape, steps):
 
 def test_lbm_tile_variants_identical(hip, oracle):
     """Work-group shapes (64 x 1/2/4/8, 256 x 1), nontemporal loads, four
-    cells per thread (variant 2) and the three tile orders change the
-    schedule, not the per-cell arithmetic: all results equal the default
+    cells per thread (variant 2), two steps per pass (variant 3: 7 steps =
+    3 fused pairs + 1 single step, z chunks down to 4 planes) and the three
+    tile orders change the schedule, not the per-cell arithmetic: all results equal the default
     launch's bit for bit, on a grid ragged in every direction (nx = 300: the
     last 256-cell segment of a row is partly used)."""
     nx, ny, nz = 300, 37, 11
@@ -237,7 +238,8 @@ def test_lbm_tile_variants_identical(hip, oracle):
     for kw in (dict(by=1), dict(by=2), dict(by=8), dict(bx=256), dict(variant=1), dict(flags=hip.TILE_XCD_SWIZZLE),
                dict(flags=hip.TILE_XCD_ROWS), dict(bx=256, flags=hip.TILE_XCD_SWIZZLE), dict(variant=2),
                dict(variant=2, by=2), dict(variant=2, by=4, flags=hip.TILE_XCD_SWIZZLE),
-               dict(variant=2, flags=hip.TILE_XCD_ROWS)):
+               dict(variant=2, flags=hip.TILE_XCD_ROWS), dict(variant=3), dict(variant=3, by=2),
+               dict(variant=3, bz=4), dict(variant=3, by=2, bz=5)):
         say("lbm tile", kw)
         out, _ = gpu_lbm(hip, g0, 7, tile=hip.tile(**kw))
         assert np.array_equal(out, ref), kw
@@ -250,7 +252,8 @@ def test_lbm_z_slab_layout_identical(hip, oracle):
     ux, uy, uz = synthetic.lbm_initial_velocity(nx, ny, nz)
     g0 = shifted(oracle.lbm_equilibrium(np.ones((nz, ny, nx)), ux, uy, uz))
     ref, _ = gpu_lbm(hip, g0, 7)
-    for kw in (dict(), dict(by=2), dict(bx=256), dict(variant=2), dict(flags=hip.TILE_XCD_SWIZZLE)):
+    for kw in (dict(), dict(by=2), dict(bx=256), dict(variant=2), dict(flags=hip.TILE_XCD_SWIZZLE), dict(variant=3),
+               dict(variant=3, by=2, bz=3)):
         say("lbm z slabs", kw)
         out, _ = gpu_lbm(hip, g0, 7, tile=hip.tile(**kw) if kw else None, layout=hip.LAYOUT_Z_SLABS)
         assert np.array_equal(out, ref), kw
