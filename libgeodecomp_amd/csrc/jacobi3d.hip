@@ -133,7 +133,7 @@ __device__ __forceinline__ double lane_next(double v) {
 // buffered: one barrier per plane).  Cells outside the box hold the edge
 // value at every level, exactly as the halo does between single sweeps, and
 // every cell sums in the single sweep's order, so K passes of this kernel
-// equal K single sweeps bit for bit.  Level-0 planes are prefetched PF
+// equal K single sweeps bit for bit.  Level-0 planes are prefetched PF = 3
 // planes ahead.
 //
 // Tiles are numbered x fastest, then y, then z chunk; work-groups map to
@@ -145,6 +145,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
                                                            int64_t origin, int zchunk, unsigned gx, unsigned gy,
                                                            unsigned ntiles, unsigned per, int swizzle) {
     constexpr int OX = 64 - 2 * K, OY = TY - 2 * K;
+    static_assert(PF == 3, "level-0 ring of 6: a window of 3 planes plus 3 in flight");
     __shared__ double lds[2][K][TY][64];
     const unsigned t = lgdhip::xcd_tile(blockIdx.x, per, swizzle != 0);
     if (t >= ntiles) return;  // whole work-group, before any barrier
@@ -160,61 +161,93 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
     const int yd = ty > 0 ? ty - 1 : 0;
     const int yu = ty < TY - 1 ? ty + 1 : TY - 1;
 
-    auto load0 = [&](int p) -> double {
-        return (inxy && (unsigned)p < (unsigned)nz) ? src[col + (int64_t)p * sxy] : edge;
-    };
+    // level-0 loads are unconditional: coordinates outside the box are
+    // clamped into the one-cell halo shell, which holds the edge value, so
+    // no select or branch guards them and the compiler's wait counts let
+    // PF planes stay in flight
+    const int64_t colh = origin + (int64_t)min(max(y, -1), ny) * sx + min(max(x, -1), nx);
+    auto load0 = [&](int p) -> double { return src[colh + (int64_t)min(max(p, -1), nz) * sxy]; };
 
+    // lv: level-0 planes, a ring of 6 (plane p in slot (p - pbeg) mod 6:
+    // the window p-2 .. p plus the loads of p+1 .. p+3 in flight); w[l]
+    // (l >= 1): level l's last three planes, a ring of 3.  The slot roles
+    // follow the step index j mod 6, a compile-time constant once the march
+    // is unrolled by 6, so no value is ever moved between registers.
+    double lv[6];
     double w[K][3];
 #pragma unroll
     for (int l = 0; l < K; ++l) w[l][0] = w[l][1] = w[l][2] = edge;
 #pragma unroll
     for (int l = 0; l < K; ++l) lds[0][l][ty][tx] = edge;
-    double pf[PF];
-    const int pbeg = z0 - K, pend = z1 + K;  // level-0 planes pbeg .. pend-1
+    const int pbeg = z0 - K;  // level-0 planes pbeg .. z1 + K - 1: 2K ramp steps, then z1 - z0 steps
+    lv[4] = lv[5] = edge;
 #pragma unroll
-    for (int d = 0; d < PF; ++d) pf[d] = load0(pbeg + d);
+    for (int d = 0; d < PF; ++d) lv[d] = load0(pbeg + d);
     __syncthreads();
 
-    int buf = 0;
-    for (int p = pbeg; p < pend; ++p) {
-        const double v0 = pf[0];
-#pragma unroll
-        for (int d = 0; d < PF - 1; ++d) pf[d] = pf[d + 1];
-        pf[PF - 1] = load0(p + PF);
-        w[0][0] = w[0][1];
-        w[0][1] = w[0][2];
-        w[0][2] = v0;
+    // step j at level-0 plane p = pbeg + j; JM = j mod 6; JR = j during the
+    // ramp, 2K after.  Level l's plane p - l is read by a written plane only
+    // when j >= 2l, and level l's LDS copy only when j >= 2l + 1: the ramp
+    // skips the rest (no later step reads those slots, so their stale
+    // contents never matter).
+    auto step = [&](auto jm_c, auto jr_c, int p) {
+        constexpr int JM = decltype(jm_c)::value;
+        constexpr int JR = decltype(jr_c)::value;
+        constexpr int NEW = JM % 3, OLD = (JM + 1) % 3, MID = (JM + 2) % 3;  // roles after this step's insert
+        constexpr int B = JM & 1;                                             // LDS buffer holding step j-1's planes
+        lv[(JM + PF) % 6] = load0(p + PF);
 #pragma unroll
         for (int l = 1; l <= K; ++l) {
+            if (2 * l > JR) continue;
             // level l at plane q = p - l from level l-1 at planes q-1, q, q+1
             const int q = p - l;
-            const double c = w[l - 1][1];
+            const double a = l == 1 ? lv[(JM + 4) % 6] : w[l - 1][OLD];
+            const double c = l == 1 ? lv[(JM + 5) % 6] : w[l - 1][MID];
+            const double b = l == 1 ? lv[JM] : w[l - 1][NEW];
             const double xm = lane_prev(c);
             const double xp = lane_next(c);
-            const double ym = lds[buf][l - 1][yd][tx];
-            const double yp = lds[buf][l - 1][yu][tx];
-            double s = w[l - 1][0];  // ( 0,  0, -1)
-            s += ym;                 // ( 0, -1,  0)
-            s += xm;                 // (-1,  0,  0)
-            s += c;                  // ( 0,  0,  0)
-            s += xp;                 // ( 1,  0,  0)
-            s += yp;                 // ( 0,  1,  0)
-            s += w[l - 1][2];        // ( 0,  0,  1)
+            const double ym = lds[B][l - 1][yd][tx];
+            const double yp = lds[B][l - 1][yu][tx];
+            double s = a;  // ( 0,  0, -1)
+            s += ym;       // ( 0, -1,  0)
+            s += xm;       // (-1,  0,  0)
+            s += c;        // ( 0,  0,  0)
+            s += xp;       // ( 1,  0,  0)
+            s += yp;       // ( 0,  1,  0)
+            s += b;        // ( 0,  0,  1)
             const double v = (inxy && (unsigned)q < (unsigned)nz) ? s * (1.0 / 7.0) : edge;
             if (l < K) {
-                w[l][0] = w[l][1];
-                w[l][1] = w[l][2];
-                w[l][2] = v;
-            } else if (writer && q >= z0) {
+                w[l][NEW] = v;
+            } else if (writer) {  // after the ramp q = p - K runs over z0 .. z1 - 1
                 __builtin_nontemporal_store(v, dst + col + (int64_t)q * sxy);
             }
         }
         // publish level l at plane p - l for the next step's y neighbours
+        if (JR >= 1) lds[B ^ 1][0][ty][tx] = lv[JM];
 #pragma unroll
-        for (int l = 0; l < K; ++l) lds[buf ^ 1][l][ty][tx] = w[l][2];
+        for (int l = 1; l < K; ++l)
+            if (2 * l + 1 <= JR) lds[B ^ 1][l][ty][tx] = w[l][NEW];
         __syncthreads();
-        buf ^= 1;
+    };
+    using lgdhip::ic;
+    lgdhip::static_for<0, 2 * K>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        step(ic<j % 6>{}, jc, pbeg + j);
+    });
+    constexpr int S = (2 * K) % 6;
+    const int m = z1 - z0;  // steps after the ramp
+    int p = z0 + K;
+    int i = 0;
+    for (; i + 6 <= m; i += 6, p += 6) {
+        lgdhip::static_for<0, 6>([&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+        });
     }
+    lgdhip::static_for<0, 5>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        if (i + d < m) step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+    });
 }
 
 // variant ids (lgdhip_tile::variant): 0 default, 1 single-plane march,
@@ -262,19 +295,20 @@ JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     const TbShape tb = tb_shape(L.variant);
     if (tb.k > 0) {
         // temporal blocking: tiles of (64-2K) x (TY-2K) output columns; a z
-        // chunk of C planes costs its work-group C + 2K pipeline planes.
-        // One 64 x TY work-group fits a CU at a time (register windows), so
-        // the launch takes about ceil(work-groups / CUs) rounds of C + 2K
-        // planes: pick C in {64, 32, 16, 8} minimising that (measured at
-        // 128^3: C = 32 gives 281 GLUPS where 8 gave 250; at 512^3 it picks
-        // 64, the measured best)
+        // chunk of C planes costs its work-group C full pipeline steps plus
+        // a 2K-step ramp of about K - 1 steps' work.  Count the launch as
+        // ceil(work-groups / CUs) rounds of C + K planes and take the chunk
+        // (any C up to 64, the measured best at 512^3) minimising it: at
+        // 128^3 that is 5 chunks of 26 planes, 240 work-groups.
         const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
         const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty - 2 * tb.k);
         if (!tile || tile->bz <= 0) {
             uint64_t best = ~(uint64_t)0;
-            for (int c = 64; c >= 8; c /= 2) {
-                const uint64_t groups = (uint64_t)gx * gy * lgdhip::ceil_div(g.nz, c);
-                const uint64_t cost = (groups + lgdhip::kCUs - 1) / lgdhip::kCUs * (uint64_t)(c + 2 * tb.k);
+            for (int64_t n = lgdhip::ceil_div(g.nz, 64); n <= g.nz; ++n) {
+                const int c = (int)lgdhip::ceil_div(g.nz, n);
+                if (lgdhip::ceil_div(g.nz, c) != n) continue;  // same chunk as a smaller n
+                const uint64_t groups = (uint64_t)gx * gy * n;
+                const uint64_t cost = (groups + lgdhip::kCUs - 1) / lgdhip::kCUs * (uint64_t)(c + tb.k);
                 if (cost < best) {
                     best = cost;
                     L.zchunk = c;
@@ -322,23 +356,23 @@ int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const Jacobi
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
     switch (L.variant) {
     case 4:
-        jacobi3d_f64_tb<2, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+        jacobi3d_f64_tb<2, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
     case 5:
-        jacobi3d_f64_tb<3, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+        jacobi3d_f64_tb<3, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
     case 6:
-        jacobi3d_f64_tb<4, 16, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+        jacobi3d_f64_tb<4, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
     case 7:
-        jacobi3d_f64_tb<2, 8, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+        jacobi3d_f64_tb<2, 8, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
     case 8:
-        jacobi3d_f64_tb<3, 8, 4><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+        jacobi3d_f64_tb<3, 8, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
     default: return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "not a temporal-blocking variant");
@@ -409,8 +443,8 @@ extern "C" int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *
     int t = 0;
     const int k = tb_shape(L.variant).k;
     if (k > 0) {
-        // the blocked kernel reads memory only inside the box (plus the edge
-        // value at the halo cell left of the origin)
+        // the blocked kernel reads memory only inside the box and its
+        // one-cell halo shell (validated: halo >= 1), which holds the edge
         for (; t + k <= steps; t += k) {
             rc = launch_tb(cur, nxt, *g, L, s);
             if (rc) return rc;

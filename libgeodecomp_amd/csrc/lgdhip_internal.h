@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 #include "../../include/lgdhip.h"
 
@@ -36,6 +37,17 @@ int hip_fail(hipError_t e, const char *what);
     } while (0)
 
 inline hipStream_t as_stream(lgdhip_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// compile-time int and loop: static_for<B, E>(f) calls f(ic<B>{}) .. f(ic<E-1>{})
+template <int V>
+using ic = std::integral_constant<int, V>;
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (B < E) {
+        f(ic<B>{});
+        static_for<B + 1, E>(f);
+    }
+}
 
 // Validates the internal consistency of a descriptor (extents, offsets and
 // pitches leave room for the halo).  Returns 0 or an error code.

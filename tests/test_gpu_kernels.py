@@ -74,12 +74,13 @@ def test_jacobi_tiles_agree(hip, oracle):
 def test_jacobi_temporal_blocking_bit_exact(hip, oracle, shape, steps):
     """K sweeps per launch (variants 4-8: K = 2, 3, 4 with 16-row tiles, K = 2,
     3 with 8-row tiles), step counts that leave a remainder of single
-    sweeps, ragged shapes, z chunks down to one plane, non-zero edge."""
+    sweeps, ragged shapes, z chunks down to one plane and chunks longer than
+    the kernel's 6-step unrolled march, non-zero edge."""
     nx, ny, nz = shape
     a = synthetic.jacobi_initial(nx, ny, nz, seed=nx + ny + nz)
     exp = oracle.jacobi3d(a, steps, edge=0.5, threads=4)
     for variant in (4, 5, 6, 7, 8):
-        for bz in (0, 1, 5):
+        for bz in (0, 1, 5, 13):
             say("jacobi tb", shape, steps, variant, bz)
             t = hip.tile(bz=bz, variant=variant)
             assert np.array_equal(gpu_jacobi(hip, a, steps, edge=0.5, tile=t), exp)
