@@ -162,7 +162,8 @@ __device__ __forceinline__ float lane_next(float v) {
 // origin of that written region) and marches a z chunk as a pipeline: when
 // plane p of the input arrives, level l is computed at plane p-l from level
 // l-1 (3-plane register window per level, x neighbours by DPP, y neighbours
-// through a double-buffered LDS plane per level, r from a register ring).
+// through a double-buffered LDS plane per level, r from a register ring;
+// the march is unrolled by 6 so every ring rotates by compile-time slot roles).
 // Cells outside the problem hold 0 at every level (the correction's zero
 // Dirichlet halo) and every cell sums in poisson_sweep_f32's order, so one
 // launch equals K launches of that kernel bit for bit.  With `partial`, the
@@ -173,10 +174,17 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
                                                             const float *__restrict__ r,
                                                             const Tile *__restrict__ tiles,
                                                             const ProblemDev *__restrict__ probs,
-                                                            double *__restrict__ partial, int zchunk) {
+                                                            double *__restrict__ partial, int zchunk,
+                                                            unsigned ntiles, unsigned per) {
+    constexpr int PF = 3;  // e planes in flight
+    static_assert(K + 2 <= 6, "r ring of 6 slots holds planes p-K .. p+1");
     __shared__ float lds[2][K][TY][64];
     __shared__ double red[TY];
-    const Tile t = tiles[blockIdx.x];
+    // XCD c runs the contiguous tile range [c*per, (c+1)*per), so tiles whose
+    // overlap rings share lines run on one L2 at about the same time
+    const unsigned tid = lgdhip::xcd_tile(blockIdx.x, per, true);
+    if (tid >= ntiles) return;  // whole work-group, before any barrier
+    const Tile t = tiles[tid];
     const ProblemDev P = probs[t.p];
     const int n = P.n;
     const int tx = threadIdx.x, ty = threadIdx.y;
@@ -188,56 +196,61 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
     const int64_t col = P.base + (int64_t)(y + 1) * sx + (x + 1);  // padded column, plane index = z + 1
     const int yd = ty > 0 ? ty - 1 : 0;
     const int yu = ty < TY - 1 ? ty + 1 : TY - 1;
-    auto in_plane = [&](int p) { return inxy && (unsigned)p < (unsigned)n; };
+    // loads are unconditional: coordinates outside the problem are clamped
+    // into its one-cell halo shell, where e is zero; r there only feeds cells
+    // that are masked to zero.  No guard, so PF planes stay in flight.
+    const int64_t colh = P.base + (int64_t)(min(max(y, -1), n) + 1) * sx + (min(max(x, -1), n) + 1);
+    auto at = [&](int p) { return colh + (int64_t)(min(max(p, -1), n) + 1) * sxy; };
 
+    // ev: e planes, a ring of 6 (plane p in slot (p - pbeg) mod 6: the window
+    // p-2 .. p plus p+1 .. p+3 in flight); rv: r planes p-K .. p+1, a ring
+    // of 6; w[l] (l >= 1): level l's last three planes.  Slot roles follow
+    // the step index mod 6, a compile-time constant once the march is
+    // unrolled by 6, so no value moves between registers.
+    float ev[6], rv[6];
     float w[K][3];
-    float rw[K + 1];  // rw[j] = r at plane p - j
 #pragma unroll
     for (int l = 0; l < K; ++l) w[l][0] = w[l][1] = w[l][2] = 0.0f;
 #pragma unroll
-    for (int j = 0; j <= K; ++j) rw[j] = 0.0f;
-#pragma unroll
     for (int l = 0; l < K; ++l) lds[0][l][ty][tx] = 0.0f;
-    const int pbeg = z0 - K, pend = z1 + K;
-    float ne = in_plane(pbeg) ? e[col + (int64_t)(pbeg + 1) * sxy] : 0.0f;
-    float nr = in_plane(pbeg) ? r[col + (int64_t)(pbeg + 1) * sxy] : 0.0f;
+    const int pbeg = z0 - K;  // input planes pbeg .. z1 + K - 1: 2K ramp steps, then z1 - z0 steps
+    ev[4] = ev[5] = 0.0f;
+#pragma unroll
+    for (int d = 0; d < PF; ++d) ev[d] = e[at(pbeg + d)];
+    rv[0] = r[at(pbeg)];
     __syncthreads();
 
     double acc = 0.0;
-    int buf = 0;
-    for (int p = pbeg; p < pend; ++p) {
-        const float v0 = ne, r0 = nr;
-        if (in_plane(p + 1)) {  // next plane's loads in flight during this step
-            ne = e[col + (int64_t)(p + 2) * sxy];
-            nr = r[col + (int64_t)(p + 2) * sxy];
-        } else {
-            ne = 0.0f;
-            nr = 0.0f;
-        }
-        w[0][0] = w[0][1];
-        w[0][1] = w[0][2];
-        w[0][2] = v0;
-#pragma unroll
-        for (int j = K; j > 0; --j) rw[j] = rw[j - 1];
-        rw[0] = r0;
+    // step j at input plane p = pbeg + j; JM = j mod 6; JR = j during the
+    // ramp, 2K after.  Level l's plane p - l is read by a written plane only
+    // when j >= 2l, and level l's LDS copy only when j >= 2l + 1: the ramp
+    // skips the rest (no later step reads those slots).
+    auto step = [&](auto jm_c, auto jr_c, int p) {
+        constexpr int JM = decltype(jm_c)::value;
+        constexpr int JR = decltype(jr_c)::value;
+        constexpr int NEW = JM % 3, OLD = (JM + 1) % 3, MID = (JM + 2) % 3;  // roles after this step's insert
+        constexpr int B = JM & 1;                                             // LDS buffer holding step j-1's planes
+        ev[(JM + PF) % 6] = e[at(p + PF)];
+        rv[(JM + 1) % 6] = r[at(p + 1)];
 #pragma unroll
         for (int l = 1; l <= K; ++l) {
+            if (2 * l > JR) continue;
             const int q = p - l;
-            const float c = w[l - 1][1];
-            float sum = w[l - 1][0];          // ( 0,  0, -1)
-            sum += lds[buf][l - 1][yd][tx];   // ( 0, -1,  0)
+            const float a = l == 1 ? ev[(JM + 4) % 6] : w[l - 1][OLD];
+            const float c = l == 1 ? ev[(JM + 5) % 6] : w[l - 1][MID];
+            const float b = l == 1 ? ev[JM] : w[l - 1][NEW];
+            float sum = a;                    // ( 0,  0, -1)
+            sum += lds[B][l - 1][yd][tx];     // ( 0, -1,  0)
             sum += lane_prev(c);              // (-1,  0,  0)
             sum += lane_next(c);              // ( 1,  0,  0)
-            sum += lds[buf][l - 1][yu][tx];   // ( 0,  1,  0)
-            sum += w[l - 1][2];               // ( 0,  0,  1)
-            sum += rw[l];
-            const bool live = in_plane(q);
+            sum += lds[B][l - 1][yu][tx];     // ( 0,  1,  0)
+            sum += b;                         // ( 0,  0,  1)
+            sum += rv[(JM + 6 - l) % 6];      // r at plane q
+            const bool live = inxy && (unsigned)q < (unsigned)n;
             const float v = live ? sum * (1.0f / 6.0f) : 0.0f;
             if (l < K) {
-                w[l][0] = w[l][1];
-                w[l][1] = w[l][2];
-                w[l][2] = v;
-            } else if (writer && q >= z0) {
+                w[l][NEW] = v;
+            } else if (writer) {  // after the ramp q = p - K runs over z0 .. z1 - 1
                 e2[col + (int64_t)(q + 1) * sxy] = v;
                 if (partial) {
                     const double rr = (double)sum - 6.0 * (double)c;
@@ -245,11 +258,32 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
                 }
             }
         }
+        // publish level l at plane p - l for the next step's y neighbours
+        if (JR >= 1) lds[B ^ 1][0][ty][tx] = ev[JM];
 #pragma unroll
-        for (int l = 0; l < K; ++l) lds[buf ^ 1][l][ty][tx] = w[l][2];
+        for (int l = 1; l < K; ++l)
+            if (2 * l + 1 <= JR) lds[B ^ 1][l][ty][tx] = w[l][NEW];
         __syncthreads();
-        buf ^= 1;
+    };
+    using lgdhip::ic;
+    lgdhip::static_for<0, 2 * K>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        step(ic<j % 6>{}, jc, pbeg + j);
+    });
+    constexpr int S = (2 * K) % 6;
+    const int m = z1 - z0;  // steps after the ramp
+    int p = z0 + K;
+    int i = 0;
+    for (; i + 6 <= m; i += 6, p += 6) {
+        lgdhip::static_for<0, 6>([&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+        });
     }
+    lgdhip::static_for<0, 5>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        if (i + d < m) step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+    });
     if (partial) {
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
@@ -258,7 +292,7 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
         if (tx == 0 && ty == 0) {
             double sum = 0.0;
             for (int i = 0; i < TY; ++i) sum += red[i];
-            partial[blockIdx.x] = sum;
+            partial[tid] = sum;
         }
     }
 }
@@ -457,10 +491,12 @@ int upload_tb_set(const Workspace &w, const lgdhip_poisson_problem *probs, const
 // one launch of K blocked sweeps a -> b over the uploaded tile list
 void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
                double *partial, hipStream_t s) {
+    if (ntiles <= 0) return;
+    const unsigned nt = (unsigned)ntiles, per = lgdhip::xcd_per(nt), groups = lgdhip::xcd_groups(nt);
     if (sh.k == 2)
-        poisson_sweep_tb<2, 16><<<(unsigned)ntiles, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ);
+        poisson_sweep_tb<2, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
     else
-        poisson_sweep_tb<4, 16><<<(unsigned)ntiles, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ);
+        poisson_sweep_tb<4, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
 }
 
 // Uploads problem descriptors and the tile list of the problems in `set`;
