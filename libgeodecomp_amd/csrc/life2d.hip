@@ -299,19 +299,31 @@ __global__ __launch_bounds__(kLanes *NW) void life_packed_stacked(const uint32_t
         const uint32_t above = wave > 0 ? xchg[buf][wave - 1][1][lane] : v[0];
         const uint32_t below = wave < NW - 1 ? xchg[buf][wave + 1][0][lane] : v[R - 1];
 
-        uint32_t sa, ca, sb, cb;
-        row_sums<true>(above, sa, ca);
-        row_sums<true>(v[0], sb, cb);
+        // one generation; the rule is applied to rows [lo, hi) of the wave.
+        // The block's top wave has no neighbour above, so after generation g
+        // its rows 0 .. g hold don't-care values (halo rows, read only by each
+        // other and never stored); likewise rows R-1-g .. R-1 of the bottom
+        // wave.  Those rows skip the rule (8 of the 14 operations per row:
+        // about 7% of the block's work over a launch; the row sums stay, as
+        // skipping them too costs every row a register move).  The bounds
+        // are wave-uniform, so a skipped row is a scalar branch.
+        auto generation = [&](int lo, int hi) {
+            uint32_t sa, ca, sb, cb;
+            row_sums<true>(above, sa, ca);
+            row_sums<true>(v[0], sb, cb);
 #pragma unroll
-        for (int i = 0; i < R; ++i) {
-            uint32_t sc, cc;
-            row_sums<true>(i + 1 < R ? v[i + 1] : below, sc, cc);
-            v[i] = life_rule(sa, ca, sb, cb, sc, cc, v[i]);
-            sa = sb;
-            ca = cb;
-            sb = sc;
-            cb = cc;
-        }
+            for (int i = 0; i < R; ++i) {
+                uint32_t sc, cc;
+                row_sums<true>(i + 1 < R ? v[i + 1] : below, sc, cc);
+                if (i >= lo && i < hi)
+                    v[i] = life_rule(sa, ca, sb, cb, sc, cc, v[i]);
+                sa = sb;
+                ca = cb;
+                sb = sc;
+                cb = cc;
+            }
+        };
+        generation(wave == 0 ? g + 1 : 0, wave == NW - 1 ? R - 1 - g : R);
         if (cols_edge) {
 #pragma unroll
             for (int i = 0; i < R; ++i) v[i] = (v[i] & cmask) | (edge & ~cmask);
