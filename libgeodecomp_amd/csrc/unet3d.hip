@@ -84,16 +84,23 @@ __device__ __forceinline__ Frag load_a(const u16 *__restrict__ X, int n, int ld,
 
 constexpr int kMaxNTiles = 12;  // cout up to 192 (the dgrad of a 192-channel concat input)
 
-// One wave: MT x 16 consecutive voxels (linear order, any n), all cout (NT
-// tiles of 16).  Each k step loads MT A fragments and NT B fragments for
-// MT x NT MFMAs: the weight fragments are shared by the MT voxel tiles.
+// One wave: MT x 16 consecutive voxels (linear order, any n) times NT cout
+// tiles of 16, the tiles [g NT, g NT + NT) of cout group g (groups split the
+// cout tiles when the voxel tiles alone give too few waves; consecutive waves
+// are the groups of one voxel range, so they share its A fragments in
+// cache).  Each k step loads MT A fragments and NT B fragments for MT x NT
+// MFMAs: the weight fragments are shared by the MT voxel tiles.  Every output
+// accumulates over k in the same order whatever the split.
 template <int NT, int MT>
 __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
                                                        int ksize, const u16 *__restrict__ W, int Kp,
                                                        const float *__restrict__ bias, int cout, u16 *Y, int ldy,
-                                                       int offy, float *Y32, int ld32, int relu, int accumulate) {
+                                                       int offy, float *Y32, int ld32, int relu, int accumulate,
+                                                       int groups) {
     const int lane = threadIdx.x & 63;
-    const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int64_t wave_id = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
+    const int tile0 = (int)(wave_id % groups) * NT;  // first cout tile of this wave
+    const int64_t wave = wave_id / groups;
     const int64_t nvox = (int64_t)n * n * n;
     const int64_t v0 = wave * 16 * MT;
     if (v0 >= nvox) return;
@@ -128,8 +135,9 @@ __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X
         }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
+            const int co = (tile0 + t) * 16 + i;
             b[t].q = make_uint4(0, 0, 0, 0);
-            if (t * 16 + i < cout) b[t].q = *reinterpret_cast<const uint4 *>(W + (int64_t)(t * 16 + i) * Kp + k0);
+            if (co < cout) b[t].q = *reinterpret_cast<const uint4 *>(W + (int64_t)co * Kp + k0);
         }
 #pragma unroll
         for (int m = 0; m < MT; ++m)
@@ -137,10 +145,10 @@ __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X
             for (int t = 0; t < NT; ++t)
                 acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, b[t].v, acc[m][t], 0, 0, 0);
     }
-    // D[r][c] in lane c + 16*(r/4), element r%4: voxel v0 + 16m + 4*(lane/16) + e, channel t*16 + lane%16
+    // D[r][c] in lane c + 16*(r/4), element r%4: voxel v0 + 16m + 4*(lane/16) + e, channel (tile0+t)*16 + lane%16
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        const int co = t * 16 + i;
+        const int co = (tile0 + t) * 16 + i;
         if (co >= cout) continue;
         const float bv = bias ? bias[co] : 0.0f;
 #pragma unroll
@@ -164,6 +172,37 @@ __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X
 // voxel tiles per wave of the forward kernel: weight-fragment reuse bounded
 // by the accumulator registers (MT x NT x 4)
 constexpr int fwd_mt(int nt) { return nt <= 4 ? 4 : (nt <= 8 ? 2 : 1); }
+
+// Forward launch shape for nvox voxels and nt cout tiles: NT tiles per wave
+// (a divisor of nt), MT voxel tiles per wave (at most fwd_mt(NT)), nt / NT
+// cout groups.  Takes the fewest fragment loads per MFMA (1/NT + 1/MT) among
+// the shapes that give at least kFwdWaves waves (8 per CU: the 16^3 and 32^3
+// levels of the U-Net otherwise leave most CUs idle), else the most waves.
+constexpr int64_t kFwdWaves = 2048;
+struct FwdShape {
+    int nt, mt, groups;
+};
+FwdShape fwd_shape(int64_t nvox, int nt) {
+    FwdShape best{nt, fwd_mt(nt), 1}, most = best;
+    double best_loads = 1e30;
+    int64_t most_waves = -1;
+    for (int ntw = nt; ntw >= 1; --ntw) {
+        if (nt % ntw) continue;
+        for (int mt = fwd_mt(ntw); mt >= 1; mt /= 2) {
+            const int64_t waves = (nvox + 16 * mt - 1) / (16 * mt) * (nt / ntw);
+            const double loads = 1.0 / ntw + 1.0 / mt;
+            if (waves >= kFwdWaves && loads < best_loads) {
+                best_loads = loads;
+                best = FwdShape{ntw, mt, nt / ntw};
+            }
+            if (waves > most_waves) {
+                most_waves = waves;
+                most = FwdShape{ntw, mt, nt / ntw};
+            }
+        }
+    }
+    return best_loads < 1e30 ? best : most;
+}
 
 // dW partial over a chunk of voxels: P[chunk][co][k] = sum_v dY[v][co] X[v+tap(k)][ci(k)]
 // one wave per (chunk, k tile of 16), all co tiles
@@ -514,6 +553,94 @@ __global__ void upsample2_backward(const u16 *__restrict__ dY, int m, int ldy, i
     *q = f2bf(s);
 }
 
+// The same four resampling kernels over 8 channels per thread (16-byte loads
+// and stores), used when the channel count, row strides, channel offsets and
+// base pointers are multiples of 8 elements.  Each channel is summed in the
+// order of the kernels above, so the results are bit-identical.
+struct alignas(16) Bf8 {
+    u16 h[8];
+};
+__device__ __forceinline__ Bf8 ld8(const u16 *p) { return *reinterpret_cast<const Bf8 *>(p); }
+__device__ __forceinline__ void st8(u16 *p, const Bf8 &v) { *reinterpret_cast<Bf8 *>(p) = v; }
+
+__global__ void avgpool2_v8(const u16 *__restrict__ X, int n, int ldx, int offx, u16 *Y, int ldy, int offy, int c8) {
+    const int m = n / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * m * m * c8) return;
+    const int ch = (int)(idx % c8) * 8;
+    const int64_t v = idx / c8;
+    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    float s[8] = {};
+    for (int d = 0; d < 8; ++d) {
+        const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
+        const Bf8 a = ld8(X + ((int64_t)((zz * n + yy) * n + xx)) * ldx + offx + ch);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += bf2f(a.h[k]);
+    }
+    Bf8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o.h[k] = f2bf(s[k] * 0.125f);
+    st8(Y + v * ldy + offy + ch, o);
+}
+
+__global__ void avgpool2_backward_v8(const u16 *__restrict__ dY, int n, int ldy, int offy, u16 *dX, int ldx, int offx,
+                                     int c8, int accumulate) {
+    const int m = n / 2;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n * n * c8) return;
+    const int ch = (int)(idx % c8) * 8;
+    const int64_t v = idx / c8;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+    const Bf8 a = ld8(dY + p * ldy + offy + ch);
+    u16 *q = dX + v * ldx + offx + ch;
+    Bf8 old;
+    if (accumulate) old = ld8(q);
+    Bf8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        float g = bf2f(a.h[k]) * 0.125f;
+        if (accumulate) g += bf2f(old.h[k]);
+        o.h[k] = f2bf(g);
+    }
+    st8(q, o);
+}
+
+__global__ void upsample2_v8(const u16 *__restrict__ X, int m, int ldx, int offx, u16 *Y, int ldy, int offy, int c8) {
+    const int n = 2 * m;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)n * n * n * c8) return;
+    const int ch = (int)(idx % c8) * 8;
+    const int64_t v = idx / c8;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+    st8(Y + v * ldy + offy + ch, ld8(X + p * ldx + offx + ch));
+}
+
+__global__ void upsample2_backward_v8(const u16 *__restrict__ dY, int m, int ldy, int offy, u16 *dX, int ldx, int offx,
+                                      int c8, int accumulate) {
+    const int n = 2 * m;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (int64_t)m * m * m * c8) return;
+    const int ch = (int)(idx % c8) * 8;
+    const int64_t v = idx / c8;
+    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    float s[8] = {};
+    for (int d = 0; d < 8; ++d) {
+        const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
+        const Bf8 a = ld8(dY + ((int64_t)((zz * n + yy) * n + xx)) * ldy + offy + ch);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s[k] += bf2f(a.h[k]);
+    }
+    u16 *q = dX + v * ldx + offx + ch;
+    Bf8 old;
+    if (accumulate) old = ld8(q);
+    Bf8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o.h[k] = f2bf(accumulate ? s[k] + bf2f(old.h[k]) : s[k]);
+    st8(q, o);
+}
+
 // FP32 -> BF16 into a channel slot of a channels-last tensor
 __global__ void pack_bf16(const float *__restrict__ src, int64_t nvox, u16 *Y, int ldy, int offy) {
     const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -563,6 +690,12 @@ __global__ void adam_step(float *__restrict__ w, float *__restrict__ m, float *_
 }
 
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+// 8-channel (16-byte) resampling applies: channel count, strides and channel
+// offsets multiples of 8 elements, both base pointers 16-byte aligned
+bool vec8(const void *a, const void *b, int c, int lda, int offa, int ldb, int offb) {
+    return c % 8 == 0 && lda % 8 == 0 && offa % 8 == 0 && ldb % 8 == 0 && offb % 8 == 0 &&
+           ((uintptr_t)a & 15) == 0 && ((uintptr_t)b & 15) == 0;
+}
 
 // shapes the channels-first weight gradient handles
 bool wgrad_cf_shape(int n, int cin, int ksize, int cout) {
@@ -608,28 +741,26 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
     const u16 *Wb = (const u16 *)w;
     u16 *Y = (u16 *)y;
     const int64_t nvox = (int64_t)n * n * n;
-#define LGD_CONV_CASE(T)                                                                                    \
-    case T: {                                                                                               \
-        constexpr int MT = fwd_mt(T);                                                                       \
-        const int64_t waves = (nvox + 16 * MT - 1) / (16 * MT);                                             \
-        conv3d_fwd_mfma<T, MT><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(                                 \
-            X, n, ldx, offx, cin, ksize, Wb, Kp, bias, cout, Y, ldy, offy, y32, ld32, relu, accumulate);    \
-        break;                                                                                              \
+    const FwdShape sh = fwd_shape(nvox, nt);
+#define LGD_CONV_CASE(T, M)                                                                                   \
+    if (sh.nt == T && sh.mt == M) {                                                                           \
+        const int64_t waves = (nvox + 16 * M - 1) / (16 * M) * sh.groups;                                     \
+        conv3d_fwd_mfma<T, M><<<(unsigned)((waves + 3) / 4), 256, 0, s>>>(                                    \
+            X, n, ldx, offx, cin, ksize, Wb, Kp, bias, cout, Y, ldy, offy, y32, ld32, relu, accumulate,        \
+            sh.groups);                                                                                       \
     }
-    switch (nt) {
-        LGD_CONV_CASE(1)
-        LGD_CONV_CASE(2)
-        LGD_CONV_CASE(3)
-        LGD_CONV_CASE(4)
-        LGD_CONV_CASE(5)
-        LGD_CONV_CASE(6)
-        LGD_CONV_CASE(7)
-        LGD_CONV_CASE(8)
-        LGD_CONV_CASE(9)
-        LGD_CONV_CASE(10)
-        LGD_CONV_CASE(11)
-        LGD_CONV_CASE(12)
-    }
+    LGD_CONV_CASE(1, 4) LGD_CONV_CASE(1, 2) LGD_CONV_CASE(1, 1)
+    LGD_CONV_CASE(2, 4) LGD_CONV_CASE(2, 2) LGD_CONV_CASE(2, 1)
+    LGD_CONV_CASE(3, 4) LGD_CONV_CASE(3, 2) LGD_CONV_CASE(3, 1)
+    LGD_CONV_CASE(4, 4) LGD_CONV_CASE(4, 2) LGD_CONV_CASE(4, 1)
+    LGD_CONV_CASE(5, 2) LGD_CONV_CASE(5, 1)
+    LGD_CONV_CASE(6, 2) LGD_CONV_CASE(6, 1)
+    LGD_CONV_CASE(7, 2) LGD_CONV_CASE(7, 1)
+    LGD_CONV_CASE(8, 2) LGD_CONV_CASE(8, 1)
+    LGD_CONV_CASE(9, 1)
+    LGD_CONV_CASE(10, 1)
+    LGD_CONV_CASE(11, 1)
+    LGD_CONV_CASE(12, 1)
 #undef LGD_CONV_CASE
     LGD_LAUNCHED("conv3d_fwd_mfma");
     return 0;
@@ -772,6 +903,12 @@ int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int l
                          lgdhip_stream_t stream) {
     LGD_REQUIRE(x && y && n % 2 == 0, "bad arguments");
     const int64_t m = n / 2;
+    if (vec8(x, y, c, ldx, offx, ldy, offy)) {
+        avgpool2_v8<<<blocks_for(m * m * m * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+            (const u16 *)x, n, ldx, offx, (u16 *)y, ldy, offy, c / 8);
+        LGD_LAUNCHED("avgpool2_v8");
+        return 0;
+    }
     avgpool2<<<blocks_for(m * m * m * c), 256, 0, lgdhip::as_stream(stream)>>>((const u16 *)x, n, ldx, offx, (u16 *)y,
                                                                                ldy, offy, c);
     LGD_LAUNCHED("avgpool2");
@@ -781,6 +918,12 @@ int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int l
 int lgdhip_avgpool2_backward_bf16(const void *dy, int n, int ldy, int offy, void *dx, int ldx, int offx, int c,
                                   int accumulate, lgdhip_stream_t stream) {
     LGD_REQUIRE(dy && dx && n % 2 == 0, "bad arguments");
+    if (vec8(dy, dx, c, ldy, offy, ldx, offx)) {
+        avgpool2_backward_v8<<<blocks_for((int64_t)n * n * n * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+            (const u16 *)dy, n, ldy, offy, (u16 *)dx, ldx, offx, c / 8, accumulate);
+        LGD_LAUNCHED("avgpool2_backward_v8");
+        return 0;
+    }
     avgpool2_backward<<<blocks_for((int64_t)n * n * n * c), 256, 0, lgdhip::as_stream(stream)>>>(
         (const u16 *)dy, n, ldy, offy, (u16 *)dx, ldx, offx, c, accumulate);
     LGD_LAUNCHED("avgpool2_backward");
@@ -791,6 +934,12 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
                           lgdhip_stream_t stream) {
     LGD_REQUIRE(x && y && m >= 1, "bad arguments");
     const int64_t n = 2 * (int64_t)m;
+    if (vec8(x, y, c, ldx, offx, ldy, offy)) {
+        upsample2_v8<<<blocks_for(n * n * n * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+            (const u16 *)x, m, ldx, offx, (u16 *)y, ldy, offy, c / 8);
+        LGD_LAUNCHED("upsample2_v8");
+        return 0;
+    }
     upsample2<<<blocks_for(n * n * n * c), 256, 0, lgdhip::as_stream(stream)>>>((const u16 *)x, m, ldx, offx, (u16 *)y,
                                                                                 ldy, offy, c);
     LGD_LAUNCHED("upsample2");
@@ -800,6 +949,12 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
 int lgdhip_upsample2_backward_bf16(const void *dy, int m, int ldy, int offy, void *dx, int ldx, int offx, int c,
                                    int accumulate, lgdhip_stream_t stream) {
     LGD_REQUIRE(dy && dx && m >= 1, "bad arguments");
+    if (vec8(dy, dx, c, ldy, offy, ldx, offx)) {
+        upsample2_backward_v8<<<blocks_for((int64_t)m * m * m * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+            (const u16 *)dy, m, ldy, offy, (u16 *)dx, ldx, offx, c / 8, accumulate);
+        LGD_LAUNCHED("upsample2_backward_v8");
+        return 0;
+    }
     upsample2_backward<<<blocks_for((int64_t)m * m * m * c), 256, 0, lgdhip::as_stream(stream)>>>(
         (const u16 *)dy, m, ldy, offy, (u16 *)dx, ldx, offx, c, accumulate);
     LGD_LAUNCHED("upsample2_backward");

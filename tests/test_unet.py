@@ -222,29 +222,74 @@ def test_conv_wgrad_strided_tensors(hip, oracle):
     assert np.allclose(outb, refb, rtol=1e-5, atol=1e-3)
 
 
+def children_sum(a, m, c):
+    """FP32 sum of the 8 children of each parent voxel in the kernels' order
+    (children z-major, then y, then x) of an (2m)^3 x c array."""
+    blocks = a.reshape(m, 2, m, 2, m, 2, c)
+    s = np.zeros((m, m, m, c), np.float32)
+    for d in range(8):
+        s = s + blocks[:, d >> 2, :, (d >> 1) & 1, :, d & 1, :]
+    return s
+
+
 @pytest.mark.gpu
-def test_pool_upsample_relu_mse_adam(hip):
+@pytest.mark.parametrize("c,pad,off", [(16, 0, 0), (24, 8, 8), (3, 2, 1), (16, 3, 0)])
+def test_pool_upsample_forward_backward(hip, c, pad, off):
+    """2x2x2 average pool, nearest upsample and both gradients, in channel
+    slices of wider rows (row stride c + pad, channel offset off): the
+    8-channel path (c, stride and offset multiples of 8) and the per-element
+    path give the numpy restatement of the kernels' FP32 order bit for bit,
+    with and without accumulation, and leave the other channels untouched."""
+    rng = np.random.default_rng(c * 10 + pad)
+    n, m = 8, 4
+    ld = c + pad
+
+    def wide(shape_vox):
+        return bf16_round(rng.standard_normal(shape_vox + (ld,)).astype(np.float32))
+
+    def run(name, src_w, dst_w, *args):
+        sb, db = dev(hip, to_bf16_bits(src_w)), dev(hip, to_bf16_bits(dst_w))
+        hip.call(name, sb.ptr, *args[:1], ld, off, db.ptr, ld, off, c, *args[1:], None)
+        out = np.empty(dst_w.size, np.uint16)
+        db.download(out)
+        return from_bf16_bits(out).reshape(dst_w.shape)
+
+    sl = slice(off, off + c)
+    x = wide((n, n, n))
+    y0 = wide((m, m, m))
+    print(f"[case] pool/upsample c {c} ld {ld} off {off}", flush=True)
+    y = run("lgdhip_avgpool2_bf16", x, y0, n)
+    exp = y0.copy()
+    exp[..., sl] = bf16_round(children_sum(x[..., sl], m, c) * np.float32(0.125))
+    assert np.array_equal(y, exp)
+    u0 = wide((n, n, n))
+    u = run("lgdhip_upsample2_bf16", y, u0, m)
+    exp = u0.copy()
+    exp[..., sl] = np.repeat(np.repeat(np.repeat(y[..., sl], 2, 0), 2, 1), 2, 2)
+    assert np.array_equal(u, exp)
+    for acc in (0, 1):
+        gy = wide((m, m, m))
+        gx0 = wide((n, n, n))
+        gx = run("lgdhip_avgpool2_backward_bf16", gy, gx0, n, acc)
+        g = np.repeat(np.repeat(np.repeat(gy[..., sl], 2, 0), 2, 1), 2, 2) * np.float32(0.125)
+        exp = gx0.copy()
+        exp[..., sl] = bf16_round(g + gx0[..., sl] if acc else g)
+        assert np.array_equal(gx, exp), ("avgpool backward", acc)
+        gu = wide((n, n, n))
+        gp0 = wide((m, m, m))
+        gp = run("lgdhip_upsample2_backward_bf16", gu, gp0, m, acc)
+        s = children_sum(gu[..., sl], m, c)
+        exp = gp0.copy()
+        exp[..., sl] = bf16_round(s + gp0[..., sl] if acc else s)
+        assert np.array_equal(gp, exp), ("upsample backward", acc)
+
+
+@pytest.mark.gpu
+def test_relu_mse_adam(hip):
     rng = np.random.default_rng(4)
     n, c = 8, 16
     x = bf16_round(rng.standard_normal((n, n, n, c)).astype(np.float32))
     xb = dev(hip, to_bf16_bits(x))
-    m = n // 2
-    yb = hip.DeviceBuffer(m ** 3 * c * 2)
-    hip.call("lgdhip_avgpool2_bf16", xb.ptr, n, c, 0, yb.ptr, c, 0, c, None)
-    y = np.empty(m ** 3 * c, np.uint16)
-    yb.download(y)
-    blocks = x.reshape(m, 2, m, 2, m, 2, c)
-    s = np.zeros((m, m, m, c), np.float32)
-    for d in range(8):     # the kernel's FP32 order: children z-major, then y, then x
-        s = s + blocks[:, d >> 2, :, (d >> 1) & 1, :, d & 1, :]
-    exp = bf16_round(s * np.float32(0.125))
-    assert np.array_equal(from_bf16_bits(y).reshape(exp.shape), exp)
-    ub = hip.DeviceBuffer(n ** 3 * c * 2)
-    hip.call("lgdhip_upsample2_bf16", yb.ptr, m, c, 0, ub.ptr, c, 0, c, None)
-    u = np.empty(n ** 3 * c, np.uint16)
-    ub.download(u)
-    up = np.repeat(np.repeat(np.repeat(from_bf16_bits(y).reshape(m, m, m, c), 2, 0), 2, 1), 2, 2)
-    assert np.array_equal(from_bf16_bits(u).reshape(up.shape), up)
     # relu backward: zero where y <= 0
     g = bf16_round(rng.standard_normal((n, n, n, c)).astype(np.float32))
     gb = dev(hip, to_bf16_bits(g))
