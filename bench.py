@@ -184,8 +184,11 @@ class Jacobi:
         self.host = host
 
     def config(self):
-        return {"workload": f"Jacobi3D 7-point FP64 {self.n}^3, zero Cube edge, U[0,1) interior, {self.sweeps} steps",
-                "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed, "parallelism": "single GPU"}
+        c = {"workload": f"Jacobi3D 7-point FP64 {self.n}^3, zero Cube edge, U[0,1) interior, {self.sweeps} steps",
+             "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed, "parallelism": "single GPU"}
+        if getattr(self, "plan", None):
+            c["launch"] = self.plan  # the library's plan: kernel variant and z planes per work-group
+        return c
 
     def lups_per_step(self):
         return float(self.n) ** 3 * self.sweeps
@@ -196,22 +199,26 @@ class Jacobi:
         lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps,
                     C.byref(self.tile), None, s)
 
-    # sweeps per launch of the temporal-blocking variants; variant 0 is the
-    # library default, 6 (4 sweeps per launch)
-    TB_DEPTH = {0: 4, 4: 2, 5: 3, 6: 4, 7: 2, 8: 3}
+    # sweeps per launch of the temporal-blocking variants (9 and 10 hold two
+    # rows per thread); the library resolves variant 0 by its launch plan
+    TB_DEPTH = {4: 2, 5: 3, 6: 4, 7: 2, 8: 3, 9: 4, 10: 4}
 
     def dominant_kernel(self):
-        k = self.TB_DEPTH.get(self.tile.variant, 1)
+        v, zc = C.c_int(0), C.c_int(0)
+        lgdhip.call("lgdhip_jacobi3d_f64_plan", C.byref(self.g), C.byref(self.tile), C.byref(v), C.byref(zc))
+        variant = v.value
+        k = self.TB_DEPTH.get(variant, 1)
         sweeps = 240   # a multiple of every blocking depth: launches of one kernel only
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr,
                                                     C.byref(self.g), sweeps, C.byref(self.tile), None,
                                                     self.stream.h))
         if k > 1:
-            name = "jacobi3d_f64_tb"
+            name = "jacobi3d_f64_tb2" if variant in (9, 10) else "jacobi3d_f64_tb"
         else:
             name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
         # one launch advances k sweeps: algorithmic bytes are k sweeps' worth
         self.shape = f"{self.n}^3,K{k}" if k > 1 else f"{self.n}^3"
+        self.plan = {"variant": variant, "zchunk": zc.value}
         return name, ms / (sweeps // k), k * self.bytes_per_lup * float(self.n) ** 3
 
     def cpu_baseline(self, budget_s=15.0):
@@ -221,7 +228,7 @@ class Jacobi:
         t0 = time.perf_counter()
         oracle.jacobi3d(self.host, steps, 0.0, th)
         dt = time.perf_counter() - t0
-        steps = max(4, min(2000, int(steps * budget_s / max(dt, 1e-3))))
+        steps = max(4, min(200000, int(steps * budget_s / max(dt, 1e-3))))
         t0 = time.perf_counter()
         oracle.jacobi3d(self.host, steps, 0.0, th)
         dt = time.perf_counter() - t0

@@ -232,11 +232,19 @@ int lgdhip_grid_store(const void *grid, const lgdhip_grid *g, void *host,
 /* ------------------------------------------------------------------ */
 /* Runs `steps` sweeps ping-ponging between a and b (both carry the edge
  * value in their halo).  *result_in_b is set to 1 when the final state is
- * in b, 0 when it is in a (tile->variant 4-8 run K = 2-4 sweeps per launch
+ * in b, 0 when it is in a (tile->variant 4-10 run K = 2-4 sweeps per launch
  * and swap once per launch, so this is not steps % 2 there). */
 int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
                             const lgdhip_tile *tile, int *result_in_b,
                             lgdhip_stream_t stream);
+
+/* The launch plan lgdhip_jacobi3d_f64_run takes for this grid and tile
+ * (tile may be NULL): *variant gets the kernel variant (the default resolves
+ * to 6 or 10 by the launch cost model), *zchunk the z planes per work-group.
+ * No device work.  Build-side query for tools (benchmarks, tuners); the
+ * reference's CUDASimulator has no counterpart (its launch shape is the
+ * block size it is given, cudasimulator.h:314-429). */
+int lgdhip_jacobi3d_f64_plan(const lgdhip_grid *g, const lgdhip_tile *tile, int *variant, int *zchunk);
 
 /* ------------------------------------------------------------------ */
 /* Game of Life, uint8 cells (0/1), Moore<2,1>, Cube topology          */

@@ -250,28 +250,165 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
     });
 }
 
+// Same K-sweep pipeline with two adjacent rows per thread: a 64 x TY
+// work-group holds a 64 x 2TY tile (written region (64-2K) x (2TY-2K)).  A
+// thread's rows 2ty and 2ty+1 are each other's y neighbours in registers;
+// only the tile-row pairs' outer neighbours come through LDS (the first row
+// of thread ty+1 and the last row of thread ty-1), which halves the LDS
+// reads per cell, and the taller tile recomputes a smaller overlap ring.
+// Cells, sum order, ramp and rings as in jacobi3d_f64_tb: bit-exact with K
+// single sweeps.
+template <int K, int TY>
+__global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__restrict__ src, double *__restrict__ dst,
+                                                            int nx, int ny, int nz, int64_t sx, int64_t sxy,
+                                                            int64_t origin, int zchunk, unsigned gx, unsigned gy,
+                                                            unsigned ntiles, unsigned per, int swizzle) {
+    constexpr int PF = 3;
+    constexpr int CY = 2 * TY;  // tile rows
+    constexpr int OX = 64 - 2 * K, OY = CY - 2 * K;
+    static_assert(OY > 0, "tile taller than its overlap ring");
+    // [buffer][level][thread row][0: first row, 1: last row of the pair][lane]
+    __shared__ double lds[2][K][TY][2][64];
+    const unsigned t = lgdhip::xcd_tile(blockIdx.x, per, swizzle != 0);
+    if (t >= ntiles) return;  // whole work-group, before any barrier
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = (int)(t % gx) * OX - K + tx;
+    const int y0 = (int)((t / gx) % gy) * OY - K + 2 * ty;  // rows y0, y0 + 1
+    const int z0 = (int)(t / (gx * gy)) * zchunk;
+    const int z1 = min(z0 + zchunk, nz);
+    const bool xin = (unsigned)x < (unsigned)nx;
+    const bool inxy[2] = {xin && (unsigned)y0 < (unsigned)ny, xin && (unsigned)(y0 + 1) < (unsigned)ny};
+    const bool xw = tx >= K && tx < 64 - K;
+    const bool writer[2] = {inxy[0] && xw && 2 * ty >= K && 2 * ty < CY - K,
+                            inxy[1] && xw && 2 * ty + 1 >= K && 2 * ty + 1 < CY - K};
+    const double edge = src[origin - 1];  // the Cube edge value held by every halo cell
+    const int64_t col[2] = {origin + (int64_t)y0 * sx + x, origin + (int64_t)(y0 + 1) * sx + x};
+    const int xc = min(max(x, -1), nx);
+    const int64_t colh[2] = {origin + (int64_t)min(max(y0, -1), ny) * sx + xc,
+                             origin + (int64_t)min(max(y0 + 1, -1), ny) * sx + xc};
+    // the ring rows (tile rows 0 and CY-1) read their own pair instead of a
+    // missing neighbour: their values are never written
+    const int tyd = ty > 0 ? ty - 1 : 0;
+    const int tyu = ty < TY - 1 ? ty + 1 : TY - 1;
+    auto load0 = [&](int r, int p) -> double { return src[colh[r] + (int64_t)min(max(p, -1), nz) * sxy]; };
+
+    double lv[2][6];
+    double w[K][2][3];
+#pragma unroll
+    for (int l = 0; l < K; ++l)
+#pragma unroll
+        for (int r = 0; r < 2; ++r) w[l][r][0] = w[l][r][1] = w[l][r][2] = edge;
+#pragma unroll
+    for (int l = 0; l < K; ++l) lds[0][l][ty][0][tx] = lds[0][l][ty][1][tx] = edge;
+    const int pbeg = z0 - K;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        lv[r][4] = lv[r][5] = edge;
+#pragma unroll
+        for (int d = 0; d < PF; ++d) lv[r][d] = load0(r, pbeg + d);
+    }
+    __syncthreads();
+
+    auto step = [&](auto jm_c, auto jr_c, int p) {
+        constexpr int JM = decltype(jm_c)::value;
+        constexpr int JR = decltype(jr_c)::value;
+        constexpr int NEW = JM % 3, OLD = (JM + 1) % 3, MID = (JM + 2) % 3;
+        constexpr int B = JM & 1;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) lv[r][(JM + PF) % 6] = load0(r, p + PF);
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            if (2 * l > JR) continue;
+            const int q = p - l;
+            const bool zin = (unsigned)q < (unsigned)nz;
+            double a[2], c[2], b[2];
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                a[r] = l == 1 ? lv[r][(JM + 4) % 6] : w[l - 1][r][OLD];
+                c[r] = l == 1 ? lv[r][(JM + 5) % 6] : w[l - 1][r][MID];
+                b[r] = l == 1 ? lv[r][JM] : w[l - 1][r][NEW];
+            }
+            const double ym[2] = {lds[B][l - 1][tyd][1][tx], c[0]};
+            const double yp[2] = {c[1], lds[B][l - 1][tyu][0][tx]};
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                double s = a[r];          // ( 0,  0, -1)
+                s += ym[r];               // ( 0, -1,  0)
+                s += lane_prev(c[r]);     // (-1,  0,  0)
+                s += c[r];                // ( 0,  0,  0)
+                s += lane_next(c[r]);     // ( 1,  0,  0)
+                s += yp[r];               // ( 0,  1,  0)
+                s += b[r];                // ( 0,  0,  1)
+                const double v = (inxy[r] && zin) ? s * (1.0 / 7.0) : edge;
+                if (l < K) {
+                    w[l][r][NEW] = v;
+                } else if (writer[r]) {
+                    __builtin_nontemporal_store(v, dst + col[r] + (int64_t)q * sxy);
+                }
+            }
+        }
+        if (JR >= 1) {
+            lds[B ^ 1][0][ty][0][tx] = lv[0][JM];
+            lds[B ^ 1][0][ty][1][tx] = lv[1][JM];
+        }
+#pragma unroll
+        for (int l = 1; l < K; ++l)
+            if (2 * l + 1 <= JR) {
+                lds[B ^ 1][l][ty][0][tx] = w[l][0][NEW];
+                lds[B ^ 1][l][ty][1][tx] = w[l][1][NEW];
+            }
+        __syncthreads();
+    };
+    using lgdhip::ic;
+    lgdhip::static_for<0, 2 * K>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        step(ic<j % 6>{}, jc, pbeg + j);
+    });
+    constexpr int S = (2 * K) % 6;
+    const int m = z1 - z0;
+    int p = z0 + K;
+    int i = 0;
+    for (; i + 6 <= m; i += 6, p += 6) {
+        lgdhip::static_for<0, 6>([&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+        });
+    }
+    lgdhip::static_for<0, 5>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        if (i + d < m) step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+    });
+}
+
 // variant ids (lgdhip_tile::variant): 0 default, 1 single-plane march,
 // 2 four-plane batches, 3 eight-plane batches; temporal blocking (K sweeps
 // per launch, tile 64 x TY): 4 K=2 TY=16, 5 K=3 TY=16, 6 K=4 TY=16,
-// 7 K=2 TY=8, 8 K=3 TY=8.  The default blocks 4 sweeps per launch (fastest
-// on 128^3 and 512^3 on MI355X); step counts not divisible by K finish with
-// single sweeps of kSingleSweepVariant.
+// 7 K=2 TY=8, 8 K=3 TY=8; two rows per thread (tile 64 x 2TY, work-group
+// 64 x TY): 9 K=4 TY=8, 10 K=4 TY=16.  The default blocks 4 sweeps per
+// launch with variant 6 or 10, whichever the launch cost model rates cheaper
+// for the grid (6 at 128^3, 10 at 512^3, the faster ones there on MI355X);
+// step counts not divisible by K finish with single sweeps of
+// kSingleSweepVariant.
 constexpr int kJacobiDefaultVariant = 6;
 constexpr int kSingleSweepVariant = 2;
 
 struct TbShape {
-    int k, ty;
+    int k, ty, rows;  // sweeps per launch, work-group rows, tile rows per work-group row
 };
 TbShape tb_shape(int variant) {
     switch (variant) {
-    case 4: return {2, 16};
-    case 5: return {3, 16};
-    case 6: return {4, 16};
-    case 7: return {2, 8};
-    case 8: return {3, 8};
-    default: return {0, 0};
+    case 4: return {2, 16, 1};
+    case 5: return {3, 16, 1};
+    case 6: return {4, 16, 1};
+    case 7: return {2, 8, 1};
+    case 8: return {3, 8, 1};
+    case 9: return {4, 8, 2};
+    case 10: return {4, 16, 2};
+    default: return {0, 0, 1};
     }
 }
+constexpr int kJacobiVariants = 10;
+constexpr int kJacobiTallVariant = 10;  // the default's choice for grids that fill several rounds
 
 struct JacobiLaunch {
     dim3 grid, block;
@@ -282,6 +419,33 @@ struct JacobiLaunch {
     int swizzle;
 };
 
+// Temporal blocking: tiles of (64-2K) x (TY*rows-2K) output columns; a z
+// chunk of C planes costs its work-group C full pipeline steps plus a 2K-step
+// ramp of about K - 1 steps' work.  One work-group fits a CU at a time, so a
+// launch takes about ceil(work-groups / CUs) rounds of C + K plane steps, a
+// step of a two-rows-per-thread kernel costing about 2.2 single-row steps
+// (two rows' arithmetic, half the LDS reads per row; 2.2 matches the MI355X
+// timings: equal at 128^3, the taller tile 15% faster at 512^3).  Returns
+// that cost for the best chunk (any C up to 64, the measured best at 512^3)
+// and sets *zchunk: at 128^3, variant 6 takes 5 chunks of 26 planes.
+double tb_cost(const lgdhip_grid &g, const TbShape &tb, int *zchunk) {
+    const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
+    const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty * tb.rows - 2 * tb.k);
+    double best = 1e300;
+    for (int64_t n = lgdhip::ceil_div(g.nz, 64); n <= g.nz; ++n) {
+        const int c = (int)lgdhip::ceil_div(g.nz, n);
+        if (lgdhip::ceil_div(g.nz, c) != n) continue;  // same chunk as a smaller n
+        const uint64_t groups = (uint64_t)gx * gy * n;
+        const double cost = (double)((groups + lgdhip::kCUs - 1) / lgdhip::kCUs) * (c + tb.k) *
+                            (tb.rows == 2 ? 2.2 : 1.0);
+        if (cost < best) {
+            best = cost;
+            *zchunk = c;
+        }
+    }
+    return best;
+}
+
 JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     JacobiLaunch L;
     L.by = 4;
@@ -290,31 +454,20 @@ JacobiLaunch plan(const lgdhip_grid &g, const lgdhip_tile *tile) {
     if (tile) {
         if (tile->by == 1 || tile->by == 2 || tile->by == 4 || tile->by == 8) L.by = tile->by;
         if (tile->bz > 0) L.zchunk = tile->bz;
-        if (tile->variant >= 1 && tile->variant <= 8) L.variant = tile->variant;
+        if (tile->variant >= 1 && tile->variant <= kJacobiVariants) L.variant = tile->variant;
+    }
+    if (L.variant == kJacobiDefaultVariant && !(tile && tile->variant != 0)) {
+        // the default: 4 sweeps per launch, one or two rows per thread by the
+        // cost model (two rows win once the grid fills several rounds)
+        int c6 = 0, c10 = 0;
+        if (tb_cost(g, tb_shape(kJacobiTallVariant), &c10) < tb_cost(g, tb_shape(kJacobiDefaultVariant), &c6))
+            L.variant = kJacobiTallVariant;
     }
     const TbShape tb = tb_shape(L.variant);
     if (tb.k > 0) {
-        // temporal blocking: tiles of (64-2K) x (TY-2K) output columns; a z
-        // chunk of C planes costs its work-group C full pipeline steps plus
-        // a 2K-step ramp of about K - 1 steps' work.  Count the launch as
-        // ceil(work-groups / CUs) rounds of C + K planes and take the chunk
-        // (any C up to 64, the measured best at 512^3) minimising it: at
-        // 128^3 that is 5 chunks of 26 planes, 240 work-groups.
         const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
-        const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty - 2 * tb.k);
-        if (!tile || tile->bz <= 0) {
-            uint64_t best = ~(uint64_t)0;
-            for (int64_t n = lgdhip::ceil_div(g.nz, 64); n <= g.nz; ++n) {
-                const int c = (int)lgdhip::ceil_div(g.nz, n);
-                if (lgdhip::ceil_div(g.nz, c) != n) continue;  // same chunk as a smaller n
-                const uint64_t groups = (uint64_t)gx * gy * n;
-                const uint64_t cost = (groups + lgdhip::kCUs - 1) / lgdhip::kCUs * (uint64_t)(c + tb.k);
-                if (cost < best) {
-                    best = cost;
-                    L.zchunk = c;
-                }
-            }
-        }
+        const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty * tb.rows - 2 * tb.k);
+        if (!tile || tile->bz <= 0) tb_cost(g, tb, &L.zchunk);
         L.by = tb.ty;
         L.block = dim3(64, tb.ty, 1);
         L.gx = gx;
@@ -375,6 +528,14 @@ int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const Jacobi
         jacobi3d_f64_tb<3, 8, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
                                                             L.gy, L.ntiles, L.per, L.swizzle);
         break;
+    case 9:
+        jacobi3d_f64_tb2<4, 8><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+                                                         L.gy, L.ntiles, L.per, L.swizzle);
+        break;
+    case 10:
+        jacobi3d_f64_tb2<4, 16><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
+                                                          L.gy, L.ntiles, L.per, L.swizzle);
+        break;
     default: return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "not a temporal-blocking variant");
     }
     LGD_LAUNCHED("jacobi3d_f64_tb");
@@ -418,6 +579,26 @@ int launch_sweep(const double *src, double *dst, const lgdhip_grid &g, const Jac
 }
 
 } // namespace
+
+extern "C" int lgdhip_jacobi3d_f64_plan(const lgdhip_grid *g, const lgdhip_tile *tile, int *variant, int *zchunk) {
+    int rc = lgdhip::check_grid(g, 3, 8, 1, true);
+    if (rc) return rc;
+    LGD_REQUIRE(variant && zchunk, "NULL output pointer");
+    lgdhip_tile view_tile;
+    if (g->flags & LGDHIP_GRID_VIEW) {  // as lgdhip_jacobi3d_f64_run treats views
+        if (tile && tb_shape(tile->variant).k > 0)
+            return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "temporal blocking needs a whole grid, not a view");
+        if (!tile || tile->variant == 0) {
+            view_tile = tile ? *tile : lgdhip_tile{};
+            view_tile.variant = kSingleSweepVariant;
+            tile = &view_tile;
+        }
+    }
+    const JacobiLaunch L = plan(*g, tile);
+    *variant = L.variant;
+    *zchunk = L.zchunk;
+    return 0;
+}
 
 extern "C" int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
                                        const lgdhip_tile *tile, int *result_in_b, lgdhip_stream_t stream) {

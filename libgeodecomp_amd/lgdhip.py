@@ -134,6 +134,7 @@ SIGNATURES = {
     "lgdhip_grid_load": (_I, [_P, _GP, _P, _P]),
     "lgdhip_grid_store": (_I, [_P, _GP, _P, _P]),
     "lgdhip_jacobi3d_f64_run": (_I, [_P, _P, _GP, _I, _TP, _IP, _P]),
+    "lgdhip_jacobi3d_f64_plan": (_I, [_GP, _TP, _IP, _IP]),
     "lgdhip_life2d_u8_run": (_I, [_P, _P, _GP, _I, _IP, _P]),
     "lgdhip_life2d_packed_words": (_I64, [_GP]),
     "lgdhip_life2d_pack": (_I, [_P, _GP, _P, _P]),

@@ -3,8 +3,9 @@
 // parameters are the tuning space of the cell's kernel family (the
 // reference's three block dimensions become what each family can actually
 // vary: work-group shape, temporal-blocking depth, kernel variant):
-//   Jacobi3DF64   Variant in the order 1 2 3 7 4 8 5 6: single sweeps, then
-//                 temporal blocking of growing depth (K = 2, 2, 3, 3, 4)
+//   Jacobi3DF64   Variant in the order 1 2 3 7 4 8 5 6 9 10: single sweeps,
+//                 then temporal blocking of growing depth (K = 2, 2, 3, 3,
+//                 4), then K = 4 with two rows per thread (16-, 32-row tiles)
 //   Life2D        StepsPerLaunch [1, 33), Variant {1..8}
 //   LBMD3Q19F32   BlockDimY {1, 2, 4, 8}, Variant {0, 1}
 //   Generic       BlockDimX {64, 128, 256}, BlockDimY {1, 2, 4, 8}
@@ -51,7 +52,7 @@ template<typename CELL, typename VALUE, VALUE CELL::*MEMBER>
 class TuningSpace<Jacobi3DF64<CELL, VALUE, MEMBER> >
 {
 public:
-    static void add(SimulationParameters *p) { p->addParameter("Variant", std::vector<int>{1, 2, 3, 7, 4, 8, 5, 6}); }
+    static void add(SimulationParameters *p) { p->addParameter("Variant", std::vector<int>{1, 2, 3, 7, 4, 8, 5, 6, 9, 10}); }
     static LaunchChoice choose(const SimulationParameters& p)
     {
         LaunchChoice c;

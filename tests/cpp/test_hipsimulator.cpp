@@ -468,7 +468,7 @@ LGD_TEST(hip_simulation_factory_every_jacobi_variant)
     SerialSimulator<JacobiCell> ref(proxy->clone());
     ref.run();
     SimulationParameters p = factory.parameters();
-    for (int k = 0; k < 8; ++k) {
+    for (int k = 0; k < 10; ++k) {
         p["Variant"].setValue(k);
         const double fit = factory(p);
         CHECK(fit < 0 && fit > -60);

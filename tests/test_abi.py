@@ -69,3 +69,26 @@ def test_lbm_x4_variant_rejects_ragged_rows_without_gpu():
     fake = C.c_void_p(0x1000), C.c_void_p(0x2000), C.c_void_p(0x3000)
     rc = lib.lgdhip_lbm_d3q19_f32_run(fake[0], fake[1], fake[2], C.byref(g), C.byref(p), 1, C.byref(t), None, None)
     assert rc == -2 and "nx % 4" in lib.lgdhip_last_error().decode()
+
+
+def test_jacobi_launch_plan_without_gpu():
+    """lgdhip_jacobi3d_f64_plan (host-only): the default picks 4 sweeps per
+    launch with one row per thread on a 128^3 cube (5 chunks of 26 planes)
+    and two rows per thread on 512^3 (64-plane chunks); an explicit variant
+    and z chunk are kept; a view falls back to single sweeps and rejects
+    temporal blocking, as lgdhip_jacobi3d_f64_run does."""
+    lib = lgdhip.load()
+
+    def plan(g, tile=None):
+        v, z = C.c_int(-1), C.c_int(-1)
+        rc = lib.lgdhip_jacobi3d_f64_plan(C.byref(g), C.byref(tile) if tile else None, C.byref(v), C.byref(z))
+        return rc, v.value, z.value
+
+    assert plan(lgdhip.make_grid(3, 128, 128, 128, halo=1, elem_bytes=8)) == (0, 6, 26)
+    assert plan(lgdhip.make_grid(3, 512, 512, 512, halo=1, elem_bytes=8)) == (0, 10, 64)
+    g = lgdhip.make_grid(3, 96, 70, 40, halo=1, elem_bytes=8)
+    assert plan(g, lgdhip.tile(variant=9, bz=7)) == (0, 9, 7)
+    v = lgdhip.lgdhip_grid()
+    lgdhip.call("lgdhip_grid_view", C.byref(g), 1, 1, 1, 20, 20, 20, C.byref(v))
+    assert plan(v)[:2] == (0, 2)
+    assert plan(v, lgdhip.tile(variant=10))[0] == -2

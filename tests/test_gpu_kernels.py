@@ -73,13 +73,14 @@ def test_jacobi_tiles_agree(hip, oracle):
                                          ((64, 64, 64), 8)])
 def test_jacobi_temporal_blocking_bit_exact(hip, oracle, shape, steps):
     """K sweeps per launch (variants 4-8: K = 2, 3, 4 with 16-row tiles, K = 2,
-    3 with 8-row tiles), step counts that leave a remainder of single
+    3 with 8-row tiles; 9, 10: K = 4 with two rows per thread, 16- and
+    32-row tiles), step counts that leave a remainder of single
     sweeps, ragged shapes, z chunks down to one plane and chunks longer than
     the kernel's 6-step unrolled march, non-zero edge."""
     nx, ny, nz = shape
     a = synthetic.jacobi_initial(nx, ny, nz, seed=nx + ny + nz)
     exp = oracle.jacobi3d(a, steps, edge=0.5, threads=4)
-    for variant in (4, 5, 6, 7, 8):
+    for variant in (4, 5, 6, 7, 8, 9, 10):
         for bz in (0, 1, 5, 13):
             say("jacobi tb", shape, steps, variant, bz)
             t = hip.tile(bz=bz, variant=variant)

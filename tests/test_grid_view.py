@@ -41,7 +41,7 @@ def test_views_rejected_where_the_halo_is_the_edge_without_gpu():
     assert lib.lgdhip_grid_fill_edge(fake[0], C.byref(v), C.byref(edge), None) == -2
     assert lib.lgdhip_grid_fill_halo(fake[0], C.byref(v), C.byref(edge), 1, None) == -2
     assert lib.lgdhip_grid_set_layout(C.byref(v), 1) == -2
-    for variant in (4, 5, 6, 7, 8):
+    for variant in (4, 5, 6, 7, 8, 9, 10):
         t = lgdhip.tile(variant=variant)
         assert lib.lgdhip_jacobi3d_f64_run(fake[0], fake[1], C.byref(v), 2, C.byref(t), None, None) == -2
     gl = lgdhip.make_grid(3, 16, 16, 16, halo=1, elem_bytes=4, members=19)
