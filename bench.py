@@ -331,16 +331,22 @@ def equilibrium_f32(rho, jx, jy, jz):
     return f
 
 
-def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, wl, shape):
+def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, wl, shape, variant=0):
     """(name, ms per launch, algorithmic bytes per launch) of the Poisson
-    solver's default sweep kernel: temporal blocking, 4 sweeps per launch
-    (12 B per sweep-LUP, so 4 x 12 B x cells per launch)."""
+    solver's sweep kernel: temporal blocking, 4 sweeps per launch (12 B per
+    sweep-LUP, so 4 x 12 B x cells per launch); variant 0 is the library
+    default (4, two rows per thread, on large launches, else 3, one row)."""
     k, sweeps = 4, 48
     in_b = C.c_int(0)
+    tile = lgdhip.tile(variant=variant)
     ms = timed(stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, f.ptr, probs, count, sweeps,
-                                           None, ws.ptr, ws_bytes, C.byref(in_b), stream.h))
+                                           C.byref(tile), ws.ptr, ws_bytes, C.byref(in_b), stream.h))
     wl.shape = shape
-    return "poisson_sweep_tb", ms / (sweeps // k), k * 12.0 * cells
+    planned = C.c_int(0)  # the library resolves the default by launch size
+    lgdhip.call("lgdhip_poisson_sweep_plan", probs, count, variant, C.byref(planned))
+    wl.sweep_plan = planned.value
+    return ("poisson_sweep_tb2" if planned.value in (4, 5) else "poisson_sweep_tb"), ms / (sweeps // k), \
+        k * 12.0 * cells
 
 
 # ----------------------------------------------------------------------
@@ -356,10 +362,11 @@ class Poisson:
     dtype = "f32"   # the sweeps; residuals and updates are FP64
     bytes_per_lup = 12.0  # read e + read r + write e (FP32)
 
-    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, use_graph=1, tol=1e-6, **_):
+    def __init__(self, n=512, steps=0, seed=synthetic.DEFAULT_SEED, use_graph=1, tol=1e-6, variant=0, **_):
         self.count, self.seed = n, seed   # --size overrides the problem count
         self.use_graph = use_graph
         self.tol = tol
+        self.variant = variant            # sweep variant (0: the library default)
         self.stream = lgdhip.Stream()
         s = self.stream.h
         t0 = time.perf_counter()
@@ -395,7 +402,7 @@ class Poisson:
                 "sweeps_max": int(sw.max()) if sw is not None else None,
                 "sweep_lups": self.last_lups,
                 "converged": int(sum(r.converged for r in self.last)) if self.last is not None else None,
-                "hip_graph": bool(self.use_graph), "parallelism": "single GPU"}
+                "hip_graph": bool(self.use_graph), "sweep_variant": self.variant, "parallelism": "single GPU"}
 
     def lups_per_step(self):
         return self.last_lups
@@ -416,6 +423,7 @@ class Poisson:
                     self.ws_bytes, ref, s)
         opt = lgdhip.lgdhip_poisson_options()
         opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = self.tol, 64, self.SWEEPS_PER_CALL, self.use_graph
+        opt.sweep_variant = self.variant
         opt.reference_norms = ref
         total = np.zeros(self.count, dtype=np.int64)
         res = (lgdhip.lgdhip_poisson_result * self.count)()
@@ -432,14 +440,14 @@ class Poisson:
         self.last_lups = float(np.sum(self.sizes.astype(np.float64) ** 3 * total))
 
     def dominant_kernel(self):
-        """The solver's sweep kernel over every problem (the library default:
-        poisson_sweep_tb, 4 sweeps per launch)."""
+        """The solver's sweep kernel over every problem (the library default,
+        4 sweeps per launch)."""
         e0 = lgdhip.DeviceBuffer(self.elems * 4)
         e1 = lgdhip.DeviceBuffer(self.elems * 4)
         for b in (e0, e1):
             lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.elems * 4, self.stream.h)
         return poisson_sweep_kernel(self.stream, e0, e1, self.f, self.probs, self.count, self.ws, self.ws_bytes,
-                                    self.cells, self, f"{self.count}problems")
+                                    self.cells, self, f"{self.count}problems", self.variant)
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -641,7 +649,7 @@ def main():
         if args.layout:
             kw.update(layout=lgdhip.LAYOUT_Z_SLABS if args.layout == "zslabs" else lgdhip.LAYOUT_MEMBER_PLANES)
     if args.workload == "poisson":
-        kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol)
+        kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol, variant=max(0, args.variant))
     wl = WORKLOADS[args.workload](**kw)
 
     for i in range(args.warmup):

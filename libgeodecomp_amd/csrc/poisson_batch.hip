@@ -297,6 +297,157 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
     }
 }
 
+// The same K-sweep pipeline with two adjacent rows per thread: a 64 x TY
+// work-group holds a 64 x 2TY tile of one problem (written region
+// (64-2K) x (2TY-2K); tile.x0/y0 its origin).  A thread's rows are each
+// other's y neighbours in registers, LDS carries only the row pairs' outer
+// rows, and the taller tile recomputes a smaller overlap ring.  Cells, sum
+// order, rings, ramp and residual partials as in poisson_sweep_tb: bit-exact
+// with K launches of poisson_sweep_f32.
+template <int K, int TY>
+__global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__restrict__ e, float *__restrict__ e2,
+                                                             const float *__restrict__ r,
+                                                             const Tile *__restrict__ tiles,
+                                                             const ProblemDev *__restrict__ probs,
+                                                             double *__restrict__ partial, int zchunk,
+                                                             unsigned ntiles, unsigned per) {
+    constexpr int PF = 3;
+    constexpr int CY = 2 * TY;
+    static_assert(K + 2 <= 6, "r ring of 6 slots holds planes p-K .. p+1");
+    __shared__ float lds[2][K][TY][2][64];
+    __shared__ double red[TY];
+    const unsigned tid = lgdhip::xcd_tile(blockIdx.x, per, true);
+    if (tid >= ntiles) return;  // whole work-group, before any barrier
+    const Tile t = tiles[tid];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int x = t.x0 - K + tx, y0 = t.y0 - K + 2 * ty;  // rows y0, y0 + 1
+    const int z0 = t.z0, z1 = min(t.z0 + zchunk, n);
+    const bool xin = (unsigned)x < (unsigned)n;
+    const bool inxy[2] = {xin && (unsigned)y0 < (unsigned)n, xin && (unsigned)(y0 + 1) < (unsigned)n};
+    const bool xw = tx >= K && tx < 64 - K;
+    const bool writer[2] = {inxy[0] && xw && 2 * ty >= K && 2 * ty < CY - K,
+                            inxy[1] && xw && 2 * ty + 1 >= K && 2 * ty + 1 < CY - K};
+    const int64_t sx = n + 2, sxy = sx * sx;
+    const int64_t col[2] = {P.base + (int64_t)(y0 + 1) * sx + (x + 1), P.base + (int64_t)(y0 + 2) * sx + (x + 1)};
+    // unconditional loads clamped into the problem's halo shell (e is zero
+    // there; r there only feeds cells masked to zero)
+    const int xc = min(max(x, -1), n) + 1;
+    const int64_t colh[2] = {P.base + (int64_t)(min(max(y0, -1), n) + 1) * sx + xc,
+                             P.base + (int64_t)(min(max(y0 + 1, -1), n) + 1) * sx + xc};
+    auto at = [&](int rr, int p) { return colh[rr] + (int64_t)(min(max(p, -1), n) + 1) * sxy; };
+    const int tyd = ty > 0 ? ty - 1 : 0;
+    const int tyu = ty < TY - 1 ? ty + 1 : TY - 1;
+
+    float ev[2][6], rv[2][6];
+    float w[K][2][3];
+#pragma unroll
+    for (int l = 0; l < K; ++l)
+#pragma unroll
+        for (int q = 0; q < 2; ++q) w[l][q][0] = w[l][q][1] = w[l][q][2] = 0.0f;
+#pragma unroll
+    for (int l = 0; l < K; ++l) lds[0][l][ty][0][tx] = lds[0][l][ty][1][tx] = 0.0f;
+    const int pbeg = z0 - K;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        ev[q][4] = ev[q][5] = 0.0f;
+#pragma unroll
+        for (int d = 0; d < PF; ++d) ev[q][d] = e[at(q, pbeg + d)];
+        rv[q][0] = r[at(q, pbeg)];
+    }
+    __syncthreads();
+
+    double acc = 0.0;
+    auto step = [&](auto jm_c, auto jr_c, int p) {
+        constexpr int JM = decltype(jm_c)::value;
+        constexpr int JR = decltype(jr_c)::value;
+        constexpr int NEW = JM % 3, OLD = (JM + 1) % 3, MID = (JM + 2) % 3;
+        constexpr int B = JM & 1;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            ev[q][(JM + PF) % 6] = e[at(q, p + PF)];
+            rv[q][(JM + 1) % 6] = r[at(q, p + 1)];
+        }
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            if (2 * l > JR) continue;
+            const int zq = p - l;
+            const bool zin = (unsigned)zq < (unsigned)n;
+            float a[2], c[2], b[2];
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                a[q] = l == 1 ? ev[q][(JM + 4) % 6] : w[l - 1][q][OLD];
+                c[q] = l == 1 ? ev[q][(JM + 5) % 6] : w[l - 1][q][MID];
+                b[q] = l == 1 ? ev[q][JM] : w[l - 1][q][NEW];
+            }
+            const float ym[2] = {lds[B][l - 1][tyd][1][tx], c[0]};
+            const float yp[2] = {c[1], lds[B][l - 1][tyu][0][tx]};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                float sum = a[q];               // ( 0,  0, -1)
+                sum += ym[q];                   // ( 0, -1,  0)
+                sum += lane_prev(c[q]);         // (-1,  0,  0)
+                sum += lane_next(c[q]);         // ( 1,  0,  0)
+                sum += yp[q];                   // ( 0,  1,  0)
+                sum += b[q];                    // ( 0,  0,  1)
+                sum += rv[q][(JM + 6 - l) % 6]; // r at plane zq
+                const float v = (inxy[q] && zin) ? sum * (1.0f / 6.0f) : 0.0f;
+                if (l < K) {
+                    w[l][q][NEW] = v;
+                } else if (writer[q]) {
+                    e2[col[q] + (int64_t)(zq + 1) * sxy] = v;
+                    if (partial) {
+                        const double rr = (double)sum - 6.0 * (double)c[q];
+                        acc += rr * rr;
+                    }
+                }
+            }
+        }
+        if (JR >= 1) {
+            lds[B ^ 1][0][ty][0][tx] = ev[0][JM];
+            lds[B ^ 1][0][ty][1][tx] = ev[1][JM];
+        }
+#pragma unroll
+        for (int l = 1; l < K; ++l)
+            if (2 * l + 1 <= JR) {
+                lds[B ^ 1][l][ty][0][tx] = w[l][0][NEW];
+                lds[B ^ 1][l][ty][1][tx] = w[l][1][NEW];
+            }
+        __syncthreads();
+    };
+    using lgdhip::ic;
+    lgdhip::static_for<0, 2 * K>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        step(ic<j % 6>{}, jc, pbeg + j);
+    });
+    constexpr int S = (2 * K) % 6;
+    const int m = z1 - z0;
+    int p = z0 + K;
+    int i = 0;
+    for (; i + 6 <= m; i += 6, p += 6) {
+        lgdhip::static_for<0, 6>([&](auto dc) {
+            constexpr int d = decltype(dc)::value;
+            step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+        });
+    }
+    lgdhip::static_for<0, 5>([&](auto dc) {
+        constexpr int d = decltype(dc)::value;
+        if (i + d < m) step(ic<(S + d) % 6>{}, ic<2 * K>{}, p + d);
+    });
+    if (partial) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        if (tx == 0) red[ty] = acc;
+        __syncthreads();
+        if (tx == 0 && ty == 0) {
+            double sum = 0.0;
+            for (int k = 0; k < TY; ++k) sum += red[k];
+            partial[tid] = sum;
+        }
+    }
+}
+
 // u += e over the interior and reset both correction buffers to zero
 __global__ __launch_bounds__(kThreads) void poisson_update_f64(double *__restrict__ u, float *e, float *eother,
                                                                const Tile *__restrict__ tiles,
@@ -369,28 +520,51 @@ void append_tiles(std::vector<Tile> *out, int p, int n) {
 // temporal blocking shapes: variant -> (K, TY); tiles write (64-2K) x (TY-2K)
 // columns and march z chunks of kTbZ planes
 struct TbShape {
-    int k, ty;
+    int k, ty, rows;  // sweeps per launch, work-group rows, tile rows per work-group row
 };
 constexpr int kTbZ = 64;
-constexpr int kDefaultSweepVariant = 3;
+// the default: 4 sweeps per launch, two rows per thread in 32-row tiles
+// (variant 4) for launches of at least kTallMinTiles tiles, else one row per
+// thread (variant 3).  On MI355X the full 512-problem ensemble (19k two-row
+// tiles) runs 3.97 ms per launch against 4.72 ms with one row (51.5 s
+// against 62.0 s per solve), while 16 problems of 128^3 (576 two-row tiles,
+// two rounds and a quarter of one work-group per CU) run 186 us against 173
+// us: the one-row kernel holds two work-groups per CU, which hides more
+// latency when a launch is only a few rounds long.
+constexpr int kDefaultSweepVariant = 4;
+constexpr int kSmallSweepVariant = 3;
+constexpr int64_t kTallMinTiles = 2048;
+constexpr int kSweepVariants = 5;
 TbShape tb_shape(int variant) {
     switch (variant) {
-    case 2: return {2, 16};
-    case 3: return {4, 16};
-    default: return {0, 0};  // 1: plain sweeps
+    case 2: return {2, 16, 1};
+    case 3: return {4, 16, 1};
+    case 4: return {4, 16, 2};  // two rows per thread, 32-row tiles
+    case 5: return {4, 8, 2};   // two rows per thread, 16-row tiles
+    default: return {0, 0, 1};  // 1: plain sweeps
     }
 }
 
 int64_t tb_tiles_of(int n, TbShape sh) {
-    const int ox = 64 - 2 * sh.k, oy = sh.ty - 2 * sh.k;
+    const int ox = 64 - 2 * sh.k, oy = sh.ty * sh.rows - 2 * sh.k;
     return (int64_t)((n + ox - 1) / ox) * ((n + oy - 1) / oy) * ((n + kTbZ - 1) / kTbZ);
 }
 
 void append_tb_tiles(std::vector<Tile> *out, int p, int n, TbShape sh) {
-    const int ox = 64 - 2 * sh.k, oy = sh.ty - 2 * sh.k;
+    const int ox = 64 - 2 * sh.k, oy = sh.ty * sh.rows - 2 * sh.k;
     for (int z = 0; z < n; z += kTbZ)
         for (int y = 0; y < n; y += oy)
             for (int x = 0; x < n; x += ox) out->push_back(Tile{p, x, y, z});
+}
+
+// the blocked sweep shape of a launch over the problems in `set`: the
+// requested variant, or for 0 the default by launch size
+TbShape sweep_shape(int requested, const lgdhip_poisson_problem *probs, const std::vector<int> &set) {
+    if (requested >= 1 && requested <= kSweepVariants) return tb_shape(requested);
+    const TbShape tall = tb_shape(kDefaultSweepVariant);
+    int64_t tiles = 0;
+    for (int k : set) tiles += tb_tiles_of(probs[k].n, tall);
+    return tiles >= kTallMinTiles ? tall : tb_shape(kSmallSweepVariant);
 }
 
 int check_problems(const lgdhip_poisson_problem *probs, int count) {
@@ -459,8 +633,11 @@ int64_t total_tiles(const lgdhip_poisson_problem *probs, int count) {
 // tile capacity for every temporal-blocking shape
 int64_t total_tb_tiles(const lgdhip_poisson_problem *probs, int count) {
     int64_t t = 0;
-    for (int k = 0; k < count; ++k)
-        t += std::max(tb_tiles_of(probs[k].n, tb_shape(2)), tb_tiles_of(probs[k].n, tb_shape(3)));
+    for (int k = 0; k < count; ++k) {
+        int64_t t_max = 0;  // tile lists of every blocked variant fit
+        for (int v = 2; v <= kSweepVariants; ++v) t_max = std::max(t_max, tb_tiles_of(probs[k].n, tb_shape(v)));
+        t += t_max;
+    }
     return t;
 }
 
@@ -493,7 +670,11 @@ void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Works
                double *partial, hipStream_t s) {
     if (ntiles <= 0) return;
     const unsigned nt = (unsigned)ntiles, per = lgdhip::xcd_per(nt), groups = lgdhip::xcd_groups(nt);
-    if (sh.k == 2)
+    if (sh.rows == 2 && sh.ty == 16)
+        poisson_sweep_tb2<4, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+    else if (sh.rows == 2)
+        poisson_sweep_tb2<4, 8><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+    else if (sh.k == 2)
         poisson_sweep_tb<2, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
     else
         poisson_sweep_tb<4, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
@@ -595,8 +776,7 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
     std::vector<Tile> tiles;
     rc = upload_set(w, probs, all, &tiles, s);
     if (rc) return rc;
-    const int variant = (tile && tile->variant >= 1 && tile->variant <= 3) ? tile->variant : kDefaultSweepVariant;
-    const TbShape sh = tb_shape(variant);
+    const TbShape sh = sweep_shape(tile ? tile->variant : 0, probs, all);
     float *cur = ea, *nxt = eb;
     int t = 0;
     if (sh.k > 0 && sweeps >= sh.k) {
@@ -643,6 +823,21 @@ int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_
     return 0;
 }
 
+int lgdhip_poisson_sweep_plan(const lgdhip_poisson_problem *probs, int count, int requested, int *variant) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(variant, "NULL output pointer");
+    if (requested >= 1 && requested <= kSweepVariants) {
+        *variant = requested;
+        return 0;
+    }
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    const TbShape sh = sweep_shape(0, probs, all);
+    *variant = sh.rows == 2 ? kDefaultSweepVariant : kSmallSweepVariant;
+    return 0;
+}
+
 int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
                                const lgdhip_poisson_options *opt, void *workspace, size_t workspace_bytes,
                                lgdhip_poisson_result *results, lgdhip_stream_t stream) {
@@ -658,9 +853,9 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
     hipStream_t s = lgdhip::as_stream(stream);
     const bool graphs = opt->use_graph != 0 && s != nullptr;
     const double theta = opt->inner_reduction > 0 ? opt->inner_reduction : 1e-2;
-    const int variant =
-        (opt->sweep_variant >= 1 && opt->sweep_variant <= 3) ? opt->sweep_variant : kDefaultSweepVariant;
-    const TbShape sh = tb_shape(variant);
+    // the blocked shape is chosen per active set (sweep_shape); the default's
+    // two candidates both run K = 4, so the chunk length below holds for any
+    TbShape sh = sweep_shape(opt->sweep_variant, probs, std::vector<int>());
     // sweeps per chunk: even (plain) or a multiple of 2K (blocked), so every
     // chunk leaves the correction in the same buffer
     const int unit = sh.k > 0 ? 2 * sh.k : 2;
@@ -736,6 +931,7 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
         rc = upload_set(w, probs, active, &tiles, s);
         if (rc) return rc;
         int64_t ntb = 0;
+        sh = sweep_shape(opt->sweep_variant, probs, active);
         if (sh.k > 0) {
             rc = upload_tb_set(w, probs, active, sh, &ntb, s);
             if (rc) return rc;

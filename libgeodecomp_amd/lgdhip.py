@@ -146,6 +146,7 @@ SIGNATURES = {
     "lgdhip_lbm_cavity_flags": (_I, [_P, _GP, _P]),
     "lgdhip_lbm_d3q19_f32_moments": (_I, [_P, _GP, _P, _P, _P, _P, _P]),
     "lgdhip_poisson_layout": (_I64, [C.POINTER(lgdhip_poisson_problem), _I]),
+    "lgdhip_poisson_sweep_plan": (_I, [C.POINTER(lgdhip_poisson_problem), _I, _I, _IP]),
     "lgdhip_poisson_workspace_bytes": (_SZ, [C.POINTER(lgdhip_poisson_problem), _I]),
     "lgdhip_poisson_set_faces": (_I, [_P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ, _P]),
     "lgdhip_poisson_residual_norms": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I, _P, _SZ,

@@ -92,3 +92,25 @@ def test_jacobi_launch_plan_without_gpu():
     lgdhip.call("lgdhip_grid_view", C.byref(g), 1, 1, 1, 20, 20, 20, C.byref(v))
     assert plan(v)[:2] == (0, 2)
     assert plan(v, lgdhip.tile(variant=10))[0] == -2
+
+
+def test_poisson_sweep_plan_without_gpu():
+    """lgdhip_poisson_sweep_plan (host-only): the default takes two rows per
+    thread (variant 4) for launches of at least 2048 of its tiles (the
+    512-problem ensemble) and one row (variant 3) for small sets (16 problems
+    of 128^3: 576 tiles); requested variants are kept."""
+    lib = lgdhip.load()
+
+    def plan(sizes, requested=0):
+        probs = (lgdhip.lgdhip_poisson_problem * len(sizes))()
+        for k, n in enumerate(sizes):
+            probs[k].n = n
+        lib.lgdhip_poisson_layout(probs, len(sizes))
+        v = C.c_int(-1)
+        assert lib.lgdhip_poisson_sweep_plan(probs, len(sizes), requested, C.byref(v)) == 0
+        return v.value
+
+    ensemble = [32 + 16 * (k % 11) for k in range(512)]
+    assert plan(ensemble) == 4
+    assert plan([128] * 16) == 3
+    assert plan([128] * 16, 5) == 5 and plan(ensemble, 3) == 3

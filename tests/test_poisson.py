@@ -153,10 +153,11 @@ class Batch:
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant,sweeps", [(1, 5), (2, 5), (3, 9), (3, 8), (0, 4)])
+@pytest.mark.parametrize("variant,sweeps", [(1, 5), (2, 5), (3, 9), (3, 8), (0, 4), (4, 9), (4, 8), (5, 12)])
 def test_gpu_sweeps_bit_exact(hip, oracle, variant, sweeps):
     """Plain sweeps (variant 1) and temporal blocking with 2 (variant 2) or 4
-    (variant 3, the default) sweeps per launch, sweep counts with remainders,
+    (variant 3) sweeps per launch, 4 with two rows per thread (variants 4,
+    the default, and 5: 32- and 16-row tiles), sweep counts with remainders,
     sizes from 1 to several tiles per axis, noise in r's halo: bit-identical
     to the oracle."""
     rng = np.random.default_rng(3 + variant)
