@@ -371,6 +371,14 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
  * device work; for tools.  No counterpart in the reference (the ensemble
  * solver is new with this library). */
 int lgdhip_poisson_sweep_plan(const lgdhip_poisson_problem *probs, int count, int requested, int *variant);
+
+/* Weighted (damped) sweeps e' = e + omega (J(e) - e), omega in (0, 1]: as
+ * lgdhip_poisson_sweeps_f32 (same variants and buffer convention), each cell
+ * computing fma(omega, fma(sum, 1/6, -e), e) with single roundings.  The
+ * smoothing phase of lgdhip_poisson_batch_solve runs these. */
+int lgdhip_poisson_damped_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs,
+                                     int count, int sweeps, float omega, const lgdhip_tile *tile, void *workspace,
+                                     size_t workspace_bytes, int *result_in_b, lgdhip_stream_t stream);
 /* Solves every problem from the guess in u (interior) to opt->rel_tol:
  * FP64 residual and update, FP32 Jacobi sweeps on the correction, all
  * active problems swept by one launch per sweep. */

@@ -129,8 +129,8 @@ __global__ __launch_bounds__(kThreads) void poisson_sweep_f32(const float *__res
             s += e[i + sx];
             s += above;
             s += r[i];
-            if (DAMPED)
-                e2[i] = center + omega * (s * (1.0f / 6.0f) - center);
+            if (DAMPED)  // single roundings, the same in every sweep kernel
+                e2[i] = __builtin_fmaf(omega, __builtin_fmaf(s, 1.0f / 6.0f, -center), center);
             else
                 e2[i] = s * (1.0f / 6.0f);
             if (partial) {
@@ -169,13 +169,13 @@ __device__ __forceinline__ float lane_next(float v) {
 // launch equals K launches of that kernel bit for bit.  With `partial`, the
 // last level adds the FP64 square of the residual of its input iterate over
 // the written cells into one partial per tile.
-template <int K, int TY>
+template <int K, int TY, bool DAMPED>
 __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restrict__ e, float *__restrict__ e2,
                                                             const float *__restrict__ r,
                                                             const Tile *__restrict__ tiles,
                                                             const ProblemDev *__restrict__ probs,
                                                             double *__restrict__ partial, int zchunk,
-                                                            unsigned ntiles, unsigned per) {
+                                                            unsigned ntiles, unsigned per, float omega) {
     constexpr int PF = 3;  // e planes in flight
     static_assert(K + 2 <= 6, "r ring of 6 slots holds planes p-K .. p+1");
     __shared__ float lds[2][K][TY][64];
@@ -247,7 +247,9 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
             sum += b;                         // ( 0,  0,  1)
             sum += rv[(JM + 6 - l) % 6];      // r at plane q
             const bool live = inxy && (unsigned)q < (unsigned)n;
-            const float v = live ? sum * (1.0f / 6.0f) : 0.0f;
+            const float jv = DAMPED ? __builtin_fmaf(omega, __builtin_fmaf(sum, 1.0f / 6.0f, -c), c)
+                                    : sum * (1.0f / 6.0f);
+            const float v = live ? jv : 0.0f;
             if (l < K) {
                 w[l][NEW] = v;
             } else if (writer) {  // after the ramp q = p - K runs over z0 .. z1 - 1
@@ -304,13 +306,13 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
 // rows, and the taller tile recomputes a smaller overlap ring.  Cells, sum
 // order, rings, ramp and residual partials as in poisson_sweep_tb: bit-exact
 // with K launches of poisson_sweep_f32.
-template <int K, int TY>
+template <int K, int TY, bool DAMPED>
 __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__restrict__ e, float *__restrict__ e2,
                                                              const float *__restrict__ r,
                                                              const Tile *__restrict__ tiles,
                                                              const ProblemDev *__restrict__ probs,
                                                              double *__restrict__ partial, int zchunk,
-                                                             unsigned ntiles, unsigned per) {
+                                                             unsigned ntiles, unsigned per, float omega) {
     constexpr int PF = 3;
     constexpr int CY = 2 * TY;
     static_assert(K + 2 <= 6, "r ring of 6 slots holds planes p-K .. p+1");
@@ -392,7 +394,9 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
                 sum += yp[q];                   // ( 0,  1,  0)
                 sum += b[q];                    // ( 0,  0,  1)
                 sum += rv[q][(JM + 6 - l) % 6]; // r at plane zq
-                const float v = (inxy[q] && zin) ? sum * (1.0f / 6.0f) : 0.0f;
+                const float jv = DAMPED ? __builtin_fmaf(omega, __builtin_fmaf(sum, 1.0f / 6.0f, -c[q]), c[q])
+                                        : sum * (1.0f / 6.0f);
+                const float v = (inxy[q] && zin) ? jv : 0.0f;
                 if (l < K) {
                     w[l][q][NEW] = v;
                 } else if (writer[q]) {
@@ -665,19 +669,33 @@ int upload_tb_set(const Workspace &w, const lgdhip_poisson_problem *probs, const
     return 0;
 }
 
-// one launch of K blocked sweeps a -> b over the uploaded tile list
-void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
-               double *partial, hipStream_t s) {
-    if (ntiles <= 0) return;
+template <bool DAMPED>
+void launch_tb_kind(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
+                    double *partial, float omega, hipStream_t s) {
     const unsigned nt = (unsigned)ntiles, per = lgdhip::xcd_per(nt), groups = lgdhip::xcd_groups(nt);
     if (sh.rows == 2 && sh.ty == 16)
-        poisson_sweep_tb2<4, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+        poisson_sweep_tb2<4, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+                                                                         per, omega);
     else if (sh.rows == 2)
-        poisson_sweep_tb2<4, 8><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+        poisson_sweep_tb2<4, 8, DAMPED><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+                                                                       per, omega);
     else if (sh.k == 2)
-        poisson_sweep_tb<2, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+        poisson_sweep_tb<2, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+                                                                        per, omega);
     else
-        poisson_sweep_tb<4, 16><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt, per);
+        poisson_sweep_tb<4, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+                                                                        per, omega);
+}
+
+// one launch of K blocked sweeps a -> b over the uploaded tile list; omega
+// > 0 runs weighted sweeps e' = e + omega (J(e) - e)
+void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
+               double *partial, hipStream_t s, float omega = 0.0f) {
+    if (ntiles <= 0) return;
+    if (omega > 0.0f)
+        launch_tb_kind<true>(sh, a, b, r, w, ntiles, partial, omega, s);
+    else
+        launch_tb_kind<false>(sh, a, b, r, w, ntiles, partial, 0.0f, s);
 }
 
 // Uploads problem descriptors and the tile list of the problems in `set`;
@@ -714,6 +732,52 @@ int residual_norms(const double *u, const float *f, const Workspace &w, const st
 }
 
 } // namespace
+
+// `sweeps` plain (omega == 0) or weighted sweeps ea <-> eb over every
+// problem: blocked launches of K sweeps, then single sweeps for the rest
+int run_sweeps(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count, int sweeps,
+               float omega, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes, int *result_in_b,
+               lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
+    const Workspace w = carve_for(workspace, probs, count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    std::vector<ProblemDev> pd(count);
+    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    std::vector<Tile> tiles;
+    rc = upload_set(w, probs, all, &tiles, s);
+    if (rc) return rc;
+    const TbShape sh = sweep_shape(tile ? tile->variant : 0, probs, all);
+    float *cur = ea, *nxt = eb;
+    int t = 0;
+    if (sh.k > 0 && sweeps >= sh.k) {
+        int64_t ntb = 0;
+        rc = upload_tb_set(w, probs, all, sh, &ntb, s);
+        if (rc) return rc;
+        for (; t + sh.k <= sweeps; t += sh.k) {
+            launch_tb(sh, cur, nxt, r, w, ntb, nullptr, s, omega);
+            LGD_LAUNCHED("poisson_sweep_tb");
+            std::swap(cur, nxt);
+        }
+    }
+    for (; t < sweeps; ++t) {
+        if (omega > 0.0f)
+            poisson_sweep_f32<true>
+                <<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr, omega);
+        else
+            poisson_sweep_f32<false>
+                <<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr, 1.0f);
+        LGD_LAUNCHED("poisson_sweep_f32");
+        std::swap(cur, nxt);
+    }
+    if (result_in_b) *result_in_b = cur == eb ? 1 : 0;
+    return 0;
+}
 
 extern "C" {
 
@@ -762,41 +826,14 @@ int lgdhip_poisson_set_faces(double *u, const lgdhip_poisson_problem *probs, int
 int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count,
                               int sweeps, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes,
                               int *result_in_b, lgdhip_stream_t stream) {
-    int rc = check_problems(probs, count);
-    if (rc) return rc;
-    LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
-    const Workspace w = carve_for(workspace, probs, count);
-    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
-    hipStream_t s = lgdhip::as_stream(stream);
-    std::vector<ProblemDev> pd(count);
-    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
-    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
-    std::vector<int> all(count);
-    for (int k = 0; k < count; ++k) all[k] = k;
-    std::vector<Tile> tiles;
-    rc = upload_set(w, probs, all, &tiles, s);
-    if (rc) return rc;
-    const TbShape sh = sweep_shape(tile ? tile->variant : 0, probs, all);
-    float *cur = ea, *nxt = eb;
-    int t = 0;
-    if (sh.k > 0 && sweeps >= sh.k) {
-        int64_t ntb = 0;
-        rc = upload_tb_set(w, probs, all, sh, &ntb, s);
-        if (rc) return rc;
-        for (; t + sh.k <= sweeps; t += sh.k) {
-            launch_tb(sh, cur, nxt, r, w, ntb, nullptr, s);
-            LGD_LAUNCHED("poisson_sweep_tb");
-            std::swap(cur, nxt);
-        }
-    }
-    for (; t < sweeps; ++t) {
-        poisson_sweep_f32<false>
-            <<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, r, w.tiles, w.probs, nullptr, 1.0f);
-        LGD_LAUNCHED("poisson_sweep_f32");
-        std::swap(cur, nxt);
-    }
-    if (result_in_b) *result_in_b = cur == eb ? 1 : 0;
-    return 0;
+    return run_sweeps(ea, eb, r, probs, count, sweeps, 0.0f, tile, workspace, workspace_bytes, result_in_b, stream);
+}
+
+int lgdhip_poisson_damped_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs,
+                                     int count, int sweeps, float omega, const lgdhip_tile *tile, void *workspace,
+                                     size_t workspace_bytes, int *result_in_b, lgdhip_stream_t stream) {
+    LGD_REQUIRE(omega > 0.0f && omega <= 1.0f, "omega must be in (0, 1]");
+    return run_sweeps(ea, eb, r, probs, count, sweeps, omega, tile, workspace, workspace_bytes, result_in_b, stream);
 }
 
 int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
@@ -909,7 +946,23 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
         if (!active.empty()) {
             rc = upload_set(w, probs, active, &tiles, s);
             if (rc) return rc;
-            for (int t = 0; t < smooth; ++t) {
+            // blocked weighted sweeps in pairs of launches (each launch swaps
+            // the buffers once), single weighted sweeps for the rest
+            const TbShape ssh = sweep_shape(opt->sweep_variant, probs, active);
+            int t = 0;
+            if (ssh.k > 0 && smooth >= 2 * ssh.k) {
+                int64_t ntb = 0;
+                rc = upload_tb_set(w, probs, active, ssh, &ntb, s);
+                if (rc) return rc;
+                for (; t + 2 * ssh.k <= smooth; t += 2 * ssh.k) {
+                    for (int half = 0; half < 2; ++half) {
+                        launch_tb(ssh, ecur, enxt, w.r, w, ntb, nullptr, s, opt->smooth_omega);
+                        LGD_LAUNCHED("poisson_sweep_tb");
+                        std::swap(ecur, enxt);
+                    }
+                }
+            }
+            for (; t < smooth; ++t) {
                 poisson_sweep_f32<true><<<(unsigned)tiles.size(), kThreads, 0, s>>>(
                     ecur, enxt, w.r, w.tiles, w.probs, nullptr, opt->smooth_omega);
                 LGD_LAUNCHED("poisson_sweep_f32");

@@ -276,6 +276,26 @@ void lgdo_poisson_set_faces(double *u, long n, const double face[6]) {
         }
 }
 
+/* one weighted (damped) FP32 Jacobi sweep e' = e + omega (J(e) - e): the
+ * plain sweep's sum, then e' = fma(omega, fma(sum, 1/6, -e), e) with single
+ * roundings (C fmaf, correctly rounded), the device kernels' form */
+void lgdo_poisson_sweep_damped_f32(const float *e, float *e2, const float *r, long n, float omega) {
+    const long s = n + 2, sxy = s * s;
+    for (long z = 1; z <= n; ++z)
+        for (long y = 1; y <= n; ++y)
+            for (long x = 1; x <= n; ++x) {
+                const long i = (z * s + y) * s + x;
+                float v = e[i - sxy];
+                v += e[i - s];
+                v += e[i - 1];
+                v += e[i + 1];
+                v += e[i + s];
+                v += e[i + sxy];
+                v += r[i];
+                e2[i] = fmaf(omega, fmaf(v, 1.0f / 6.0f, -e[i]), e[i]);
+            }
+}
+
 /* one FP32 Jacobi sweep on A e = r, in the summation order of the kernel */
 void lgdo_poisson_sweep_f32(const float *e, float *e2, const float *r, long n) {
     const long s = n + 2, sxy = s * s;

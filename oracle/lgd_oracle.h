@@ -72,6 +72,7 @@ void lgdo_lbm_moments_f64(const double *f, long ncells, double *rho, double *jx,
 /* Poisson problems: padded (n+2)^3 blocks, x fastest, faces in u's halo */
 void lgdo_poisson_set_faces(double *u, long n, const double face[6]);
 void lgdo_poisson_sweep_f32(const float *e, float *e2, const float *r, long n);
+void lgdo_poisson_sweep_damped_f32(const float *e, float *e2, const float *r, long n, float omega);
 double lgdo_poisson_residual_norm(const double *u, const float *f, long n);
 long lgdo_poisson_jacobi_f64(double *u, const float *f, long n, double rel_tol, double reference, long max_sweeps,
                              long check, double *final_norm);

@@ -38,6 +38,8 @@ def load():
         lib.lgdo_poisson_set_faces.restype = None
         lib.lgdo_poisson_sweep_f32.argtypes = [P, P, P, L]
         lib.lgdo_poisson_sweep_f32.restype = None
+        lib.lgdo_poisson_sweep_damped_f32.argtypes = [P, P, P, L, C.c_float]
+        lib.lgdo_poisson_sweep_damped_f32.restype = None
         lib.lgdo_poisson_residual_norm.argtypes = [P, P, L]
         lib.lgdo_poisson_residual_norm.restype = D
         lib.lgdo_poisson_jacobi_f64.argtypes = [P, P, L, D, D, L, L, C.POINTER(D)]
@@ -120,6 +122,12 @@ def poisson_faces(u: np.ndarray, faces) -> None:
 def poisson_sweep_f32(e: np.ndarray, r: np.ndarray) -> np.ndarray:
     out = np.zeros_like(e)
     load().lgdo_poisson_sweep_f32(_p(e), _p(out), _p(r), e.shape[0] - 2)
+    return out
+
+
+def poisson_sweep_damped_f32(e: np.ndarray, r: np.ndarray, omega: float) -> np.ndarray:
+    out = np.zeros_like(e)
+    load().lgdo_poisson_sweep_damped_f32(_p(e), _p(out), _p(r), e.shape[0] - 2, omega)
     return out
 
 

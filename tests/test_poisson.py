@@ -57,6 +57,27 @@ def test_oracle_sweep_matches_brute_force(oracle):
     assert np.array_equal(oracle.poisson_sweep_f32(e, r), brute_sweep_f32(e, r))
 
 
+def test_oracle_damped_sweep_pins(oracle):
+    """The weighted sweep e' = fma(w, fma(sum, 1/6, -e), e): w = 0 returns e
+    bit for bit; any w agrees with e + w (J(e) - e) evaluated in FP64 from the
+    oracle's own plain sweep to a few FP32 ulps; the halo stays untouched."""
+    rng = np.random.default_rng(21)
+    n = 9
+    e = np.zeros((n + 2,) * 3, np.float32)
+    e[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, n))
+    r = rng.standard_normal((n + 2,) * 3).astype(np.float32)
+    assert np.array_equal(oracle.poisson_sweep_damped_f32(e, r, 0.0), e)
+    plain = oracle.poisson_sweep_f32(e, r).astype(np.float64)
+    for w in (2.0 / 3.0, 0.5, 1.0):
+        got = oracle.poisson_sweep_damped_f32(e, r, w).astype(np.float64)
+        exp = e.astype(np.float64) + np.float64(np.float32(w)) * (plain - e)
+        inner = (slice(1, -1),) * 3
+        assert np.all(np.abs(got[inner] - exp[inner]) <= 4 * 2.0 ** -23 * (np.abs(exp[inner]) + np.abs(e[inner]) + 1e-30))
+        halo = np.ones(e.shape, bool)
+        halo[inner] = False
+        assert np.all(got[halo] == 0)
+
+
 def test_oracle_jacobi_decay_rate_closed_form(oracle):
     """For the smoothest eigenvector v of A (product of sines, zero faces),
     f = A v and u0 = 0, the residual after k sweeps is rho^k r0 with
@@ -191,6 +212,46 @@ def test_gpu_sweeps_bit_exact(hip, oracle, variant, sweeps):
     for k, (e, rr) in enumerate(blocks):
         for _ in range(sweeps):
             e = oracle.poisson_sweep_f32(e, rr)
+        o = b.probs[k].offset
+        assert np.array_equal(out[o:o + e.size].reshape(e.shape), e), sizes[k]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant,sweeps", [(1, 3), (3, 9), (4, 10), (5, 8), (0, 12)])
+def test_gpu_damped_sweeps_bit_exact(hip, oracle, variant, sweeps):
+    """Weighted sweeps (omega = 2/3, the smoothing phase's) through every
+    kernel -- single sweeps, blocked with one or two rows per thread, sweep
+    counts with remainders -- equal the oracle's damped sweep bit for bit."""
+    rng = np.random.default_rng(40 + variant)
+    sizes = [1, 6, 17, 33, 70]
+    problems = [(n, np.zeros(6), np.zeros((n + 2,) * 3, np.float32)) for n in sizes]
+    b = Batch(hip, problems)
+    e_host = np.zeros(b.elems, np.float32)
+    r_host = np.zeros(b.elems, np.float32)
+    blocks = []
+    for k, n in enumerate(sizes):
+        o = b.probs[k].offset
+        e = np.zeros((n + 2,) * 3, np.float32)
+        r = (10 * rng.standard_normal((n + 2,) * 3)).astype(np.float32)
+        e[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, n))
+        e_host[o:o + e.size] = e.ravel()
+        r_host[o:o + r.size] = r.ravel()
+        blocks.append((e, r))
+    ea, eb, r = (hip.DeviceBuffer(b.elems * 4) for _ in range(3))
+    ea.upload(e_host)
+    eb.upload(np.zeros(b.elems, np.float32))
+    r.upload(r_host)
+    in_b = C.c_int(-1)
+    omega = 2.0 / 3.0
+    print("[case] poisson damped sweeps", sizes, variant, sweeps, flush=True)
+    tile = hip.tile(variant=variant)
+    hip.call("lgdhip_poisson_damped_sweeps_f32", ea.ptr, eb.ptr, r.ptr, b.probs, b.count, sweeps, omega,
+             C.byref(tile), b.ws.ptr, b.ws_bytes, C.byref(in_b), None)
+    out = np.empty(b.elems, np.float32)
+    (eb if in_b.value else ea).download(out)
+    for k, (e, rr) in enumerate(blocks):
+        for _ in range(sweeps):
+            e = oracle.poisson_sweep_damped_f32(e, rr, omega)
         o = b.probs[k].offset
         assert np.array_equal(out[o:o + e.size].reshape(e.shape), e), sizes[k]
 
