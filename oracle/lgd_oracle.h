@@ -32,7 +32,9 @@
  *    spectrum); integer manufactured solutions with integer faces give an
  *    exact right-hand side and are recovered; the FP32 sweep is the FP64
  *    sweep rounded (compared to tolerance) and a brute-force Python loop on
- *    tiny cubes.
+ *    tiny cubes; the weighted sweep returns e unchanged for omega = 0 and
+ *    agrees with an FP64 evaluation of e + omega (J(e) - e) from the plain
+ *    sweep to a few ulps.
  *  - lgdo_conv3d_*: a delta kernel reproduces (shifted) inputs, linearity,
  *    and the adjoint identities <conv(x), g> = <x, dgrad(g)> and
  *    <conv_w(x), g> = <w, wgrad(x, g)> (exact up to FP64 rounding), plus a
