@@ -50,7 +50,9 @@ __device__ __forceinline__ void tap_offset(int tap, int ksize, int &dz, int &dy,
 }
 
 // A fragment: 8 consecutive k of voxel (z, y, x) -- one 16-byte load when
-// the 8 k share a tap (cin % 8 == 0), else element-wise
+// the 8 k share a tap (cin % 8 == 0); for even cin with even stride and
+// offset, 4 loads of two channels (k and k+1 always share a tap there);
+// else element-wise.  The fragment's values are the same in every case.
 __device__ __forceinline__ Frag load_a(const u16 *__restrict__ X, int n, int ld, int off, int cin, int ksize,
                                        int z, int y, int x, int k0, int K) {
     Frag f;
@@ -63,6 +65,23 @@ __device__ __forceinline__ Frag load_a(const u16 *__restrict__ X, int n, int ld,
             const int zz = z + dz, yy = y + dy, xx = x + dx;
             if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n) {
                 f.q = *reinterpret_cast<const uint4 *>(X + ((int64_t)((zz * n + yy) * n + xx)) * ld + off + ci);
+            }
+        }
+    } else if ((cin & 1) == 0 && ((ld | off) & 1) == 0 && ((uintptr_t)X & 3) == 0) {
+#pragma unroll
+        for (int e = 0; e < 8; e += 2) {
+            const int k = k0 + e;  // even: channels ci, ci + 1 of one tap
+            if (k < K) {
+                const int tap = k / cin, ci = k % cin;
+                int dz, dy, dx;
+                tap_offset(tap, ksize, dz, dy, dx);
+                const int zz = z + dz, yy = y + dy, xx = x + dx;
+                if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n) {
+                    const uint32_t pr = *reinterpret_cast<const uint32_t *>(
+                        X + ((int64_t)((zz * n + yy) * n + xx)) * ld + off + ci);
+                    f.h[e] = (u16)(pr & 0xFFFFu);
+                    f.h[e + 1] = (u16)(pr >> 16);
+                }
             }
         }
     } else {
