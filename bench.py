@@ -335,7 +335,7 @@ def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, w
     """(name, ms per launch, algorithmic bytes per launch) of the Poisson
     solver's sweep kernel: temporal blocking, 4 sweeps per launch (12 B per
     sweep-LUP, so 4 x 12 B x cells per launch); variant 0 is the library
-    default (4, two rows per thread, on large launches, else 3, one row)."""
+    default (6, three rows per thread, on large launches, else 3, one row)."""
     k, sweeps = 4, 48
     in_b = C.c_int(0)
     tile = lgdhip.tile(variant=variant)
@@ -345,7 +345,7 @@ def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, w
     planned = C.c_int(0)  # the library resolves the default by launch size
     lgdhip.call("lgdhip_poisson_sweep_plan", probs, count, variant, C.byref(planned))
     wl.sweep_plan = planned.value
-    return ("poisson_sweep_tb2" if planned.value in (4, 5) else "poisson_sweep_tb"), ms / (sweeps // k), \
+    return ("poisson_sweep_tb2" if planned.value in (4, 5, 6) else "poisson_sweep_tb"), ms / (sweeps // k), \
         k * 12.0 * cells
 
 

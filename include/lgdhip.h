@@ -326,11 +326,12 @@ typedef struct lgdhip_poisson_options {
     double inner_reduction;      /* FP64 refresh once the correction's residual
                                     fell by this factor (0: 1e-2)                 */
     int32_t use_graph;           /* replay chunks as a captured hipGraph           */
-    int32_t sweep_variant;       /* 0 default (4 for active sets of at least
+    int32_t sweep_variant;       /* 0 default (6 for active sets of at least
                                     2048 of its tiles, else 3), 1 plain sweeps,
                                     2 / 3 temporal blocking with 2 / 4 sweeps per
                                     launch, 4 / 5 4 sweeps with two rows per
-                                    thread (32- / 16-row tiles)                   */
+                                    thread (32- / 16-row tiles), 6 with three
+                                    rows (48-row tiles)                           */
     const double *reference_norms; /* host array, one per problem, or NULL: the
                                       residual norm of the initial guess          */
     int32_t smooth_sweeps;       /* weighted-Jacobi sweeps before the plain ones
@@ -367,7 +368,7 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
                               int *result_in_b, lgdhip_stream_t stream);
 
 /* The blocked sweep variant a launch over these problems uses for
- * `requested` (0: the default, resolved by launch size; 1-5 are kept).  No
+ * `requested` (0: the default, resolved by launch size; 1-6 are kept).  No
  * device work; for tools.  No counterpart in the reference (the ensemble
  * solver is new with this library). */
 int lgdhip_poisson_sweep_plan(const lgdhip_poisson_problem *probs, int count, int requested, int *variant);

@@ -299,14 +299,15 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb(const float *__restr
     }
 }
 
-// The same K-sweep pipeline with two adjacent rows per thread: a 64 x TY
-// work-group holds a 64 x 2TY tile of one problem (written region
-// (64-2K) x (2TY-2K); tile.x0/y0 its origin).  A thread's rows are each
-// other's y neighbours in registers, LDS carries only the row pairs' outer
-// rows, and the taller tile recomputes a smaller overlap ring.  Cells, sum
-// order, rings, ramp and residual partials as in poisson_sweep_tb: bit-exact
-// with K launches of poisson_sweep_f32.
-template <int K, int TY, bool DAMPED>
+// The same K-sweep pipeline with R adjacent rows per thread (R = 2, 3): a
+// 64 x TY work-group holds a 64 x R*TY tile of one problem (written region
+// (64-2K) x (R*TY-2K); tile.x0/y0 its origin).  A thread's rows are each
+// other's y neighbours in registers, LDS carries only each thread's first
+// and last row for the neighbouring threads, and the taller tile recomputes
+// a smaller overlap ring.  Cells, sum order, rings, ramp and residual
+// partials as in poisson_sweep_tb: bit-exact with K launches of
+// poisson_sweep_f32 (DAMPED: of its weighted form).
+template <int K, int TY, int R, bool DAMPED>
 __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__restrict__ e, float *__restrict__ e2,
                                                              const float *__restrict__ r,
                                                              const Tile *__restrict__ tiles,
@@ -314,9 +315,10 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
                                                              double *__restrict__ partial, int zchunk,
                                                              unsigned ntiles, unsigned per, float omega) {
     constexpr int PF = 3;
-    constexpr int CY = 2 * TY;
+    constexpr int CY = R * TY;
     static_assert(K + 2 <= 6, "r ring of 6 slots holds planes p-K .. p+1");
-    __shared__ float lds[2][K][TY][2][64];
+    static_assert(R >= 2 && R <= 4, "rows per thread");
+    __shared__ float lds[2][K][TY][2][64];  // [buffer][level][thread row][first, last row][lane]
     __shared__ double red[TY];
     const unsigned tid = lgdhip::xcd_tile(blockIdx.x, per, true);
     if (tid >= ntiles) return;  // whole work-group, before any barrier
@@ -324,35 +326,39 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
     const ProblemDev P = probs[t.p];
     const int n = P.n;
     const int tx = threadIdx.x, ty = threadIdx.y;
-    const int x = t.x0 - K + tx, y0 = t.y0 - K + 2 * ty;  // rows y0, y0 + 1
+    const int x = t.x0 - K + tx, y0 = t.y0 - K + R * ty;  // rows y0 .. y0 + R - 1
     const int z0 = t.z0, z1 = min(t.z0 + zchunk, n);
     const bool xin = (unsigned)x < (unsigned)n;
-    const bool inxy[2] = {xin && (unsigned)y0 < (unsigned)n, xin && (unsigned)(y0 + 1) < (unsigned)n};
     const bool xw = tx >= K && tx < 64 - K;
-    const bool writer[2] = {inxy[0] && xw && 2 * ty >= K && 2 * ty < CY - K,
-                            inxy[1] && xw && 2 * ty + 1 >= K && 2 * ty + 1 < CY - K};
     const int64_t sx = n + 2, sxy = sx * sx;
-    const int64_t col[2] = {P.base + (int64_t)(y0 + 1) * sx + (x + 1), P.base + (int64_t)(y0 + 2) * sx + (x + 1)};
     // unconditional loads clamped into the problem's halo shell (e is zero
     // there; r there only feeds cells masked to zero)
     const int xc = min(max(x, -1), n) + 1;
-    const int64_t colh[2] = {P.base + (int64_t)(min(max(y0, -1), n) + 1) * sx + xc,
-                             P.base + (int64_t)(min(max(y0 + 1, -1), n) + 1) * sx + xc};
-    auto at = [&](int rr, int p) { return colh[rr] + (int64_t)(min(max(p, -1), n) + 1) * sxy; };
+    bool inxy[R], writer[R];
+    int64_t col[R], colh[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        const int yq = y0 + q, tr = R * ty + q;
+        inxy[q] = xin && (unsigned)yq < (unsigned)n;
+        writer[q] = inxy[q] && xw && tr >= K && tr < CY - K;
+        col[q] = P.base + (int64_t)(yq + 1) * sx + (x + 1);
+        colh[q] = P.base + (int64_t)(min(max(yq, -1), n) + 1) * sx + xc;
+    }
+    auto at = [&](int q, int p) { return colh[q] + (int64_t)(min(max(p, -1), n) + 1) * sxy; };
     const int tyd = ty > 0 ? ty - 1 : 0;
     const int tyu = ty < TY - 1 ? ty + 1 : TY - 1;
 
-    float ev[2][6], rv[2][6];
-    float w[K][2][3];
+    float ev[R][6], rv[R][6];
+    float w[K][R][3];
 #pragma unroll
     for (int l = 0; l < K; ++l)
 #pragma unroll
-        for (int q = 0; q < 2; ++q) w[l][q][0] = w[l][q][1] = w[l][q][2] = 0.0f;
+        for (int q = 0; q < R; ++q) w[l][q][0] = w[l][q][1] = w[l][q][2] = 0.0f;
 #pragma unroll
     for (int l = 0; l < K; ++l) lds[0][l][ty][0][tx] = lds[0][l][ty][1][tx] = 0.0f;
     const int pbeg = z0 - K;
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < R; ++q) {
         ev[q][4] = ev[q][5] = 0.0f;
 #pragma unroll
         for (int d = 0; d < PF; ++d) ev[q][d] = e[at(q, pbeg + d)];
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
         constexpr int NEW = JM % 3, OLD = (JM + 1) % 3, MID = (JM + 2) % 3;
         constexpr int B = JM & 1;
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
+        for (int q = 0; q < R; ++q) {
             ev[q][(JM + PF) % 6] = e[at(q, p + PF)];
             rv[q][(JM + 1) % 6] = r[at(q, p + 1)];
         }
@@ -376,22 +382,24 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
             if (2 * l > JR) continue;
             const int zq = p - l;
             const bool zin = (unsigned)zq < (unsigned)n;
-            float a[2], c[2], b[2];
+            float a[R], c[R], b[R];
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < R; ++q) {
                 a[q] = l == 1 ? ev[q][(JM + 4) % 6] : w[l - 1][q][OLD];
                 c[q] = l == 1 ? ev[q][(JM + 5) % 6] : w[l - 1][q][MID];
                 b[q] = l == 1 ? ev[q][JM] : w[l - 1][q][NEW];
             }
-            const float ym[2] = {lds[B][l - 1][tyd][1][tx], c[0]};
-            const float yp[2] = {c[1], lds[B][l - 1][tyu][0][tx]};
+            const float below = lds[B][l - 1][tyd][1][tx];  // last row of the thread row above
+            const float above = lds[B][l - 1][tyu][0][tx];  // first row of the thread row below
 #pragma unroll
-            for (int q = 0; q < 2; ++q) {
+            for (int q = 0; q < R; ++q) {
+                const float ym = q == 0 ? below : c[q - 1];
+                const float yp = q == R - 1 ? above : c[q + 1];
                 float sum = a[q];               // ( 0,  0, -1)
-                sum += ym[q];                   // ( 0, -1,  0)
+                sum += ym;                      // ( 0, -1,  0)
                 sum += lane_prev(c[q]);         // (-1,  0,  0)
                 sum += lane_next(c[q]);         // ( 1,  0,  0)
-                sum += yp[q];                   // ( 0,  1,  0)
+                sum += yp;                      // ( 0,  1,  0)
                 sum += b[q];                    // ( 0,  0,  1)
                 sum += rv[q][(JM + 6 - l) % 6]; // r at plane zq
                 const float jv = DAMPED ? __builtin_fmaf(omega, __builtin_fmaf(sum, 1.0f / 6.0f, -c[q]), c[q])
@@ -410,13 +418,13 @@ __global__ __launch_bounds__(64 * TY) void poisson_sweep_tb2(const float *__rest
         }
         if (JR >= 1) {
             lds[B ^ 1][0][ty][0][tx] = ev[0][JM];
-            lds[B ^ 1][0][ty][1][tx] = ev[1][JM];
+            lds[B ^ 1][0][ty][1][tx] = ev[R - 1][JM];
         }
 #pragma unroll
         for (int l = 1; l < K; ++l)
             if (2 * l + 1 <= JR) {
                 lds[B ^ 1][l][ty][0][tx] = w[l][0][NEW];
-                lds[B ^ 1][l][ty][1][tx] = w[l][1][NEW];
+                lds[B ^ 1][l][ty][1][tx] = w[l][R - 1][NEW];
             }
         __syncthreads();
     };
@@ -527,24 +535,26 @@ struct TbShape {
     int k, ty, rows;  // sweeps per launch, work-group rows, tile rows per work-group row
 };
 constexpr int kTbZ = 64;
-// the default: 4 sweeps per launch, two rows per thread in 32-row tiles
-// (variant 4) for launches of at least kTallMinTiles tiles, else one row per
-// thread (variant 3).  On MI355X the full 512-problem ensemble (19k two-row
-// tiles) runs 3.97 ms per launch against 4.72 ms with one row (51.5 s
-// against 62.0 s per solve), while 16 problems of 128^3 (576 two-row tiles,
-// two rounds and a quarter of one work-group per CU) run 186 us against 173
-// us: the one-row kernel holds two work-groups per CU, which hides more
-// latency when a launch is only a few rounds long.
-constexpr int kDefaultSweepVariant = 4;
+// the default: 4 sweeps per launch, three rows per thread in 48-row tiles
+// (variant 6) for launches of at least kTallMinTiles of its tiles, else one
+// row per thread (variant 3).  On MI355X the full 512-problem ensemble runs
+// 3.81 ms per launch with three rows, 3.99 ms with two, 4.72 ms with one
+// (50.0 / 52.2 / 62.0 s per solve), while 16 problems of 128^3 (576 two-row
+// tiles, two rounds and a quarter of one work-group per CU) ran 186 us with
+// two rows against 173 us with one: the one-row kernel holds two
+// work-groups per CU, which hides more latency when a launch is only a few
+// rounds long.
+constexpr int kDefaultSweepVariant = 6;
 constexpr int kSmallSweepVariant = 3;
 constexpr int64_t kTallMinTiles = 2048;
-constexpr int kSweepVariants = 5;
+constexpr int kSweepVariants = 6;
 TbShape tb_shape(int variant) {
     switch (variant) {
     case 2: return {2, 16, 1};
     case 3: return {4, 16, 1};
     case 4: return {4, 16, 2};  // two rows per thread, 32-row tiles
     case 5: return {4, 8, 2};   // two rows per thread, 16-row tiles
+    case 6: return {4, 16, 3};  // three rows per thread, 48-row tiles
     default: return {0, 0, 1};  // 1: plain sweeps
     }
 }
@@ -673,12 +683,15 @@ template <bool DAMPED>
 void launch_tb_kind(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
                     double *partial, float omega, hipStream_t s) {
     const unsigned nt = (unsigned)ntiles, per = lgdhip::xcd_per(nt), groups = lgdhip::xcd_groups(nt);
-    if (sh.rows == 2 && sh.ty == 16)
-        poisson_sweep_tb2<4, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
-                                                                         per, omega);
+    if (sh.rows == 3)
+        poisson_sweep_tb2<4, 16, 3, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial,
+                                                                            kTbZ, nt, per, omega);
+    else if (sh.rows == 2 && sh.ty == 16)
+        poisson_sweep_tb2<4, 16, 2, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial,
+                                                                            kTbZ, nt, per, omega);
     else if (sh.rows == 2)
-        poisson_sweep_tb2<4, 8, DAMPED><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
-                                                                       per, omega);
+        poisson_sweep_tb2<4, 8, 2, DAMPED><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ,
+                                                                          nt, per, omega);
     else if (sh.k == 2)
         poisson_sweep_tb<2, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
                                                                         per, omega);
@@ -871,7 +884,7 @@ int lgdhip_poisson_sweep_plan(const lgdhip_poisson_problem *probs, int count, in
     std::vector<int> all(count);
     for (int k = 0; k < count; ++k) all[k] = k;
     const TbShape sh = sweep_shape(0, probs, all);
-    *variant = sh.rows == 2 ? kDefaultSweepVariant : kSmallSweepVariant;
+    *variant = sh.rows > 1 ? kDefaultSweepVariant : kSmallSweepVariant;
     return 0;
 }
 

@@ -95,10 +95,10 @@ def test_jacobi_launch_plan_without_gpu():
 
 
 def test_poisson_sweep_plan_without_gpu():
-    """lgdhip_poisson_sweep_plan (host-only): the default takes two rows per
-    thread (variant 4) for launches of at least 2048 of its tiles (the
+    """lgdhip_poisson_sweep_plan (host-only): the default takes three rows per
+    thread (variant 6) for launches of at least 2048 of its tiles (the
     512-problem ensemble) and one row (variant 3) for small sets (16 problems
-    of 128^3: 576 tiles); requested variants are kept."""
+    of 128^3: 384 tiles); requested variants are kept."""
     lib = lgdhip.load()
 
     def plan(sizes, requested=0):
@@ -111,6 +111,6 @@ def test_poisson_sweep_plan_without_gpu():
         return v.value
 
     ensemble = [32 + 16 * (k % 11) for k in range(512)]
-    assert plan(ensemble) == 4
+    assert plan(ensemble) == 6
     assert plan([128] * 16) == 3
     assert plan([128] * 16, 5) == 5 and plan(ensemble, 3) == 3

@@ -174,11 +174,13 @@ class Batch:
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant,sweeps", [(1, 5), (2, 5), (3, 9), (3, 8), (0, 4), (4, 9), (4, 8), (5, 12)])
+@pytest.mark.parametrize("variant,sweeps", [(1, 5), (2, 5), (3, 9), (3, 8), (0, 4), (4, 9), (4, 8), (5, 12),
+                                            (6, 9), (6, 16)])
 def test_gpu_sweeps_bit_exact(hip, oracle, variant, sweeps):
     """Plain sweeps (variant 1) and temporal blocking with 2 (variant 2) or 4
-    (variant 3) sweeps per launch, 4 with two rows per thread (variants 4,
-    the default, and 5: 32- and 16-row tiles), sweep counts with remainders,
+    (variant 3) sweeps per launch, 4 with two rows per thread (variants 4
+    and 5: 32- and 16-row tiles) or three (variant 6, the default for large
+    launches: 48-row tiles), sweep counts with remainders,
     sizes from 1 to several tiles per axis, noise in r's halo: bit-identical
     to the oracle."""
     rng = np.random.default_rng(3 + variant)
@@ -217,7 +219,7 @@ def test_gpu_sweeps_bit_exact(hip, oracle, variant, sweeps):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("variant,sweeps", [(1, 3), (3, 9), (4, 10), (5, 8), (0, 12)])
+@pytest.mark.parametrize("variant,sweeps", [(1, 3), (3, 9), (4, 10), (5, 8), (0, 12), (6, 9)])
 def test_gpu_damped_sweeps_bit_exact(hip, oracle, variant, sweeps):
     """Weighted sweeps (omega = 2/3, the smoothing phase's) through every
     kernel -- single sweeps, blocked with one or two rows per thread, sweep
