@@ -224,7 +224,7 @@ class Jacobi:
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
         th = cpu_threads()
-        steps = 4
+        steps = 64  # a probe long enough that thread start-up does not dominate the rate
         t0 = time.perf_counter()
         oracle.jacobi3d(self.host, steps, 0.0, th)
         dt = time.perf_counter() - t0
