@@ -8,6 +8,7 @@
 //
 // Compiled with hipcc (tests/cpp/Makefile): the cells' update() runs on the
 // GPU.
+#include <algorithm>
 #include "devicecells.h"
 #include "harness.h"
 
@@ -353,12 +354,20 @@ double runSeconds(Initializer<CELL> *init)
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// GLUPS from the difference of two whole runs of s1 and s2 steps (the host
+// initializer and the uploads cancel); the step spans below keep the GPU part
+// of the difference well above that host time's jitter, and the best of three
+// differences is taken
 template<typename CELL>
 double glups(std::function<Initializer<CELL> *(unsigned)> make, double cells, unsigned s1, unsigned s2)
 {
-    const double t1 = runSeconds<CELL>(make(s1));
-    const double t2 = runSeconds<CELL>(make(s2));
-    return cells * (double)(s2 - s1) / (t2 - t1) * 1e-9;
+    double best = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {
+        const double t1 = runSeconds<CELL>(make(s1));
+        const double t2 = runSeconds<CELL>(make(s2));
+        if (t2 > t1) best = std::max(best, cells * (double)(s2 - s1) / (t2 - t1) * 1e-9);
+    }
+    return best;
 }
 
 LGD_TEST(generic_vs_family_throughput)
@@ -370,13 +379,13 @@ LGD_TEST(generic_vs_family_throughput)
             return new RandomCellInitializer<DeviceJacobiCell, 3>(
                 j, s, 1, [](double u) { return DeviceJacobiCell(u); }, DeviceJacobiCell(0.0));
         },
-        jc, 100, 1100);
+        jc, 100, 8100);
     const double jf = glups<JacobiCell>(
         [&](unsigned s) {
             return new RandomCellInitializer<JacobiCell, 3>(
                 j, s, 1, [](double u) { return JacobiCell(u); }, JacobiCell(0.0));
         },
-        jc, 100, 1100);
+        jc, 100, 8100);
     std::printf("  Jacobi3D FP64 128^3: generic %.1f GLUPS, Jacobi3DF64 family %.1f GLUPS\n", jg, jf);
     const Coord<2> l(4096, 4096);
     const double lc = 4096.0 * 4096;
@@ -385,9 +394,9 @@ LGD_TEST(generic_vs_family_throughput)
             return new RandomCellInitializer<DeviceConwayCell, 2>(
                 l, s, 2, [](double u) { return DeviceConwayCell(u < 0.3); }, DeviceConwayCell(false));
         },
-        lc, 200, 2200);
+        lc, 200, 4200);
     const double lf = glups<ConwayCell>([&](unsigned s) { return new RandomLifeInitializer(l, s, 2, 0.3); }, lc, 200,
-                                        2200);
+                                        20200);
     std::printf("  Life 4096^2: generic %.1f GLUPS, Life2D family %.1f GLUPS\n", lg, lf);
     CHECK(jg > 0 && jf > 0 && lg > 0 && lf > 0);
 }
