@@ -114,3 +114,27 @@ def test_poisson_sweep_plan_without_gpu():
     assert plan(ensemble) == 6
     assert plan([128] * 16) == 3
     assert plan([128] * 16, 5) == 5 and plan(ensemble, 3) == 3
+
+
+def test_new_entry_points_reject_bad_arguments_without_gpu():
+    """Argument checks that return before any device work: the plan queries
+    need their output pointers and valid problems; weighted sweeps need
+    omega in (0, 1]."""
+    lib = lgdhip.load()
+    g = lgdhip.make_grid(3, 32, 32, 32, halo=1, elem_bytes=8)
+    z = C.c_int()
+    assert lib.lgdhip_jacobi3d_f64_plan(C.byref(g), None, None, C.byref(z)) == -1
+    assert "NULL" in lib.lgdhip_last_error().decode()
+    probs = (lgdhip.lgdhip_poisson_problem * 2)()
+    probs[0].n, probs[1].n = 16, 0
+    v = C.c_int()
+    assert lib.lgdhip_poisson_sweep_plan(probs, 2, 0, C.byref(v)) == -1      # n = 0
+    assert lib.lgdhip_poisson_sweep_plan(probs, 0, 0, C.byref(v)) == -1      # no problems
+    probs[1].n = 8
+    lib.lgdhip_poisson_layout(probs, 2)
+    assert lib.lgdhip_poisson_sweep_plan(probs, 2, 0, None) == -1
+    fake = [C.c_void_p(0x1000 * (i + 1)) for i in range(4)]
+    for omega in (0.0, -0.5, 1.5):
+        rc = lib.lgdhip_poisson_damped_sweeps_f32(fake[0], fake[1], fake[2], probs, 2, 4, C.c_float(omega), None,
+                                                  fake[3], C.c_size_t(1 << 20), None, None)
+        assert rc == -1 and "omega" in lib.lgdhip_last_error().decode()
