@@ -271,3 +271,68 @@ def test_lbm_float_matches_double(oracle):
     a = oracle.lbm_run(f, fl, 40, omega=1.7, lid=(0.05, 0.0, 0.0))
     b = oracle.lbm_run(f.astype(np.float32), fl, 40, omega=1.7, lid=(0.05, 0.0, 0.0))
     assert np.abs(a - b).max() < 2e-6
+
+
+def pulled(f, d):
+    """The populations each cell pulls before collision: f_i(x - e_i), for
+    the cells whose pull sources are all inside the box ([1:-1] on every
+    axis), in numpy."""
+    out = np.empty((19,) + tuple(s - 2 for s in f.shape[1:]))
+    nz, ny, nx = f.shape[1:]
+    for i, (ex, ey, ez, _) in enumerate(d):
+        out[i] = f[i, 1 - ez:nz - 1 - ez, 1 - ey:ny - 1 - ey, 1 - ex:nx - 1 - ex]
+    return out
+
+
+def test_lbm_uniform_flow_equilibrium_unchanged_in_the_interior(oracle):
+    """A uniform flow at equilibrium: every interior cell pulls the same
+    equilibrium populations and the collision of an equilibrium returns it,
+    so one step leaves the cells away from the walls unchanged.  A collision
+    whose equilibrium used the wrong momentum (a sign, a dropped e.u or
+    (e.u)^2 term, a transposed direction) would move them."""
+    nx, ny, nz = 10, 9, 8
+    shape = (nz, ny, nx)
+    j = (0.02, -0.013, 0.017)
+    f = oracle.lbm_equilibrium(np.full(shape, 1.02), *(np.full(shape, c) for c in j))
+    fl = oracle.lbm_cavity_flags(nx, ny, nz)
+    out = oracle.lbm_run(f, fl, 1, omega=1.7, lid=(0.05, 0.0, 0.0))
+    inner = (slice(None), slice(1, -1), slice(1, -1), slice(1, -1))
+    np.testing.assert_allclose(out[inner], f[inner], rtol=0, atol=2e-16)
+    # ... while the cells next to a wall do change (bounce-back is active)
+    assert np.abs(out[:, 0] - f[:, 0]).max() > 1e-3
+
+
+def test_lbm_collision_conserves_mass_and_momentum(oracle):
+    """Random non-equilibrium populations: for every cell away from the
+    walls, the post-collision moments equal those of the populations it
+    pulled, rho = sum f and j = sum e f (the BGK collision relaxes only the
+    non-conserved moments)."""
+    d = oracle.lbm_directions()
+    nx, ny, nz = 9, 8, 7
+    rng = np.random.default_rng(17)
+    w = oracle.lbm_equilibrium(np.ones(1), np.zeros(1), np.zeros(1), np.zeros(1))[:, 0]
+    f = w[:, None, None, None] * (1.0 + 0.2 * rng.standard_normal((19, nz, ny, nx)))
+    fl = oracle.lbm_cavity_flags(nx, ny, nz)
+    out = oracle.lbm_run(f, fl, 1, omega=1.7)
+    pin = pulled(f, d)
+    got = oracle.lbm_moments(out[:, 1:-1, 1:-1, 1:-1])
+    exp = oracle.lbm_moments(pin)
+    for a, b in zip(got, exp):
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-15)
+    # the non-conserved part did relax: output differs from the pulled input
+    assert np.abs(out[:, 1:-1, 1:-1, 1:-1] - pin).max() > 1e-3
+
+
+def test_lbm_full_relaxation_gives_the_equilibrium_of_the_pulled_moments(oracle):
+    """omega = 1 replaces every population by the equilibrium of the pulled
+    moments: out = f_eq(rho, j) in closed form (lbm_equilibrium is itself
+    pinned by its moments above)."""
+    d = oracle.lbm_directions()
+    nx, ny, nz = 8, 7, 6
+    rng = np.random.default_rng(23)
+    w = oracle.lbm_equilibrium(np.ones(1), np.zeros(1), np.zeros(1), np.zeros(1))[:, 0]
+    f = w[:, None, None, None] * (1.0 + 0.1 * rng.standard_normal((19, nz, ny, nx)))
+    fl = oracle.lbm_cavity_flags(nx, ny, nz)
+    out = oracle.lbm_run(f, fl, 1, omega=1.0)
+    rho, jx, jy, jz = oracle.lbm_moments(pulled(f, d))
+    np.testing.assert_allclose(out[:, 1:-1, 1:-1, 1:-1], oracle.lbm_equilibrium(rho, jx, jy, jz), rtol=0, atol=1e-15)

@@ -150,10 +150,11 @@ class Batch:
         self.f.upload(f_host)
         hip.call("lgdhip_poisson_set_faces", self.u.ptr, self.probs, self.count, self.ws.ptr, self.ws_bytes, None)
 
-    def solve(self, rel_tol=1e-6, chunk=32, use_graph=1, reference=None, stream=None):
+    def solve(self, rel_tol=1e-6, chunk=32, use_graph=1, reference=None, stream=None, sweep_variant=0):
         hip = self.hip
         opt = hip.lgdhip_poisson_options()
         opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = rel_tol, chunk, 2_000_000, use_graph
+        opt.sweep_variant = sweep_variant
         ref = None
         if reference is not None:
             ref = (C.c_double * self.count)(*reference)
@@ -258,27 +259,42 @@ def test_gpu_damped_sweeps_bit_exact(hip, oracle, variant, sweeps):
         assert np.array_equal(out[o:o + e.size].reshape(e.shape), e), sizes[k]
 
 
+def check_solution(oracle, problem, u, result):
+    """A solved problem (padded FP64 block u) meets the FP64 residual
+    criterion as recomputed on the CPU and reports that residual; returns
+    (zero-guess block u0, its residual norm r0, the solution's rk)."""
+    n, faces, f = problem
+    assert result.converged == 1
+    u0 = padded(n, faces, oracle)
+    r0 = oracle.poisson_residual_norm(u0, f)
+    rk = oracle.poisson_residual_norm(u, f)
+    assert rk <= 1e-6 * r0 * 1.0000001, (n, rk / r0)
+    assert abs(result.residual_norm - rk) <= 1e-9 * r0
+    return u0, r0, rk
+
+
 @pytest.mark.gpu
-@pytest.mark.parametrize("use_graph", [1, 0])
-def test_gpu_ensemble_solve(hip, oracle, use_graph):
-    """A small uneven ensemble (config-4 generator, sizes 8..40): every GPU
-    solution meets the FP64 residual criterion as recomputed on the CPU,
-    takes about as many sweeps as plain FP64 Jacobi, and agrees with the
-    oracle's solution within the error bound the tolerance implies."""
+@pytest.mark.parametrize("use_graph,variant", [(1, 0), (0, 0), (1, 1), (1, 2), (0, 3), (1, 4), (1, 5), (1, 6),
+                                               (0, 6)])
+def test_gpu_ensemble_solve(hip, oracle, use_graph, variant):
+    """A small uneven ensemble (config-4 generator, sizes 8..40) through the
+    solver with the default sweep kernel and each forced one (1 plain
+    sweeps, 2 / 3 blocked one-row, 4 / 5 two rows, 6 three rows), graph and
+    direct launches: every GPU solution meets the FP64 residual criterion as
+    recomputed on the CPU, takes about as many sweeps as plain FP64 Jacobi,
+    and agrees with the oracle's solution within the error bound the
+    tolerance implies."""
     problems = synthetic.poisson_ensemble(10, sizes=(8, 16, 24, 32, 40), seed=11)
-    print("[case] poisson ensemble", [p[0] for p in problems], "graph", use_graph, flush=True)
+    print("[case] poisson ensemble", [p[0] for p in problems], "graph", use_graph, "variant", variant, flush=True)
     b = Batch(hip, problems)
     # graphs are captured on a non-null stream only
     stream = hip.Stream() if use_graph else None
-    res = b.solve(rel_tol=1e-6, chunk=16, use_graph=use_graph, stream=stream.h if stream else None)
+    res = b.solve(rel_tol=1e-6, chunk=16, use_graph=use_graph, stream=stream.h if stream else None,
+                  sweep_variant=variant)
     ublocks = b.u_blocks()
     for k, (n, faces, f) in enumerate(problems):
-        assert res[k].converged == 1
-        u0 = padded(n, faces, oracle)
-        r0 = oracle.poisson_residual_norm(u0, f)
-        rk = oracle.poisson_residual_norm(ublocks[k], f)
-        assert rk <= 1e-6 * r0 * 1.0000001, (k, rk / r0)
-        assert abs(res[k].residual_norm - rk) <= 1e-9 * r0
+        ublock = ublocks[k]
+        u0, r0, rk = check_solution(oracle, problems[k], ublock, res[k])
         uo = u0.copy()
         sweeps_o, rn_o = oracle.poisson_jacobi(uo, f, rel_tol=1e-6, check=1)
         # same linear iteration: the sweep count may exceed the oracle's by
@@ -287,7 +303,7 @@ def test_gpu_ensemble_solve(hip, oracle, use_graph):
             (k, n, sweeps_o, res[k].sweeps)
         lam = 6.0 - 6.0 * np.cos(np.pi / (n + 1))
         bound = (rk + rn_o) / lam
-        assert np.linalg.norm(ublocks[k] - uo) <= 1.01 * bound + 1e-12
+        assert np.linalg.norm(ublock - uo) <= 1.01 * bound + 1e-12
         print(f"  n={n}: sweeps gpu {res[k].sweeps} oracle {sweeps_o}, outer {res[k].outer_iterations}, "
               f"rel residual {rk / r0:.3e}", flush=True)
 
@@ -335,3 +351,45 @@ def test_gpu_guess_with_reference_norm(hip, oracle):
     for k in range(3):
         assert res1[k].converged == 1 and res1[k].reference_norm == refs[k]
         assert res1[k].sweeps < res0[k].sweeps
+
+
+@pytest.mark.gpu
+def test_gpu_ensemble_with_largest_size_and_kernel_switch(hip, oracle):
+    """2048 small problems (n = 8..16) and one of the configuration's
+    largest size, n = 192: the active set starts at or above the three-row
+    kernel's launch threshold (the plan query reports variant 6) and falls
+    below it once the small problems have converged, so the solve re-plans
+    to the one-row kernel (variant 3), re-uploads its tiles and re-captures
+    its graph mid-solve.  Every problem meets the FP64 residual criterion
+    recomputed on the CPU; a sample of the small ones also agrees with the
+    oracle's plain FP64 Jacobi solution within the tolerance's bound (the
+    oracle needs ~10^5 sweeps of 7 M cells for n = 192, so that one is held
+    to the residual criterion alone)."""
+    small = synthetic.poisson_ensemble(2048, sizes=(8, 12, 16), seed=71)
+    big = synthetic.poisson_ensemble(1, sizes=(192,), seed=72)
+    problems = small + big
+    b = Batch(hip, problems)
+    plan = C.c_int(0)
+    hip.call("lgdhip_poisson_sweep_plan", b.probs, b.count, 0, C.byref(plan))
+    assert plan.value == 6
+    one = (hip.lgdhip_poisson_problem * 1)(b.probs[b.count - 1])
+    hip.call("lgdhip_poisson_sweep_plan", one, 1, 0, C.byref(plan))
+    assert plan.value == 3
+    print("[case] poisson 2048 small + n=192, kernel switch", flush=True)
+    stream = hip.Stream()
+    res = b.solve(rel_tol=1e-6, chunk=64, use_graph=1, stream=stream.h)
+    ublocks = b.u_blocks()
+    for k in range(b.count):
+        ublock = ublocks[k]
+        u0, r0, rk = check_solution(oracle, problems[k], ublock, res[k])
+        if k % 256 == 0:
+            n, _, f = problems[k]
+            uo = u0.copy()
+            sweeps_o, rn_o = oracle.poisson_jacobi(uo, f, rel_tol=1e-6, check=1)
+            lam = 6.0 - 6.0 * np.cos(np.pi / (n + 1))
+            assert np.linalg.norm(ublock - uo) <= 1.01 * (rk + rn_o) / lam + 1e-12
+    big_sweeps = res[b.count - 1].sweeps
+    # the smoothest mode decays by cos(pi/193) per sweep: ~1e5 sweeps for 1e-6
+    print(f"  n=192: {big_sweeps} sweeps, {res[b.count - 1].outer_iterations} outer steps", flush=True)
+    assert 3e4 < big_sweeps < 2e5
+    assert max(r.sweeps for r in res[:-1]) < 5000
