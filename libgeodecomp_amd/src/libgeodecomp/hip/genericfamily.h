@@ -116,6 +116,65 @@ public:
     }
 };
 
+// Neighbourhood of an array-of-structs line update (a cell with
+// HasFixedCoordsOnlyUpdate and HasUpdateLineX but no SoA):
+// hood[FixedCoord<X,Y,Z>()] is the cell at (*x + X, y + Y, z + Z) of the
+// row the update walks, x shared with the cell's loop, as the reference's
+// LinePointerNeighborhood reads its line pointers at the current x
+// (storage/linepointerneighborhood.h).
+template<typename CELL, int DIM>
+class GenericLineNeighborhood
+{
+public:
+    __device__ GenericLineNeighborhood(const CELL *row, const long *x, long px, long pxy) : row(row), x(x)
+    {
+        stride[0] = 1;
+        stride[1] = px;
+        stride[2] = pxy;
+    }
+
+    template<int X, int Y, int Z>
+    __device__ const CELL& operator[](FixedCoord<X, Y, Z>) const
+    {
+        return row[*x + X + (DIM > 1 ? Y * stride[1] : 0) + (DIM > 2 ? Z * stride[2] : 0)];
+    }
+
+private:
+    const CELL *row;
+    const long *x;
+    long stride[3];
+};
+
+// update of the cell at index i (row start `row`, x coordinate x): the
+// cell-wise update() path, or one element of a line update
+template<typename CELL, int DIM, bool LINE = APITraits::SelectLinePointerUpdate<CELL>::VALUE>
+class UpdateAt
+{
+public:
+    __device__ static void apply(const CELL *src, CELL *dst, long row, int x, long px, long pxy, unsigned nanoStep)
+    {
+        typedef Update<CELL, typename APITraits::SelectSeparateHIPUpdate<CELL>::Value,
+                       typename APITraits::SelectSeparateCUDAUpdate<CELL>::Value> U;
+        const long i = row + x;
+        GenericNeighborhood<CELL, DIM> hood(src + i, px, pxy);
+        CELL cell = dst[i];
+        U::apply(cell, hood, nanoStep);
+        dst[i] = cell;
+    }
+};
+
+template<typename CELL, int DIM>
+class UpdateAt<CELL, DIM, true>
+{
+public:
+    __device__ static void apply(const CELL *src, CELL *dst, long row, int x, long px, long pxy, unsigned nanoStep)
+    {
+        long cursor = x;
+        GenericLineNeighborhood<CELL, DIM> hood(src + row, &cursor, px, pxy);
+        CELL::updateLineX(dst + row, &cursor, (long)x + 1, hood, nanoStep);
+    }
+};
+
 // One thread per cell.  Work-group (bx, by): bx consecutive x cells of by
 // consecutive rows (a row = one (y, z) pair); blockIdx.x enumerates
 // (row group, x block) with the x block fastest.
@@ -124,8 +183,6 @@ __global__ void __launch_bounds__(1024) sweep(const CELL *__restrict__ src, CELL
                                               long rows, int ny, int h, int hy, int hz, long px, long pxy,
                                               unsigned xblocks, unsigned nanoStep)
 {
-    typedef Update<CELL, typename APITraits::SelectSeparateHIPUpdate<CELL>::Value,
-                   typename APITraits::SelectSeparateCUDAUpdate<CELL>::Value> U;
     const unsigned xb = blockIdx.x % xblocks;
     const long row = (long)(blockIdx.x / xblocks) * blockDim.y + threadIdx.y;
     const int x = (int)(xb * blockDim.x + threadIdx.x);
@@ -134,11 +191,7 @@ __global__ void __launch_bounds__(1024) sweep(const CELL *__restrict__ src, CELL
     }
     const long y = row % ny;
     const long z = row / ny;
-    const long i = (z + hz) * pxy + (y + hy) * px + (x + h);
-    GenericNeighborhood<CELL, DIM> hood(src + i, px, pxy);
-    CELL cell = dst[i];
-    U::apply(cell, hood, nanoStep);
-    dst[i] = cell;
+    UpdateAt<CELL, DIM>::apply(src, dst, (z + hz) * pxy + (y + hy) * px + h, x, px, pxy, nanoStep);
 }
 
 inline void checkHip(hipError_t e, const char *what)
@@ -230,7 +283,41 @@ public:
         check(lgdhip_cell_grid_store(a->get(), &g, host->data(), stream.get()), "lgdhip_cell_grid_store");
     }
 
+    // nanoSteps is a multiple of NANO_STEPS (whole steps).  A period of
+    // lcm(2, NANO_STEPS) nano steps returns the buffers to their places and
+    // is replayed as one captured graph; the rest is launched directly.
     void run(unsigned nanoSteps)
+    {
+        const unsigned period = NANO_STEPS % 2 ? 2 * NANO_STEPS : NANO_STEPS;
+        void *pa = a->get(), *pb = b->get();
+        loop.run(stream.get(), pa, nanoSteps / period, [&]() {
+            void *src = pa, *dst = pb;
+            for (unsigned t = 0; t < period; ++t) {
+                nanoStep(src, dst, t % NANO_STEPS);
+                std::swap(src, dst);
+            }
+        });
+        for (unsigned t = 0; t < nanoSteps % period; ++t) {
+            nanoStep(a->get(), b->get(), t % NANO_STEPS);
+            a.swap(b);
+        }
+    }
+
+    void sync() { stream.sync(); }
+    const lgdhip_cell_grid& descriptor() const { return g; }
+    const GraphLoop& graphs() const { return loop; }
+
+private:
+    Stream stream;
+    lgdhip_cell_grid g;
+    std::unique_ptr<DeviceMemory> a, b, edge;
+    int wrap;
+    bool uploaded;
+    int bx, by;
+    GraphLoop loop;
+
+    // one nano step src -> dst: wrapped halo refresh, then the sweep
+    void nanoStep(void *src, void *dst, unsigned nano)
     {
         const long rows = g.ny * g.nz;
         const unsigned xblocks = (unsigned)((g.nx + bx - 1) / bx);
@@ -241,29 +328,14 @@ public:
         const int hy = DIM >= 2 ? RADIUS : 0;
         const int hz = DIM == 3 ? RADIUS : 0;
         hipStream_t s = reinterpret_cast<hipStream_t>(stream.get());
-        for (unsigned t = 0; t < nanoSteps; ++t) {
-            if (wrap) {
-                check(lgdhip_cell_grid_fill_halo(a->get(), &g, edge->get(), wrap, stream.get()),
-                      "lgdhip_cell_grid_fill_halo");
-            }
-            GenericHelpers::sweep<CELL, DIM><<<dim3((unsigned)(xblocks * groups)), dim3(bx, by), 0, s>>>(
-                static_cast<const CELL *>(a->get()), static_cast<CELL *>(b->get()), (int)g.nx, rows, (int)g.ny,
-                RADIUS, hy, hz, (long)g.px, (long)(g.px * g.py), xblocks, t % NANO_STEPS);
-            GenericHelpers::checkHip(hipGetLastError(), "generic HIP kernel launch");
-            a.swap(b);
+        if (wrap) {
+            check(lgdhip_cell_grid_fill_halo(src, &g, edge->get(), wrap, stream.get()), "lgdhip_cell_grid_fill_halo");
         }
+        GenericHelpers::sweep<CELL, DIM><<<dim3((unsigned)(xblocks * groups)), dim3(bx, by), 0, s>>>(
+            static_cast<const CELL *>(src), static_cast<CELL *>(dst), (int)g.nx, rows, (int)g.ny, RADIUS, hy, hz,
+            (long)g.px, (long)(g.px * g.py), xblocks, nano);
+        GenericHelpers::checkHip(hipGetLastError(), "generic HIP kernel launch");
     }
-
-    void sync() { stream.sync(); }
-    const lgdhip_cell_grid& descriptor() const { return g; }
-
-private:
-    Stream stream;
-    lgdhip_cell_grid g;
-    std::unique_ptr<DeviceMemory> a, b, edge;
-    int wrap;
-    bool uploaded;
-    int bx, by;
 
     void requireBox(const Grid<CELL, Topology>& host) const
     {
@@ -276,5 +348,8 @@ private:
 }
 
 }
+
+// struct-of-arrays line cells: the SoA family, from the same include
+#include <libgeodecomp/hip/soafamily.h>
 
 #endif

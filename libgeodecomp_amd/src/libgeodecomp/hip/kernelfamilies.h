@@ -86,6 +86,109 @@ private:
     lgdhip_stream_t s;
 };
 
+// A fixed sequence of launches (one "period" of a family's time loop, e.g.
+// two nano steps that return the ping-pong buffers to their places)
+// captured once into a hipGraph per buffer arrangement and replayed: the
+// multi-launch steps of the time loop go to the GPU as one graph launch per
+// period instead of one launch call per kernel (the reference launches
+// every nano step separately, cudasimulator.h:467-470).  Only kernel
+// launches are captured (no memset or copy nodes).
+class GraphLoop
+{
+public:
+    GraphLoop() { clear(); }
+    ~GraphLoop() { reset(); }
+    GraphLoop(const GraphLoop&) = delete;
+    GraphLoop& operator=(const GraphLoop&) = delete;
+
+    // enqueue() `count` times on `stream`; key names the arrangement the
+    // launches of one period depend on (e.g. the current buffer's address)
+    template<typename ENQUEUE>
+    void run(lgdhip_stream_t stream, const void *key, unsigned long long count, ENQUEUE enqueue)
+    {
+        if (count < 2) {
+            for (unsigned long long i = 0; i < count; ++i) {
+                enqueue();
+            }
+            return;
+        }
+        lgdhip_graph_t graph = find(key);
+        if (!graph) {
+            check(lgdhip_graph_begin(stream), "lgdhip_graph_begin");
+            try {
+                enqueue();
+            } catch (...) {
+                lgdhip_graph_t broken = 0;
+                if (lgdhip_graph_end(stream, &broken) == 0) {
+                    lgdhip_graph_destroy(broken);
+                }
+                throw;
+            }
+            check(lgdhip_graph_end(stream, &graph), "lgdhip_graph_end");
+            store(key, graph);
+            ++captures;
+        }
+        for (unsigned long long i = 0; i < count; ++i) {
+            check(lgdhip_graph_launch(graph, stream), "lgdhip_graph_launch");
+        }
+        replays += count;
+    }
+
+    // destroys the captured graphs (after the buffers they name changed)
+    void reset()
+    {
+        for (int i = 0; i < SLOTS; ++i) {
+            if (graphs[i]) {
+                lgdhip_graph_destroy(graphs[i]);
+            }
+        }
+        clear();
+    }
+
+    unsigned long long graphsCaptured() const { return captures; }
+    unsigned long long graphLaunches() const { return replays; }
+
+private:
+    static const int SLOTS = 2;
+    const void *keys[SLOTS];
+    lgdhip_graph_t graphs[SLOTS];
+    unsigned long long captures = 0, replays = 0;
+
+    void clear()
+    {
+        for (int i = 0; i < SLOTS; ++i) {
+            keys[i] = 0;
+            graphs[i] = 0;
+        }
+    }
+
+    lgdhip_graph_t find(const void *key) const
+    {
+        for (int i = 0; i < SLOTS; ++i) {
+            if (graphs[i] && keys[i] == key) {
+                return graphs[i];
+            }
+        }
+        return 0;
+    }
+
+    void store(const void *key, lgdhip_graph_t graph)
+    {
+        for (int i = 0; i < SLOTS; ++i) {
+            if (!graphs[i]) {
+                keys[i] = key;
+                graphs[i] = graph;
+                return;
+            }
+        }
+        lgdhip_graph_destroy(graphs[0]);
+        keys[0] = keys[1];
+        graphs[0] = graphs[1];
+        keys[1] = key;
+        graphs[1] = graph;
+    }
+};
+
 // Tuning knobs handed down by HipSimulator (blockSize -> bx/by/bz).
 struct Tuning
 {
@@ -221,15 +324,15 @@ public:
     void run(unsigned nanoSteps)
     {
         if (wrap) {
-            lgdhip_tile single;
-            std::memset(&single, 0, sizeof(single));
-            single.variant = 2;  // a single sweep that reads its halo from memory
-            for (unsigned t = 0; t < nanoSteps; ++t) {
-                int inB = 0;
-                check(lgdhip_grid_fill_halo(a->get(), &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
-                check(lgdhip_jacobi3d_f64_run((double *)a->get(), (double *)b->get(), &g, 1, &single, &inB,
-                                              stream.get()),
-                      "lgdhip_jacobi3d_f64_run");
+            // halo refresh + single sweep per step, two steps per captured
+            // graph (the buffers are back in place after two)
+            void *pa = a->get(), *pb = b->get();
+            loop.run(stream.get(), pa, nanoSteps / 2, [&]() {
+                wrappedSweep(pa, pb);
+                wrappedSweep(pb, pa);
+            });
+            if (nanoSteps % 2) {
+                wrappedSweep(pa, pb);
                 a.swap(b);
             }
             return;
@@ -245,6 +348,7 @@ public:
 
     void sync() { stream.sync(); }
     const lgdhip_grid& descriptor() const { return g; }
+    const GraphLoop& graphs() const { return loop; }
 
 private:
     Tuning tuning;
@@ -253,6 +357,20 @@ private:
     std::unique_ptr<DeviceMemory> a, b;
     int wrap;
     double edge;
+    GraphLoop loop;
+
+    // one step of a periodic grid: wrapped halo from the interior, then a
+    // single sweep that reads its halo from memory
+    void wrappedSweep(void *src, void *dst)
+    {
+        lgdhip_tile single;
+        std::memset(&single, 0, sizeof(single));
+        single.variant = 2;
+        int inB = 0;
+        check(lgdhip_grid_fill_halo(src, &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
+        check(lgdhip_jacobi3d_f64_run((double *)src, (double *)dst, &g, 1, &single, &inB, stream.get()),
+              "lgdhip_jacobi3d_f64_run");
+    }
 };
 
 // ---------------------------------------------------------------------
@@ -353,13 +471,15 @@ public:
     void run(unsigned nanoSteps)
     {
         if (wrap) {
-            const uint8_t edge = (uint8_t)edgeAlive;
-            for (unsigned t = 0; t < nanoSteps; ++t) {
-                int inB = 0;
-                check(lgdhip_grid_fill_halo(grid->get(), &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
-                check(lgdhip_life2d_u8_run((uint8_t *)grid->get(), (uint8_t *)grid2->get(), &g, 1, &inB,
-                                           stream.get()),
-                      "lgdhip_life2d_u8_run");
+            // halo refresh + one generation per step, two steps per
+            // captured graph
+            void *pa = grid->get(), *pb = grid2->get();
+            loop.run(stream.get(), pa, nanoSteps / 2, [&]() {
+                wrappedGeneration(pa, pb);
+                wrappedGeneration(pb, pa);
+            });
+            if (nanoSteps % 2) {
+                wrappedGeneration(pa, pb);
                 grid.swap(grid2);
             }
             return;
@@ -373,6 +493,7 @@ public:
     }
 
     void sync() { stream.sync(); }
+    const GraphLoop& graphs() const { return loop; }
 
 private:
     Tuning tuning;
@@ -382,6 +503,16 @@ private:
     uint32_t *pa, *pb;
     int edgeAlive;
     int wrap;
+    GraphLoop loop;
+
+    void wrappedGeneration(void *src, void *dst)
+    {
+        const uint8_t edge = (uint8_t)edgeAlive;
+        int inB = 0;
+        check(lgdhip_grid_fill_halo(src, &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
+        check(lgdhip_life2d_u8_run((uint8_t *)src, (uint8_t *)dst, &g, 1, &inB, stream.get()),
+              "lgdhip_life2d_u8_run");
+    }
 };
 
 // ---------------------------------------------------------------------

@@ -33,6 +33,11 @@ public:
         typedef void Type;
     };
 
+    template<typename T, typename DUMMY = void>
+    class IsTrue { public: static const bool VALUE = false; };
+    template<typename DUMMY>
+    class IsTrue<TrueType, DUMMY> { public: static const bool VALUE = true; };
+
     // ---- struct-of-arrays storage
     template<typename CELL, typename HAS = void>
     class SelectSoA { public: typedef FalseType Value; };
@@ -107,6 +112,30 @@ public:
     { public: typedef typename CELL::API::StaticData Value; };
     template<typename DATA>
     class HasStaticData { public: typedef void SupportsStaticData; typedef DATA StaticData; };
+
+    // ---- the update path the reference's UpdateFunctor selector picks
+    // (storage/updatefunctor.h:96-139, :140-174): a struct-of-arrays cell with
+    // a FixedCoord-only neighbourhood and a line update runs streak by streak
+    // through static updateLineX(hoodOld, indexEnd, hoodNew, nanoStep); an
+    // array-of-structs cell with both of those traits runs static
+    // updateLineX(target, &x, endX, hood, nanoStep) over line pointers; every
+    // other cell runs update(hood, nanoStep) cell by cell.
+    template<typename CELL>
+    class SelectSoAStreakUpdate
+    {
+    public:
+        static const bool VALUE = IsTrue<typename SelectSoA<CELL>::Value>::VALUE &&
+                                  IsTrue<typename SelectUpdateLineX<CELL>::Value>::VALUE &&
+                                  IsTrue<typename SelectFixedCoordsOnlyUpdate<CELL>::Value>::VALUE;
+    };
+    template<typename CELL>
+    class SelectLinePointerUpdate
+    {
+    public:
+        static const bool VALUE = !IsTrue<typename SelectSoA<CELL>::Value>::VALUE &&
+                                  IsTrue<typename SelectUpdateLineX<CELL>::Value>::VALUE &&
+                                  IsTrue<typename SelectFixedCoordsOnlyUpdate<CELL>::Value>::VALUE;
+    };
 
     // ---- hand-written HIP kernel family implementing the cell's update
     template<typename CELL, typename HAS = void>

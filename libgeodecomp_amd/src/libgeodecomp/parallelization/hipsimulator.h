@@ -17,8 +17,12 @@
 // (libgeodecomp/hip/kernelfamilies.h), reached through the lgdhip C-ABI, or,
 // for a cell without one, the cell's own update() compiled for the GPU by
 // the generic family (libgeodecomp/hip/genericfamily.h, needs hipcc); the
-// host only holds the Grid that Writers and Steerers see.  Differences from
-// CUDASimulator, all in the user's favour:
+// host only holds the Grid that Writers and Steerers see.  A
+// struct-of-arrays cell (HasSoA + HasFixedCoordsOnlyUpdate + HasUpdateLineX,
+// registered with LIBFLATARRAY_REGISTER_SOA) runs its static updateLineX
+// over device SoA grids (libgeodecomp/hip/soafamily.h), the path the
+// reference's CUDASimulator leaves empty (cudasimulator.h:592-594).
+// Differences from CUDASimulator, all in the user's favour:
 //  * steerer edits are uploaded before the next sweep (the reference leaves
 //    them on the host, cudasimulator.h:450-465);
 //  * between two plugin callbacks run() advances all steps in one call into
@@ -44,9 +48,12 @@ namespace LibGeoDecomp {
 
 namespace HipKernels {
 
-// defined in libgeodecomp/hip/genericfamily.h (hipcc translation units)
+// defined in libgeodecomp/hip/genericfamily.h and libgeodecomp/hip/soafamily.h
+// (hipcc translation units)
 template<typename CELL>
 class Generic;
+template<typename CELL>
+class SoA;
 
 template<typename T, typename = void>
 class IsComplete : public std::false_type {};
@@ -70,7 +77,14 @@ class HipSimulator : public MonolithicSimulator<CELL_TYPE>
 public:
     typedef typename MonolithicSimulator<CELL_TYPE>::Topology Topology;
     typedef typename APITraits::SelectHipKernel<CELL_TYPE>::Value RegisteredFamily;
-    typedef typename std::conditional<std::is_void<RegisteredFamily>::value, HipKernels::Generic<CELL_TYPE>,
+    // a registered hand-written family; else the cell's own code: its static
+    // updateLineX over struct-of-arrays grids for SoA line cells, its
+    // update() (or array-of-structs updateLineX) for every other cell -- the
+    // choice of the reference's UpdateFunctor selector
+    // (storage/updatefunctor.h:96-174)
+    typedef typename std::conditional<APITraits::SelectSoAStreakUpdate<CELL_TYPE>::VALUE, HipKernels::SoA<CELL_TYPE>,
+                                      HipKernels::Generic<CELL_TYPE> >::type OwnCodeFamily;
+    typedef typename std::conditional<std::is_void<RegisteredFamily>::value, OwnCodeFamily,
                                       RegisteredFamily>::type Family;
     typedef TrackingGrid<CELL_TYPE, Topology> GridType;
     typedef typename Steerer<CELL_TYPE>::SteererFeedback SteererFeedback;
@@ -79,8 +93,9 @@ public:
 
     static_assert(HipKernels::IsComplete<Family>::value,
                   "no hand-written HIP kernel family is registered for this cell: include "
-                  "<libgeodecomp/hip/genericfamily.h> and compile with hipcc to run the cell's own update() on "
-                  "the GPU, or name a family with APITraits::HasHipKernel<...> / LIBGEODECOMP_REGISTER_HIP_KERNEL");
+                  "<libgeodecomp/hip/genericfamily.h> and compile with hipcc to run the cell's own update() or "
+                  "updateLineX() on the GPU, or name a family with APITraits::HasHipKernel<...> / "
+                  "LIBGEODECOMP_REGISTER_HIP_KERNEL");
 
     using MonolithicSimulator<CELL_TYPE>::chronometer;
     using MonolithicSimulator<CELL_TYPE>::initializer;
@@ -167,6 +182,9 @@ public:
 
     // number of device sweeps issued so far (for tests and tracing)
     unsigned long long sweepsIssued() const { return sweeps; }
+
+    // the kernel family holding the device state (for tests and tuning)
+    const Family& deviceFamily() const { return *device; }
 
     // host -> device transfers so far: whole-grid uploads, region uploads
     // after steerers, and the cells those region uploads carried

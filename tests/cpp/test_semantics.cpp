@@ -4,6 +4,7 @@
 #include "cells.h"
 #include "devicecells.h"
 #include "harness.h"
+#include "soacells.h"
 
 #include <libgeodecomp/geometry/region.h>
 #include <libgeodecomp/io/mpiioinitializer.h>
@@ -707,6 +708,162 @@ LGD_TEST(simulation_factory_mechanics)
     std::unique_ptr<Simulator<ConwayCell> > sim(factory());
     sim->run();
     CHECK_EQ(sim->getStep(), 12u);
+}
+
+// ---------------------------------------------------------------------
+// struct-of-arrays storage and the oracle's SoA / line update paths
+// ---------------------------------------------------------------------
+
+// the registration's addressing: member m of element i at
+// data + DIM_PROD * offset(m) + i * sizeof(m), offsets the summed sizes of
+// the members before (libflatarray detail/macros.hpp:99-106)
+LGD_TEST(soa_registration_member_addresses)
+{
+    typedef LibFlatArray::soa_traits<SoAMixedCell> T;
+    CHECK_EQ(T::MEMBERS, 3);
+    CHECK_EQ(T::BYTES_PER_CELL, (std::size_t)(3 * 4 + 1 + 8));
+    std::vector<std::size_t> offsets, sizes, counts;
+    T::each_member([&](std::size_t o, std::size_t s, std::size_t c) {
+        offsets.push_back(o);
+        sizes.push_back(s);
+        counts.push_back(c);
+    });
+    CHECK(offsets == std::vector<std::size_t>({0, 12, 13}));
+    CHECK(sizes == std::vector<std::size_t>({4, 1, 8}));
+    CHECK(counts == std::vector<std::size_t>({3, 1, 1}));
+    const long prod = 40;
+    std::vector<char> buf(prod * T::BYTES_PER_CELL, 0);
+    LibFlatArray::soa_accessor<SoAMixedCell> acc(buf.data(), 7, 5, 20, prod);
+    acc << SoAMixedCell(1.5f, true, 2.25);
+    float f1;
+    std::memcpy(&f1, buf.data() + prod * (0 + 1 * 4) + 7 * 4, 4);   // f[1], the second plane of f
+    CHECK_EQ(f1, 3.0f);
+    CHECK_EQ(buf[prod * 12 + 7], (char)1);                          // on
+    double d;
+    std::memcpy(&d, buf.data() + prod * 13 + 7 * 8, 8);
+    CHECK_EQ(d, 2.25);
+    const SoAMixedCell back = acc;
+    CHECK(back == SoAMixedCell(1.5f, true, 2.25));
+    // coord<X, Y, Z> moves the index by X + Y DIM_X + Z DIM_X DIM_Y
+    const long moved = acc[LibFlatArray::coord<1, -1, 1>()].index();
+    CHECK_EQ(moved, 7L + 1 - 5 + 20);
+}
+
+// edge radii only on unwrapped axes (soagrid.h:578-584); the padding holds
+// the edge cell; get / set through the topology
+LGD_TEST(soa_grid_edges_and_topology)
+{
+    typedef SoAGrid<SoAJacobiCell, Topologies::Cube<3>::Topology> CubeGrid;
+    typedef SoAGrid<SoATorusJacobiCell, Topologies::Torus<3>::Topology> TorusGrid;
+    typedef TopologiesHelpers::Topology<3, true, false, true> Slab;
+    typedef SoAGrid<SoASlabJacobiCell, Slab> SlabGrid;
+    CHECK(CubeGrid::calcEdgeRadii() == Coord<3>(1, 1, 1));
+    CHECK(TorusGrid::calcEdgeRadii() == Coord<3>(0, 0, 0));
+    CHECK(SlabGrid::calcEdgeRadii() == Coord<3>(0, 1, 0));
+    SoAGrid<SoAJacobiCell, Topologies::Cube<3>::Topology> g(CoordBox<3>(Coord<3>(), Coord<3>(4, 3, 2)),
+                                                            SoAJacobiCell(1.0), SoAJacobiCell(-5.0));
+    CHECK(g.soaLayout().dimX == 6 && g.soaLayout().dimY == 5 && g.soaLayout().dimZ == 4);
+    g.set(Coord<3>(3, 2, 1), SoAJacobiCell(7.0));
+    CHECK_EQ(g.get(Coord<3>(3, 2, 1)).temp, 7.0);
+    CHECK_EQ(g.get(Coord<3>(4, 2, 1)).temp, -5.0);
+    // the padding element right of (3, 2, 1) holds the edge cell
+    CHECK_EQ(g.accessor(g.soaLayout().index(4, 2, 1)).temp(), -5.0);
+    g.setEdge(SoAJacobiCell(9.0));
+    CHECK_EQ(g.accessor(g.soaLayout().index(-1, 0, 0)).temp(), 9.0);
+    CHECK_EQ(g.get(Coord<3>(0, 0, 0)).temp, 1.0);
+    SoAGrid<SoATorusJacobiCell, Topologies::Torus<3>::Topology> t(CoordBox<3>(Coord<3>(), Coord<3>(4, 3, 2)));
+    t.set(Coord<3>(-1, 3, 2), SoATorusJacobiCell(4.0));     // wraps to (3, 0, 0)
+    CHECK_EQ(t.get(Coord<3>(3, 0, 0)).temp, 4.0);
+}
+
+// the reference functor test's closed forms on the oracle's SoA path
+template<typename CELL>
+int oracleFunctorMismatches(const Coord<3>& dim, double edgeA, bool torus)
+{
+    SerialSimulator<CELL> sim(new EncodedInitializer<CELL>(dim, 1, edgeA));
+    sim.run();
+    const GridBase<CELL, 3> *g = sim.getGrid();
+    int bad = 0;
+    CoordBox<3> box(Coord<3>(), dim);
+    const int dx[6] = {0, 0, -1, 1, 0, 0}, dy[6] = {0, -1, 0, 0, 1, 0}, dz[6] = {-1, 0, 0, 0, 0, 1};
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+        Coord<3> sum;
+        for (int k = 0; k < 6; ++k) {
+            Coord<3> n = *i + Coord<3>(dx[k], dy[k], dz[k]);
+            if (torus) {
+                n = Topologies::Torus<3>::Topology::normalize(n, dim);
+            } else if (Topologies::Cube<3>::Topology::isOutOfBounds(n, dim)) {
+                n = Coord<3>(-1, -1, -1);
+            }
+            sum = sum + n;
+        }
+        bad += g->get(*i).valB != sum.x() + sum.y() * 1000.0 + sum.z() * 1000.0 * 1000.0;
+    }
+    return bad;
+}
+
+LGD_TEST(oracle_soa_functor_known_answers)
+{
+    CHECK_EQ(oracleFunctorMismatches<TwoDoublesTorus>(Coord<3>(200, 100, 50), -1.0, true), 0);
+    CHECK_EQ(oracleFunctorMismatches<TwoDoublesTorus>(Coord<3>(1, 10, 20), -1.0, true), 0);
+    CHECK_EQ(oracleFunctorMismatches<TwoDoublesCube>(Coord<3>(200, 100, 50), -1001001.0, false), 0);
+}
+
+// the three update paths of the oracle agree on the same arithmetic: SoA
+// streaks, array-of-structs lines and cell-wise update()
+template<typename A, typename B>
+int jacobiDifferences(const Coord<3>& dim, unsigned steps, double edge)
+{
+    SerialSimulator<A> a(new RandomCellInitializer<A, 3>(dim, steps, 9, [](double u) { return A(u); }, A(edge)));
+    SerialSimulator<B> b(new RandomCellInitializer<B, 3>(dim, steps, 9, [](double u) { return B(u); }, B(edge)));
+    a.run();
+    b.run();
+    int differ = 0;
+    CoordBox<3> box(Coord<3>(), dim);
+    for (CoordBox<3>::Iterator i = box.begin(); i != box.end(); ++i) {
+        differ += a.getGrid()->get(*i).temp != b.getGrid()->get(*i).temp;
+    }
+    return differ;
+}
+
+LGD_TEST(oracle_soa_and_line_paths_equal_cellwise_update)
+{
+    static_assert(SerialSimulator<SoAJacobiCell>::SOA && !SerialSimulator<SoAJacobiCell>::LINE, "SoA path");
+    static_assert(SerialSimulator<LineJacobiCell>::LINE && !SerialSimulator<LineJacobiCell>::SOA, "line path");
+    static_assert(!SerialSimulator<JacobiCell>::SOA && !SerialSimulator<JacobiCell>::LINE, "cell-wise path");
+    CHECK_EQ((jacobiDifferences<SoAJacobiCell, JacobiCell>(Coord<3>(23, 17, 11), 6, 0.25)), 0);
+    CHECK_EQ((jacobiDifferences<LineJacobiCell, JacobiCell>(Coord<3>(23, 17, 11), 6, 0.25)), 0);
+    CHECK_EQ((jacobiDifferences<SoATorusJacobiCell, TorusJacobiCell>(Coord<3>(9, 7, 5), 5, 0.0)), 0);
+    CHECK_EQ((jacobiDifferences<SoASlabJacobiCell, SlabJacobiCell>(Coord<3>(9, 7, 5), 5, 0.5)), 0);
+    // one-cell axes: the first and last cell are the same
+    CHECK_EQ((jacobiDifferences<SoATorusJacobiCell, TorusJacobiCell>(Coord<3>(1, 3, 1), 4, 0.0)), 0);
+    CHECK_EQ((jacobiDifferences<SoAJacobiCell, JacobiCell>(Coord<3>(1, 1, 1), 3, 0.5)), 0);
+}
+
+// SoA probe cells pass their own checks on the oracle, and a probe built
+// to assume the wrong topology fails them (the checks see the wrap)
+template<typename CELL>
+int oracleProbeInvalid(const Coord<CELL::API::Topology::DIM>& dim, unsigned steps)
+{
+    SerialSimulator<CELL> sim(new ProbeInitializer<CELL>(dim, steps, 3));
+    sim.run();
+    int invalid = 0;
+    CoordBox<CELL::API::Topology::DIM> box(Coord<CELL::API::Topology::DIM>(), dim);
+    for (typename CoordBox<CELL::API::Topology::DIM>::Iterator i = box.begin(); i != box.end(); ++i) {
+        const CELL c = sim.getGrid()->get(*i);
+        invalid += !c.valid || c.cycle != steps * APITraits::SelectNanoSteps<CELL>::VALUE;
+    }
+    return invalid;
+}
+
+LGD_TEST(oracle_soa_probe_cells)
+{
+    CHECK_EQ(oracleProbeInvalid<SoAProbe3Cube>(Coord<3>(9, 7, 5), 3), 0);
+    CHECK_EQ(oracleProbeInvalid<SoAProbe3Torus>(Coord<3>(9, 7, 5), 3), 0);
+    CHECK_EQ(oracleProbeInvalid<SoAProbe3Tube>(Coord<3>(9, 7, 5), 3), 0);
+    CHECK_EQ(oracleProbeInvalid<SoAProbe2Torus>(Coord<2>(11, 6), 4), 0);
+    CHECK_EQ(oracleProbeInvalid<SoAProbe1Ring>(Coord<1>(13), 4), 0);
+    CHECK_EQ(oracleProbeInvalid<SoAProbe3Torus>(Coord<3>(1, 2, 1), 2), 0);
 }
 
 int main(int argc, char **argv)

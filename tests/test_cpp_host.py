@@ -6,6 +6,9 @@ test_hipsimulator (GPU): HipSimulator against the oracle SerialSimulator on
 the same user cells and plugins.
 test_generic (GPU): cells without a hand-written kernel family, their own
 update() compiled by hipcc (the generic kernel family), against the oracle.
+test_soa (GPU): struct-of-arrays cells and their static updateLineX (the
+SoA kernel family) and array-of-structs line cells, against the oracle and
+the reference's closed forms.
 """
 import subprocess
 from pathlib import Path
@@ -38,3 +41,8 @@ def test_cpp_hipsimulator(tmp_path):
 @pytest.mark.gpu
 def test_cpp_generic(tmp_path):
     run("test_generic", 600, tmp_path)
+
+
+@pytest.mark.gpu
+def test_cpp_soa(tmp_path):
+    run("test_soa", 600, tmp_path)
