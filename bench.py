@@ -54,24 +54,122 @@ def timed(stream, fn, reps=1):
     return e0.elapsed_ms(e1)
 
 
-def load_traffic(key: str):
-    """HBM bytes per launch of a dominant kernel on one workload shape
-    ("kernel@shape") from the committed rocprof PMC summaries
-    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE scaled by the copy
-    calibration), or None when that shape was not profiled."""
-    p = REPO / "profiles" / "traffic.json"
+# MI355X published peaks: HBM3E 8 TB/s; VALU issue = 256 CUs x 4 SIMDs, each
+# retiring one wave64 instruction per 4 cycles (16 lanes), at the 2.4 GHz
+# peak engine clock
+VALU_PEAK_GINST = 256 * 4 / 4 * 2.4   # 614.4 G wave64 instructions per second
+MFMA_BF16_PEAK_TFLOPS = 2500.0         # dense BF16 matrix cores
+
+
+def load_counters(key: str):
+    """Counter figures per launch of a dominant kernel on one workload shape
+    ("kernel@shape") from the committed rocprofv3 PMC summaries
+    (profiles/kernel_counters.json): HBM bytes (request-size-weighted
+    read/write requests at the memory controllers) and wave64 VALU
+    instructions (SQ_INSTS_VALU), or {} when that shape was not profiled."""
+    p = REPO / "profiles" / "kernel_counters.json"
     if not p.exists():
-        return None
+        return {}
     try:
-        entry = json.loads(p.read_text()).get(key)
+        return json.loads(p.read_text()).get(key) or {}
     except ValueError:
-        return None
-    return entry.get("hbm_bytes_per_launch") if entry else None
+        return {}
+
+
+def roofline(name, shape, ms, alg_bytes, sweeps=1, min_bytes=None):
+    """The dominant kernel's roofline from its live average launch time `ms`.
+
+    `alg_bytes` are the algorithmic bytes of the `sweeps` sweeps (or
+    generations, or time steps) one launch advances.  A single-sweep kernel
+    is held to HBM with achieved = alg_bytes / time.  A temporally blocked
+    launch must move the state through HBM only once, in the kernel's own
+    storage format (`min_bytes`, default alg_bytes / sweeps), and its sweeps
+    are VALU work on chip: both fractions are reported, and the roofline is
+    the one the kernel is closer to (the binding one).  The per-sweep rate
+    alg_bytes / time goes into `effective_bw` only."""
+    t = ms * 1e-3
+    c = load_counters(f"{name}@{shape}")
+    if min_bytes is None:
+        min_bytes = alg_bytes / sweeps
+    hbm_alg = min_bytes / t / 1e9
+    hbm = {"achieved": round(hbm_alg, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(hbm_alg / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_launch": min_bytes}
+    traffic = c.get("hbm_bytes_per_launch")
+    if traffic:
+        hbm["traffic_GBs"] = round(traffic / t / 1e9, 2)
+    out = {"kernel": name, "bound": "hbm", **hbm, "traffic": traffic, "avg_launch_us": round(ms * 1e3, 3),
+           "sweeps_per_launch": sweeps, "hbm": dict(hbm)}
+    valu = c.get("valu_insts_per_launch")
+    if valu:
+        rate = valu / t / 1e9
+        out["valu"] = {"achieved": round(rate, 2), "peak": VALU_PEAK_GINST, "unit": "Ginst/s",
+                       "frac": round(rate / VALU_PEAK_GINST, 4), "insts_per_launch": valu}
+        if out["valu"]["frac"] > out["frac"]:
+            out.update(bound="valu", achieved=out["valu"]["achieved"], peak=VALU_PEAK_GINST, unit="Ginst/s",
+                       frac=out["valu"]["frac"])
+    if sweeps > 1:
+        out["effective_bw"] = {"value": round(alg_bytes / t / 1e9, 2), "unit": "GB/s",
+                               "meaning": f"{sweeps} sweeps' algorithmic bytes per launch / launch time; "
+                                          "above HBM because the intermediate sweeps stay on chip"}
+    if c.get("summary"):
+        out["profile"] = "profiles/" + c["summary"]
+    if out["frac"] > 1.0:
+        # a working set held in the 256 MB Infinity Cache can be served
+        # above the HBM peak; say so instead of reporting a fraction > 1
+        out["error"] = "frac > 1: the launch is served from the Infinity Cache or its counter figures are stale"
+        log(f"roofline of {name}@{shape}: {out['error']}")
+    return out
 
 
 def cpu_threads() -> int:
-    n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    """Host threads for the OpenMP baseline: the CPU share of this process
+    (OMP_NUM_THREADS when the environment sets one, else every core in the
+    affinity mask)."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return max(1, len(os.sched_getaffinity(0)))
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_rates(run, units0, budget_s, max_units, unit_lups):
+    """Times `run(threads, units)` (the oracle over `units` steps) serially
+    and on cpu_threads() threads, each sized to about `budget_s` seconds
+    from a probe of `units0` steps.  Returns {threads: (GLUPS, units, s)}."""
+    out = {}
+    for th in (1, cpu_threads()):
+        if th in out:
+            continue
+        t0 = time.perf_counter()
+        run(th, units0)
+        dt = time.perf_counter() - t0
+        units = max(units0, min(max_units, int(units0 * budget_s / max(dt, 1e-3))))
+        t0 = time.perf_counter()
+        run(th, units)
+        dt = time.perf_counter() - t0
+        out[th] = (1e-9 * unit_lups * units / dt, units, dt)
+    return out
+
+
+def cpu_entry(rates, what):
+    """The cpu_baseline object: the OpenMP rate as value, the serial one
+    beside it, host facts stated."""
+    th = max(rates)
+    v, units, dt = rates[th]
+    sv, sunits, sdt = rates[1]
+    return {"value": round(v, 4), "unit": "GLUPS", "cores": th, "kind": "port",
+            "sample": f"{what}: OpenMP {th} threads {units} steps in {dt:.1f}s; serial {sunits} steps in {sdt:.1f}s",
+            "serial": {"value": round(sv, 4), "unit": "GLUPS", "cores": 1},
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
 
 
 def load_oracle():
@@ -124,7 +222,8 @@ class Life:
                     self.ws_bytes, C.byref(self.tile), s)
 
     def dominant_kernel(self):
-        """(kernel name, avg ms per launch, algorithmic bytes per launch)."""
+        """The roofline of the generation kernel, timed over a block of
+        launches of `per_launch` generations each on the packed state."""
         s = self.stream.h
         half = self.ws_bytes // 2
         pa, pb = self.ws.value, self.ws.value + half
@@ -138,26 +237,16 @@ class Life:
         per_launch_bytes = self.bytes_per_lup * self.n * self.n * self.per_launch
         # variant 0 is the library default (a stacked-wave kernel)
         name = "life_packed_tb" if 1 <= self.tile.variant <= 4 else "life_packed_stacked"
-        self.shape = f"{self.n}x{self.n},{self.per_launch}gen"
-        return name, ms / launches, per_launch_bytes
+        # one launch reads and writes the bit-packed state once (1 bit per cell)
+        packed = 2.0 * self.n * self.n / 8
+        return roofline(name, f"{self.n}x{self.n},{self.per_launch}gen", ms / launches, per_launch_bytes,
+                        self.per_launch, packed)
 
-    def cpu_baseline(self, budget_s=15.0):
+    def cpu_baseline(self, budget_s=10.0):
         oracle = load_oracle()
-        th = cpu_threads()
         sample = self.host[:4096, :4096].copy()
-        gens = 8
-        t0 = time.perf_counter()
-        oracle.life2d(sample, gens, 0, th)
-        dt = time.perf_counter() - t0
-        # scale the sample to roughly the time budget
-        gens = max(8, int(gens * budget_s / max(dt, 1e-3)))
-        gens = min(gens, 2000)
-        t0 = time.perf_counter()
-        oracle.life2d(sample, gens, 0, th)
-        dt = time.perf_counter() - t0
-        return {"value": 1e-9 * sample.size * gens / dt, "unit": "GLUPS", "cores": th, "kind": "port",
-                "sample": f"C oracle (OpenMP, {th} threads) on the top-left 4096x4096 of the same input, "
-                          f"{gens} generations, {dt:.1f}s"}
+        rates = cpu_rates(lambda th, k: oracle.life2d(sample, k, 0, th), 4, budget_s, 4000, sample.size)
+        return cpu_entry(rates, "C oracle (byte-per-cell rule) on the top-left 4096x4096 of the same input")
 
 
 # ----------------------------------------------------------------------
@@ -216,24 +305,17 @@ class Jacobi:
             name = "jacobi3d_f64_tb2" if variant in (9, 10) else "jacobi3d_f64_tb"
         else:
             name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
-        # one launch advances k sweeps: algorithmic bytes are k sweeps' worth
-        self.shape = f"{self.n}^3,K{k}" if k > 1 else f"{self.n}^3"
         self.plan = {"variant": variant, "zchunk": zc.value}
-        return name, ms / (sweeps // k), k * self.bytes_per_lup * float(self.n) ** 3
+        # one launch advances k sweeps: algorithmic bytes are k sweeps' worth
+        shape = f"{self.n}^3,K{k}" if k > 1 else f"{self.n}^3"
+        return roofline(name, shape, ms / (sweeps // k), k * self.bytes_per_lup * float(self.n) ** 3, k)
 
-    def cpu_baseline(self, budget_s=15.0):
+    def cpu_baseline(self, budget_s=10.0):
         oracle = load_oracle()
-        th = cpu_threads()
-        steps = 64  # a probe long enough that thread start-up does not dominate the rate
-        t0 = time.perf_counter()
-        oracle.jacobi3d(self.host, steps, 0.0, th)
-        dt = time.perf_counter() - t0
-        steps = max(4, min(200000, int(steps * budget_s / max(dt, 1e-3))))
-        t0 = time.perf_counter()
-        oracle.jacobi3d(self.host, steps, 0.0, th)
-        dt = time.perf_counter() - t0
-        return {"value": 1e-9 * self.host.size * steps / dt, "unit": "GLUPS", "cores": th, "kind": "port",
-                "sample": f"C oracle (OpenMP, {th} threads), same {self.n}^3 input, {steps} steps, {dt:.1f}s"}
+        probe = max(1, int(16 * (128.0 / self.n) ** 3))  # about 1e-1 s serial
+        rates = cpu_rates(lambda th, k: oracle.jacobi3d(self.host, k, 0.0, th), probe, budget_s, 200000,
+                          self.host.size)
+        return cpu_entry(rates, f"C oracle, same {self.n}^3 input")
 
 
 # ----------------------------------------------------------------------
@@ -291,28 +373,20 @@ class LBM:
         ms = timed(self.stream, lambda: lgdhip.call("lgdhip_lbm_d3q19_f32_run", self.a.ptr, self.b.ptr,
                                                     self.flags.ptr, C.byref(self.g), C.byref(self.p),
                                                     launches * per, C.byref(self.tile), None, self.stream.h))
-        self.shape = f"{self.n}^3" + (",zslabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "") + \
+        shape = f"{self.n}^3" + (",zslabs" if self.g.layout == lgdhip.LAYOUT_Z_SLABS else "") + \
             (",2steps" if per == 2 else "")
         name = "lbm_d3q19_pull2" if per == 2 else "lbm_d3q19_pull"
-        return name, ms / launches, per * self.bytes_per_lup * float(self.n) ** 3
+        return roofline(name, shape, ms / launches, per * self.bytes_per_lup * float(self.n) ** 3, per)
 
-    def cpu_baseline(self, budget_s=15.0):
+    def cpu_baseline(self, budget_s=10.0):
         oracle = load_oracle()
-        th = cpu_threads()
         m = 64
         ux, uy, uz = synthetic.lbm_initial_velocity(m, m, m, 1e-3, self.seed)
         f0 = oracle.lbm_equilibrium(np.ones_like(ux), ux, uy, uz)
         fl = oracle.lbm_cavity_flags(m, m, m)
-        steps = 2
-        t0 = time.perf_counter()
-        oracle.lbm_run(f0, fl, steps, 1.7, (0.05, 0.0, 0.0), th)
-        dt = time.perf_counter() - t0
-        steps = max(2, min(2000, int(steps * budget_s / max(dt, 1e-3))))
-        t0 = time.perf_counter()
-        oracle.lbm_run(f0, fl, steps, 1.7, (0.05, 0.0, 0.0), th)
-        dt = time.perf_counter() - t0
-        return {"value": 1e-9 * m ** 3 * steps / dt, "unit": "GLUPS", "cores": th, "kind": "port",
-                "sample": f"C oracle FP64 (OpenMP, {th} threads), 64^3 cavity, {steps} steps, {dt:.1f}s"}
+        rates = cpu_rates(lambda th, k: oracle.lbm_run(f0, fl, k, 1.7, (0.05, 0.0, 0.0), th), 2, budget_s, 2000,
+                          m ** 3)
+        return cpu_entry(rates, "C oracle FP64 D3Q19, 64^3 cavity")
 
 
 def equilibrium_f32(rho, jx, jy, jz):
@@ -331,22 +405,20 @@ def equilibrium_f32(rho, jx, jy, jz):
     return f
 
 
-def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, wl, shape, variant=0):
-    """(name, ms per launch, algorithmic bytes per launch) of the Poisson
-    solver's sweep kernel: temporal blocking, 4 sweeps per launch (12 B per
-    sweep-LUP, so 4 x 12 B x cells per launch); variant 0 is the library
-    default (6, three rows per thread, on large launches, else 3, one row)."""
+def poisson_sweep_kernel(stream, e0, e1, f, probs, count, ws, ws_bytes, cells, shape, variant=0):
+    """The roofline of the Poisson solver's sweep kernel: temporal blocking,
+    4 sweeps per launch (12 B per sweep-LUP, so 4 x 12 B x cells per launch);
+    variant 0 is the library default (6, three rows per thread, on large
+    launches, else 3, one row).  Returns (roofline, planned variant)."""
     k, sweeps = 4, 48
     in_b = C.c_int(0)
     tile = lgdhip.tile(variant=variant)
     ms = timed(stream, lambda: lgdhip.call("lgdhip_poisson_sweeps_f32", e0.ptr, e1.ptr, f.ptr, probs, count, sweeps,
                                            C.byref(tile), ws.ptr, ws_bytes, C.byref(in_b), stream.h))
-    wl.shape = shape
     planned = C.c_int(0)  # the library resolves the default by launch size
     lgdhip.call("lgdhip_poisson_sweep_plan", probs, count, variant, C.byref(planned))
-    wl.sweep_plan = planned.value
-    return ("poisson_sweep_tb2" if planned.value in (4, 5, 6) else "poisson_sweep_tb"), ms / (sweeps // k), \
-        k * 12.0 * cells
+    name = "poisson_sweep_tb2" if planned.value in (4, 5, 6) else "poisson_sweep_tb"
+    return roofline(name, shape, ms / (sweeps // k), k * 12.0 * cells, k), planned.value
 
 
 # ----------------------------------------------------------------------
@@ -446,8 +518,10 @@ class Poisson:
         e1 = lgdhip.DeviceBuffer(self.elems * 4)
         for b in (e0, e1):
             lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.elems * 4, self.stream.h)
-        return poisson_sweep_kernel(self.stream, e0, e1, self.f, self.probs, self.count, self.ws, self.ws_bytes,
-                                    self.cells, self, f"{self.count}problems", self.variant)
+        roof, self.sweep_plan = poisson_sweep_kernel(self.stream, e0, e1, self.f, self.probs, self.count, self.ws,
+                                                     self.ws_bytes, self.cells, f"{self.count}problems",
+                                                     self.variant)
+        return roof
 
     def cpu_baseline(self, budget_s=15.0):
         oracle = load_oracle()
@@ -463,8 +537,9 @@ class Poisson:
         oracle.poisson_jacobi(u.copy(), f, 1e-300, max_sweeps=sweeps, check=sweeps)
         dt = time.perf_counter() - t0
         return {"value": 1e-9 * m ** 3 * sweeps / dt, "unit": "GLUPS", "cores": 1, "kind": "port",
-                "sample": f"C oracle plain FP64 Jacobi (1 thread), one {m}^3 problem of the ensemble, {sweeps} "
-                          f"sweeps, {dt:.1f}s"}
+                "sample": f"C oracle plain FP64 Jacobi (1 thread; the oracle's Poisson sweep is serial), one {m}^3 "
+                          f"problem of the ensemble, {sweeps} sweeps, {dt:.1f}s", "cpu_model": cpu_model(),
+                "host_cpus": os.cpu_count()}
 
 
 # ----------------------------------------------------------------------
@@ -532,14 +607,15 @@ class Learned:
         return 0.0
 
     def dominant_kernel(self):
-        """The polishing sweep over the held-out set (poisson_sweep_f32)."""
+        """The polishing sweep over the held-out set (the blocked sweep)."""
         e0 = lgdhip.DeviceBuffer(self.eval.elems * 4)
         e1 = lgdhip.DeviceBuffer(self.eval.elems * 4)
         for b in (e0, e1):
             lgdhip.call("lgdhip_fill_u8", b.ptr, 0, self.eval.elems * 4, self.stream.h)
-        return poisson_sweep_kernel(self.stream, e0, e1, self.eval.f, self.eval.probs, self.eval.count, self.eval.ws,
-                                    self.eval.ws_bytes, self.neval * float(self.n) ** 3, self,
-                                    f"{self.neval}x{self.n}^3")
+        roof, self.sweep_plan = poisson_sweep_kernel(self.stream, e0, e1, self.eval.f, self.eval.probs,
+                                                     self.eval.count, self.eval.ws, self.eval.ws_bytes,
+                                                     self.neval * float(self.n) ** 3, f"{self.neval}x{self.n}^3")
+        return roof
 
     def cpu_baseline(self, budget_s=15.0):
         """The oracle's plain FP64 Jacobi on one held-out problem (sweep rate)."""
@@ -557,7 +633,74 @@ class Learned:
         dt = time.perf_counter() - t0
         return {"value": 1e-9 * self.n ** 3 * sweeps / dt, "unit": "GLUPS", "cores": 1, "kind": "port",
                 "sample": f"C oracle plain FP64 Jacobi (1 thread), one held-out {self.n}^3 problem, {sweeps} sweeps, "
-                          f"{dt:.1f}s (sweep rate; the zero-guess solve needs ~3e4 sweeps per problem)"}
+                          f"{dt:.1f}s (sweep rate; the zero-guess solve needs ~3e4 sweeps per problem)",
+                "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+class UNet:
+    """The learned guess's network alone: 3D U-Net 16-32-64-128 on one 128^3
+    sample (3 input channels: f, b and the coarse base), BF16 operands on
+    the MFMA cores with FP32 accumulation, random weights and inputs.  One
+    step = one training step on one sample (forward, MSE loss, backward,
+    Adam); the forward alone is timed beside it.  FLOPs count the
+    convolutions only (2 x 27 x cin x cout per voxel; backward = 2 x
+    forward)."""
+    name = "unet"
+    dtype = "bf16"
+
+    def __init__(self, n=128, seed=synthetic.DEFAULT_SEED, **_):
+        from libgeodecomp_amd.unet import UNet3D
+        self.n = n
+        self.stream = lgdhip.Stream()
+        s = self.stream.h
+        self.net = UNet3D(n=n, in_ch=3, seed=seed % 1000, stream=s, bias=False)
+        rng = np.random.default_rng(seed)
+        self.fields = []
+        for _ in range(3):
+            b = lgdhip.DeviceBuffer(n ** 3 * 4)
+            b.upload(rng.standard_normal(n ** 3).astype(np.float32), s)
+            self.fields.append(b)
+        self.target = lgdhip.DeviceBuffer(n ** 3 * 4)
+        self.target.upload(rng.standard_normal(n ** 3).astype(np.float32), s)
+        self.net.load_input(self.fields)
+        self.fwd_flops = self.net.flops_forward()
+        self.fwd_ms = None
+
+    def config(self):
+        return {"workload": f"3D U-Net 16-32-64-128 training step on one {self.n}^3 sample (3 input channels), "
+                            "BF16 MFMA convolutions, FP32 accumulation", "grid": [self.n] * 3,
+                "forward_gflop": round(self.fwd_flops / 1e9, 1), "forward_ms": self.fwd_ms,
+                "forward_tflops": round(self.fwd_flops / self.fwd_ms * 1e-9, 1) if self.fwd_ms else None,
+                "parallelism": "single GPU"}
+
+    def step(self):
+        self.net.forward()
+        self.net.loss(self.target, read=False)
+        self.net.backward()
+        self.net.adam(lr=1e-6)
+
+    def value(self):
+        """(TFLOP/s of the training step, unit, ms per step) -- set by
+        measure() through lups_per_step()."""
+        return self._value
+
+    def lups_per_step(self):
+        return 0.0
+
+    def dominant_kernel(self):
+        """The forward pass as one unit (its 15 convolutions, 3 poolings, 3
+        upsamplings): BF16 MFMA roofline over the convolution FLOPs."""
+        reps = 10
+        ms = timed(self.stream, self.net.forward, reps=reps) / reps
+        self.fwd_ms = round(ms, 4)
+        achieved = self.fwd_flops / (ms * 1e-3) / 1e12
+        return {"kernel": "unet3d_forward (conv3d_fwd_mfma and friends)", "bound": "mfma",
+                "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
+                "avg_launch_us": round(ms * 1e3, 3), "algorithmic_flops_per_launch": self.fwd_flops}
+
+    def cpu_baseline(self, budget_s=0.0):
+        return None
 
 
 class Copy:
@@ -587,14 +730,14 @@ class Copy:
     def dominant_kernel(self):
         launches = 10
         ms = timed(self.stream, self.step, reps=launches)
-        self.shape = f"{self.n}B"
-        return "lgd_copy16_kernel", ms / launches, 2.0 * self.n
+        return roofline("lgd_copy16_kernel", f"{self.n}B", ms / launches, 2.0 * self.n)
 
     def cpu_baseline(self, budget_s=0.0):
         return None
 
 
-WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "poisson": Poisson, "learned": Learned, "copy": Copy}
+WORKLOADS = {"life": Life, "jacobi": Jacobi, "lbm": LBM, "poisson": Poisson, "learned": Learned, "copy": Copy,
+             "unet": UNet}
 
 
 def main():
@@ -602,7 +745,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="life")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default=None,
+                    help="one workload (default: life, plus the secondary Jacobi and LBM entries)")
     ap.add_argument("--size", type=int, default=0, help="override the grid edge length")
     ap.add_argument("--iters", type=int, default=0, help="override generations / steps per run")
     ap.add_argument("--per-launch", type=int, default=31, help="life: generations per launch")
@@ -618,7 +762,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true", help="poisson: plain launches instead of hipGraph chunks")
     ap.add_argument("--tol", type=float, default=1e-6, help="poisson: relative residual to solve to")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true", help="default run: the main workload only")
     args = ap.parse_args()
+    default_run = args.workload is None and not (args.size or args.iters or args.variant >= 0)
+    if args.workload is None:
+        args.workload = "life"
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the library has no multi-GPU path; the kernels shard as "
@@ -651,56 +799,79 @@ def main():
     if args.workload == "poisson":
         kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol, variant=max(0, args.variant))
     wl = WORKLOADS[args.workload](**kw)
-
-    for i in range(args.warmup):
-        wl.step()
-    wl.stream.sync()
-    log(f"warmup {args.warmup} done")
-
-    lgdhip.call("lgdhip_device_synchronize")
-    t0 = time.perf_counter()
-    ms = timed(wl.stream, wl.step, reps=args.steps)
-    lgdhip.call("lgdhip_device_synchronize")
-    wall = time.perf_counter() - t0
-    ms_per_step = ms / args.steps
-    unit = "GLUPS"
-    if hasattr(wl, "value"):
-        value, unit, ms_per_step = wl.value()
-    else:
-        value = 1e-9 * wl.lups_per_step() / (ms_per_step * 1e-3)
-    log(f"timed {args.steps} steps: {ms:.3f} ms (wall {wall * 1e3:.1f} ms), {value:.3f} {unit}")
-
-    kname, k_ms, k_bytes = wl.dominant_kernel()
-    achieved = k_bytes / (k_ms * 1e-3) / 1e9
-    log(f"dominant kernel {kname}: {k_ms * 1e3:.2f} us/launch, {achieved:.1f} GB/s algorithmic")
-    roofline = {"kernel": kname, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(f"{kname}@{getattr(wl, 'shape', '')}"),
-                "avg_launch_us": round(k_ms * 1e3, 3), "algorithmic_bytes_per_launch": k_bytes}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        cpu = wl.cpu_baseline()
-        log(f"cpu baseline: {cpu}")
+    main_entry = measure(wl, args.steps, args.warmup, not args.no_cpu_baseline, 10.0)
+    secondary = []
+    if default_run and not args.no_secondary:
+        # the other sweep configurations, each timed the same way on its own
+        # shape, so the line carries their GLUPS and rooflines too
+        for sub, skw in SECONDARY:
+            w2 = WORKLOADS[sub](**skw)
+            e = measure(w2, max(args.steps, 2), max(args.warmup, 1), not args.no_cpu_baseline, 5.0)
+            secondary.append({"workload": e["config"]["workload"], "value": e["value"], "unit": e["unit"],
+                              "ms_per_step": e["ms_per_step"], "steps": e["steps"], "dtype": w2.dtype,
+                              "config": e["config"], "roofline": e["roofline"], "cpu_baseline": e["cpu_baseline"]})
+            del w2
+            lgdhip.call("lgdhip_device_synchronize")
 
     line = {
         "metric": METRIC,
-        "value": round(value, 3),
-        "unit": unit,
+        "value": main_entry["value"],
+        "unit": main_entry["unit"],
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": main_entry["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": wl.dtype,
         "data": "synthetic (SplitMix64 seeded initializer)",
-        "config": wl.config(),
-        "roofline": roofline,
-        "cpu_baseline": cpu,
+        "config": main_entry["config"],
+        "roofline": main_entry["roofline"],
+        "cpu_baseline": main_entry["cpu_baseline"],
     }
+    if secondary:
+        line["secondary"] = secondary
     print(json.dumps(line), flush=True)
+
+
+# the secondary entries of the default run: Jacobi3D FP64 128^3 x 100 (the
+# configured shape, Infinity-Cache resident), Jacobi3D 512^3 x 100 (the
+# HBM-bound shape), D3Q19 LBM FP32 256^3 x 2000
+SECONDARY = [("jacobi", dict(n=128, steps=100)), ("jacobi", dict(n=512, steps=100)),
+             ("lbm", dict(n=256, steps=2000))]
+
+
+def measure(wl, steps, warmup, cpu, cpu_budget):
+    """Warm-up, `steps` timed steps, the dominant kernel's roofline and the
+    CPU baseline of one workload."""
+    for _ in range(warmup):
+        wl.step()
+    wl.stream.sync()
+    log(f"{wl.name}: warmup {warmup} done")
+    lgdhip.call("lgdhip_device_synchronize")
+    t0 = time.perf_counter()
+    ms = timed(wl.stream, wl.step, reps=steps)
+    lgdhip.call("lgdhip_device_synchronize")
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / steps
+    unit = "GLUPS"
+    if isinstance(wl, UNet):
+        wl._value = (3.0 * wl.fwd_flops / (ms_per_step * 1e-3) / 1e12, "TFLOP/s (training step)", ms_per_step)
+    if hasattr(wl, "value"):
+        value, unit, ms_per_step = wl.value()
+    else:
+        value = 1e-9 * wl.lups_per_step() / (ms_per_step * 1e-3)
+    log(f"{wl.name}: timed {steps} steps: {ms:.3f} ms (wall {wall * 1e3:.1f} ms), {value:.3f} {unit}")
+    roof = wl.dominant_kernel()
+    log(f"{wl.name}: dominant kernel {roof['kernel']}: {roof['avg_launch_us']:.2f} us/launch, bound {roof['bound']}, "
+        f"{roof['achieved']} {roof['unit']}, frac {roof['frac']}")
+    base = None
+    if cpu:
+        base = wl.cpu_baseline(cpu_budget)
+        log(f"{wl.name}: cpu baseline: {base}")
+    return {"value": round(value, 3), "unit": unit, "ms_per_step": round(ms_per_step, 4), "steps": steps,
+            "config": wl.config(), "roofline": roof, "cpu_baseline": base}
 
 
 if __name__ == "__main__":
