@@ -127,6 +127,11 @@ int lgdhip_host_free(void *ptr);
 int lgdhip_memcpy_h2d(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
 int lgdhip_memcpy_d2h(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
 int lgdhip_memcpy_d2d(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream);
+/* `height` rows of `width` bytes, device rows `spitch` bytes apart, host rows
+ * `dpitch` apart; synchronous like lgdhip_memcpy_d2h (one transfer for, e.g.,
+ * one x-row of every member plane). */
+int lgdhip_memcpy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                        lgdhip_stream_t stream);
 /* byte-pattern fill by a kernel (safe inside graph capture) */
 int lgdhip_fill_u8(void *dst, uint8_t value, size_t bytes, lgdhip_stream_t stream);
 /* streaming copy kernel (roofline / PMC calibration); bytes % 16 == 0 */

@@ -987,7 +987,15 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             }
         }
     }
-    hipGraphExec_t graph = nullptr;
+    // the chunk graph, destroyed on every return path
+    struct GraphExec {
+        hipGraphExec_t h = nullptr;
+        void reset() {
+            if (h) hipGraphExecDestroy(h);
+            h = nullptr;
+        }
+        ~GraphExec() { reset(); }
+    } graph;
     bool graph_valid = false;
     while (true) {
         active.clear();
@@ -1029,18 +1037,20 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             };
             if (graphs) {
                 if (!graph_valid) {
-                    if (graph) hipGraphExecDestroy(graph);
-                    graph = nullptr;
+                    graph.reset();
                     LGD_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                     enqueue(ecur, enxt);
                     hipGraph_t g = nullptr;
                     LGD_HIP(hipStreamEndCapture(s, &g));
-                    hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+                    hipError_t e = hipGraphInstantiate(&graph.h, g, nullptr, nullptr, 0);
                     hipGraphDestroy(g);
-                    if (e != hipSuccess) return lgdhip::hip_fail(e, "hipGraphInstantiate");
+                    if (e != hipSuccess) {
+                        graph.h = nullptr;
+                        return lgdhip::hip_fail(e, "hipGraphInstantiate");
+                    }
                     graph_valid = true;
                 }
-                LGD_HIP(hipGraphLaunch(graph, s));
+                LGD_HIP(hipGraphLaunch(graph.h, s));
             } else {
                 enqueue(ecur, enxt);
                 LGD_LAUNCHED("poisson_sweep_f32");
@@ -1085,7 +1095,7 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             graph_valid = false;
         }
     }
-    if (graph) hipGraphExecDestroy(graph);
+    graph.reset();
     for (int k = 0; k < count; ++k) {
         results[k].sweeps = st[k].sweeps;
         results[k].outer_iterations = st[k].outer_iters;

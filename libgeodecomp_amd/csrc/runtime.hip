@@ -228,6 +228,15 @@ int lgdhip_memcpy_d2h(void *dst, const void *src, size_t bytes, lgdhip_stream_t 
     return 0;
 }
 
+int lgdhip_memcpy2d_d2h(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height,
+                        lgdhip_stream_t stream) {
+    if (width == 0 || height == 0) return 0;
+    LGD_REQUIRE(dst && src && dpitch >= width && spitch >= width, "NULL pointer or pitch below the row width");
+    LGD_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDeviceToHost, as_stream(stream)));
+    LGD_HIP(hipStreamSynchronize(as_stream(stream)));
+    return 0;
+}
+
 int lgdhip_memcpy_d2d(void *dst, const void *src, size_t bytes, lgdhip_stream_t stream) {
     if (bytes == 0) return 0;
     LGD_REQUIRE(dst && src, "NULL pointer");

@@ -283,6 +283,23 @@ public:
         check(lgdhip_cell_grid_store(a->get(), &g, host->data(), stream.get()), "lgdhip_cell_grid_store");
     }
 
+    // one contiguous copy per streak straight into the host cells
+    void downloadRegion(Grid<CELL, Topology> *host, const Region<DIM>& region)
+    {
+        requireBox(*host);
+        const std::vector<Streak<DIM> > streaks = relativeStreaks(region, host->boundingBox());
+        const long hy = DIM >= 2 ? RADIUS : 0, hz = DIM == 3 ? RADIUS : 0;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<DIM>& st = streaks[k];
+            const long y = coordOf(st.origin, 1), z = coordOf(st.origin, 2);
+            const long i = ((z + hz) * g.py + (y + hy)) * g.px + (st.origin.x() + RADIUS);
+            CELL *dst = host->data() + st.origin.toIndex(host->boundingBox().dimensions);
+            check(lgdhip_memcpy_d2h(dst, static_cast<const CELL *>(a->get()) + i,
+                                    sizeof(CELL) * (std::size_t)(st.endX - st.origin.x()), stream.get()),
+                  "lgdhip_memcpy_d2h");
+        }
+    }
+
     // nanoSteps is a multiple of NANO_STEPS (whole steps).  A period of
     // lcm(2, NANO_STEPS) nano steps returns the buffers to their places and
     // is replayed as one captured graph; the rest is launched directly.

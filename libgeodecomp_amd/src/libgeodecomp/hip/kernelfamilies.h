@@ -7,6 +7,9 @@
 //     uploadRegion(hostGrid, region)    only the region's cells (optional;
 //                                       HipSimulator uses it after steerers)
 //     download(&grid)                   device state -> host cells
+//     downloadRegion(&grid, region)     only the region's cells (optional;
+//                                       HipSimulator uses it for the rows
+//                                       its Writers read)
 //     run(nanoSteps)                    advance the device state
 // A cell opts in with APITraits::HasHipKernel<FAMILY> or, after its
 // definition, LIBGEODECOMP_REGISTER_HIP_KERNEL(CELL, FAMILY).  The family's
@@ -321,6 +324,26 @@ public:
         }
     }
 
+    // one contiguous copy per streak
+    void downloadRegion(Grid<CELL, Topology> *host, const Region<3>& region)
+    {
+        const Coord<3> dim = host->boundingBox().dimensions;
+        const std::vector<Streak<3> > streaks = relativeStreaks(region, host->boundingBox());
+        std::vector<double> line;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<3>& st = streaks[k];
+            line.resize(st.endX - st.origin.x());
+            const double *src = (const double *)a->get() + elementIndex(g, 0, st.origin.x(), st.origin.y(),
+                                                                           st.origin.z());
+            check(lgdhip_memcpy_d2h(line.data(), src, line.size() * sizeof(double), stream.get()),
+                  "lgdhip_memcpy_d2h");
+            CELL *cells = host->data() + st.origin.toIndex(dim);
+            for (std::size_t i = 0; i < line.size(); ++i) {
+                cells[i].*MEMBER = line[i];
+            }
+        }
+    }
+
     void run(unsigned nanoSteps)
     {
         if (wrap) {
@@ -468,6 +491,36 @@ public:
         }
     }
 
+    // wrapped axes: the streak's bytes of the byte grid; otherwise the
+    // packed words covering the streak, unpacked on the host
+    void downloadRegion(Grid<CELL, Topology> *host, const Region<2>& region)
+    {
+        const Coord<2> dim = host->boundingBox().dimensions;
+        const std::vector<Streak<2> > streaks = relativeStreaks(region, host->boundingBox());
+        const int64_t nw = (g.nx + 31) / 32;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<2>& st = streaks[k];
+            const int y = st.origin.y();
+            CELL *row = host->data() + (std::size_t)y * dim.x();
+            if (wrap) {
+                std::vector<uint8_t> bytes(st.endX - st.origin.x());
+                const uint8_t *src = (const uint8_t *)grid->get() + elementIndex(g, 0, st.origin.x(), y, 0);
+                check(lgdhip_memcpy_d2h(bytes.data(), src, bytes.size(), stream.get()), "lgdhip_memcpy_d2h");
+                for (std::size_t i = 0; i < bytes.size(); ++i) {
+                    row[st.origin.x() + i].*MEMBER = static_cast<VALUE>(bytes[i]);
+                }
+                continue;
+            }
+            const int64_t w0 = st.origin.x() / 32, w1 = (st.endX - 1) / 32;
+            std::vector<uint32_t> words(w1 - w0 + 1);
+            check(lgdhip_memcpy_d2h(words.data(), pa + y * nw + w0, words.size() * 4, stream.get()),
+                  "lgdhip_memcpy_d2h");
+            for (int x = st.origin.x(); x < st.endX; ++x) {
+                row[x].*MEMBER = static_cast<VALUE>((words[x / 32 - w0] >> (x % 32)) & 1u);
+            }
+        }
+    }
+
     void run(unsigned nanoSteps)
     {
         if (wrap) {
@@ -603,6 +656,31 @@ public:
         for (std::size_t i = 0; i < n; ++i) {
             for (int q = 0; q < 19; ++q) {
                 host->data()[i].f[q] = (float)((double)dense[q * n + i] + weight(q));
+            }
+        }
+    }
+
+    // one strided copy per streak: the streak's segment of all 19 members
+    void downloadRegion(Grid<CELL, Topology> *host, const Region<3>& region)
+    {
+        const Coord<3> dim = host->boundingBox().dimensions;
+        const std::vector<Streak<3> > streaks = relativeStreaks(region, host->boundingBox());
+        const std::size_t memberStride = (std::size_t)(elementIndex(g, 1, 0, 0, 0) - elementIndex(g, 0, 0, 0, 0));
+        std::vector<float> line;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<3>& st = streaks[k];
+            const std::size_t n = st.endX - st.origin.x();
+            line.resize(19 * n);
+            const float *src = (const float *)a->get() + elementIndex(g, 0, st.origin.x(), st.origin.y(),
+                                                                         st.origin.z());
+            check(lgdhip_memcpy2d_d2h(line.data(), n * sizeof(float), src, memberStride * sizeof(float),
+                                      n * sizeof(float), 19, stream.get()),
+                  "lgdhip_memcpy2d_d2h");
+            CELL *cells = host->data() + st.origin.toIndex(dim);
+            for (std::size_t i = 0; i < n; ++i) {
+                for (int q = 0; q < 19; ++q) {
+                    cells[i].f[q] = (float)((double)line[q * n + i] + weight(q));
+                }
             }
         }
     }

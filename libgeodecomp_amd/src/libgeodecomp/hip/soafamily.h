@@ -237,6 +237,35 @@ public:
         }
     }
 
+    // the region's cells only: per streak one strided copy of its segment
+    // of every member plane, converted back to cells on the host
+    void downloadRegion(Grid<CELL, Topology> *host, const Region<DIM>& region)
+    {
+        requireBox(*host);
+        const std::vector<Streak<DIM> > streaks = relativeStreaks(region, host->boundingBox());
+        std::vector<char> line;
+        for (std::size_t k = 0; k < streaks.size(); ++k) {
+            const Streak<DIM>& st = streaks[k];
+            const long n = st.endX - st.origin.x();
+            const long y = coordOf(st.origin, 1), z = coordOf(st.origin, 2);
+            const long first = layout.index(st.origin.x(), y, z);
+            line.assign((std::size_t)n * Traits::BYTES_PER_CELL, 0);
+            Traits::each_member([&](std::size_t offset, std::size_t size, std::size_t count) {
+                // the `count` planes of this member lie geo.prod * size bytes apart
+                const std::size_t plane = offset;
+                const char *src = static_cast<const char *>(a->get()) + (std::size_t)geo.prod * plane + first * size;
+                check(lgdhip_memcpy2d_d2h(line.data() + (std::size_t)n * plane, (std::size_t)n * size, src,
+                                          (std::size_t)geo.prod * size, (std::size_t)n * size, count, stream.get()),
+                      "lgdhip_memcpy2d_d2h");
+            });
+            CELL *cells = host->data() + (z * geo.ny + y) * geo.nx + st.origin.x();
+            for (long i = 0; i < n; ++i) {
+                Accessor acc(line.data(), i, n, n, n);
+                acc >> cells[i];
+            }
+        }
+    }
+
     // nanoSteps is a multiple of NANO_STEPS (whole steps); a period of
     // lcm(2, NANO_STEPS) nano steps is replayed as one captured graph
     void run(unsigned nanoSteps)

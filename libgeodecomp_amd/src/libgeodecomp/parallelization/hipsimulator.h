@@ -28,7 +28,10 @@
 //  * between two plugin callbacks run() advances all steps in one call into
 //    the family (temporal blocking / multi-step launches), instead of one
 //    launch per nano-step;
-//  * the device state is downloaded only when a plugin looks at it;
+//  * the device state is downloaded only when a plugin looks at it, and for
+//    Writers only the x-rows they read (TrackingGrid's on-demand reads over
+//    the family's downloadRegion; the reference copies the whole grid back
+//    for every writer call, cudasimulator.h:490-497);
 //  * after steerers, only the cells they changed are uploaded (the host grid
 //    records every write through GridBase, see storage/trackinggrid.h),
 //    when the family supports region uploads and the change is small; a
@@ -59,6 +62,15 @@ template<typename T, typename = void>
 class IsComplete : public std::false_type {};
 template<typename T>
 class IsComplete<T, decltype(void(sizeof(T)))> : public std::true_type {};
+
+// does FAMILY offer downloadRegion(&grid, region)?
+template<typename FAMILY, typename GRID, typename REGION, typename = void>
+class HasDownloadRegion : public std::false_type {};
+template<typename FAMILY, typename GRID, typename REGION>
+class HasDownloadRegion<FAMILY, GRID, REGION,
+                        decltype(std::declval<FAMILY&>().downloadRegion(std::declval<GRID *>(),
+                                                                        std::declval<const REGION&>()))>
+    : public std::true_type {};
 
 // does FAMILY offer uploadRegion(grid, region)?
 template<typename FAMILY, typename GRID, typename REGION, typename = void>
@@ -147,7 +159,9 @@ public:
     virtual void run()
     {
         initializer->grid(&hostGrid);
+        hostGrid.markCurrent();
         hostCurrent = true;
+        writerView = false;
         deviceCurrent = false;
         stepNum = initializer->startStep();
         setIORegions();
@@ -191,6 +205,10 @@ public:
     unsigned long long fullUploads() const { return nFull; }
     unsigned long long regionUploads() const { return nRegion; }
     unsigned long long regionCellsUploaded() const { return regionCells; }
+    // device -> host transfers so far: whole-grid downloads and the x-rows
+    // Writers brought in one by one
+    unsigned long long fullDownloads() const { return nFullDown; }
+    std::size_t rowsDownloaded() const { return hostGrid.rowsFetchedOnDemand(); }
 
     // a region upload is used when it carries at most this share of the grid
     static constexpr double REGION_UPLOAD_MAX_SHARE = 0.125;
@@ -201,14 +219,52 @@ protected:
     bool hostCurrent;
     bool deviceCurrent;
     unsigned long long sweeps = 0;
+    bool writerView = false;     // host rows stale, fetched as Writers read them
     Region<DIM> pendingRegion;   // host edits not yet on the device
-    unsigned long long nFull = 0, nRegion = 0, regionCells = 0;
+    unsigned long long nFull = 0, nRegion = 0, regionCells = 0, nFullDown = 0;
 
     void syncHost()
     {
         if (!hostCurrent) {
             device->download(&hostGrid);
+            ++nFullDown;
+            hostGrid.markCurrent();
             hostCurrent = true;
+            writerView = false;
+        }
+    }
+
+    // before Writers look at a stale host grid: rows come in as they read
+    // them (whole grid at once for a family without region downloads)
+    void prepareWriterView()
+    {
+        if (hostCurrent || writerView) {
+            return;
+        }
+        prepareWriterView(HipKernels::HasDownloadRegion<Family, Grid<CELL_TYPE, Topology>, Region<DIM> >());
+    }
+
+    void prepareWriterView(std::true_type)
+    {
+        hostGrid.markStale([this](const Region<DIM>& rows) { device->downloadRegion(&hostGrid, rows); },
+                           [this]() {
+                               device->download(&hostGrid);
+                               ++nFullDown;
+                           });
+        writerView = true;
+    }
+
+    void prepareWriterView(std::false_type)
+    {
+        syncHost();
+    }
+
+    // after the Writers: every row fetched means the host copy is whole
+    void finishWriterView()
+    {
+        if (writerView && !hostGrid.hasStaleRows()) {
+            hostCurrent = true;
+            writerView = false;
         }
     }
 
@@ -270,6 +326,7 @@ protected:
         device->sync();
         sweeps += (unsigned long long)steps * NANO_STEPS;
         hostCurrent = false;
+        writerView = false;
     }
 
     // how many steps from stepNum on can run without a plugin callback in
@@ -308,10 +365,11 @@ protected:
         TimeOutput t(&chronometer);
         for (std::size_t i = 0; i < writers.size(); ++i) {
             if ((event != WRITER_STEP_FINISHED) || ((stepNum % writers[i]->getPeriod()) == 0)) {
-                syncHost();
+                prepareWriterView();
                 writers[i]->stepFinished(hostGrid, stepNum, event);
             }
         }
+        finishWriterView();
     }
 
     void handleInput(SteererEvent event, SteererFeedback *feedback)

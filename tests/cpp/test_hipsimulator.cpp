@@ -11,6 +11,7 @@
 #include <libgeodecomp/io/mpiioinitializer.h>
 #include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
+#include <libgeodecomp/io/tracingwriter.h>
 #include <libgeodecomp/misc/patternoptimizer.h>
 #include <libgeodecomp/parallelization/autotuningsimulator.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
@@ -625,6 +626,124 @@ LGD_TEST(steerer_region_uploads)
     regionUploadCase<JacobiCell>(&makeSteeredJacobi, 6, [](GridBase<JacobiCell, 3> *g, unsigned step) {
         g->setEdge(JacobiCell(0.1 * step));
     }, false);
+}
+
+// ---------------------------------------------------------------------
+// writers read on demand: a writer that samples a few cells brings in only
+// their x-rows (device -> host), a TracingWriter nothing at all
+// ---------------------------------------------------------------------
+template<typename CELL>
+struct Samples
+{
+    typedef typename Writer<CELL>::GridType GridType;
+    static const int DIM = GridType::DIM;
+    std::vector<Coord<DIM> > at;
+    std::vector<std::vector<CELL> > seen;
+
+    static void hook(const GridType& g, unsigned, void *ctx)
+    {
+        Samples *s = static_cast<Samples *>(ctx);
+        std::vector<CELL> cells;
+        for (std::size_t i = 0; i < s->at.size(); ++i) {
+            cells.push_back(g.get(s->at[i]));
+        }
+        s->seen.push_back(cells);
+    }
+};
+
+// runs `make` on HipSimulator and the oracle with a sampling writer of
+// period `period`; the samples must agree and the device must have sent only
+// `rows` rows per callback after the first (whose grid is the host's own)
+template<typename CELL, typename SAME>
+void sampleCase(std::function<Initializer<CELL> *()> make, unsigned period,
+                const std::vector<Coord<Samples<CELL>::DIM> >& at, std::size_t rowsPerCall, SAME same)
+{
+    std::vector<Event> eh, er;
+    Samples<CELL> sh, sr;
+    sh.at = sr.at = at;
+    HipSimulator<CELL> hip(make());
+    SerialSimulator<CELL> ref(make());
+    hip.addWriter(new RecordingWriter<CELL>(&eh, period, &Samples<CELL>::hook, &sh));
+    ref.addWriter(new RecordingWriter<CELL>(&er, period, &Samples<CELL>::hook, &sr));
+    hip.run();
+    ref.run();
+    CHECK(eh == er);
+    CHECK_EQ(sh.seen.size(), sr.seen.size());
+    int differ = 0;
+    for (std::size_t k = 0; k < sh.seen.size() && k < sr.seen.size(); ++k) {
+        for (std::size_t i = 0; i < at.size(); ++i) {
+            differ += !same(sh.seen[k][i], sr.seen[k][i]);
+        }
+    }
+    CHECK_EQ(differ, 0);
+    CHECK_EQ(hip.fullDownloads(), 0ull);
+    CHECK_EQ(hip.rowsDownloaded(), rowsPerCall * (sh.seen.size() - 1));
+    std::printf("  %zu writer calls, %zu rows downloaded, %llu whole-grid downloads\n", sh.seen.size(),
+                hip.rowsDownloaded(), hip.fullDownloads());
+}
+
+LGD_TEST(writer_reads_only_its_rows)
+{
+    auto eq = [](const auto& a, const auto& b) { return a == b; };
+    // Life, bit-packed device state: three cells in two rows (one of them
+    // straddling a 32-cell word), an edge cell (no transfer)
+    sampleCase<ConwayCell>([]() -> Initializer<ConwayCell> * { return makeRandom(); }, 10,
+                           {Coord<2>(3, 4), Coord<2>(31, 4), Coord<2>(32, 4), Coord<2>(500, 290), Coord<2>(-1, 7)},
+                           2, eq);
+    // Torus Life, byte device state: a wrapped coordinate reads row ny - 1
+    sampleCase<TorusLifeCell>([]() -> Initializer<TorusLifeCell> * { return makeTorusLife(); }, 6,
+                              {Coord<2>(-1, -1), Coord<2>(130, 76), Coord<2>(5, 9)}, 2, eq);
+    // Jacobi
+    sampleCase<JacobiCell>([]() -> Initializer<JacobiCell> * { return makeJacobi(); }, 7,
+                           {Coord<3>(0, 0, 0), Coord<3>(63, 47, 39), Coord<3>(10, 47, 39), Coord<3>(5, 5, 40)}, 2,
+                           eq);
+    // LBM: all 19 member planes of a row in one strided copy, FP32 rounding
+    sampleCase<LBMCavityCell<float> >(
+        []() -> Initializer<LBMCavityCell<float> > * { return makeSteeredLBM(); }, 5,
+        {Coord<3>(1, 2, 3), Coord<3>(23, 19, 15), Coord<3>(0, 19, 15)}, 2,
+        [](const LBMCavityCell<float>& a, const LBMCavityCell<float>& b) {
+            for (int q = 0; q < 19; ++q) {
+                if (std::fabs(a.f[q] - b.f[q]) > 1e-5f) {
+                    return false;
+                }
+            }
+            return true;
+        });
+}
+
+LGD_TEST(writer_reading_everything_switches_to_one_download)
+{
+    // the snapshot writer reads every cell: the first rows come one by one,
+    // the rest in one whole-grid transfer per callback
+    std::vector<Event> eh;
+    Snapshots<JacobiCell> sh;
+    HipSimulator<JacobiCell> hip(makeJacobi());
+    hip.addWriter(new RecordingWriter<JacobiCell>(&eh, 7, &Snapshots<JacobiCell>::hook, &sh));
+    hip.run();
+    const std::size_t calls = sh.grids.size() - 1;  // the first callback reads the host's own grid
+    const std::size_t limit = std::max<std::size_t>(16, 48 * 40 / 32);
+    CHECK_EQ(hip.fullDownloads(), (unsigned long long)calls);
+    CHECK_EQ(hip.rowsDownloaded(), limit * calls);
+}
+
+LGD_TEST(tracing_writer_costs_no_transfer)
+{
+    std::ostringstream out;
+    HipSimulator<ConwayCell> hip(makeRandom());
+    hip.addWriter(new TracingWriter<ConwayCell>(5, 95, TracingWriter<ConwayCell>::OUTPUT_ON_ALL_RANKS, out));
+    hip.run();
+    const std::string log = out.str();
+    std::size_t steps = 0;
+    for (std::size_t p = log.find("TracingWriter::stepFinished()"); p != std::string::npos;
+         p = log.find("TracingWriter::stepFinished()", p + 1)) {
+        ++steps;
+    }
+    CHECK_EQ(steps, (std::size_t)16);   // the run starts at step 13: steps 15..90; 95 is ALL_DONE
+    CHECK(log.find("TracingWriter::initialized()") != std::string::npos);
+    CHECK(log.find("TracingWriter::allDone()") != std::string::npos);
+    CHECK(log.find(" GLUPS") != std::string::npos);
+    CHECK_EQ(hip.fullDownloads(), 0ull);
+    CHECK_EQ(hip.rowsDownloaded(), (std::size_t)0);
 }
 
 int main(int argc, char **argv)

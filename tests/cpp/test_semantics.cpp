@@ -10,9 +10,11 @@
 #include <libgeodecomp/io/mpiioinitializer.h>
 #include <libgeodecomp/io/mpiiowriter.h>
 #include <libgeodecomp/io/serialbovwriter.h>
+#include <libgeodecomp/io/tracingwriter.h>
 #include <libgeodecomp/io/varstepinitializerproxy.h>
 #include <libgeodecomp/misc/patternoptimizer.h>
 #include <libgeodecomp/misc/simulationfactory.h>
+#include <libgeodecomp/storage/trackinggrid.h>
 
 #include <cstdio>
 #include <cstring>
@@ -869,4 +871,117 @@ LGD_TEST(oracle_soa_probe_cells)
 int main(int argc, char **argv)
 {
     return lgdtest::runAll(argc, argv);
+}
+
+// ---------------------------------------------------------------------
+// TrackingGrid's on-demand reads (the host side of HipSimulator's partial
+// device-to-host transfers for writers), with a host array as the device
+// ---------------------------------------------------------------------
+LGD_TEST(tracking_grid_reads_rows_on_demand)
+{
+    typedef Topologies::Cube<3>::Topology Cube3;
+    const CoordBox<3> box(Coord<3>(2, 3, 4), Coord<3>(10, 40, 30));   // 1200 rows, one-by-one limit 37
+    auto truth = [](const Coord<3>& c) { return 1000.0 * c.z() + 10.0 * c.y() + 0.5 * c.x(); };
+    TrackingGrid<double, Cube3> g(box);
+    g.setEdge(-1.0);
+    std::vector<Region<3> > calls;
+    int alls = 0;
+    auto fill = [&](const Streak<3>& s) {
+        for (Coord<3> c = s.origin; c.x() < s.endX; ++c.x()) {
+            g.data()[(c - box.origin).toIndex(box.dimensions)] = truth(c);
+        }
+    };
+    g.markStale([&](const Region<3>& r) {
+                    calls.push_back(r);
+                    for (const Streak<3>& s : r.streaks()) {
+                        fill(s);
+                    }
+                },
+                [&]() {
+                    ++alls;
+                    for (int z = 0; z < 30; ++z) {
+                        for (int y = 0; y < 40; ++y) {
+                            fill(Streak<3>(Coord<3>(2, 3 + y, 4 + z), 12));
+                        }
+                    }
+                });
+    CHECK(g.hasStaleRows());
+    CHECK_EQ(g.rowLimit(), (std::size_t)37);
+    // the first read of a row brings in exactly that row
+    CHECK_EQ(g.get(Coord<3>(5, 13, 11)), truth(Coord<3>(5, 13, 11)));
+    CHECK_EQ(calls.size(), (std::size_t)1);
+    Region<3> row;
+    row << Streak<3>(Coord<3>(2, 13, 11), 12);
+    CHECK(calls[0] == row);
+    CHECK_EQ(g.get(Coord<3>(11, 13, 11)), truth(Coord<3>(11, 13, 11)));
+    CHECK_EQ(calls.size(), (std::size_t)1);
+    // edge reads never fetch
+    CHECK_EQ(g.get(Coord<3>(1, 13, 11)), -1.0);
+    CHECK_EQ(g.get(Coord<3>(5, 43, 11)), -1.0);
+    CHECK_EQ(calls.size(), (std::size_t)1);
+    // a streak read fetches its row once
+    double line[4];
+    g.get(Streak<3>(Coord<3>(4, 20, 33), 8), line);
+    CHECK_EQ(line[3], truth(Coord<3>(7, 20, 33)));
+    CHECK_EQ(calls.size(), (std::size_t)2);
+    // past the limit the rest comes in one transfer
+    for (int y = 3; calls.size() < 37; ++y) {
+        g.get(Coord<3>(2, y, 4));
+    }
+    CHECK_EQ(alls, 0);
+    CHECK(g.hasStaleRows());
+    CHECK_EQ(g.get(Coord<3>(3, 42, 33)), truth(Coord<3>(3, 42, 33)));
+    CHECK_EQ(alls, 1);
+    CHECK(!g.hasStaleRows());
+    CHECK_EQ(g.rowsFetchedOnDemand(), (std::size_t)37);
+    // after markCurrent nothing is fetched, and markStale starts a new count
+    g.markCurrent();
+    g.get(Coord<3>(2, 3, 4));
+    CHECK_EQ(calls.size(), (std::size_t)37);
+    bool fetched = false;
+    g.markStale([&](const Region<3>&) { fetched = true; }, [&]() {});
+    g.get(Coord<3>(2, 3, 4));
+    CHECK(fetched);
+}
+
+LGD_TEST(tracking_grid_on_demand_torus_rows)
+{
+    typedef Topologies::Torus<2>::Topology Torus2;
+    TrackingGrid<int, Torus2> g(CoordBox<2>(Coord<2>(), Coord<2>(7, 5)));
+    std::vector<int> rows;
+    g.markStale([&](const Region<2>& r) { rows.push_back(r.streaks()[0].origin.y()); }, [&]() {});
+    g.get(Coord<2>(-1, -1));   // wraps to (6, 4)
+    g.get(Coord<2>(3, 9));     // wraps to (3, 4): same row
+    g.get(Coord<2>(0, 5));     // wraps to (0, 0)
+    CHECK(rows == std::vector<int>({4, 0}));
+}
+
+LGD_TEST(tracing_writer_output_and_rates)
+{
+    typedef TracingWriter<ConwayCell> TW;
+    CHECK_EQ(TW::renderDuration(3661.5), std::string("01:01:01.500000"));
+    CHECK_EQ(TW::renderDuration(59.0), std::string("00:00:59"));
+    // GLUPS = steps * cells / seconds / 1e9; bandwidth = GLUPS * 2 * sizeof(cell)
+    CHECK(std::fabs(TW::glups(100, 1e6, 0.5) - 0.2) < 1e-12);
+    CHECK(std::fabs(TW::bandwidthGBs(0.2) - 0.4 * sizeof(ConwayCell)) < 1e-12);
+    CHECK_THROWS(TW(0, 10), std::invalid_argument);
+    std::ostringstream out;
+    SerialSimulator<ConwayCell> sim(new ExampleLifeInitializer(20));
+    sim.addWriter(new TW(5, 20, TW::OUTPUT_ON_ALL_RANKS, out));
+    sim.run();
+    const std::string log = out.str();
+    std::size_t steps = 0;
+    for (std::size_t p = log.find("TracingWriter::stepFinished()"); p != std::string::npos;
+         p = log.find("TracingWriter::stepFinished()", p + 1)) {
+        ++steps;
+    }
+    CHECK_EQ(steps, (std::size_t)3);   // 5, 10, 15; step 20 is ALL_DONE
+    CHECK(log.find("step: 15 of 20") != std::string::npos);
+    CHECK(log.rfind("TracingWriter::allDone()") > log.rfind("stepFinished"));
+    // a writer meant for another rank prints nothing
+    std::ostringstream none;
+    SerialSimulator<ConwayCell> sim2(new ExampleLifeInitializer(10));
+    sim2.addWriter(new TW(5, 10, 3, none));
+    sim2.run();
+    CHECK(none.str().empty());
 }
