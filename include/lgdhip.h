@@ -372,6 +372,18 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
                               int sweeps, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes,
                               int *result_in_b, lgdhip_stream_t stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under hipGraph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  lgdhip_poisson_sweeps_f32 without the workspace: the
+ * problem descriptors and tile lists go into a static 8 MiB device array
+ * per GPU (the whole 512-problem ensemble of n in 32..192 needs ~5 MiB);
+ * a problem set that needs more fails with LGDHIP_ERR_WORKSPACE.  Calls no
+ * allocating routine.  Calls that overlap on one GPU share that array and
+ * race; serialise them (one stream, or one host thread per GPU). */
+int lgdhip_poisson_sweeps_f32_noalloc(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs,
+                                      int count, int sweeps, const lgdhip_tile *tile, int *result_in_b,
+                                      lgdhip_stream_t stream);
+
 /* The blocked sweep variant a launch over these problems uses for
  * `requested` (0: the default, resolved by launch size; 1-6 are kept).  No
  * device work; for tools.  No counterpart in the reference (the ensemble

@@ -29,6 +29,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace {
@@ -747,16 +749,12 @@ int residual_norms(const double *u, const float *f, const Workspace &w, const st
 } // namespace
 
 // `sweeps` plain (omega == 0) or weighted sweeps ea <-> eb over every
-// problem: blocked launches of K sweeps, then single sweeps for the rest
-int run_sweeps(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count, int sweeps,
-               float omega, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes, int *result_in_b,
-               lgdhip_stream_t stream) {
-    int rc = check_problems(probs, count);
-    if (rc) return rc;
-    LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
-    const Workspace w = carve_for(workspace, probs, count);
-    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
-    hipStream_t s = lgdhip::as_stream(stream);
+// problem: blocked launches of K sweeps, then single sweeps for the rest.
+// Uses the descriptor and tile-list pieces of `w` (probs, tiles, begin, end,
+// tb_*), which must hold every problem's lists.
+int run_sweeps_in(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count, int sweeps,
+                  float omega, const lgdhip_tile *tile, const Workspace &w, int *result_in_b, hipStream_t s) {
+    int rc = 0;
     std::vector<ProblemDev> pd(count);
     for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
     LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
@@ -791,6 +789,78 @@ int run_sweeps(float *ea, float *eb, const float *r, const lgdhip_poisson_proble
     if (result_in_b) *result_in_b = cur == eb ? 1 : 0;
     return 0;
 }
+
+int run_sweeps(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs, int count, int sweeps,
+               float omega, const lgdhip_tile *tile, void *workspace, size_t workspace_bytes, int *result_in_b,
+               lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(ea && eb && ea != eb && r && workspace && sweeps >= 0, "bad arguments");
+    const Workspace w = carve_for(workspace, probs, count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    return run_sweeps_in(ea, eb, r, probs, count, sweeps, omega, tile, w, result_in_b, lgdhip::as_stream(stream));
+}
+
+// ---------------------------------------------------------------------------
+// the no-allocation sweep entry point
+// ---------------------------------------------------------------------------
+// Every device buffer of lgdhip_poisson_sweeps_f32_noalloc beyond the
+// caller's e, r pools is the problem descriptors and tile lists of one call;
+// they live in this static array.  Each device holds its own copy for the
+// life of the process; its address on a device comes from
+// hipGetSymbolAddress (the array's name is a host-side stand-in, not a
+// device address).  Calls on one device share it: overlapping calls there
+// race.  Nothing in it needs to start at zero (every piece a call reads is
+// uploaded by that call first).
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(8) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices];
+
+// The device address of `scratch` on the current device, resolved once per
+// device on its first use.
+int scratch_on_current_device(void **out) {
+    int dev = -1;
+    LGD_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        return lgdhip::set_error(LGDHIP_ERR_INVALID_ARG, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return lgdhip::hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// The descriptor and tile-list pieces run_sweeps_in uses, carved from
+// `base` (nullptr: sizes only) for `ntiles` single-sweep and `ntb` blocked
+// tiles of `count` problems; the pool pieces stay null.
+Workspace carve_lists(void *base, int64_t ntiles, int64_t ntb, int count) {
+    Workspace w{};
+    char *p = (char *)base;
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        char *q = p ? p + off : nullptr;
+        off += align256(bytes);
+        return q;
+    };
+    w.tiles = (Tile *)take(ntiles * sizeof(Tile));
+    w.probs = (ProblemDev *)take((size_t)count * sizeof(ProblemDev));
+    w.begin = (int *)take((size_t)count * 4);
+    w.end = (int *)take((size_t)count * 4);
+    w.tb_tiles = (Tile *)take(ntb * sizeof(Tile));
+    w.tb_begin = (int *)take((size_t)count * 4);
+    w.tb_end = (int *)take((size_t)count * 4);
+    w.bytes = off;
+    return w;
+}
+} // namespace
 
 extern "C" {
 
@@ -847,6 +917,32 @@ int lgdhip_poisson_damped_sweeps_f32(float *ea, float *eb, const float *r, const
                                      size_t workspace_bytes, int *result_in_b, lgdhip_stream_t stream) {
     LGD_REQUIRE(omega > 0.0f && omega <= 1.0f, "omega must be in (0, 1]");
     return run_sweeps(ea, eb, r, probs, count, sweeps, omega, tile, workspace, workspace_bytes, result_in_b, stream);
+}
+
+int lgdhip_poisson_sweeps_f32_noalloc(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs,
+                                      int count, int sweeps, const lgdhip_tile *tile, int *result_in_b,
+                                      lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(ea && eb && ea != eb && r && sweeps >= 0, "bad arguments");
+    // the lists this call uploads: every problem's single-sweep tiles and the
+    // blocked tiles of the shape the call runs
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    const TbShape sh = sweep_shape(tile ? tile->variant : 0, probs, all);
+    int64_t ntb = 0;
+    if (sh.k > 0)
+        for (int k = 0; k < count; ++k) ntb += tb_tiles_of(probs[k].n, sh);
+    const size_t need = carve_lists(nullptr, total_tiles(probs, count), ntb, count).bytes;
+    if (need > kNoAllocScratchBytes)
+        return lgdhip::set_error(LGDHIP_ERR_WORKSPACE,
+                                 "problem set needs " + std::to_string(need) + " bytes of tile lists; the static "
+                                 "scratch holds " + std::to_string(kNoAllocScratchBytes));
+    void *base = nullptr;
+    rc = scratch_on_current_device(&base);
+    if (rc) return rc;
+    const Workspace w = carve_lists(base, total_tiles(probs, count), ntb, count);
+    return run_sweeps_in(ea, eb, r, probs, count, sweeps, 0.0f, tile, w, result_in_b, lgdhip::as_stream(stream));
 }
 
 int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_poisson_problem *probs, int count,

@@ -154,6 +154,8 @@ SIGNATURES = {
                                            C.POINTER(C.c_double), _P]),
     "lgdhip_poisson_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _TP, _P, _SZ, _IP,
                                        _P]),
+    "lgdhip_poisson_sweeps_f32_noalloc": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _TP, _IP,
+                                               _P]),
     "lgdhip_poisson_damped_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, C.c_float, _TP,
                                               _P, _SZ, _IP, _P]),
     "lgdhip_poisson_batch_solve": (_I, [_P, _P, C.POINTER(lgdhip_poisson_problem), _I,
