@@ -188,6 +188,219 @@ __global__ __launch_bounds__(256) void conv3d_fwd_mfma(const u16 *__restrict__ X
     }
 }
 
+// LDS-staged implicit GEMM for 3x3x3 convolutions with at most 32 output
+// channels (NT = 1 or 2 tiles of 16): the U-Net's 128^3 and 64^3 levels,
+// where a wave's A fragments (16 voxels x 32 k) feed only NT MFMAs each, so
+// fetching them from global memory per MFMA is bound by L1 delivery.
+// A work-group computes a 16 x 8 x TZ block of output voxels (TZ = 8 / NT);
+// each of its 4 waves owns RT = 16 / NT rows of 16 voxels along x.  The
+// input channels are taken 8 at a time (a "group"): the block's input tile
+// with its one-voxel halo, (16+2) x (8+2) x (TZ+2) voxels x 8 channels, is
+// staged in LDS (two buffers: the next group's global loads are in flight
+// while the current one is computed), zeros outside the volume giving the
+// faces' zero padding.  A group's 27 taps x 8 channels are seven k steps of
+// v_mfma_f32_16x16x32_bf16: in step j, lane group q (lanes 16q..16q+15)
+// holds tap 4j + q (the 28th tap is zero), so every A fragment is one
+// 16-byte LDS read per lane, voxel (x0 + i + dx, y + dy, z + dz).  The
+// group's weight fragments (7 x NT) are loaded once and used for all RT
+// rows.  The k order (group-major, then tap) differs from
+// conv3d_fwd_mfma's; each output is still one FP32 accumulation of the same
+// BF16 products.
+constexpr int kLdsTX = 16, kLdsTY = 8;
+// 16 zero bytes in device memory: the source of loads that would fall
+// outside the volume (or past cout), so no load needs a guard or a select.
+// Loads go through global-address-space pointers (global_load, counted by
+// vmcnt only; a generic pointer would give flat loads, which LDS waits
+// would also wait for).
+__device__ __align__(16) const unsigned int kZero16[4] = {0u, 0u, 0u, 0u};
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef const u32x4 __attribute__((address_space(1))) *gq_ptr;
+typedef const u16 __attribute__((address_space(1))) *gh_ptr;
+__device__ __forceinline__ uint4 load_global16(gh_ptr p) {
+    const u32x4 v = *(gq_ptr)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+
+template <int NT, bool VEC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv3d_fwd_lds(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
+                                                      const u16 *__restrict__ W, int Kp,
+                                                      const float *__restrict__ bias, int cout, u16 *Y, int ldy,
+                                                      int offy, float *Y32, int ld32, int relu, int accumulate,
+                                                      int bx, int by) {
+    constexpr int TZ = 8 / NT;                // output planes per block
+    constexpr int RT = 16 / NT;               // rows per wave: 8 y rows x TZ / 4 planes
+    constexpr int SX = kLdsTX + 2, SY = kLdsTY + 2, SZ = TZ + 2;
+    constexpr int SV = SX * SY * SZ;          // staged voxels
+    constexpr int PER = (SV + 255) / 256;     // staging loads per thread
+    __shared__ uint4 tile[2][SV];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, q = lane >> 4;
+    int b = blockIdx.x;
+    const int x0 = (b % bx) * kLdsTX;
+    b /= bx;
+    const int y0 = (b % by) * kLdsTY;
+    const int z0 = (b / by) * TZ;
+    const int groups = (cin + 7) / 8;
+
+    // staged voxel p of this thread: its element offset in X (0 and a zero
+    // mask outside the volume, so every load is issued unconditionally and
+    // all of a thread's loads are in flight together)
+    int64_t soff[PER];
+    bool sok[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int sv = tid + 256 * p;
+        const int xx = x0 + sv % SX - 1, yy = y0 + (sv / SX) % SY - 1, zz = z0 + sv / (SX * SY) - 1;
+        sok[p] = sv < SV && (unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n;
+        soff[p] = sok[p] ? (((int64_t)zz * n + yy) * n + xx) * ldx : 0;
+    }
+    uint4 stage[PER];
+    const gh_ptr zero = (gh_ptr)(const void *)kZero16;
+    const gh_ptr xg = (gh_ptr)X;
+    auto stage_load = [&](int c) {
+        const gh_ptr base = xg + offx + 8 * c;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            Frag f;
+            if (VEC) {
+                f.q = load_global16(sok[p] ? base + soff[p] : zero);
+            } else {
+                // channels beyond cin (and voxels outside) read zeros
+#pragma unroll
+                for (int e = 0; e < 8; ++e) f.h[e] = *(sok[p] && 8 * c + e < cin ? base + soff[p] + e : zero);
+            }
+            stage[p] = f.q;
+        }
+    };
+    auto stage_store = [&](int buf) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int sv = tid + 256 * p;
+            if (sv < SV) tile[buf][sv] = stage[p];
+        }
+    };
+
+    // this lane group's tap of each k step, as an offset in the staged tile
+    // relative to the row's first voxel (x0 - 1, y - 1, z - 1)
+    int toff[7];
+    bool tok[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const int tap = 4 * j + q;
+        tok[j] = tap < 27;
+        int dz, dy, dx;
+        tap_offset(tok[j] ? tap : 13, 3, dz, dy, dx);
+        toff[j] = ((dz + 1) * SY + (dy + 1)) * SX + (dx + 1) + i;
+    }
+
+    f32x4 acc[RT][NT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[r][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // a group's weight fragments: channels 8c..8c+7 of tap 4j + q for output
+    // channel t * 16 + i (zero for the 28th tap, channels past cout or cin)
+    Frag bf[7][NT];
+    auto load_b = [&](int c) {
+#pragma unroll
+        for (int j = 0; j < 7; ++j)
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int co = t * 16 + i;
+                const bool ok = tok[j] && co < cout;
+                const gh_ptr src = (gh_ptr)W + (int64_t)co * Kp + (4 * j + q) * cin + 8 * c;
+                if (VEC) {
+                    bf[j][t].q = load_global16(ok ? src : zero);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) bf[j][t].h[e] = *(ok && 8 * c + e < cin ? src + e : zero);
+                }
+            }
+    };
+
+    // Software pipeline over the groups.  Loads complete in issue order
+    // (vmcnt), so each group's loads are issued after the ones it waits for
+    // last: the staged input of group c + 1 goes out first, the MFMAs of
+    // group c then need only the weights loaded in the previous iteration;
+    // the weights of group c + 1 go out after the MFMAs, and writing the
+    // staged input waits only for the older staging loads.
+    stage_load(0);
+    load_b(0);
+    stage_store(0);
+    __syncthreads();
+    for (int c = 0; c < groups; ++c) {
+        const int cur = c & 1;
+        // the next group's loads are issued unconditionally (the last group
+        // re-reads its own, never stored), so the wait counts are the same on
+        // every iteration and exact
+        const int cn = c + 1 < groups ? c + 1 : c;
+        stage_load(cn);  // in flight during the MFMAs below
+        // k step outermost: consecutive MFMAs update different rows'
+        // accumulators, so none waits for the previous one's result
+        // (the 28th tap's zero weights make its A values irrelevant: it reads
+        // the block's centre tap, always inside the staged tile)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            Frag a[RT];  // all rows' fragments of this k step in flight together
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const int yl = r % 8, zl = wave * (TZ / 4) + r / 8;
+                a[r].q = tile[cur][(zl * SY + yl) * SX + toff[j]];
+            }
+#pragma unroll
+            for (int r = 0; r < RT; ++r)
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[r].v, bf[j][t].v, acc[r][t], 0, 0, 0);
+        }
+        // schedule: kLdsAhead fragment reads in flight ahead of the MFMAs
+        // that consume them (the compiler alone keeps two, and each MFMA
+        // then waits out an LDS latency)
+        constexpr int kReads = 7 * RT, kLdsAhead = 8;
+        __builtin_amdgcn_sched_group_barrier(0x100, kLdsAhead, 0);
+        lgdhip::static_for<0, kReads - kLdsAhead>([&](auto) {
+            __builtin_amdgcn_sched_group_barrier(0x008, NT, 0);
+            __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        });
+        __builtin_amdgcn_sched_group_barrier(0x008, NT * kLdsAhead, 0);
+        load_b(cn);
+        if (c + 1 < groups) stage_store(cur ^ 1);
+        __syncthreads();
+    }
+    // D[r][c] in lane c + 16 (r / 4), element r % 4: voxel x0 + 4q + e of the
+    // row, channel t * 16 + i
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        const int co = t * 16 + i;
+        if (co >= cout) continue;
+        const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int y = y0 + r % 8, z = z0 + wave * (TZ / 4) + r / 8;
+            if (y >= n || z >= n) continue;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int x = x0 + 4 * q + e;
+                if (x >= n) continue;
+                const int64_t v = ((int64_t)z * n + y) * n + x;
+                float o = acc[r][t][e] + bv;
+                if (relu) o = o > 0.0f ? o : 0.0f;
+                if (Y) {
+                    u16 *pp = Y + v * ldy + offy + co;
+                    if (accumulate) o += bf2f(*pp);
+                    *pp = f2bf(o);
+                }
+                if (Y32) Y32[v * ld32 + co] = o;
+            }
+        }
+    }
+}
+
+// the LDS-staged kernel takes 3x3x3 layers with up to 32 output channels on
+// volumes of at least 16^3
+constexpr int kLdsMinN = 16;
+
 // voxel tiles per wave of the forward kernel: weight-fragment reuse bounded
 // by the accumulator registers (MT x NT x 4)
 constexpr int fwd_mt(int nt) { return nt <= 4 ? 4 : (nt <= 8 ? 2 : 1); }
@@ -760,6 +973,20 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
     const u16 *Wb = (const u16 *)w;
     u16 *Y = (u16 *)y;
     const int64_t nvox = (int64_t)n * n * n;
+    if (ksize == 3 && nt <= 2 && n >= kLdsMinN) {
+        const int bx = (n + kLdsTX - 1) / kLdsTX, by = (n + kLdsTY - 1) / kLdsTY;
+        const int bz = (n + 8 / nt - 1) / (8 / nt);
+        const unsigned blocks = (unsigned)((int64_t)bx * by * bz);
+        const bool vec = (cin & 7) == 0;  // ldx, offx and the alignment were checked above
+#define LGD_LDS_CASE(T, V)                                                                                  \
+    if (nt == T && vec == V)                                                                                \
+        conv3d_fwd_lds<T, V><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, y32, \
+                                                    ld32, relu, accumulate, bx, by);
+        LGD_LDS_CASE(1, true) LGD_LDS_CASE(1, false) LGD_LDS_CASE(2, true) LGD_LDS_CASE(2, false)
+#undef LGD_LDS_CASE
+        LGD_LAUNCHED("conv3d_fwd_lds");
+        return 0;
+    }
     const FwdShape sh = fwd_shape(nvox, nt);
 #define LGD_CONV_CASE(T, M)                                                                                   \
     if (sh.nt == T && sh.mt == M) {                                                                           \

@@ -89,8 +89,11 @@ def run_conv(hip, x, w, bias, ksize, ld_out=None, off_out=0, relu=False, y_init=
     return out32.reshape(n, n, n, cout), from_bf16_bits(outb).reshape(n, n, n, ld)
 
 
+# n >= 16 with up to 32 output channels runs the LDS-staged kernel (odd
+# sizes leave partial blocks; cin 2 and 3 take its element-load path)
 CONV_CASES = [(8, 2, 16, 3), (16, 16, 16, 3), (12, 48, 16, 3), (8, 192, 64, 3), (16, 16, 1, 1), (4, 64, 128, 3),
-              (8, 32, 192, 3), (6, 8, 24, 3)]
+              (8, 32, 192, 3), (6, 8, 24, 3), (32, 16, 16, 3), (33, 48, 16, 3), (20, 3, 16, 3), (16, 2, 16, 3),
+              (24, 32, 32, 3), (17, 16, 24, 3)]
 
 
 @pytest.mark.gpu
@@ -113,9 +116,10 @@ def test_conv_forward_vs_oracle(hip, oracle, n, cin, cout, ksize):
 
 
 @pytest.mark.gpu
-def test_conv_relu_slice_accumulate(hip, oracle):
-    rng = np.random.default_rng(9)
-    n, cin, cout, ld, off = 8, 16, 16, 48, 32
+@pytest.mark.parametrize("n", [8, 19])
+def test_conv_relu_slice_accumulate(hip, oracle, n):
+    rng = np.random.default_rng(9 + n)
+    cin, cout, ld, off = 16, 16, 48, 32
     x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
     w = bf16_round((rng.standard_normal((cout, 27, cin)) / 12).astype(np.float32))
     b = np.zeros(cout, np.float32)
@@ -130,7 +134,8 @@ def test_conv_relu_slice_accumulate(hip, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,cin,cout", [(8, 16, 16), (8, 48, 16), (4, 192, 64), (8, 16, 1)])
+@pytest.mark.parametrize("n,cin,cout", [(8, 16, 16), (8, 48, 16), (4, 192, 64), (8, 16, 1), (16, 16, 16),
+                                        (18, 32, 16)])
 def test_conv_dgrad_vs_oracle(hip, oracle, n, cin, cout):
     print("[case] conv dgrad", n, cin, cout, flush=True)
     rng = np.random.default_rng(cin + cout)
