@@ -590,7 +590,17 @@ class Learned:
                 "peak_lr": self.lr,
                 "generate_s": round(self.generate_s, 1), "train_s": round(self.train_s, 1),
                 "final_loss": self.log[-1][1] if self.log else None, "arms": r.get("arms"),
-                "sweep_ratio": r.get("sweep_ratio"), "seed": self.seed, "parallelism": "single GPU"}
+                "sweep_ratio": r.get("sweep_ratio"), "fp64_polish_sweeps": r.get("fp64_polish_sweeps"),
+                "network_gain_over_coarse_base": self.network_gain(), "seed": self.seed,
+                "parallelism": "single GPU"}
+
+    def network_gain(self):
+        """The network's own share: wall time of the coarse-base guess alone
+        over the learned guess (both polished by the same solver)."""
+        arms = {a["arm"]: a for a in (self.result or {}).get("arms", [])}
+        if "coarse" not in arms or "learned" not in arms:
+            return None
+        return round(arms["coarse"]["wall_s"] / arms["learned"]["wall_s"], 3)
 
     def step(self):
         self.result = self.lgm.evaluate(self.model, self.eval, self.m)
