@@ -342,6 +342,10 @@ typedef struct lgdhip_poisson_options {
     int32_t smooth_sweeps;       /* weighted-Jacobi sweeps before the plain ones
                                     (0: none; rounded up to even)                 */
     float smooth_omega;          /* their weight in (0, 1], e.g. 2/3              */
+    int32_t final_fp64_sweeps;   /* plain FP64 Jacobi sweeps on u after every
+                                    problem met the tolerance (0: none; rounded up
+                                    to even); the FP64 residual is recomputed and
+                                    reported after them                           */
 } lgdhip_poisson_options;
 
 typedef struct lgdhip_poisson_result {
@@ -383,6 +387,16 @@ int lgdhip_poisson_sweeps_f32(float *ea, float *eb, const float *r, const lgdhip
 int lgdhip_poisson_sweeps_f32_noalloc(float *ea, float *eb, const float *r, const lgdhip_poisson_problem *probs,
                                       int count, int sweeps, const lgdhip_tile *tile, int *result_in_b,
                                       lgdhip_stream_t stream);
+
+/* `sweeps` plain FP64 Jacobi sweeps on u (ping-pong ua/ub, both with the
+ * faces in their halos, e.g. from lgdhip_poisson_set_faces), each cell
+ * (u[z-1] + u[y-1] + u[x-1] + u[x+1] + u[y+1] + u[z+1] + f) * (1/6) summed in
+ * that order; the final polishing sweeps of lgdhip_poisson_batch_solve.
+ * *result_in_b = sweeps % 2.  No reference counterpart (the ensemble
+ * solver is new with this library). */
+int lgdhip_poisson_sweeps_f64(double *ua, double *ub, const float *f, const lgdhip_poisson_problem *probs,
+                              int count, int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
+                              lgdhip_stream_t stream);
 
 /* The blocked sweep variant a launch over these problems uses for
  * `requested` (0: the default, resolved by launch size; 1-6 are kept).  No

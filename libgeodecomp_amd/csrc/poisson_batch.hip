@@ -482,6 +482,38 @@ __global__ __launch_bounds__(kThreads) void poisson_update_f64(double *__restric
     }
 }
 
+// one plain FP64 Jacobi sweep on u (the final polishing sweeps):
+// u'[c] = (u[z-1] + u[y-1] + u[x-1] + u[x+1] + u[y+1] + u[z+1] + f[c]) * (1/6),
+// the six neighbours summed in that order, then f added, then one multiply
+// (the oracle's plain FP64 Jacobi, lgdo_poisson_jacobi_f64)
+__global__ __launch_bounds__(kThreads) void poisson_sweep_f64(const double *__restrict__ u, double *__restrict__ u2,
+                                                              const float *__restrict__ f,
+                                                              const Tile *__restrict__ tiles,
+                                                              const ProblemDev *__restrict__ probs) {
+    const Tile t = tiles[blockIdx.x];
+    const ProblemDev P = probs[t.p];
+    const int n = P.n;
+    const int x = t.x0 + (int)(threadIdx.x % kTX);
+    const int y = t.y0 + (int)(threadIdx.x / kTX);
+    if (x >= n || y >= n) return;
+    const int64_t sx = n + 2, sxy = sx * sx;
+    const int z1 = min(t.z0 + kTZ, n);
+    int64_t i = P.base + ((int64_t)(t.z0 + 1) * sx + (y + 1)) * sx + (x + 1);
+    double below = u[i - sxy], center = u[i];
+    for (int z = t.z0; z < z1; ++z, i += sxy) {
+        const double above = u[i + sxy];
+        double nb = below;
+        nb += u[i - sx];
+        nb += u[i - 1];
+        nb += u[i + 1];
+        nb += u[i + sx];
+        nb += above;
+        u2[i] = (nb + (double)f[i]) * (1.0 / 6.0);
+        below = center;
+        center = above;
+    }
+}
+
 // norms[k] = sum of partial over the tiles [begin[k], end[k])  (fixed order)
 __global__ __launch_bounds__(256) void poisson_reduce(const double *__restrict__ partial, const int *__restrict__ begin,
                                                       const int *__restrict__ end, double *__restrict__ norms) {
@@ -945,6 +977,33 @@ int lgdhip_poisson_sweeps_f32_noalloc(float *ea, float *eb, const float *r, cons
     return run_sweeps_in(ea, eb, r, probs, count, sweeps, 0.0f, tile, w, result_in_b, lgdhip::as_stream(stream));
 }
 
+int lgdhip_poisson_sweeps_f64(double *ua, double *ub, const float *f, const lgdhip_poisson_problem *probs,
+                              int count, int sweeps, void *workspace, size_t workspace_bytes, int *result_in_b,
+                              lgdhip_stream_t stream) {
+    int rc = check_problems(probs, count);
+    if (rc) return rc;
+    LGD_REQUIRE(ua && ub && ua != ub && f && workspace && sweeps >= 0, "bad arguments");
+    const Workspace w = carve_for(workspace, probs, count);
+    if (workspace_bytes < w.bytes) return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "poisson workspace too small");
+    hipStream_t s = lgdhip::as_stream(stream);
+    std::vector<ProblemDev> pd(count);
+    for (int k = 0; k < count; ++k) pd[k] = ProblemDev{probs[k].n, 0, probs[k].offset};
+    LGD_HIP(hipMemcpyAsync(w.probs, pd.data(), pd.size() * sizeof(ProblemDev), hipMemcpyHostToDevice, s));
+    std::vector<int> all(count);
+    for (int k = 0; k < count; ++k) all[k] = k;
+    std::vector<Tile> tiles;
+    rc = upload_set(w, probs, all, &tiles, s);
+    if (rc) return rc;
+    double *cur = ua, *nxt = ub;
+    for (int t = 0; t < sweeps; ++t) {
+        poisson_sweep_f64<<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, f, w.tiles, w.probs);
+        LGD_LAUNCHED("poisson_sweep_f64");
+        std::swap(cur, nxt);
+    }
+    if (result_in_b) *result_in_b = cur == ub ? 1 : 0;
+    return 0;
+}
+
 int lgdhip_poisson_residual_norms(const double *u, const float *f, const lgdhip_poisson_problem *probs, int count,
                                   void *workspace, size_t workspace_bytes, double *norms,
                                   lgdhip_stream_t stream) {
@@ -1192,6 +1251,31 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
         }
     }
     graph.reset();
+    // final FP64 polishing: plain FP64 Jacobi sweeps on u itself for every
+    // problem (an even count, so u holds the result), the second buffer being
+    // the correction pools' storage (no longer needed), then fresh FP64
+    // residuals
+    const int polish = opt->final_fp64_sweeps + (opt->final_fp64_sweeps & 1);
+    if (polish > 0) {
+        double *u2 = reinterpret_cast<double *>(w.e0);  // e0 and e1 together hold pool doubles
+        LGD_HIP(hipMemcpyAsync(u2, u, (size_t)pool * sizeof(double), hipMemcpyDeviceToDevice, s));
+        active.resize(count);
+        for (int k = 0; k < count; ++k) active[k] = k;
+        rc = upload_set(w, probs, active, &tiles, s);
+        if (rc) return rc;
+        double *cur = u, *nxt = u2;
+        for (int t = 0; t < polish; ++t) {
+            poisson_sweep_f64<<<(unsigned)tiles.size(), kThreads, 0, s>>>(cur, nxt, f, w.tiles, w.probs);
+            LGD_LAUNCHED("poisson_sweep_f64");
+            std::swap(cur, nxt);
+        }
+        rc = residual_norms(u, f, w, tiles, count, &rn, s);
+        if (rc) return rc;
+        for (int k = 0; k < count; ++k) {
+            st[k].outer = rn[k];
+            st[k].sweeps += polish;
+        }
+    }
     for (int k = 0; k < count; ++k) {
         results[k].sweeps = st[k].sweeps;
         results[k].outer_iterations = st[k].outer_iters;

@@ -99,12 +99,14 @@ class Ensemble:
         return out
 
     def solve(self, rel_tol: float = 1e-6, chunk: int = 64, use_graph: int = 1, max_sweeps: int = 10_000_000,
-              smooth_sweeps: int = 0, smooth_omega: float = 2.0 / 3.0):
+              smooth_sweeps: int = 0, smooth_omega: float = 2.0 / 3.0, polish: int = 0):
         """Solves every problem from the current u to rel_tol relative to the
-        zero guess's residual norm; returns the per-problem results."""
+        zero guess's residual norm, then `polish` plain FP64 Jacobi sweeps on
+        u; returns the per-problem results."""
         opt = lgdhip.lgdhip_poisson_options()
         opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = rel_tol, chunk, max_sweeps, use_graph
         opt.smooth_sweeps, opt.smooth_omega = smooth_sweeps, smooth_omega
+        opt.final_fp64_sweeps = polish
         ref = (C.c_double * self.count)(*self.reference)
         opt.reference_norms = ref
         res = (lgdhip.lgdhip_poisson_result * self.count)()
@@ -244,17 +246,18 @@ def _arm(name: str, exact: list, results, t_guess: float, t_solve: float, r_gues
 
 
 def evaluate(model: LearnedGuess, data: Ensemble, coarse_m: int = 0, rel_tol: float = 1e-6, chunk: int = 64,
-             smooth_sweeps: int = 128) -> dict:
+             smooth_sweeps: int = 128, polish: int = 16) -> dict:
     """Zero guess (plain Jacobi); with a coarse grid, the prolongated coarse
     solution alone; and the learned guess -- on the same problems with the
-    same solver, tolerance and reference norms (the zero guess's residual).
+    same solver, tolerance and reference norms (the zero guess's residual),
+    every arm ending with `polish` plain FP64 Jacobi sweeps on u.
     Guessed arms start with `smooth_sweeps` weighted (2/3) Jacobi sweeps,
     counted in their sweeps; their wall times include building the guess
     (restriction, coarse solve, prolongation, network)."""
     data.reset()
     _sync()
     t0 = time.perf_counter()
-    res0 = data.solve(rel_tol, chunk)
+    res0 = data.solve(rel_tol, chunk, polish=polish)
     _sync()
     t_zero = time.perf_counter() - t0
     exact = [data.interior(k) for k in range(data.count)]
@@ -272,7 +275,7 @@ def evaluate(model: LearnedGuess, data: Ensemble, coarse_m: int = 0, rel_tol: fl
         guess = [data.interior(k) for k in range(data.count)]
         r_guess = data.residual_norms() / data.reference
         t0 = time.perf_counter()
-        res = data.solve(rel_tol, chunk, smooth_sweeps=smooth_sweeps)
+        res = data.solve(rel_tol, chunk, smooth_sweeps=smooth_sweeps, polish=polish)
         _sync()
         arms.append(_arm(name, exact, res, t_g, time.perf_counter() - t0, r_guess, guess))
 
@@ -292,7 +295,7 @@ def evaluate(model: LearnedGuess, data: Ensemble, coarse_m: int = 0, rel_tol: fl
     guessed("learned", learned)
     best = arms[-1]
     return {"problems": data.count, "n": data.n, "coarse_m": coarse_m, "rel_tol": rel_tol,
-            "smooth_sweeps": smooth_sweeps, "arms": arms,
+            "smooth_sweeps": smooth_sweeps, "fp64_polish_sweeps": polish, "arms": arms,
             "sweep_ratio": arms[0]["sweeps_total"] / max(best["sweeps_total"], 1.0),
             "wall_speedup": arms[0]["wall_s"] / best["wall_s"]}
 

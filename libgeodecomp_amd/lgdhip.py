@@ -67,7 +67,7 @@ class lgdhip_poisson_options(C.Structure):
     _fields_ = [("rel_tol", C.c_double), ("chunk", C.c_int32), ("max_sweeps", C.c_int32),
                 ("inner_reduction", C.c_double), ("use_graph", C.c_int32), ("sweep_variant", C.c_int32),
                 ("reference_norms", C.POINTER(C.c_double)), ("smooth_sweeps", C.c_int32),
-                ("smooth_omega", C.c_float)]
+                ("smooth_omega", C.c_float), ("final_fp64_sweeps", C.c_int32)]
 
 
 class lgdhip_poisson_result(C.Structure):
@@ -154,6 +154,7 @@ SIGNATURES = {
                                            C.POINTER(C.c_double), _P]),
     "lgdhip_poisson_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _TP, _P, _SZ, _IP,
                                        _P]),
+    "lgdhip_poisson_sweeps_f64": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _P, _SZ, _IP, _P]),
     "lgdhip_poisson_sweeps_f32_noalloc": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, _TP, _IP,
                                                _P]),
     "lgdhip_poisson_damped_sweeps_f32": (_I, [_P, _P, _P, C.POINTER(lgdhip_poisson_problem), _I, _I, C.c_float, _TP,

@@ -150,11 +150,12 @@ class Batch:
         self.f.upload(f_host)
         hip.call("lgdhip_poisson_set_faces", self.u.ptr, self.probs, self.count, self.ws.ptr, self.ws_bytes, None)
 
-    def solve(self, rel_tol=1e-6, chunk=32, use_graph=1, reference=None, stream=None, sweep_variant=0):
+    def solve(self, rel_tol=1e-6, chunk=32, use_graph=1, reference=None, stream=None, sweep_variant=0, polish=0):
         hip = self.hip
         opt = hip.lgdhip_poisson_options()
         opt.rel_tol, opt.chunk, opt.max_sweeps, opt.use_graph = rel_tol, chunk, 2_000_000, use_graph
         opt.sweep_variant = sweep_variant
+        opt.final_fp64_sweeps = polish
         ref = None
         if reference is not None:
             ref = (C.c_double * self.count)(*reference)
@@ -393,3 +394,58 @@ def test_gpu_ensemble_with_largest_size_and_kernel_switch(hip, oracle):
     print(f"  n=192: {big_sweeps} sweeps, {res[b.count - 1].outer_iterations} outer steps", flush=True)
     assert 3e4 < big_sweeps < 2e5
     assert max(r.sweeps for r in res[:-1]) < 5000
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sweeps", [1, 4, 7])
+def test_gpu_fp64_sweeps_bit_exact(hip, oracle, sweeps):
+    """Plain FP64 Jacobi sweeps on u (the solver's final polishing sweeps):
+    bit-identical to the oracle's plain FP64 Jacobi, faces kept, for sizes
+    from one cell to several tiles per axis."""
+    rng = np.random.default_rng(60 + sweeps)
+    sizes = [1, 5, 17, 33, 40]
+    problems = [(n, rng.uniform(-1, 1, size=6), np.zeros((n + 2,) * 3, np.float32)) for n in sizes]
+    for _, _, f in problems:
+        f[1:-1, 1:-1, 1:-1] = rng.standard_normal(f[1:-1, 1:-1, 1:-1].shape)
+    guesses = []
+    for n, faces, _ in problems:
+        g = padded(n, faces, oracle)
+        g[1:-1, 1:-1, 1:-1] = rng.standard_normal((n, n, n))
+        guesses.append(g)
+    b = Batch(hip, problems, guesses)
+    ub = hip.DeviceBuffer(b.elems * 8)
+    host = np.zeros(b.elems)
+    b.u.download(host)
+    ub.upload(host)  # the same faces in both buffers
+    in_b = C.c_int(-1)
+    print("[case] poisson fp64 sweeps", sizes, sweeps, flush=True)
+    hip.call("lgdhip_poisson_sweeps_f64", b.u.ptr, ub.ptr, b.f.ptr, b.probs, b.count, sweeps, b.ws.ptr, b.ws_bytes,
+             C.byref(in_b), None)
+    assert in_b.value == sweeps % 2
+    out = np.empty(b.elems)
+    (ub if in_b.value else b.u).download(out)
+    for k, (n, _, f) in enumerate(problems):
+        exp = guesses[k].copy()
+        oracle.poisson_jacobi(exp, f, 1e-300, max_sweeps=sweeps, check=sweeps)
+        o = b.probs[k].offset
+        assert np.array_equal(out[o:o + exp.size].reshape(exp.shape), exp), n
+
+
+@pytest.mark.gpu
+def test_gpu_solve_with_fp64_polishing(hip, oracle):
+    """final_fp64_sweeps = 8: the solve ends with 8 FP64 Jacobi sweeps on u --
+    the polished u equals the unpolished solve's u plus 8 oracle FP64 sweeps,
+    bit for bit; the reported residual is the polished one (recomputed on the
+    CPU) and the sweep counts include the 8."""
+    problems = synthetic.poisson_ensemble(5, sizes=(8, 16, 24), seed=12)
+    plain, polished = Batch(hip, problems), Batch(hip, problems)
+    r0 = plain.solve(rel_tol=1e-6, chunk=8)
+    r1 = polished.solve(rel_tol=1e-6, chunk=8, polish=7)   # rounded up to 8
+    u0, u1 = plain.u_blocks(), polished.u_blocks()
+    for k, (n, faces, f) in enumerate(problems):
+        exp = u0[k].copy()
+        oracle.poisson_jacobi(exp, f, 1e-300, max_sweeps=8, check=8)
+        assert np.array_equal(u1[k], exp), n
+        assert r1[k].sweeps == r0[k].sweeps + 8
+        check_solution(oracle, problems[k], u1[k], r1[k])
+        assert r1[k].residual_norm <= r0[k].residual_norm
