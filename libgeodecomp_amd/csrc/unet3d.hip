@@ -221,7 +221,14 @@ __device__ __forceinline__ uint4 load_global16(gh_ptr p) {
     return make_uint4(v.x, v.y, v.z, v.w);
 }
 
-template <int NT, bool VEC>
+// IN: how the input is staged -- kInVec, 16-byte loads (cin % 8 == 0);
+// kInMasked, 16-byte loads of a row whose channels past cin (< 8) are
+// another tensor's or padding and are zeroed; kInElem, one load per
+// channel.  VST: the output tile goes out as 16-byte stores through LDS
+// (plain BF16 output of full cout tiles, no accumulation, no FP32 copy).
+constexpr int kInElem = 0, kInVec = 1, kInMasked = 2;
+
+template <int NT, int IN, bool VST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv3d_fwd_lds(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
                                                       const u16 *__restrict__ W, int Kp,
                                                       const float *__restrict__ bias, int cout, u16 *Y, int ldy,
@@ -262,8 +269,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
 #pragma unroll
         for (int p = 0; p < PER; ++p) {
             Frag f;
-            if (VEC) {
+            if (IN != kInElem) {
                 f.q = load_global16(sok[p] ? base + soff[p] : zero);
+                if (IN == kInMasked) {
+#pragma unroll
+                    for (int e = 0; e < 8; ++e)
+                        if (e >= cin) f.h[e] = 0;
+                }
             } else {
                 // channels beyond cin (and voxels outside) read zeros
 #pragma unroll
@@ -310,7 +322,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
                 const int co = t * 16 + i;
                 const bool ok = tok[j] && co < cout;
                 const gh_ptr src = (gh_ptr)W + (int64_t)co * Kp + (4 * j + q) * cin + 8 * c;
-                if (VEC) {
+                if (IN == kInVec) {
                     bf[j][t].q = load_global16(ok ? src : zero);
                 } else {
 #pragma unroll
@@ -370,6 +382,40 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
     // D[r][c] in lane c + 16 (r / 4), element r % 4: voxel x0 + 4q + e of the
     // row, channel t * 16 + i
+    if (VST) {
+        // through LDS (the staging buffers are free after the last barrier):
+        // [voxel of the block][16 NT channels] BF16, then 16-byte stores
+        constexpr int CH = 16 * NT;
+        u16 *otile = reinterpret_cast<u16 *>(&tile[0][0]);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int co = t * 16 + i;
+            const float bv = bias ? bias[co] : 0.0f;
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const int vt = ((wave * (TZ / 4) + r / 8) * 8 + r % 8) * 16 + 4 * q;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float o = acc[r][t][e] + bv;
+                    if (relu) o = o > 0.0f ? o : 0.0f;
+                    otile[(vt + e) * CH + co] = f2bf(o);
+                }
+            }
+        }
+        __syncthreads();
+        constexpr int CHUNKS = 16 * 8 * TZ * CH / 8;  // 16-byte pieces of the block
+#pragma unroll
+        for (int k = tid; k < CHUNKS; k += 256) {
+            const int vt = k / (CH / 8), part = k % (CH / 8);
+            const int x = x0 + vt % 16, y = y0 + (vt / 16) % 8, z = z0 + vt / 128;
+            if (x < n && y < n && z < n) {
+                const int64_t v = ((int64_t)z * n + y) * n + x;
+                *reinterpret_cast<uint4 *>(Y + v * ldy + offy + 8 * part) =
+                    *reinterpret_cast<const uint4 *>(otile + vt * CH + 8 * part);
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         const int co = t * 16 + i;
@@ -395,6 +441,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
             }
         }
     }
+}
+
+// 1x1x1 convolution to a single output channel (the U-Net's output layer):
+// one thread per voxel, its cin channels in 16-byte loads (cin % 8 == 0), an
+// FP32 dot product with the cin weights.  An MFMA tile would compute 16
+// output channels to keep one, and the layer is memory bound.
+__global__ __launch_bounds__(256) void conv1x1_to1(const u16 *__restrict__ X, int64_t nvox, int ldx, int offx,
+                                                   int cin, const u16 *__restrict__ W,
+                                                   const float *__restrict__ bias, u16 *Y, int ldy, int offy,
+                                                   float *Y32, int ld32, int relu, int accumulate) {
+    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (v >= nvox) return;
+    const u16 *xr = X + v * ldx + offx;
+    float acc = 0.0f;
+    for (int c = 0; c < cin; c += 8) {
+        Frag f, w;
+        f.q = *reinterpret_cast<const uint4 *>(xr + c);
+        w.q = *reinterpret_cast<const uint4 *>(W + c);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) acc += bf2f(f.h[e]) * bf2f(w.h[e]);
+    }
+    float o = acc + (bias ? bias[0] : 0.0f);
+    if (relu) o = o > 0.0f ? o : 0.0f;
+    if (Y) {
+        u16 *pp = Y + v * ldy + offy;
+        if (accumulate) o += bf2f(*pp);
+        *pp = f2bf(o);
+    }
+    if (Y32) Y32[v * ld32] = o;
 }
 
 // the LDS-staged kernel takes 3x3x3 layers with up to 32 output channels on
@@ -973,16 +1048,33 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
     const u16 *Wb = (const u16 *)w;
     u16 *Y = (u16 *)y;
     const int64_t nvox = (int64_t)n * n * n;
+    if (ksize == 1 && cout == 1 && (cin & 7) == 0) {  // ldx, offx, alignment checked above
+        conv1x1_to1<<<(unsigned)((nvox + 255) / 256), 256, 0, s>>>(X, nvox, ldx, offx, cin, Wb, bias, Y, ldy, offy,
+                                                                    y32, ld32, relu, accumulate);
+        LGD_LAUNCHED("conv1x1_to1");
+        return 0;
+    }
     if (ksize == 3 && nt <= 2 && n >= kLdsMinN) {
         const int bx = (n + kLdsTX - 1) / kLdsTX, by = (n + kLdsTY - 1) / kLdsTY;
         const int bz = (n + 8 / nt - 1) / (8 / nt);
         const unsigned blocks = (unsigned)((int64_t)bx * by * bz);
-        const bool vec = (cin & 7) == 0;  // ldx, offx and the alignment were checked above
-#define LGD_LDS_CASE(T, V)                                                                                  \
-    if (nt == T && vec == V)                                                                                \
-        conv3d_fwd_lds<T, V><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, y32, \
-                                                    ld32, relu, accumulate, bx, by);
-        LGD_LDS_CASE(1, true) LGD_LDS_CASE(1, false) LGD_LDS_CASE(2, true) LGD_LDS_CASE(2, false)
+        // input: 16-byte loads when cin % 8 == 0 (ldx, offx and the alignment
+        // were checked above), or for cin < 8 when the 8-channel row
+        // [offx, offx + 8) lies inside the voxel's ldx channels and is aligned
+        const bool aligned_row = cin < 8 && (ldx & 7) == 0 && (offx & 7) == 0 && offx + 8 <= ldx &&
+                                 ((uintptr_t)x & 15) == 0;
+        const int in = (cin & 7) == 0 ? kInVec : (aligned_row ? kInMasked : kInElem);
+        // output: 16-byte stores of full cout tiles into an aligned slice
+        const bool vst = y && !y32 && !accumulate && cout == 16 * nt && (ldy & 7) == 0 && (offy & 7) == 0 &&
+                         ((uintptr_t)y & 15) == 0;
+#define LGD_LDS_CASE(T, I, V)                                                                                   \
+    if (nt == T && in == I && vst == V)                                                                         \
+        conv3d_fwd_lds<T, I, V><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, y32, \
+                                                       ld32, relu, accumulate, bx, by);
+        LGD_LDS_CASE(1, kInVec, true) LGD_LDS_CASE(1, kInVec, false) LGD_LDS_CASE(1, kInMasked, true)
+        LGD_LDS_CASE(1, kInMasked, false) LGD_LDS_CASE(1, kInElem, true) LGD_LDS_CASE(1, kInElem, false)
+        LGD_LDS_CASE(2, kInVec, true) LGD_LDS_CASE(2, kInVec, false) LGD_LDS_CASE(2, kInMasked, true)
+        LGD_LDS_CASE(2, kInMasked, false) LGD_LDS_CASE(2, kInElem, true) LGD_LDS_CASE(2, kInElem, false)
 #undef LGD_LDS_CASE
         LGD_LAUNCHED("conv3d_fwd_lds");
         return 0;

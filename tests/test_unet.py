@@ -116,6 +116,62 @@ def test_conv_forward_vs_oracle(hip, oracle, n, cin, cout, ksize):
 
 
 @pytest.mark.gpu
+def test_conv_input_row_wider_than_cin(hip, oracle):
+    """3 input channels in an 8-channel row whose other 5 hold junk (the
+    U-Net's input tensor keeps its 3 fields in an 8-channel row): the LDS
+    kernel loads whole 16-byte rows and must zero the channels past cin."""
+    rng = np.random.default_rng(31)
+    n, cin, ld, cout = 20, 3, 8, 16
+    row = bf16_round(rng.standard_normal((n, n, n, ld)).astype(np.float32) * 100)
+    x = row[..., :cin].copy()
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / 9).astype(np.float32))
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, 3)
+    xb, wb = dev(hip, to_bf16_bits(row)), dev(hip, to_bf16_bits(conv_weights_layout(w, kp)))
+    y32 = hip.DeviceBuffer(n ** 3 * cout * 4)
+    hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, ld, 0, cin, 3, wb.ptr, None, cout, None, cout, 0, y32.ptr,
+             cout, 0, 0, None)
+    out = np.empty(n ** 3 * cout, np.float32)
+    y32.download(out)
+    ref = oracle.conv3d_forward(x.astype(np.float64), w.astype(np.float64), None, 3)
+    mag = oracle.conv3d_forward(np.abs(x).astype(np.float64), np.abs(w).astype(np.float64), None, 3)
+    assert np.all(np.abs(out.reshape(ref.shape) - ref) <= (27 * cin + 2) * 2.0 ** -24 * mag + 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout,ld,off", [(17, 16, 16, 16, 0), (19, 48, 16, 48, 32), (18, 32, 32, 32, 0),
+                                               (16, 16, 32, 96, 64)])
+def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout, ld, off):
+    """BF16 output alone (written through LDS as 16-byte stores) is byte-for-
+    byte the output written next to an FP32 copy (per-element stores), bias
+    and ReLU included; the slice's other channels are untouched."""
+    rng = np.random.default_rng(n + cin + cout + ld)
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / 10).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, 3)
+    xb, wb, bb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(conv_weights_layout(w, kp))), dev(hip, b)
+    init = to_bf16_bits(rng.standard_normal((n, n, n, ld)).astype(np.float32))
+    outs = []
+    for with32 in (False, True):
+        y = dev(hip, init)
+        y32 = hip.DeviceBuffer(n ** 3 * cout * 4) if with32 else None
+        hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, cin, 0, cin, 3, wb.ptr, bb.ptr, cout, y.ptr, ld, off,
+                 y32.ptr if with32 else None, cout, 1, 0, None)
+        host = np.empty(n ** 3 * ld, np.uint16)
+        y.download(host)
+        outs.append(host.reshape(n, n, n, ld))
+    assert np.array_equal(outs[0], outs[1])
+    other = np.ones(ld, bool)
+    other[off:off + cout] = False
+    assert np.array_equal(outs[0][..., other], init.reshape(n, n, n, ld)[..., other])
+    ref = np.maximum(oracle.conv3d_forward(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64), 3), 0)
+    got = from_bf16_bits(outs[0][..., off:off + cout]).astype(np.float64)
+    assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-3)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("n", [8, 19])
 def test_conv_relu_slice_accumulate(hip, oracle, n):
     rng = np.random.default_rng(9 + n)
