@@ -443,6 +443,164 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
+// Z-marching LDS convolution for 3x3x3 layers with up to 16 output channels
+// and cin = 8 G input channels (G <= 6; G = 1 also takes cin < 8 rows, see
+// IN).  A work-group owns a 16 x 8 column of output voxels and marches z
+// over a chunk of ZC planes.  Input planes (18 x 10 voxels with their halo,
+// all cin channels) pass through a ring of 4 LDS slots: output plane z reads
+// input planes z-1, z, z+1 while plane z+2, loaded into registers one step
+// earlier, is written into the fourth slot -- every input voxel is fetched
+// once per column, whole channel rows together (the tile kernel above takes
+// the channels 8 at a time, re-reading each voxel's row G times).  The G x 7
+// weight fragments of the lane stay in registers for the whole march; wave w
+// computes output rows y0 + 2w and y0 + 2w + 1 of each plane.  The output
+// plane goes out through LDS as 16-byte stores one step later.  k order as
+// conv3d_fwd_lds (group-major, then tap).
+constexpr int kZmChunk = 16;  // output planes per work-group
+
+template <int G, int IN>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void conv3d_fwd_zm(
+    const u16 *__restrict__ X, int n, int ldx, int offx, int cin, const u16 *__restrict__ W, int Kp,
+    const float *__restrict__ bias, int cout, u16 *Y, int ldy, int offy, int relu, int bx, int by) {
+    constexpr int SX = 18, SY = 10, PV = SX * SY;  // staged voxels per plane
+    constexpr int CHUNKS = PV * G;                 // 16-byte pieces per plane
+    constexpr int PER = (CHUNKS + 255) / 256;
+    __shared__ uint4 ring[4][CHUNKS];               // [slot][group][voxel]
+    __shared__ u16 oplane[2][8 * 16 * 16];          // [y][x][channel] of an output plane
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, q = lane >> 4;
+    int b = blockIdx.x;
+    const int x0 = (b % bx) * 16;
+    b /= bx;
+    const int y0 = (b % by) * 8;
+    const int z0 = (b / by) * kZmChunk;
+    const int zend = min(z0 + kZmChunk, n);
+
+    const gh_ptr zero = (gh_ptr)(const void *)kZero16;
+    const gh_ptr xg = (gh_ptr)X + offx;
+    // this thread's pieces of a plane: voxel sv = k / G, group g = k % G
+    // (consecutive threads read consecutive 16 bytes of the voxels' rows)
+    int pofs[PER];   // element offset within a plane's (x, y) footprint
+    bool pok[PER];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int k = tid + 256 * p;
+        const int sv = k / G, g = k % G;
+        const int xx = x0 + sv % SX - 1, yy = y0 + sv / SX - 1;
+        pok[p] = k < CHUNKS && (unsigned)xx < (unsigned)n && (unsigned)yy < (unsigned)n;
+        pofs[p] = pok[p] ? (yy * n + xx) * ldx + 8 * g : 0;
+    }
+    uint4 stage[PER];
+    auto plane_load = [&](int z) {
+        const bool zok = (unsigned)z < (unsigned)n;
+        const gh_ptr base = xg + (int64_t)(zok ? z : 0) * n * n * ldx;
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            Frag f;
+            f.q = load_global16(pok[p] && zok ? base + pofs[p] : zero);
+            if (IN == kInMasked) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    if (e >= cin) f.h[e] = 0;
+            }
+            stage[p] = f.q;
+        }
+    };
+    auto plane_store = [&](int slot) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int k = tid + 256 * p;
+            if (k < CHUNKS) ring[slot][(k % G) * PV + k / G] = stage[p];
+        }
+    };
+
+    // weights: group g, k step j -> tap 4j + q, channels 8g..8g+7 of output
+    // channel i (zero for the 28th tap and past cout / cin)
+    Frag bf[G][7];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int j = 0; j < 7; ++j) {
+            const bool ok = 4 * j + q < 27 && i < cout;
+            const gh_ptr src = (gh_ptr)W + (int64_t)i * Kp + (4 * j + q) * cin + 8 * g;
+            if (IN == kInVec) {
+                bf[g][j].q = load_global16(ok ? src : zero);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bf[g][j].h[e] = *(ok && e < cin ? src + e : zero);
+            }
+        }
+    // this lane group's taps: ring slot offset (dz + 1), row / column offset
+    int tslot[7], toff[7];
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const int tap = 4 * j + q;
+        int dz, dy, dx;
+        tap_offset(tap < 27 ? tap : 13, 3, dz, dy, dx);
+        tslot[j] = dz + 1;
+        toff[j] = (dy + 1) * SX + (dx + 1) + i;
+    }
+    const float bv = bias && i < cout ? bias[i] : 0.0f;
+
+    // prologue: planes z0-1, z0, z0+1 in slots 0..2, z0+2 in flight
+    plane_load(z0 - 1);
+    plane_store(0);
+    plane_load(z0);
+    plane_store(1);
+    plane_load(z0 + 1);
+    plane_store(2);
+    plane_load(z0 + 2);
+    __syncthreads();
+    for (int z = z0, st = 0; z < zend; ++z, ++st) {
+        // the previous plane's output tile goes out (written before the barrier)
+        if (st > 0) {
+            const int ob = (st - 1) & 1;
+            const int vt = tid >> 1, part = tid & 1;  // 128 voxels x 2 halves of 16 channels
+            const int x = x0 + vt % 16, y = y0 + vt / 16;
+            if (x < n && y < n)
+                *reinterpret_cast<uint4 *>(Y + (((int64_t)(z - 1) * n + y) * n + x) * ldy + offy + 8 * part) =
+                    *reinterpret_cast<const uint4 *>(&oplane[ob][vt * 16 + 8 * part]);
+        }
+        f32x4 acc[2];
+        acc[0] = acc[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+#pragma unroll
+            for (int j = 0; j < 7; ++j) {
+                const uint4 *pl = &ring[(st + tslot[j]) & 3][g * PV];
+#pragma unroll
+                for (int r = 0; r < 2; ++r) {
+                    Frag a;
+                    a.q = pl[(2 * wave + r) * SX + toff[j]];
+                    acc[r] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, bf[g][j].v, acc[r], 0, 0, 0);
+                }
+            }
+        // output plane z into LDS: row 2w + r, voxels 4q + e, channel i
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                float o = acc[r][e] + bv;
+                if (relu) o = o > 0.0f ? o : 0.0f;
+                oplane[st & 1][((2 * wave + r) * 16 + 4 * q + e) * 16 + i] = f2bf(o);
+            }
+        // plane z+2 into the free slot, plane z+3 in flight
+        plane_store((st + 3) & 3);
+        plane_load(z + 3);
+        __syncthreads();
+    }
+    // the last plane's output
+    {
+        const int st = zend - z0;
+        const int ob = (st - 1) & 1;
+        const int vt = tid >> 1, part = tid & 1;
+        const int x = x0 + vt % 16, y = y0 + vt / 16;
+        if (x < n && y < n)
+            *reinterpret_cast<uint4 *>(Y + (((int64_t)(zend - 1) * n + y) * n + x) * ldy + offy + 8 * part) =
+                *reinterpret_cast<const uint4 *>(&oplane[ob][vt * 16 + 8 * part]);
+    }
+}
+
 // 1x1x1 convolution to a single output channel (the U-Net's output layer):
 // one thread per voxel, its cin channels in 16-byte loads (cin % 8 == 0), an
 // FP32 dot product with the cin weights.  An MFMA tile would compute 16
@@ -1053,6 +1211,28 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
                                                                     y32, ld32, relu, accumulate);
         LGD_LAUNCHED("conv1x1_to1");
         return 0;
+    }
+    // z-marching kernel: 16 output channels written as plain BF16 into an
+    // aligned slice, input of at most 6 groups of 8 channels (or a masked
+    // row of fewer than 8)
+    if (ksize == 3 && cout == 16 && n >= kLdsMinN && y && !y32 && !accumulate && (ldy & 7) == 0 &&
+        (offy & 7) == 0 && ((uintptr_t)y & 15) == 0) {
+        const bool masked = cin < 8 && (ldx & 7) == 0 && (offx & 7) == 0 && offx + 8 <= ldx &&
+                            ((uintptr_t)x & 15) == 0;
+        const int groups = (cin & 7) == 0 ? cin / 8 : (masked ? 1 : 0);
+        if (groups >= 1 && groups <= 6) {
+            const int bx = (n + 15) / 16, by = (n + 7) / 8, bz = (n + kZmChunk - 1) / kZmChunk;
+            const unsigned blocks = (unsigned)((int64_t)bx * by * bz);
+#define LGD_ZM_CASE(G, I)                                                                                    \
+    if (groups == G && (I == kInMasked) == masked)                                                           \
+        conv3d_fwd_zm<G, I><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, relu, \
+                                                   bx, by);
+            LGD_ZM_CASE(1, kInMasked) LGD_ZM_CASE(1, kInVec) LGD_ZM_CASE(2, kInVec) LGD_ZM_CASE(3, kInVec)
+            LGD_ZM_CASE(4, kInVec) LGD_ZM_CASE(5, kInVec) LGD_ZM_CASE(6, kInVec)
+#undef LGD_ZM_CASE
+            LGD_LAUNCHED("conv3d_fwd_zm");
+            return 0;
+        }
     }
     if (ksize == 3 && nt <= 2 && n >= kLdsMinN) {
         const int bx = (n + kLdsTX - 1) / kLdsTX, by = (n + kLdsTY - 1) / kLdsTY;

@@ -139,25 +139,32 @@ def test_conv_input_row_wider_than_cin(hip, oracle):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,cin,cout,ld,off", [(17, 16, 16, 16, 0), (19, 48, 16, 48, 32), (18, 32, 32, 32, 0),
-                                               (16, 16, 32, 96, 64)])
-def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout, ld, off):
-    """BF16 output alone (written through LDS as 16-byte stores) is byte-for-
-    byte the output written next to an FP32 copy (per-element stores), bias
-    and ReLU included; the slice's other channels are untouched."""
+@pytest.mark.parametrize("n,cin,cout,ld,off,xld", [(17, 16, 16, 16, 0, 16), (19, 48, 16, 48, 32, 48),
+                                                   (33, 40, 16, 16, 0, 40), (21, 3, 16, 16, 0, 8),
+                                                   (18, 32, 32, 32, 0, 32), (16, 16, 32, 96, 64, 16)])
+def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout, ld, off, xld):
+    """BF16 output alone (16 output channels: the z-marching kernel; 32: the
+    tile kernel's LDS-transposed 16-byte stores) is byte-for-byte the output
+    written next to an FP32 copy (the tile kernel with per-element stores),
+    bias and ReLU included, partial z chunks (n = 33) and a 3-channel input
+    in an 8-channel row of junk (xld = 8) too; the slice's other channels
+    are untouched."""
     rng = np.random.default_rng(n + cin + cout + ld)
-    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    row = bf16_round(rng.standard_normal((n, n, n, xld)).astype(np.float32) * 50)
+    row[..., :cin] = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    x = row[..., :cin].copy()
     w = bf16_round((rng.standard_normal((cout, 27, cin)) / 10).astype(np.float32))
     b = rng.standard_normal(cout).astype(np.float32)
     lib = hip.load()
     kp = lib.lgdhip_conv3d_kpad(cin, 3)
-    xb, wb, bb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(conv_weights_layout(w, kp))), dev(hip, b)
+    xb, wb, bb = dev(hip, to_bf16_bits(row)), dev(hip, to_bf16_bits(conv_weights_layout(w, kp))), dev(hip, b)
     init = to_bf16_bits(rng.standard_normal((n, n, n, ld)).astype(np.float32))
     outs = []
     for with32 in (False, True):
         y = dev(hip, init)
         y32 = hip.DeviceBuffer(n ** 3 * cout * 4) if with32 else None
-        hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, cin, 0, cin, 3, wb.ptr, bb.ptr, cout, y.ptr, ld, off,
+        print("[case] conv output paths", n, cin, cout, ld, off, xld, with32, flush=True)
+        hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, xld, 0, cin, 3, wb.ptr, bb.ptr, cout, y.ptr, ld, off,
                  y32.ptr if with32 else None, cout, 1, 0, None)
         host = np.empty(n ** 3 * ld, np.uint16)
         y.download(host)
