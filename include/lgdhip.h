@@ -116,6 +116,8 @@ const char *lgdhip_error_string(int code);
 const char *lgdhip_last_error(void);
 int lgdhip_get_device_count(int *count);
 int lgdhip_set_device(int device);
+/* The calling thread's current device. */
+int lgdhip_get_device(int *device);
 int lgdhip_device_name(int device, char *buf, int len);
 int lgdhip_device_synchronize(void);
 

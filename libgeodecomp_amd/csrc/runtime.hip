@@ -170,6 +170,12 @@ int lgdhip_set_device(int device) {
     return 0;
 }
 
+int lgdhip_get_device(int *device) {
+    LGD_REQUIRE(device, "NULL pointer");
+    LGD_HIP(hipGetDevice(device));
+    return 0;
+}
+
 int lgdhip_device_name(int device, char *buf, int len) {
     LGD_REQUIRE(buf && len > 0, "bad output buffer");
     hipDeviceProp_t prop;

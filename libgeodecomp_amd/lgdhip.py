@@ -106,6 +106,7 @@ SIGNATURES = {
     "lgdhip_memcpy_d2h": (_I, [_P, _P, _SZ, _P]),
     "lgdhip_memcpy_d2d": (_I, [_P, _P, _SZ, _P]),
     "lgdhip_memcpy2d_d2h": (_I, [_P, _SZ, _P, _SZ, _SZ, _SZ, _P]),
+    "lgdhip_get_device": (_I, [_IP]),
     "lgdhip_fill_u8": (_I, [_P, C.c_uint8, _SZ, _P]),
     "lgdhip_copy_kernel": (_I, [_P, _P, _SZ, _P]),
     "lgdhip_stream_create": (_I, [C.POINTER(_P)]),

@@ -5,7 +5,9 @@
 // vary: work-group shape, temporal-blocking depth, kernel variant):
 //   Jacobi3DF64   Variant in the order 1 2 3 7 4 8 5 6 9 10: single sweeps,
 //                 then temporal blocking of growing depth (K = 2, 2, 3, 3,
-//                 4), then K = 4 with two rows per thread (16-, 32-row tiles)
+//                 4), then K = 4 with two rows per thread (16-, 32-row tiles);
+//                 ZChunk {0, 8, 16, 26, 32, 48, 64}: z planes per work-group
+//                 (0: the launch cost model's choice, the starting point)
 //   Life2D        StepsPerLaunch [1, 33), Variant {1..8}
 //   LBMD3Q19F32   BlockDimY {1, 2, 4, 8}, Variant {0, 1}
 //   Generic       BlockDimX {64, 128, 256}, BlockDimY {1, 2, 4, 8}
@@ -52,11 +54,16 @@ template<typename CELL, typename VALUE, VALUE CELL::*MEMBER>
 class TuningSpace<Jacobi3DF64<CELL, VALUE, MEMBER> >
 {
 public:
-    static void add(SimulationParameters *p) { p->addParameter("Variant", std::vector<int>{1, 2, 3, 7, 4, 8, 5, 6, 9, 10}); }
+    static void add(SimulationParameters *p)
+    {
+        p->addParameter("Variant", std::vector<int>{1, 2, 3, 7, 4, 8, 5, 6, 9, 10});
+        p->addParameter("ZChunk", std::vector<int>{0, 8, 16, 26, 32, 48, 64});
+    }
     static LaunchChoice choose(const SimulationParameters& p)
     {
         LaunchChoice c;
         c.variant = (int)p["Variant"];
+        c.blockSize = Coord<3>(0, 0, (int)p["ZChunk"]);
         return c;
     }
 };

@@ -227,6 +227,16 @@ public:
     const Parameter& operator[](std::size_t i) const { return *parameters.at(i); }
 
     std::size_t size() const { return parameters.size(); }
+    // the name of parameter i (parameters are indexed in the order added)
+    std::string name(std::size_t i) const
+    {
+        for (std::map<std::string, std::size_t>::const_iterator n = names.begin(); n != names.end(); ++n) {
+            if (n->second == i) {
+                return n->first;
+            }
+        }
+        throw std::out_of_range("no parameter with this index");
+    }
     bool contains(const std::string& name) const { return names.count(name) != 0; }
 
     std::string toString() const

@@ -524,6 +524,46 @@ LGD_TEST(autotuning_simulator_jacobi_and_life)
     CHECK(sameGrid(life.getGrid(), lref.getGrid()));
 }
 
+LGD_TEST(autotuning_cache_reuses_parameters)
+{
+    // the first run tunes and stores its result; a second run of the same
+    // cell, grid and device takes it from the file without tuning and still
+    // equals the oracle; another grid is another key
+    const std::string path = "tuning_cache.txt";
+    std::remove(path.c_str());
+    AutoTuningSimulator<JacobiCell, PatternOptimizer> first(new JacobiInitializer(Coord<3>(48, 40, 32), 20, 2, 0.5),
+                                                            4);
+    first.setCacheFile(path);
+    first.run();
+    CHECK(!first.usedCache());
+    CHECK(first.stepsPerTuningRun() >= 1u);
+    AutoTuningSimulator<JacobiCell, PatternOptimizer> second(new JacobiInitializer(Coord<3>(48, 40, 32), 20, 2, 0.5),
+                                                             4);
+    second.setCacheFile(path);
+    second.run();
+    CHECK(second.usedCache());
+    CHECK_EQ(second.stepsPerTuningRun(), 0u);
+    CHECK(second.bestParameters().toString() == first.bestParameters().toString());
+    std::printf("  cached %s\n", second.bestParameters().toString().c_str());
+    SerialSimulator<JacobiCell> ref(new JacobiInitializer(Coord<3>(48, 40, 32), 20, 2, 0.5));
+    ref.run();
+    CHECK(sameGrid(second.getGrid(), ref.getGrid()));
+    AutoTuningSimulator<JacobiCell, PatternOptimizer> other(new JacobiInitializer(Coord<3>(40, 40, 32), 20, 2, 0.5),
+                                                            2);
+    other.setCacheFile(path);
+    other.run();
+    CHECK(!other.usedCache());
+    std::ifstream f(path.c_str());
+    std::string line;
+    int lines = 0;
+    while (std::getline(f, line)) {
+        ++lines;
+        CHECK(line.find('\t') != std::string::npos);
+    }
+    CHECK_EQ(lines, 2);
+    std::remove(path.c_str());
+}
+
 // ---------------------------------------------------------------------
 // steerer edits travel as region uploads (only the changed streaks)
 // ---------------------------------------------------------------------
