@@ -189,8 +189,8 @@ class Life:
     dtype = "u8"
     bytes_per_lup = 2.0  # 1 B read + 1 B write of the uint8 cell format
 
-    def __init__(self, n=16384, generations=1000, seed=synthetic.DEFAULT_SEED, per_launch=31, variant=7):
-        self.n, self.gens, self.seed = n, generations, seed
+    def __init__(self, n=16384, generations=1000, seed=synthetic.DEFAULT_SEED, per_launch=31, variant=7, wrap=0):
+        self.n, self.gens, self.seed, self.wrap = n, generations, seed, wrap
         self.tile = lgdhip.tile(steps_per_launch=per_launch, variant=variant)
         self.per_launch = per_launch
         self.stream = lgdhip.Stream()
@@ -207,10 +207,17 @@ class Life:
         self.ws = lgdhip.DeviceBuffer(self.ws_bytes)
         self.host = host
 
+    def topology(self):
+        return {0: "dead Cube edge", 1: "x wrapped, dead edge in y", 2: "y wrapped, dead edge in x",
+                3: "Torus"}[self.wrap]
+
     def config(self):
-        return {"workload": f"Conway Game of Life 2D uint8 {self.n}x{self.n}, Bernoulli(0.3), dead Cube edge, "
-                            f"{self.gens} generations", "grid": [self.n, self.n], "generations": self.gens,
-                "seed": self.seed, "generations_per_launch": self.per_launch, "parallelism": "single GPU"}
+        c = {"workload": f"Conway Game of Life 2D uint8 {self.n}x{self.n}, Bernoulli(0.3), {self.topology()}, "
+                         f"{self.gens} generations", "grid": [self.n, self.n], "generations": self.gens,
+             "seed": self.seed, "generations_per_launch": self.per_launch, "parallelism": "single GPU"}
+        if self.wrap:
+            c["wrap_mask"] = self.wrap
+        return c
 
     def lups_per_step(self):
         return float(self.n) * self.n * self.gens
@@ -218,8 +225,17 @@ class Life:
     def step(self):
         s = self.stream.h
         lgdhip.call("lgdhip_memcpy_d2d", self.work.ptr, self.pristine.ptr, self.nbytes, s)
-        lgdhip.call("lgdhip_life2d_run", self.work.ptr, C.byref(self.g), 0, self.gens, self.ws.ptr,
-                    self.ws_bytes, C.byref(self.tile), s)
+        if not self.wrap:
+            lgdhip.call("lgdhip_life2d_run", self.work.ptr, C.byref(self.g), 0, self.gens, self.ws.ptr,
+                        self.ws_bytes, C.byref(self.tile), s)
+            return
+        # periodic: the same pack -> generations -> unpack as lgdhip_life2d_run
+        pa, pb = self.ws.value, self.ws.value + self.ws_bytes // 2
+        in_b = C.c_int()
+        lgdhip.call("lgdhip_life2d_pack", self.work.ptr, C.byref(self.g), C.c_void_p(pa), s)
+        lgdhip.call("lgdhip_life2d_packed_run_periodic", C.c_void_p(pa), C.c_void_p(pb), C.byref(self.g), 0,
+                    self.gens, self.wrap, C.byref(self.tile), C.byref(in_b), s)
+        lgdhip.call("lgdhip_life2d_unpack", C.c_void_p(pb if in_b.value else pa), C.byref(self.g), self.work.ptr, s)
 
     def dominant_kernel(self):
         """The roofline of the generation kernel, timed over a block of
@@ -231,9 +247,14 @@ class Life:
         launches = 64
         gens = launches * self.per_launch
         in_b = C.c_int()
-        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_life2d_packed_run", C.c_void_p(pa), C.c_void_p(pb),
-                                                    C.byref(self.g), 0, gens, C.byref(self.tile),
-                                                    C.byref(in_b), s))
+        if self.wrap:
+            ms = timed(self.stream, lambda: lgdhip.call("lgdhip_life2d_packed_run_periodic", C.c_void_p(pa),
+                                                        C.c_void_p(pb), C.byref(self.g), 0, gens, self.wrap,
+                                                        C.byref(self.tile), C.byref(in_b), s))
+        else:
+            ms = timed(self.stream, lambda: lgdhip.call("lgdhip_life2d_packed_run", C.c_void_p(pa), C.c_void_p(pb),
+                                                        C.byref(self.g), 0, gens, C.byref(self.tile),
+                                                        C.byref(in_b), s))
         per_launch_bytes = self.bytes_per_lup * self.n * self.n * self.per_launch
         # variant 0 is the library default (a stacked-wave kernel)
         name = "life_packed_tb" if 1 <= self.tile.variant <= 4 else "life_packed_stacked"
@@ -245,7 +266,8 @@ class Life:
     def cpu_baseline(self, budget_s=10.0):
         oracle = load_oracle()
         sample = self.host[:4096, :4096].copy()
-        rates = cpu_rates(lambda th, k: oracle.life2d(sample, k, 0, th), 4, budget_s, 4000, sample.size)
+        rates = cpu_rates(lambda th, k: oracle.life2d(sample, k, 0, th, wrap=self.wrap), 4, budget_s, 4000,
+                          sample.size)
         return cpu_entry(rates, "C oracle (byte-per-cell rule) on the top-left 4096x4096 of the same input")
 
 
@@ -257,8 +279,8 @@ class Jacobi:
     dtype = "f64"
     bytes_per_lup = 16.0
 
-    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, variant=0, by=0, bz=0, flags=0, **_):
-        self.n, self.sweeps, self.seed = n, steps, seed
+    def __init__(self, n=128, steps=100, seed=synthetic.DEFAULT_SEED, variant=0, by=0, bz=0, flags=0, wrap=0, **_):
+        self.n, self.sweeps, self.seed, self.wrap = n, steps, seed, wrap
         self.tile = lgdhip.tile(by=by, bz=bz, variant=variant, flags=flags)
         self.stream = lgdhip.Stream()
         self.g = lgdhip.make_grid(3, n, n, n, halo=1, elem_bytes=8)
@@ -273,8 +295,11 @@ class Jacobi:
         self.host = host
 
     def config(self):
-        c = {"workload": f"Jacobi3D 7-point FP64 {self.n}^3, zero Cube edge, U[0,1) interior, {self.sweeps} steps",
+        topo = "zero Cube edge" if not self.wrap else ("Torus" if self.wrap == 7 else f"wrap mask {self.wrap}, zero edge")
+        c = {"workload": f"Jacobi3D 7-point FP64 {self.n}^3, {topo}, U[0,1) interior, {self.sweeps} steps",
              "grid": [self.n] * 3, "steps": self.sweeps, "seed": self.seed, "parallelism": "single GPU"}
+        if self.wrap:
+            c["wrap_mask"] = self.wrap
         if getattr(self, "plan", None):
             c["launch"] = self.plan  # the library's plan: kernel variant and z planes per work-group
         return c
@@ -282,15 +307,23 @@ class Jacobi:
     def lups_per_step(self):
         return float(self.n) ** 3 * self.sweeps
 
+    def run(self, sweeps, s):
+        if self.wrap:
+            lgdhip.call("lgdhip_jacobi3d_f64_run_periodic", self.a.ptr, self.b.ptr, C.byref(self.g), sweeps,
+                        self.wrap, 0.0, C.byref(self.tile), None, s)
+        else:
+            lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), sweeps,
+                        C.byref(self.tile), None, s)
+
     def step(self):
         s = self.stream.h
         lgdhip.call("lgdhip_memcpy_d2d", self.a.ptr, self.pristine.ptr, self.nbytes, s)
-        lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr, C.byref(self.g), self.sweeps,
-                    C.byref(self.tile), None, s)
+        self.run(self.sweeps, s)
 
-    # sweeps per launch of the temporal-blocking variants (9 and 10 hold two
-    # rows per thread); the library resolves variant 0 by its launch plan
-    TB_DEPTH = {4: 2, 5: 3, 6: 4, 7: 2, 8: 3, 9: 4, 10: 4}
+    # sweeps per launch of the temporal-blocking variants (9, 10 and 12 hold
+    # two rows per thread; 11 and 12 run the same pipeline for one sweep);
+    # the library resolves variant 0 by its launch plan
+    TB_DEPTH = {4: 2, 5: 3, 6: 4, 7: 2, 8: 3, 9: 4, 10: 4, 11: 1, 12: 1}
 
     def dominant_kernel(self):
         v, zc = C.c_int(0), C.c_int(0)
@@ -298,11 +331,9 @@ class Jacobi:
         variant = v.value
         k = self.TB_DEPTH.get(variant, 1)
         sweeps = 240   # a multiple of every blocking depth: launches of one kernel only
-        ms = timed(self.stream, lambda: lgdhip.call("lgdhip_jacobi3d_f64_run", self.a.ptr, self.b.ptr,
-                                                    C.byref(self.g), sweeps, C.byref(self.tile), None,
-                                                    self.stream.h))
-        if k > 1:
-            name = "jacobi3d_f64_tb2" if variant in (9, 10) else "jacobi3d_f64_tb"
+        ms = timed(self.stream, lambda: self.run(sweeps, self.stream.h))
+        if variant in self.TB_DEPTH:
+            name = "jacobi3d_f64_tb2" if variant in (9, 10, 12) else "jacobi3d_f64_tb"
         else:
             name = "jacobi3d_f64_zmarch" if self.tile.variant == 1 else "jacobi3d_f64_zbatch"
         self.plan = {"variant": variant, "zchunk": zc.value}
@@ -313,8 +344,8 @@ class Jacobi:
     def cpu_baseline(self, budget_s=10.0):
         oracle = load_oracle()
         probe = max(1, int(16 * (128.0 / self.n) ** 3))  # about 1e-1 s serial
-        rates = cpu_rates(lambda th, k: oracle.jacobi3d(self.host, k, 0.0, th), probe, budget_s, 200000,
-                          self.host.size)
+        rates = cpu_rates(lambda th, k: oracle.jacobi3d(self.host, k, 0.0, th, wrap=self.wrap), probe, budget_s,
+                          200000, self.host.size)
         return cpu_entry(rates, f"C oracle, same {self.n}^3 input")
 
 
@@ -771,10 +802,12 @@ def main():
     ap.add_argument("--xcd-rows", action="store_true", help="lbm: x rows of tiles round-robin over the XCDs")
     ap.add_argument("--no-graph", action="store_true", help="poisson: plain launches instead of hipGraph chunks")
     ap.add_argument("--tol", type=float, default=1e-6, help="poisson: relative residual to solve to")
+    ap.add_argument("--wrap", type=int, default=0,
+                    help="jacobi (bits 1, 2, 4) / life (bits 1, 2): periodic axes (0: the Cube)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="default run: the main workload only")
     args = ap.parse_args()
-    default_run = args.workload is None and not (args.size or args.iters or args.variant >= 0)
+    default_run = args.workload is None and not (args.size or args.iters or args.variant >= 0 or args.wrap)
     if args.workload is None:
         args.workload = "life"
 
@@ -797,11 +830,11 @@ def main():
     if args.iters:
         kw["generations" if args.workload == "life" else "steps"] = args.iters
     if args.workload == "life":
-        kw.update(per_launch=args.per_launch, variant=7 if args.variant < 0 else args.variant)
+        kw.update(per_launch=args.per_launch, variant=7 if args.variant < 0 else args.variant, wrap=args.wrap)
     flags = (lgdhip.TILE_NO_XCD_SWIZZLE if args.no_xcd_swizzle else 0) | \
         (lgdhip.TILE_XCD_SWIZZLE if args.xcd_swizzle else 0) | (lgdhip.TILE_XCD_ROWS if args.xcd_rows else 0)
     if args.workload == "jacobi":
-        kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags)
+        kw.update(variant=max(0, args.variant), by=args.by, bz=args.bz, flags=flags, wrap=args.wrap)
     if args.workload == "lbm":
         kw.update(by=args.by, flags=flags, variant=max(0, args.variant), bx=args.bx, bz=args.bz)
         if args.layout:

@@ -240,7 +240,8 @@ int lgdhip_grid_store(const void *grid, const lgdhip_grid *g, void *host,
 /* Runs `steps` sweeps ping-ponging between a and b (both carry the edge
  * value in their halo).  *result_in_b is set to 1 when the final state is
  * in b, 0 when it is in a (tile->variant 4-10 run K = 2-4 sweeps per launch
- * and swap once per launch, so this is not steps % 2 there). */
+ * and swap once per launch, so this is not steps % 2 there; 11 and 12 run
+ * the same tile pipelines for one sweep per launch). */
 int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int steps,
                             const lgdhip_tile *tile, int *result_in_b,
                             lgdhip_stream_t stream);
@@ -252,6 +253,20 @@ int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int step
  * reference's CUDASimulator has no counterpart (its launch shape is the
  * block size it is given, cudasimulator.h:314-429). */
 int lgdhip_jacobi3d_f64_plan(const lgdhip_grid *g, const lgdhip_tile *tile, int *variant, int *zchunk);
+
+/* The same sweeps with periodic axes: wrap_mask bits 1, 2, 4 wrap x, y, z
+ * (Topologies::Torus and the selectively wrapped topologies,
+ * geometry/topologies.h:225-247,267-280: a neighbour past a wrapped face is
+ * the cell on the opposite face; past an unwrapped face it is `edge`).  The
+ * temporal-blocking variants take the wrapped neighbours straight from the
+ * interior; steps % K are single sweeps after a periodic halo refresh
+ * (lgdhip_grid_fill_halo).  Both buffers' halos are rewritten.  A wrapped z
+ * axis shorter than 8 cells runs single sweeps throughout.  Results equal
+ * `steps` single sweeps of the periodic stencil bit for bit; *result_in_b
+ * as for lgdhip_jacobi3d_f64_run.  Views are refused. */
+int lgdhip_jacobi3d_f64_run_periodic(double *a, double *b, const lgdhip_grid *g, int steps, int wrap_mask,
+                                     double edge, const lgdhip_tile *tile, int *result_in_b,
+                                     lgdhip_stream_t stream);
 
 /* ------------------------------------------------------------------ */
 /* Game of Life, uint8 cells (0/1), Moore<2,1>, Cube topology          */
@@ -273,6 +288,14 @@ int lgdhip_life2d_unpack(const uint32_t *packed, const lgdhip_grid *g, uint8_t *
 int lgdhip_life2d_packed_run(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g,
                              int edge_alive, int steps, const lgdhip_tile *tile,
                              int *result_in_b, lgdhip_stream_t stream);
+/* The same generations with periodic axes: wrap_mask bit 1 wraps x (needs
+ * nx % 32 == 0: the packed columns wrap word by word), bit 2 wraps y
+ * (Topologies::Torus and the cylinders, geometry/topologies.h:225-247,
+ * 267-280); cells past an unwrapped face hold the edge state.  The blocked
+ * kernels load the periodic images of rows and word columns directly. */
+int lgdhip_life2d_packed_run_periodic(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g, int edge_alive, int steps,
+                                      int wrap_mask, const lgdhip_tile *tile, int *result_in_b,
+                                      lgdhip_stream_t stream);
 /* uint8 grid in, uint8 grid out (in place): pack -> packed_run -> unpack.
  * `workspace` must hold lgdhip_life2d_workspace_bytes(g) bytes. */
 size_t lgdhip_life2d_workspace_bytes(const lgdhip_grid *g);

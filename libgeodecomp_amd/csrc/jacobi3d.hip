@@ -122,6 +122,22 @@ __device__ __forceinline__ double lane_next(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// Coordinates of the temporal-blocking kernels.  A tile reaches K cells
+// past the box; on a wrapped axis (Torus topologies) those cells are the
+// periodic images of interior cells and are computed like them, on the
+// others they hold the edge value.  Level-0 loads read the image on a
+// wrapped axis and clamp into the one-cell halo shell on the others.
+__device__ __forceinline__ bool in_axis(int c, int n, int wraps) { return wraps || (unsigned)c < (unsigned)n; }
+__device__ __forceinline__ int level0_coord(int c, int n, int wraps) {
+    return wraps ? ((c % n) + n) % n : min(max(c, -1), n);
+}
+// planes run from K before a z chunk to K + PF past it: one period's
+// correction reaches them when nz >= 8 (the host's condition for wrapped z)
+__device__ __forceinline__ int level0_plane(int p, int n, int wraps) {
+    if (wraps) return p < 0 ? p + n : (p >= n ? p - n : p);
+    return min(max(p, -1), n);
+}
+
 // K sweeps per pass over HBM (temporal blocking).  A work-group is a 64 x TY
 // tile of (x, y) columns whose outer K lanes/rows are an overlap ring
 // recomputed by the neighbouring tiles, so it writes (64-2K) x (TY-2K)
@@ -143,7 +159,8 @@ template <int K, int TY, int PF>
 __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restrict__ src, double *__restrict__ dst,
                                                            int nx, int ny, int nz, int64_t sx, int64_t sxy,
                                                            int64_t origin, int zchunk, unsigned gx, unsigned gy,
-                                                           unsigned ntiles, unsigned per, int swizzle) {
+                                                           unsigned ntiles, unsigned per, int swizzle, int wrap,
+                                                           int64_t edge_at) {
     constexpr int OX = 64 - 2 * K, OY = TY - 2 * K;
     static_assert(PF == 3, "level-0 ring of 6: a window of 3 planes plus 3 in flight");
     __shared__ double lds[2][K][TY][64];
@@ -154,19 +171,22 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
     const int y = (int)((t / gx) % gy) * OY - K + ty;
     const int z0 = (int)(t / (gx * gy)) * zchunk;
     const int z1 = min(z0 + zchunk, nz);
-    const bool inxy = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny;
-    const bool writer = inxy && tx >= K && tx < 64 - K && ty >= K && ty < TY - K;
-    const double edge = src[origin - 1];  // the Cube edge value held by every halo cell
+    // a cell counts as inside when every coordinate is in range or its axis wraps
+    const bool inxy = in_axis(x, nx, wrap & 1) && in_axis(y, ny, wrap & 2);
+    const bool writer = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && tx >= K && tx < 64 - K &&
+                        ty >= K && ty < TY - K;
+    const double edge = src[edge_at];  // a halo cell of an unwrapped axis: the edge value
     const int64_t col = origin + (int64_t)y * sx + x;
     const int yd = ty > 0 ? ty - 1 : 0;
     const int yu = ty < TY - 1 ? ty + 1 : TY - 1;
 
-    // level-0 loads are unconditional: coordinates outside the box are
-    // clamped into the one-cell halo shell, which holds the edge value, so
-    // no select or branch guards them and the compiler's wait counts let
-    // PF planes stay in flight
-    const int64_t colh = origin + (int64_t)min(max(y, -1), ny) * sx + min(max(x, -1), nx);
-    auto load0 = [&](int p) -> double { return src[colh + (int64_t)min(max(p, -1), nz) * sxy]; };
+    // level-0 loads are unconditional: coordinates outside the box map to
+    // their periodic image on a wrapped axis and are clamped into the
+    // one-cell halo shell (which holds the edge value) on the others, so no
+    // select or branch guards them and the compiler's wait counts let PF
+    // planes stay in flight
+    const int64_t colh = origin + (int64_t)level0_coord(y, ny, wrap & 2) * sx + level0_coord(x, nx, wrap & 1);
+    auto load0 = [&](int p) -> double { return src[colh + (int64_t)level0_plane(p, nz, wrap & 4) * sxy]; };
 
     // lv: level-0 planes, a ring of 6 (plane p in slot (p - pbeg) mod 6:
     // the window p-2 .. p plus the loads of p+1 .. p+3 in flight); w[l]
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
             s += xp;       // ( 1,  0,  0)
             s += yp;       // ( 0,  1,  0)
             s += b;        // ( 0,  0,  1)
-            const double v = (inxy && (unsigned)q < (unsigned)nz) ? s * (1.0 / 7.0) : edge;
+            const double v = (inxy && in_axis(q, nz, wrap & 4)) ? s * (1.0 / 7.0) : edge;
             if (l < K) {
                 w[l][NEW] = v;
             } else if (writer) {  // after the ramp q = p - K runs over z0 .. z1 - 1
@@ -262,7 +282,8 @@ template <int K, int TY>
 __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__restrict__ src, double *__restrict__ dst,
                                                             int nx, int ny, int nz, int64_t sx, int64_t sxy,
                                                             int64_t origin, int zchunk, unsigned gx, unsigned gy,
-                                                            unsigned ntiles, unsigned per, int swizzle) {
+                                                            unsigned ntiles, unsigned per, int swizzle, int wrap,
+                                                            int64_t edge_at) {
     constexpr int PF = 3;
     constexpr int CY = 2 * TY;  // tile rows
     constexpr int OX = 64 - 2 * K, OY = CY - 2 * K;
@@ -276,21 +297,21 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
     const int y0 = (int)((t / gx) % gy) * OY - K + 2 * ty;  // rows y0, y0 + 1
     const int z0 = (int)(t / (gx * gy)) * zchunk;
     const int z1 = min(z0 + zchunk, nz);
-    const bool xin = (unsigned)x < (unsigned)nx;
-    const bool inxy[2] = {xin && (unsigned)y0 < (unsigned)ny, xin && (unsigned)(y0 + 1) < (unsigned)ny};
-    const bool xw = tx >= K && tx < 64 - K;
-    const bool writer[2] = {inxy[0] && xw && 2 * ty >= K && 2 * ty < CY - K,
-                            inxy[1] && xw && 2 * ty + 1 >= K && 2 * ty + 1 < CY - K};
-    const double edge = src[origin - 1];  // the Cube edge value held by every halo cell
+    const bool xin = in_axis(x, nx, wrap & 1);
+    const bool inxy[2] = {xin && in_axis(y0, ny, wrap & 2), xin && in_axis(y0 + 1, ny, wrap & 2)};
+    const bool xw = (unsigned)x < (unsigned)nx && tx >= K && tx < 64 - K;
+    const bool writer[2] = {(unsigned)y0 < (unsigned)ny && xw && 2 * ty >= K && 2 * ty < CY - K,
+                            (unsigned)(y0 + 1) < (unsigned)ny && xw && 2 * ty + 1 >= K && 2 * ty + 1 < CY - K};
+    const double edge = src[edge_at];  // a halo cell of an unwrapped axis: the edge value
     const int64_t col[2] = {origin + (int64_t)y0 * sx + x, origin + (int64_t)(y0 + 1) * sx + x};
-    const int xc = min(max(x, -1), nx);
-    const int64_t colh[2] = {origin + (int64_t)min(max(y0, -1), ny) * sx + xc,
-                             origin + (int64_t)min(max(y0 + 1, -1), ny) * sx + xc};
+    const int xc = level0_coord(x, nx, wrap & 1);
+    const int64_t colh[2] = {origin + (int64_t)level0_coord(y0, ny, wrap & 2) * sx + xc,
+                             origin + (int64_t)level0_coord(y0 + 1, ny, wrap & 2) * sx + xc};
     // the ring rows (tile rows 0 and CY-1) read their own pair instead of a
     // missing neighbour: their values are never written
     const int tyd = ty > 0 ? ty - 1 : 0;
     const int tyu = ty < TY - 1 ? ty + 1 : TY - 1;
-    auto load0 = [&](int r, int p) -> double { return src[colh[r] + (int64_t)min(max(p, -1), nz) * sxy]; };
+    auto load0 = [&](int r, int p) -> double { return src[colh[r] + (int64_t)level0_plane(p, nz, wrap & 4) * sxy]; };
 
     double lv[2][6];
     double w[K][2][3];
@@ -320,7 +341,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
         for (int l = 1; l <= K; ++l) {
             if (2 * l > JR) continue;
             const int q = p - l;
-            const bool zin = (unsigned)q < (unsigned)nz;
+            const bool zin = in_axis(q, nz, wrap & 4);
             double a[2], c[2], b[2];
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
@@ -384,7 +405,9 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
 // 2 four-plane batches, 3 eight-plane batches; temporal blocking (K sweeps
 // per launch, tile 64 x TY): 4 K=2 TY=16, 5 K=3 TY=16, 6 K=4 TY=16,
 // 7 K=2 TY=8, 8 K=3 TY=8; two rows per thread (tile 64 x 2TY, work-group
-// 64 x TY): 9 K=4 TY=8, 10 K=4 TY=16.  The default blocks 4 sweeps per
+// 64 x TY): 9 K=4 TY=8, 10 K=4 TY=16; the same pipelines for one sweep per
+// launch (a plane fetched once per tile, y neighbours through LDS, x by
+// DPP): 11 one row per thread, 12 two rows.  The default blocks 4 sweeps per
 // launch with variant 6 or 10, whichever the launch cost model rates cheaper
 // for the grid (6 at 128^3, 10 at 512^3, the faster ones there on MI355X);
 // step counts not divisible by K finish with single sweeps of
@@ -404,10 +427,12 @@ TbShape tb_shape(int variant) {
     case 8: return {3, 8, 1};
     case 9: return {4, 8, 2};
     case 10: return {4, 16, 2};
+    case 11: return {1, 16, 1};
+    case 12: return {1, 16, 2};
     default: return {0, 0, 1};
     }
 }
-constexpr int kJacobiVariants = 10;
+constexpr int kJacobiVariants = 12;
 constexpr int kJacobiTallVariant = 10;  // the default's choice for grids that fill several rounds
 
 struct JacobiLaunch {
@@ -502,42 +527,30 @@ void launch_batched(const double *src, double *dst, const lgdhip_grid &g, const 
                                                              origin, L.zchunk);
 }
 
-// one launch of the temporal-blocked kernel: K sweeps src -> dst
-int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L, hipStream_t s) {
+// one launch of the temporal-blocked kernel: K sweeps src -> dst; wrap bits
+// 1, 2, 4: x, y, z taken periodic; edge_at: a cell of src holding the edge
+// value (a halo cell of an unwrapped axis; any cell when all three wrap)
+int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const JacobiLaunch &L, hipStream_t s,
+              int wrap = 0, int64_t edge_at = -1) {
     const int64_t sxy = g.sx * g.sy;
     const int64_t origin = lgdhip::origin_index(g);
+    if (edge_at < 0) edge_at = origin - 1;
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+#define LGD_TB_ARGS                                                                                                \
+    src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx, L.gy, L.ntiles, L.per, L.swizzle, wrap, edge_at
     switch (L.variant) {
-    case 4:
-        jacobi3d_f64_tb<2, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                            L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 5:
-        jacobi3d_f64_tb<3, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                            L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 6:
-        jacobi3d_f64_tb<4, 16, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                            L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 7:
-        jacobi3d_f64_tb<2, 8, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                            L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 8:
-        jacobi3d_f64_tb<3, 8, 3><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                            L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 9:
-        jacobi3d_f64_tb2<4, 8><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                         L.gy, L.ntiles, L.per, L.swizzle);
-        break;
-    case 10:
-        jacobi3d_f64_tb2<4, 16><<<L.grid, L.block, 0, s>>>(src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx,
-                                                          L.gy, L.ntiles, L.per, L.swizzle);
-        break;
+    case 4: jacobi3d_f64_tb<2, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 5: jacobi3d_f64_tb<3, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 6: jacobi3d_f64_tb<4, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 7: jacobi3d_f64_tb<2, 8, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 8: jacobi3d_f64_tb<3, 8, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 9: jacobi3d_f64_tb2<4, 8><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 10: jacobi3d_f64_tb2<4, 16><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 11: jacobi3d_f64_tb<1, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
+    case 12: jacobi3d_f64_tb2<1, 16><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
     default: return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "not a temporal-blocking variant");
     }
+#undef LGD_TB_ARGS
     LGD_LAUNCHED("jacobi3d_f64_tb");
     return 0;
 }
@@ -643,6 +656,70 @@ extern "C" int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *
         single = plan(*g, &plain);
     }
     for (; t < steps; ++t) {
+        rc = launch_sweep(cur, nxt, *g, single, s);
+        if (rc) return rc;
+        double *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+    }
+    if (result_in_b) *result_in_b = (cur == b) ? 1 : 0;
+    return 0;
+}
+
+// Periodic axes (Torus topologies and selectively wrapped ones): the same
+// K-sweep launches with wrapped tile coordinates, so a wrapped grid costs what
+// a Cube grid does; leftover steps (steps mod K) are single sweeps, each after
+// a periodic halo refresh.  Both buffers' halos are refreshed first, so the
+// halo cells of the unwrapped axes hold `edge` whatever the caller left there.
+extern "C" int lgdhip_jacobi3d_f64_run_periodic(double *a, double *b, const lgdhip_grid *g, int steps, int wrap_mask,
+                                                double edge, const lgdhip_tile *tile, int *result_in_b,
+                                                lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(g, 3, 8, 1, true);
+    if (rc) return rc;
+    LGD_REQUIRE(a && b && a != b, "need two distinct device buffers");
+    LGD_REQUIRE(steps >= 0, "steps must be >= 0");
+    LGD_REQUIRE(wrap_mask >= 0 && wrap_mask <= 7, "wrap_mask must be a combination of bits 1, 2, 4");
+    if (g->flags & LGDHIP_GRID_VIEW)
+        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "periodic sweeps need a whole grid, not a view");
+    hipStream_t s = lgdhip::as_stream(stream);
+    if (steps == 0) {
+        if (result_in_b) *result_in_b = 0;
+        return 0;
+    }
+    JacobiLaunch L = plan(*g, tile);
+    int k = tb_shape(L.variant).k;
+    // wrapped z: a single period's correction must reach K + 3 planes past the box
+    if ((wrap_mask & 4) && g->nz < 8) k = 0;
+    const int64_t origin = lgdhip::origin_index(*g);
+    const int64_t edge_at = !(wrap_mask & 1)   ? origin - 1
+                            : !(wrap_mask & 2) ? origin - g->sx
+                            : !(wrap_mask & 4) ? origin - g->sx * g->sy
+                                               : origin;
+    double *cur = a, *nxt = b;
+    int t = 0;
+    if (k > 0 && steps >= k) {
+        rc = lgdhip_grid_fill_halo(a, g, &edge, wrap_mask, stream);
+        if (rc) return rc;
+        rc = lgdhip_grid_fill_halo(b, g, &edge, wrap_mask, stream);
+        if (rc) return rc;
+        for (; t + k <= steps; t += k) {
+            rc = launch_tb(cur, nxt, *g, L, s, wrap_mask, edge_at);
+            if (rc) return rc;
+            double *tmp = cur;
+            cur = nxt;
+            nxt = tmp;
+        }
+    }
+    lgdhip_tile plain = tile ? *tile : lgdhip_tile{};
+    if (k > 0 || tb_shape(L.variant).k > 0 || plain.variant == 0) {
+        plain.variant = kSingleSweepVariant;
+        plain.by = 0;
+        plain.bz = 0;
+    }
+    const JacobiLaunch single = plan(*g, &plain);
+    for (; t < steps; ++t) {
+        rc = lgdhip_grid_fill_halo(cur, g, &edge, wrap_mask, stream);
+        if (rc) return rc;
         rc = launch_sweep(cur, nxt, *g, single, s);
         if (rc) return rc;
         double *tmp = cur;

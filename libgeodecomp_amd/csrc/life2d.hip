@@ -180,10 +180,24 @@ __device__ __forceinline__ uint32_t life_rule(uint32_t s0, uint32_t c0, uint32_t
     return LGD_BITOP3(ones, q1, t, kResult);
 }
 
-template <int R, bool DPP>
+// Word column and row a lane loads.  A band reaches `gens` rows and one
+// word column past its output; on a wrapped axis (Torus topologies) those are
+// the periodic images of grid cells and are computed like them (columns wrap
+// word by word, so a wrapped x axis needs nx % 32 == 0), on the others the
+// load is clamped into the grid and replaced by the edge state.
+__device__ __forceinline__ int packed_col(int col, int nw, int wrap) {
+    return (wrap & 1) ? ((col % nw) + nw) % nw : (col < 0 ? 0 : (col >= nw ? nw - 1 : col));
+}
+__device__ __forceinline__ int packed_row(int row, int ny, int wrap) {
+    return (wrap & 2) ? ((row % ny) + ny) % ny : (row < 0 ? 0 : (row >= ny ? ny - 1 : row));
+}
+
+template <int R, bool DPP, bool WRAP>
 __global__ __launch_bounds__(kLanes *kWavesPerBlock) void life_packed_tb(
     const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, int nw, int ny, uint32_t last_mask,
-    uint32_t edge, int gens, int halo_rows) {
+    uint32_t edge, int gens, int halo_rows, int wrap_mask) {
+    // the Cube instantiation (WRAP false) carries no trace of the wrap logic
+    const int wrap = WRAP ? wrap_mask : 0;
     const int lane = threadIdx.x & (kLanes - 1);
     // wave index is uniform: keep it (and everything row-related) in SGPRs
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kLanes));
@@ -194,25 +208,23 @@ __global__ __launch_bounds__(kLanes *kWavesPerBlock) void life_packed_tb(
     if (row0 + halo_rows >= ny) return;  // wave-uniform, no barriers below
 
     const bool col_in = col >= 0 && col < nw;
-    const uint32_t cmask = col_in ? (col == nw - 1 ? last_mask : 0xFFFFFFFFu) : 0u;
-    // does this wave touch the grid boundary (rows or columns)?
-    const bool rows_edge = row0 < 0 || row0 + R > ny;
-    const bool cols_edge = blockIdx.x == 0 || (int)(blockIdx.x + 1) * kOutCols + 1 > nw - 1;
+    const uint32_t cmask = (wrap & 1) ? 0xFFFFFFFFu : (col_in ? (col == nw - 1 ? last_mask : 0xFFFFFFFFu) : 0u);
+    // does this wave touch an unwrapped grid boundary (rows or columns)?
+    const bool rows_edge = !(wrap & 2) && (row0 < 0 || row0 + R > ny);
+    const bool cols_edge = !(wrap & 1) && (blockIdx.x == 0 || (int)(blockIdx.x + 1) * kOutCols + 1 > nw - 1);
     // register rows [ilo, ihi) lie inside the grid
     int ilo = row0 < 0 ? -row0 : 0;
     int ihi = ny - row0 < R ? ny - row0 : R;
 
-    // every lane loads from a clamped, in-bounds column and row (uniform
-    // row base + per-lane 32-bit column offset) and then substitutes the
-    // edge state where the cell lies outside the grid
-    const int colc = col < 0 ? 0 : (col >= nw ? nw - 1 : col);
+    // every lane loads from an in-bounds column and row (uniform row base +
+    // per-lane 32-bit column offset): the periodic image on a wrapped axis,
+    // else the clamped cell, replaced by the edge state outside the grid
+    const int colc = packed_col(col, nw, wrap);
     uint32_t v[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        int rc = row0 + i;
-        rc = rc < 0 ? 0 : (rc >= ny ? ny - 1 : rc);
-        const uint32_t x = src[(int64_t)rc * nw + colc];
-        v[i] = (i >= ilo && i < ihi) ? ((x & cmask) | (edge & ~cmask)) : edge;
+        const uint32_t x = src[(int64_t)packed_row(row0 + i, ny, wrap) * nw + colc];
+        v[i] = ((wrap & 2) || (i >= ilo && i < ihi)) ? ((x & cmask) | (edge & ~cmask)) : edge;
     }
 
     for (int g = 0; g < gens; ++g) {
@@ -257,11 +269,12 @@ __global__ __launch_bounds__(kLanes *kWavesPerBlock) void life_packed_tb(
 // LDS every generation, so only the work-group's top and bottom K rows are
 // redundant halo (instead of 2K rows per wave).
 // ---------------------------------------------------------------------------
-template <int R, int NW>
+template <int R, int NW, bool WRAP>
 __global__ __launch_bounds__(kLanes *NW) void life_packed_stacked(const uint32_t *__restrict__ src,
                                                                   uint32_t *__restrict__ dst, int nw, int ny,
                                                                   uint32_t last_mask, uint32_t edge, int gens,
-                                                                  int halo_rows) {
+                                                                  int halo_rows, int wrap_mask) {
+    const int wrap = WRAP ? wrap_mask : 0;
     // [buffer][wave][0: first row, 1: last row][lane]
     __shared__ uint32_t xchg[2][NW][2][kLanes];
     const int lane = threadIdx.x & (kLanes - 1);
@@ -272,20 +285,18 @@ __global__ __launch_bounds__(kLanes *NW) void life_packed_stacked(const uint32_t
     const int row0 = block_row0 + wave * R;
 
     const bool col_in = col >= 0 && col < nw;
-    const uint32_t cmask = col_in ? (col == nw - 1 ? last_mask : 0xFFFFFFFFu) : 0u;
-    const bool rows_edge = row0 < 0 || row0 + R > ny;
-    const bool cols_edge = blockIdx.x == 0 || (int)(blockIdx.x + 1) * kOutCols + 1 > nw - 1;
+    const uint32_t cmask = (wrap & 1) ? 0xFFFFFFFFu : (col_in ? (col == nw - 1 ? last_mask : 0xFFFFFFFFu) : 0u);
+    const bool rows_edge = !(wrap & 2) && (row0 < 0 || row0 + R > ny);
+    const bool cols_edge = !(wrap & 1) && (blockIdx.x == 0 || (int)(blockIdx.x + 1) * kOutCols + 1 > nw - 1);
     int ilo = row0 < 0 ? (-row0 < R ? -row0 : R) : 0;
     int ihi = ny - row0 < R ? (ny - row0 > 0 ? ny - row0 : 0) : R;
 
-    const int colc = col < 0 ? 0 : (col >= nw ? nw - 1 : col);
+    const int colc = packed_col(col, nw, wrap);
     uint32_t v[R];
 #pragma unroll
     for (int i = 0; i < R; ++i) {
-        int rc = row0 + i;
-        rc = rc < 0 ? 0 : (rc >= ny ? ny - 1 : rc);
-        const uint32_t x = src[(int64_t)rc * nw + colc];
-        v[i] = (i >= ilo && i < ihi) ? ((x & cmask) | (edge & ~cmask)) : edge;
+        const uint32_t x = src[(int64_t)packed_row(row0 + i, ny, wrap) * nw + colc];
+        v[i] = ((wrap & 2) || (i >= ilo && i < ihi)) ? ((x & cmask) | (edge & ~cmask)) : edge;
     }
 
     for (int g = 0; g < gens; ++g) {
@@ -370,34 +381,45 @@ int rows_per_group(int variant) {
 
 template <int R>
 void launch_rows(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge, int gens,
-                 int halo_rows, bool dpp, hipStream_t s) {
+                 int halo_rows, bool dpp, int wrap, hipStream_t s) {
     const int valid = R - 2 * halo_rows;
     const int bands = (ny + valid - 1) / valid;
     dim3 grid((unsigned)((nw + kOutCols - 1) / kOutCols), (unsigned)((bands + kWavesPerBlock - 1) / kWavesPerBlock));
     dim3 block(kLanes * kWavesPerBlock);
-    if (dpp)
-        life_packed_tb<R, true><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows);
+    if (dpp && wrap)
+        life_packed_tb<R, true, true><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, wrap);
+    else if (dpp)
+        life_packed_tb<R, true, false><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, 0);
+    else if (wrap)
+        life_packed_tb<R, false, true><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows,
+                                                              wrap);
     else
-        life_packed_tb<R, false><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows);
+        life_packed_tb<R, false, false><<<grid, block, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows,
+                                                               0);
 }
 
 template <int R, int NW>
 void launch_stacked(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge, int gens,
-                    int halo_rows, hipStream_t s) {
+                    int halo_rows, int wrap, hipStream_t s) {
     const int valid = NW * R - 2 * halo_rows;
     dim3 grid((unsigned)((nw + kOutCols - 1) / kOutCols), (unsigned)((ny + valid - 1) / valid));
-    life_packed_stacked<R, NW><<<grid, kLanes * NW, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens, halo_rows);
+    if (wrap)
+        life_packed_stacked<R, NW, true><<<grid, kLanes * NW, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens,
+                                                                      halo_rows, wrap);
+    else
+        life_packed_stacked<R, NW, false><<<grid, kLanes * NW, 0, s>>>(src, dst, nw, ny, last_mask, edge, gens,
+                                                                       halo_rows, 0);
 }
 
 int packed_launch(const uint32_t *src, uint32_t *dst, int nw, int ny, uint32_t last_mask, uint32_t edge,
-                  int gens, int halo_rows, int variant, hipStream_t s) {
+                  int gens, int halo_rows, int variant, int wrap, hipStream_t s) {
     switch (variant) {
-    case 1: case 2: launch_rows<128>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 1, s); break;
-    case 3: case 4: launch_rows<64>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 3, s); break;
-    case 5: launch_stacked<64, 4>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
-    case 6: launch_stacked<64, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
-    case 7: launch_stacked<32, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
-    default: launch_stacked<32, 16>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, s); break;
+    case 1: case 2: launch_rows<128>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 1, wrap, s); break;
+    case 3: case 4: launch_rows<64>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, variant == 3, wrap, s); break;
+    case 5: launch_stacked<64, 4>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, wrap, s); break;
+    case 6: launch_stacked<64, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, wrap, s); break;
+    case 7: launch_stacked<32, 8>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, wrap, s); break;
+    default: launch_stacked<32, 16>(src, dst, nw, ny, last_mask, edge, gens, halo_rows, wrap, s); break;
     }
     LGD_LAUNCHED("life_packed_tb");
     return 0;
@@ -411,6 +433,50 @@ int check_life_grid(const lgdhip_grid *g) {
     // the SWAR and pack kernels read whole 4/32-byte groups past nx
     if (g->sx < g->ox + ((g->nx + 31) / 32) * 32 + 1)
         return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "life kernels need row padding up to a multiple of 32 cells");
+    return 0;
+}
+
+// K generations per launch over the packed words; wrap bits 1 (x) and 2 (y):
+// the axes taken periodic (x only when nx % 32 == 0: columns wrap word by word)
+int packed_run(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g, int edge_alive, int steps, int wrap,
+               const lgdhip_tile *tile, int *result_in_b, lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(g, 2, 1, 1);
+    if (rc) return rc;
+    if ((wrap & 1) && g->nx % 32 != 0)
+        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "a wrapped x axis needs nx % 32 == 0 (whole words)");
+    LGD_REQUIRE(pa && pb && pa != pb, "need two distinct device buffers");
+    LGD_REQUIRE(steps >= 0, "steps must be >= 0");
+    LGD_REQUIRE(edge_alive == 0 || edge_alive == 1, "edge_alive must be 0 or 1");
+    int per_launch = kDefaultPerLaunch;
+    int variant = kDefaultVariant;
+    if (tile) {
+        if (tile->steps_per_launch > 0) per_launch = tile->steps_per_launch;
+        if (tile->variant != 0) variant = tile->variant;
+    }
+    if (variant < 1 || variant > kNumVariants) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "unknown life variant");
+    const int rows = rows_per_group(variant);
+    // a lane's neighbour word covers 32 cells, so at most 32 generations may
+    // run before the halo lanes' garbage reaches an output column; the
+    // register rows must keep at least one valid row per band
+    if (per_launch < 1 || per_launch > 32 || 2 * per_launch >= rows)
+        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED,
+                                 "steps_per_launch must be in [1, 32] and below half the register rows");
+    const int nw = (int)((g->nx + 31) / 32);
+    const int rem = (int)(g->nx - 32 * (int64_t)(nw - 1));
+    const uint32_t last_mask = rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
+    const uint32_t edge = edge_alive ? 0xFFFFFFFFu : 0u;
+    hipStream_t s = lgdhip::as_stream(stream);
+    uint32_t *cur = pa, *nxt = pb;
+    for (int done = 0; done < steps;) {
+        const int gens = (steps - done < per_launch) ? (steps - done) : per_launch;
+        rc = packed_launch(cur, nxt, nw, (int)g->ny, last_mask, edge, gens, per_launch, variant, wrap, s);
+        if (rc) return rc;
+        uint32_t *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        done += gens;
+    }
+    if (result_in_b) *result_in_b = (cur == pb) ? 1 : 0;
     return 0;
 }
 
@@ -474,42 +540,14 @@ int lgdhip_life2d_unpack(const uint32_t *packed, const lgdhip_grid *g, uint8_t *
 
 int lgdhip_life2d_packed_run(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g, int edge_alive, int steps,
                              const lgdhip_tile *tile, int *result_in_b, lgdhip_stream_t stream) {
-    int rc = lgdhip::check_grid(g, 2, 1, 1);
-    if (rc) return rc;
-    LGD_REQUIRE(pa && pb && pa != pb, "need two distinct device buffers");
-    LGD_REQUIRE(steps >= 0, "steps must be >= 0");
-    LGD_REQUIRE(edge_alive == 0 || edge_alive == 1, "edge_alive must be 0 or 1");
-    int per_launch = kDefaultPerLaunch;
-    int variant = kDefaultVariant;
-    if (tile) {
-        if (tile->steps_per_launch > 0) per_launch = tile->steps_per_launch;
-        if (tile->variant != 0) variant = tile->variant;
-    }
-    if (variant < 1 || variant > kNumVariants) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "unknown life variant");
-    const int rows = rows_per_group(variant);
-    // a lane's neighbour word covers 32 cells, so at most 32 generations may
-    // run before the halo lanes' garbage reaches an output column; the
-    // register rows must keep at least one valid row per band
-    if (per_launch < 1 || per_launch > 32 || 2 * per_launch >= rows)
-        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED,
-                                 "steps_per_launch must be in [1, 32] and below half the register rows");
-    const int nw = (int)((g->nx + 31) / 32);
-    const int rem = (int)(g->nx - 32 * (int64_t)(nw - 1));
-    const uint32_t last_mask = rem >= 32 ? 0xFFFFFFFFu : ((1u << rem) - 1u);
-    const uint32_t edge = edge_alive ? 0xFFFFFFFFu : 0u;
-    hipStream_t s = lgdhip::as_stream(stream);
-    uint32_t *cur = pa, *nxt = pb;
-    for (int done = 0; done < steps;) {
-        const int gens = (steps - done < per_launch) ? (steps - done) : per_launch;
-        rc = packed_launch(cur, nxt, nw, (int)g->ny, last_mask, edge, gens, per_launch, variant, s);
-        if (rc) return rc;
-        uint32_t *tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-        done += gens;
-    }
-    if (result_in_b) *result_in_b = (cur == pb) ? 1 : 0;
-    return 0;
+    return packed_run(pa, pb, g, edge_alive, steps, 0, tile, result_in_b, stream);
+}
+
+int lgdhip_life2d_packed_run_periodic(uint32_t *pa, uint32_t *pb, const lgdhip_grid *g, int edge_alive, int steps,
+                                      int wrap_mask, const lgdhip_tile *tile, int *result_in_b,
+                                      lgdhip_stream_t stream) {
+    LGD_REQUIRE(wrap_mask >= 0 && wrap_mask <= 3, "wrap_mask must be a combination of bits 1 (x) and 2 (y)");
+    return packed_run(pa, pb, g, edge_alive, steps, wrap_mask, tile, result_in_b, stream);
 }
 
 size_t lgdhip_life2d_workspace_bytes(const lgdhip_grid *g) {

@@ -137,11 +137,13 @@ SIGNATURES = {
     "lgdhip_grid_store": (_I, [_P, _GP, _P, _P]),
     "lgdhip_jacobi3d_f64_run": (_I, [_P, _P, _GP, _I, _TP, _IP, _P]),
     "lgdhip_jacobi3d_f64_plan": (_I, [_GP, _TP, _IP, _IP]),
+    "lgdhip_jacobi3d_f64_run_periodic": (_I, [_P, _P, _GP, _I, _I, C.c_double, _TP, _IP, _P]),
     "lgdhip_life2d_u8_run": (_I, [_P, _P, _GP, _I, _IP, _P]),
     "lgdhip_life2d_packed_words": (_I64, [_GP]),
     "lgdhip_life2d_pack": (_I, [_P, _GP, _P, _P]),
     "lgdhip_life2d_unpack": (_I, [_P, _GP, _P, _P]),
     "lgdhip_life2d_packed_run": (_I, [_P, _P, _GP, _I, _I, _TP, _IP, _P]),
+    "lgdhip_life2d_packed_run_periodic": (_I, [_P, _P, _GP, _I, _I, _I, _TP, _IP, _P]),
     "lgdhip_life2d_workspace_bytes": (_SZ, [_GP]),
     "lgdhip_life2d_run": (_I, [_P, _GP, _I, _I, _P, _SZ, _TP, _P]),
     "lgdhip_lbm_d3q19_f32_run": (_I, [_P, _P, _P, _GP, C.POINTER(lgdhip_lbm_params), _I, _TP, _IP, _P]),

@@ -274,8 +274,8 @@ public:
     static const int DIM = 3;
     static_assert(Topology::DIM == 3, "Jacobi3DF64 is a 3D family");
 
-    // Cube: the tuned (temporally blocked) sweeps.  Wrapped axes (Torus
-    // topologies): single sweeps, each after a periodic halo refresh.
+    // The tuned (temporally blocked) sweeps; wrapped axes (Torus topologies)
+    // take their periodic neighbours inside the same blocked kernels.
     Jacobi3DF64(const Coord<3>& dim, const Tuning& tuning) :
         tuning(tuning),
         wrap(wrapMask<CELL>()),
@@ -346,24 +346,16 @@ public:
 
     void run(unsigned nanoSteps)
     {
-        if (wrap) {
-            // halo refresh + single sweep per step, two steps per captured
-            // graph (the buffers are back in place after two)
-            void *pa = a->get(), *pb = b->get();
-            loop.run(stream.get(), pa, nanoSteps / 2, [&]() {
-                wrappedSweep(pa, pb);
-                wrappedSweep(pb, pa);
-            });
-            if (nanoSteps % 2) {
-                wrappedSweep(pa, pb);
-                a.swap(b);
-            }
-            return;
-        }
         int inB = 0;
-        check(lgdhip_jacobi3d_f64_run((double *)a->get(), (double *)b->get(), &g, (int)nanoSteps, tuning.get(), &inB,
-                                      stream.get()),
-              "lgdhip_jacobi3d_f64_run");
+        if (wrap) {
+            check(lgdhip_jacobi3d_f64_run_periodic((double *)a->get(), (double *)b->get(), &g, (int)nanoSteps, wrap,
+                                                   edge, tuning.get(), &inB, stream.get()),
+                  "lgdhip_jacobi3d_f64_run_periodic");
+        } else {
+            check(lgdhip_jacobi3d_f64_run((double *)a->get(), (double *)b->get(), &g, (int)nanoSteps, tuning.get(),
+                                          &inB, stream.get()),
+                  "lgdhip_jacobi3d_f64_run");
+        }
         if (inB) {
             a.swap(b);
         }
@@ -381,19 +373,6 @@ private:
     int wrap;
     double edge;
     GraphLoop loop;
-
-    // one step of a periodic grid: wrapped halo from the interior, then a
-    // single sweep that reads its halo from memory
-    void wrappedSweep(void *src, void *dst)
-    {
-        lgdhip_tile single;
-        std::memset(&single, 0, sizeof(single));
-        single.variant = 2;
-        int inB = 0;
-        check(lgdhip_grid_fill_halo(src, &g, &edge, wrap, stream.get()), "lgdhip_grid_fill_halo");
-        check(lgdhip_jacobi3d_f64_run((double *)src, (double *)dst, &g, 1, &single, &inB, stream.get()),
-              "lgdhip_jacobi3d_f64_run");
-    }
 };
 
 // ---------------------------------------------------------------------
@@ -408,18 +387,21 @@ public:
     static const int DIM = 2;
     static_assert(Topology::DIM == 2, "Life2D is a 2D family");
 
-    // Cube: bit-packed temporally blocked sweeps.  Wrapped axes (Torus
-    // topologies): byte-grid single generations, each after a periodic halo
-    // refresh (the state then stays in the byte grids).
+    // Bit-packed temporally blocked sweeps; wrapped axes (Torus topologies)
+    // load their periodic images inside the same kernels.  Only a wrapped x
+    // axis whose length is not a multiple of 32 (the packed row would wrap
+    // inside a word) runs byte-grid single generations, each after a
+    // periodic halo refresh (the state then stays in the byte grids).
     Life2D(const Coord<2>& dim, const Tuning& tuning) :
         tuning(tuning),
         edgeAlive(0),
-        wrap(wrapMask<CELL>())
+        wrap(wrapMask<CELL>()),
+        bytes((wrap & 1) && dim.x() % 32 != 0)
     {
         requireSingleNanoStep<CELL>();
         check(lgdhip_grid_make(&g, 2, dim.x(), dim.y(), 1, 1, 1, 1), "lgdhip_grid_make");
         grid.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));
-        if (wrap) {
+        if (bytes) {
             grid2.reset(new DeviceMemory((std::size_t)lgdhip_grid_bytes(&g)));
         }
         const std::size_t ws = lgdhip_life2d_workspace_bytes(&g);
@@ -438,14 +420,14 @@ public:
         const uint8_t edge = (uint8_t)edgeAlive;
         check(lgdhip_grid_fill_edge(grid->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
         check(lgdhip_grid_load(grid->get(), &g, dense.data(), stream.get()), "lgdhip_grid_load");
-        if (wrap) {
+        if (bytes) {
             check(lgdhip_grid_fill_edge(grid2->get(), &g, &edge, stream.get()), "lgdhip_grid_fill_edge");
             return;
         }
         check(lgdhip_life2d_pack((const uint8_t *)grid->get(), &g, pa, stream.get()), "lgdhip_life2d_pack");
     }
 
-    // wrapped axes: bytes into the byte grid; otherwise the packed words
+    // byte-grid mode: bytes into the byte grid; otherwise the packed words
     // covering each streak, packed from the (current) host row
     void uploadRegion(const Grid<CELL, Topology>& host, const Region<2>& region)
     {
@@ -456,13 +438,13 @@ public:
             const Streak<2>& st = streaks[k];
             const int y = st.origin.y();
             const CELL *row = host.data() + (std::size_t)y * dim.x();
-            if (wrap) {
-                std::vector<uint8_t> bytes(st.endX - st.origin.x());
-                for (std::size_t i = 0; i < bytes.size(); ++i) {
-                    bytes[i] = (row[st.origin.x() + i].*MEMBER) ? 1 : 0;
+            if (bytes) {
+                std::vector<uint8_t> line(st.endX - st.origin.x());
+                for (std::size_t i = 0; i < line.size(); ++i) {
+                    line[i] = (row[st.origin.x() + i].*MEMBER) ? 1 : 0;
                 }
                 uint8_t *dst = (uint8_t *)grid->get() + elementIndex(g, 0, st.origin.x(), y, 0);
-                check(lgdhip_memcpy_h2d(dst, bytes.data(), bytes.size(), stream.get()), "lgdhip_memcpy_h2d");
+                check(lgdhip_memcpy_h2d(dst, line.data(), line.size(), stream.get()), "lgdhip_memcpy_h2d");
                 continue;
             }
             const int64_t w0 = st.origin.x() / 32, w1 = (st.endX - 1) / 32;
@@ -481,7 +463,7 @@ public:
 
     void download(Grid<CELL, Topology> *host)
     {
-        if (!wrap) {
+        if (!bytes) {
             check(lgdhip_life2d_unpack(pa, &g, (uint8_t *)grid->get(), stream.get()), "lgdhip_life2d_unpack");
         }
         std::vector<uint8_t> dense(host->size());
@@ -491,7 +473,7 @@ public:
         }
     }
 
-    // wrapped axes: the streak's bytes of the byte grid; otherwise the
+    // byte-grid mode: the streak's bytes of the byte grid; otherwise the
     // packed words covering the streak, unpacked on the host
     void downloadRegion(Grid<CELL, Topology> *host, const Region<2>& region)
     {
@@ -502,12 +484,12 @@ public:
             const Streak<2>& st = streaks[k];
             const int y = st.origin.y();
             CELL *row = host->data() + (std::size_t)y * dim.x();
-            if (wrap) {
-                std::vector<uint8_t> bytes(st.endX - st.origin.x());
+            if (bytes) {
+                std::vector<uint8_t> line(st.endX - st.origin.x());
                 const uint8_t *src = (const uint8_t *)grid->get() + elementIndex(g, 0, st.origin.x(), y, 0);
-                check(lgdhip_memcpy_d2h(bytes.data(), src, bytes.size(), stream.get()), "lgdhip_memcpy_d2h");
-                for (std::size_t i = 0; i < bytes.size(); ++i) {
-                    row[st.origin.x() + i].*MEMBER = static_cast<VALUE>(bytes[i]);
+                check(lgdhip_memcpy_d2h(line.data(), src, line.size(), stream.get()), "lgdhip_memcpy_d2h");
+                for (std::size_t i = 0; i < line.size(); ++i) {
+                    row[st.origin.x() + i].*MEMBER = static_cast<VALUE>(line[i]);
                 }
                 continue;
             }
@@ -523,7 +505,7 @@ public:
 
     void run(unsigned nanoSteps)
     {
-        if (wrap) {
+        if (bytes) {
             // halo refresh + one generation per step, two steps per
             // captured graph
             void *pa = grid->get(), *pb = grid2->get();
@@ -538,8 +520,14 @@ public:
             return;
         }
         int inB = 0;
-        check(lgdhip_life2d_packed_run(pa, pb, &g, edgeAlive, (int)nanoSteps, tuning.get(), &inB, stream.get()),
-              "lgdhip_life2d_packed_run");
+        if (wrap) {
+            check(lgdhip_life2d_packed_run_periodic(pa, pb, &g, edgeAlive, (int)nanoSteps, wrap, tuning.get(), &inB,
+                                                    stream.get()),
+                  "lgdhip_life2d_packed_run_periodic");
+        } else {
+            check(lgdhip_life2d_packed_run(pa, pb, &g, edgeAlive, (int)nanoSteps, tuning.get(), &inB, stream.get()),
+                  "lgdhip_life2d_packed_run");
+        }
         if (inB) {
             std::swap(pa, pb);
         }
@@ -556,6 +544,7 @@ private:
     uint32_t *pa, *pb;
     int edgeAlive;
     int wrap;
+    bool bytes;  // byte-grid single generations (wrapped x with nx % 32 != 0)
     GraphLoop loop;
 
     void wrappedGeneration(void *src, void *dst)

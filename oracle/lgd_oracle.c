@@ -32,35 +32,44 @@
 /* ------------------------------------------------------------------ */
 /* Jacobi3D FP64                                                       */
 /* ------------------------------------------------------------------ */
-static void jacobi_sweep(const double *src, double *dst, long nx, long ny, long nz, double edge, int threads) {
+/* periodic neighbour index along one axis: the opposite face's cell when
+ * the axis wraps (Topologies::Torus normalize, geometry/topologies.h:225-247) */
+static long wrap_prev(long c, long n) { return c > 0 ? c - 1 : n - 1; }
+static long wrap_next(long c, long n) { return c < n - 1 ? c + 1 : 0; }
+
+static void jacobi_sweep(const double *src, double *dst, long nx, long ny, long nz, double edge, int wrap,
+                         int threads) {
     const long sxy = nx * ny;
+    const int wx = wrap & 1, wy = wrap & 2, wz = wrap & 4;
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (long z = 0; z < nz; ++z) {
         for (long y = 0; y < ny; ++y) {
             for (long x = 0; x < nx; ++x) {
                 const long i = z * sxy + y * nx + x;
-                /* hood[FixedCoord<...>] in the order of main.cpp:1196-1202 */
-                const double zm = z > 0 ? src[i - sxy] : edge;
-                const double ym = y > 0 ? src[i - nx] : edge;
-                const double xm = x > 0 ? src[i - 1] : edge;
+                const long row = z * sxy + y * nx, col = y * nx + x;
+                /* hood[FixedCoord<...>] in the order of main.cpp:1196-1202;
+                 * a neighbour past an unwrapped face is the edge cell */
+                const double zm = z > 0 ? src[i - sxy] : wz ? src[wrap_prev(z, nz) * sxy + col] : edge;
+                const double ym = y > 0 ? src[i - nx] : wy ? src[z * sxy + wrap_prev(y, ny) * nx + x] : edge;
+                const double xm = x > 0 ? src[i - 1] : wx ? src[row + wrap_prev(x, nx)] : edge;
                 const double c = src[i];
-                const double xp = x < nx - 1 ? src[i + 1] : edge;
-                const double yp = y < ny - 1 ? src[i + nx] : edge;
-                const double zp = z < nz - 1 ? src[i + sxy] : edge;
+                const double xp = x < nx - 1 ? src[i + 1] : wx ? src[row + wrap_next(x, nx)] : edge;
+                const double yp = y < ny - 1 ? src[i + nx] : wy ? src[z * sxy + wrap_next(y, ny) * nx + x] : edge;
+                const double zp = z < nz - 1 ? src[i + sxy] : wz ? src[wrap_next(z, nz) * sxy + col] : edge;
                 dst[i] = (zm + ym + xm + c + xp + yp + zp) * (1.0 / 7.0);
             }
         }
     }
 }
 
-int lgdo_jacobi3d_f64(double *grid, long nx, long ny, long nz, double edge, int steps, int threads) {
-    if (nx < 1 || ny < 1 || nz < 1 || steps < 0) return -1;
+int lgdo_jacobi3d_f64_wrap(double *grid, long nx, long ny, long nz, double edge, int steps, int threads, int wrap) {
+    if (nx < 1 || ny < 1 || nz < 1 || steps < 0 || wrap < 0 || wrap > 7) return -1;
     const size_t n = (size_t)nx * ny * nz;
     double *tmp = (double *)malloc(n * sizeof(double));
     if (!tmp) return -2;
     double *cur = grid, *nxt = tmp;
     for (int t = 0; t < steps; ++t) {
-        jacobi_sweep(cur, nxt, nx, ny, nz, edge, threads);
+        jacobi_sweep(cur, nxt, nx, ny, nz, edge, wrap, threads);
         double *s = cur;
         cur = nxt;
         nxt = s;
@@ -70,18 +79,26 @@ int lgdo_jacobi3d_f64(double *grid, long nx, long ny, long nz, double edge, int 
     return 0;
 }
 
+int lgdo_jacobi3d_f64(double *grid, long nx, long ny, long nz, double edge, int steps, int threads) {
+    return lgdo_jacobi3d_f64_wrap(grid, nx, ny, nz, edge, steps, threads, 0);
+}
+
 /* ------------------------------------------------------------------ */
 /* Game of Life                                                        */
 /* ------------------------------------------------------------------ */
-static void life_sweep(const uint8_t *src, uint8_t *dst, long nx, long ny, int edge, int threads) {
+static void life_sweep(const uint8_t *src, uint8_t *dst, long nx, long ny, int edge, int wrap, int threads) {
 #pragma omp parallel for schedule(static) num_threads(threads) if (threads > 1)
     for (long y = 0; y < ny; ++y) {
         for (long x = 0; x < nx; ++x) {
-            /* countLivingNeighbors: all nine cells minus the centre */
+            /* countLivingNeighbors: all nine cells minus the centre; a
+             * coordinate past a wrapped face is taken modulo the extent
+             * (Topologies::Torus normalize, geometry/topologies.h:225-247) */
             int n = 0;
             for (int dy = -1; dy < 2; ++dy) {
                 for (int dx = -1; dx < 2; ++dx) {
-                    const long xx = x + dx, yy = y + dy;
+                    long xx = x + dx, yy = y + dy;
+                    if (wrap & 1) xx = (xx + nx) % nx;
+                    if (wrap & 2) yy = (yy + ny) % ny;
                     const int alive = (xx < 0 || yy < 0 || xx >= nx || yy >= ny) ? edge : (src[yy * nx + xx] != 0);
                     n += alive;
                 }
@@ -99,14 +116,14 @@ static void life_sweep(const uint8_t *src, uint8_t *dst, long nx, long ny, int e
     }
 }
 
-int lgdo_life2d_u8(uint8_t *grid, long nx, long ny, int edge, int steps, int threads) {
-    if (nx < 1 || ny < 1 || steps < 0 || (edge != 0 && edge != 1)) return -1;
+int lgdo_life2d_u8_wrap(uint8_t *grid, long nx, long ny, int edge, int steps, int threads, int wrap) {
+    if (nx < 1 || ny < 1 || steps < 0 || (edge != 0 && edge != 1) || wrap < 0 || wrap > 3) return -1;
     const size_t n = (size_t)nx * ny;
     uint8_t *tmp = (uint8_t *)malloc(n);
     if (!tmp) return -2;
     uint8_t *cur = grid, *nxt = tmp;
     for (int t = 0; t < steps; ++t) {
-        life_sweep(cur, nxt, nx, ny, edge, threads);
+        life_sweep(cur, nxt, nx, ny, edge, wrap, threads);
         uint8_t *s = cur;
         cur = nxt;
         nxt = s;
@@ -114,6 +131,10 @@ int lgdo_life2d_u8(uint8_t *grid, long nx, long ny, int edge, int steps, int thr
     if (cur != grid) memcpy(grid, cur, n);
     free(tmp);
     return 0;
+}
+
+int lgdo_life2d_u8(uint8_t *grid, long nx, long ny, int edge, int steps, int threads) {
+    return lgdo_life2d_u8_wrap(grid, nx, ny, edge, steps, threads, 0);
 }
 
 /* ------------------------------------------------------------------ */

@@ -17,11 +17,21 @@
  *    encoded coordinate sums, exact in FP64) times 1.0/7.0; constant field
  *    with equal edge is a fixed point; maximum principle; symmetry under
  *    axis permutations; brute-force pure-Python loop on tiny grids.
+ *  - lgdo_jacobi3d_f64_wrap: wrap 0 equals lgdo_jacobi3d_f64 bit for bit;
+ *    a cyclic shift of the input along wrapped axes shifts the output the
+ *    same way (bit for bit); on a full torus a plane wave cos(2 pi k x / nx)
+ *    decays by the closed-form eigenvalue (5 + 2 cos(2 pi k / nx)) / 7 per
+ *    sweep and the grid sum is conserved; brute force against a
+ *    pure-Python periodic loop on tiny grids.
  *  - lgdo_life2d_u8: blinker period 2, glider translation (1,1) per 4
  *    generations, block still life, the reference example's glider and
  *    diehard seeds (src/examples/gameoflife/main.cpp:70-89; diehard vanishes
  *    after exactly 130 generations on an unbounded plane); brute force
  *    against a numpy evaluation of the 3x3 box sum, dead and alive edges.
+ *  - lgdo_life2d_u8_wrap: wrap 0 equals lgdo_life2d_u8; a glider on a 16x16
+ *    torus crosses both seams and is back in place after 64 generations;
+ *    cyclic shifts along wrapped axes commute with the generations; numpy
+ *    brute force with np.roll (the periodic 3x3 box sum) for every wrap.
  *  - lgdo_lbm_d3q19_*: rest state is a fixed point; mass conservation with
  *    bounce-back walls and a moving lid; equilibrium moments sum rules
  *    (sum f_eq = rho, sum e f_eq = j, closed form); mirror symmetry of the
@@ -59,7 +69,12 @@ extern "C" {
 #define LGDO_LID_YHI 0x40
 
 int lgdo_jacobi3d_f64(double *grid, long nx, long ny, long nz, double edge, int steps, int threads);
+/* periodic axes: wrap bits 1, 2, 4 wrap x, y, z (Topologies::Torus and the
+ * selectively wrapped topologies); wrap 0 is lgdo_jacobi3d_f64 */
+int lgdo_jacobi3d_f64_wrap(double *grid, long nx, long ny, long nz, double edge, int steps, int threads, int wrap);
 int lgdo_life2d_u8(uint8_t *grid, long nx, long ny, int edge, int steps, int threads);
+/* periodic axes: wrap bits 1, 2 wrap x, y; wrap 0 is lgdo_life2d_u8 */
+int lgdo_life2d_u8_wrap(uint8_t *grid, long nx, long ny, int edge, int steps, int threads, int wrap);
 
 int lgdo_lbm_direction(int i, int *ex, int *ey, int *ez, int *opp);
 void lgdo_lbm_cavity_flags(uint8_t *flags, long nx, long ny, long nz);

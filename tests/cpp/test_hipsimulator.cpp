@@ -212,9 +212,12 @@ LGD_TEST(jacobi_bit_exact)
 }
 
 // ---------------------------------------------------------------------
-// periodic topologies (Torus, and axes wrapped selectively): single sweeps
-// after a periodic halo refresh, identical to the oracle at every writer
-// callback
+// periodic topologies (Torus, and axes wrapped selectively): Jacobi runs
+// its blocked sweeps with wrapped tile coordinates (leftover steps: single
+// sweeps after a periodic halo refresh), Life its bit-packed blocked
+// generations with wrapped rows and word columns (a wrapped x axis of
+// nx % 32 != 0: byte-grid generations after a halo refresh); identical to
+// the oracle at every writer callback
 // ---------------------------------------------------------------------
 template<typename CELL> CELL makeLife(double u) { return CELL(u < 0.3); }
 template<typename CELL> CELL makeTemp(double u) { return CELL(u); }
@@ -249,6 +252,23 @@ LGD_TEST(life_torus)
     CHECK(sh.grids == sr.grids);
 }
 
+// a torus whose rows are whole packed words: the blocked kernels load the
+// periodic images (a writer period of 33: launches of 31 and 2 generations)
+static RandomCellInitializer<TorusLifeCell, 2> *makeTorusLifePacked()
+{
+    return new RandomCellInitializer<TorusLifeCell, 2>(Coord<2>(128, 77), 70, 15, &makeLife<TorusLifeCell>,
+                                                       TorusLifeCell(true));
+}
+
+LGD_TEST(life_torus_packed)
+{
+    std::vector<Event> eh, er;
+    Snapshots<TorusLifeCell> sh, sr;
+    runBoth<TorusLifeCell>(&makeTorusLifePacked, 33, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
+    CHECK(sh.grids == sr.grids);
+}
+
 LGD_TEST(life_wrapped_x_only)
 {
     std::vector<Event> eh, er;
@@ -271,6 +291,17 @@ LGD_TEST(jacobi_wrapped_x_and_z)
     std::vector<Event> eh, er;
     Snapshots<SlabJacobiCell> sh, sr;
     runBoth<SlabJacobiCell>(&makeSlabJacobi, 3, &eh, &er, &sh, &sr);
+    CHECK(sh.grids == sr.grids);
+}
+
+// a writer period of 9: each run holds two 4-sweep blocked launches with
+// wrapped tile coordinates and one halo-refresh single sweep
+LGD_TEST(jacobi_wrapped_x_and_z_blocked)
+{
+    std::vector<Event> eh, er;
+    Snapshots<SlabJacobiCell> sh, sr;
+    runBoth<SlabJacobiCell>(&makeSlabJacobi, 9, &eh, &er, &sh, &sr);
+    CHECK(eh == er);
     CHECK(sh.grids == sr.grids);
 }
 
@@ -469,7 +500,7 @@ LGD_TEST(hip_simulation_factory_every_jacobi_variant)
     SerialSimulator<JacobiCell> ref(proxy->clone());
     ref.run();
     SimulationParameters p = factory.parameters();
-    for (int k = 0; k < 10; ++k) {
+    for (int k = 0; k < 12; ++k) {  // every index of the Variant set
         p["Variant"].setValue(k);
         const double fit = factory(p);
         CHECK(fit < 0 && fit > -60);

@@ -24,7 +24,9 @@ def load():
         lib = C.CDLL(str(ORACLE_LIB))
         P, L, I, D, F = C.c_void_p, C.c_long, C.c_int, C.c_double, C.c_float
         lib.lgdo_jacobi3d_f64.argtypes = [P, L, L, L, D, I, I]
+        lib.lgdo_jacobi3d_f64_wrap.argtypes = [P, L, L, L, D, I, I, I]
         lib.lgdo_life2d_u8.argtypes = [P, L, L, I, I, I]
+        lib.lgdo_life2d_u8_wrap.argtypes = [P, L, L, I, I, I, I]
         lib.lgdo_lbm_cavity_flags.argtypes = [P, L, L, L]
         lib.lgdo_lbm_cavity_flags.restype = None
         lib.lgdo_lbm_d3q19_double.argtypes = [P, P, L, L, L, D, D, D, D, I, I]
@@ -53,18 +55,26 @@ def _p(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def jacobi3d(grid: np.ndarray, steps: int, edge: float = 0.0, threads: int = 1) -> np.ndarray:
+def jacobi3d(grid: np.ndarray, steps: int, edge: float = 0.0, threads: int = 1, wrap: int = 0) -> np.ndarray:
+    """wrap: bits 1, 2, 4 make the x, y, z axes periodic (0: the Cube)."""
     out = np.ascontiguousarray(grid, dtype=np.float64).copy()
     nz, ny, nx = out.shape
-    rc = load().lgdo_jacobi3d_f64(_p(out), nx, ny, nz, edge, steps, threads)
+    if wrap:
+        rc = load().lgdo_jacobi3d_f64_wrap(_p(out), nx, ny, nz, edge, steps, threads, wrap)
+    else:
+        rc = load().lgdo_jacobi3d_f64(_p(out), nx, ny, nz, edge, steps, threads)
     assert rc == 0
     return out
 
 
-def life2d(grid: np.ndarray, steps: int, edge: int = 0, threads: int = 1) -> np.ndarray:
+def life2d(grid: np.ndarray, steps: int, edge: int = 0, threads: int = 1, wrap: int = 0) -> np.ndarray:
+    """wrap: bits 1, 2 make the x, y axes periodic (0: the Cube)."""
     out = np.ascontiguousarray(grid, dtype=np.uint8).copy()
     ny, nx = out.shape
-    rc = load().lgdo_life2d_u8(_p(out), nx, ny, edge, steps, threads)
+    if wrap:
+        rc = load().lgdo_life2d_u8_wrap(_p(out), nx, ny, edge, steps, threads, wrap)
+    else:
+        rc = load().lgdo_life2d_u8(_p(out), nx, ny, edge, steps, threads)
     assert rc == 0
     return out
 

@@ -71,6 +71,34 @@ def test_lbm_x4_variant_rejects_ragged_rows_without_gpu():
     assert rc == -2 and "nx % 4" in lib.lgdhip_last_error().decode()
 
 
+def test_jacobi_periodic_rejects_bad_arguments_without_gpu():
+    # argument checks come before any device work
+    lib = lgdhip.load()
+    g = lgdhip.make_grid(3, 12, 10, 9, halo=1, elem_bytes=8)
+    fake = C.c_void_p(0x1000), C.c_void_p(0x2000)
+    in_b = C.c_int(-1)
+    rc = lib.lgdhip_jacobi3d_f64_run_periodic(fake[0], fake[1], C.byref(g), 4, 8, 0.0, None, C.byref(in_b), None)
+    assert rc == -1 and "wrap_mask" in lib.lgdhip_last_error().decode()
+    rc = lib.lgdhip_jacobi3d_f64_run_periodic(fake[0], fake[1], C.byref(g), -1, 7, 0.0, None, C.byref(in_b), None)
+    assert rc == -1 and "steps" in lib.lgdhip_last_error().decode()
+    v = lgdhip.lgdhip_grid()
+    assert lib.lgdhip_grid_view(C.byref(g), 1, 1, 1, 4, 4, 4, C.byref(v)) == 0
+    rc = lib.lgdhip_jacobi3d_f64_run_periodic(fake[0], fake[1], C.byref(v), 4, 7, 0.0, None, C.byref(in_b), None)
+    assert rc == -2 and "view" in lib.lgdhip_last_error().decode()
+
+
+def test_life_periodic_rejects_ragged_wrapped_x_without_gpu():
+    # a wrapped x axis needs whole packed words; checked before device work
+    lib = lgdhip.load()
+    g = lgdhip.make_grid(2, 100, 40, 1, halo=1, elem_bytes=1)
+    fake = C.c_void_p(0x1000), C.c_void_p(0x2000)
+    in_b = C.c_int(-1)
+    rc = lib.lgdhip_life2d_packed_run_periodic(fake[0], fake[1], C.byref(g), 0, 4, 1, None, C.byref(in_b), None)
+    assert rc == -2 and "nx % 32" in lib.lgdhip_last_error().decode()
+    rc = lib.lgdhip_life2d_packed_run_periodic(fake[0], fake[1], C.byref(g), 0, 4, 4, None, C.byref(in_b), None)
+    assert rc == -1 and "wrap_mask" in lib.lgdhip_last_error().decode()
+
+
 def test_jacobi_launch_plan_without_gpu():
     """lgdhip_jacobi3d_f64_plan (host-only): the default picks 4 sweeps per
     launch with one row per thread on a 128^3 cube (5 chunks of 26 planes)

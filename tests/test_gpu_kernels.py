@@ -76,15 +76,59 @@ def test_jacobi_temporal_blocking_bit_exact(hip, oracle, shape, steps):
     3 with 8-row tiles; 9, 10: K = 4 with two rows per thread, 16- and
     32-row tiles), step counts that leave a remainder of single
     sweeps, ragged shapes, z chunks down to one plane and chunks longer than
-    the kernel's 6-step unrolled march, non-zero edge."""
+    the kernel's 6-step unrolled march, non-zero edge; 11, 12: the same
+    pipelines for one sweep per launch."""
     nx, ny, nz = shape
     a = synthetic.jacobi_initial(nx, ny, nz, seed=nx + ny + nz)
     exp = oracle.jacobi3d(a, steps, edge=0.5, threads=4)
-    for variant in (4, 5, 6, 7, 8, 9, 10):
+    for variant in (4, 5, 6, 7, 8, 9, 10, 11, 12):
         for bz in (0, 1, 5, 13):
             say("jacobi tb", shape, steps, variant, bz)
             t = hip.tile(bz=bz, variant=variant)
             assert np.array_equal(gpu_jacobi(hip, a, steps, edge=0.5, tile=t), exp)
+
+
+def gpu_jacobi_periodic(hip, host, steps, wrap, edge=0.0, tile=None, stale=(77.0, 99.0)):
+    """lgdhip_jacobi3d_f64_run_periodic; the buffers' halos start out holding
+    other values than `edge` (the entry point rewrites both)."""
+    import ctypes as C
+    nz, ny, nx = host.shape
+    g = hip.make_grid(3, nx, ny, nz, halo=1, elem_bytes=8)
+    a, b = hip.DeviceGrid(g, np.float64), hip.DeviceGrid(g, np.float64)
+    a.fill_edge(stale[0])
+    b.fill_edge(stale[1])
+    a.load(host)
+    in_b = C.c_int(-1)
+    t = C.byref(tile) if tile is not None else None
+    hip.call("lgdhip_jacobi3d_f64_run_periodic", a.ptr, b.ptr, C.byref(g), steps, wrap, edge, t, C.byref(in_b), None)
+    hip.call("lgdhip_device_synchronize")
+    assert in_b.value in (0, 1)
+    return (b if in_b.value else a).store()
+
+
+@pytest.mark.parametrize("shape,steps", [((96, 70, 40), 9), ((37, 22, 15), 12), ((10, 5, 9), 8), ((33, 20, 5), 5),
+                                         ((1, 1, 1), 4), ((130, 33, 8), 4)])
+def test_jacobi_periodic_bit_exact(hip, oracle, shape, steps):
+    """Torus and selectively wrapped axes (every wrap mask) through the
+    temporal-blocking kernels with wrapped tile coordinates: the default
+    (K = 4, one or two rows per thread), K = 2 and 3, a single-sweep variant,
+    step counts with a remainder of halo-refresh single sweeps, boxes
+    narrower than a tile (10 x 5), a wrapped z axis too short for blocking
+    (nz = 5: single sweeps throughout) and exactly long enough (nz = 8)."""
+    nx, ny, nz = shape
+    a = synthetic.jacobi_initial(nx, ny, nz, seed=nx * ny + nz)
+    for wrap in range(8):
+        exp = oracle.jacobi3d(a, steps, edge=0.5, threads=4, wrap=wrap)
+        for variant, bz in ((0, 0), (10, 0), (6, 5), (4, 0), (8, 3), (2, 0), (11, 0), (12, 7)):
+            say("jacobi periodic", shape, steps, "wrap", wrap, "variant", variant, "bz", bz)
+            t = hip.tile(bz=bz, variant=variant) if variant else None
+            got = gpu_jacobi_periodic(hip, a, steps, wrap, edge=0.5, tile=t)
+            assert np.array_equal(got, exp), (wrap, variant, bz)
+
+
+def test_jacobi_periodic_zero_steps(hip):
+    a = synthetic.jacobi_initial(12, 10, 9, seed=3)
+    assert np.array_equal(gpu_jacobi_periodic(hip, a, 0, 7), a)
 
 
 # ----------------------------------------------------------------------
@@ -149,6 +193,45 @@ def test_life_packed_variants(hip, oracle, variant, per_launch, shape):
     exp = oracle.life2d(g0, 40, threads=4)
     out = gpu_life_packed(hip, g0, 40, tile=hip.tile(steps_per_launch=per_launch, variant=variant))
     assert np.array_equal(out, exp)
+
+
+def gpu_life_packed_periodic(hip, host, steps, wrap, edge=0, tile=None):
+    import ctypes as C
+    ny, nx = host.shape
+    g = hip.make_grid(2, nx, ny, 1, halo=1, elem_bytes=1)
+    a = hip.DeviceGrid(g, np.uint8)
+    a.fill_edge(edge)
+    a.load(host)
+    ws_bytes = hip.load().lgdhip_life2d_workspace_bytes(C.byref(g))
+    ws = hip.DeviceBuffer(ws_bytes)
+    pa = ws.ptr.value
+    pb = pa + ws_bytes // 2
+    hip.call("lgdhip_life2d_pack", a.ptr, C.byref(g), C.c_void_p(pa), None)
+    in_b = C.c_int(-1)
+    t = C.byref(tile) if tile is not None else None
+    hip.call("lgdhip_life2d_packed_run_periodic", C.c_void_p(pa), C.c_void_p(pb), C.byref(g), edge, steps, wrap, t,
+             C.byref(in_b), None)
+    hip.call("lgdhip_life2d_unpack", C.c_void_p(pb if in_b.value else pa), C.byref(g), a.ptr, None)
+    return a.store()
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (96, 45), (32, 7), (4096, 130), (320, 1500), (61, 37), (300, 211)])
+def test_life_packed_periodic_bit_exact(hip, oracle, shape):
+    """Torus and cylinder topologies through the bit-packed blocked kernels:
+    rows wrap for any height (7 rows: a band wraps several times), word
+    columns wrap when nx % 32 == 0 (32 cells: one word, its own neighbour),
+    both edge states on the unwrapped axis, the default and the other
+    kernel variants, launch depths with a remainder."""
+    nx, ny = shape
+    g0 = synthetic.life_initial(nx, ny, seed=nx + 7 * ny)
+    for wrap in ((1, 2, 3) if nx % 32 == 0 else (2,)):
+        for edge in (0, 1):
+            exp = oracle.life2d(g0, 45, edge=edge, threads=4, wrap=wrap)
+            for variant, per_launch in ((0, 0), (7, 16), (1, 31), (3, 7), (5, 1), (8, 31), (2, 13)):
+                say("life periodic", shape, "wrap", wrap, "edge", edge, "variant", variant, "per launch", per_launch)
+                t = hip.tile(steps_per_launch=per_launch, variant=variant) if variant else None
+                out = gpu_life_packed_periodic(hip, g0, 45, wrap, edge=edge, tile=t)
+                assert np.array_equal(out, exp), (wrap, edge, variant, per_launch)
 
 
 def test_life_glider_on_gpu(hip):

@@ -107,6 +107,86 @@ def test_jacobi_openmp_matches_serial(oracle):
     assert np.array_equal(oracle.jacobi3d(a, 4, threads=1), oracle.jacobi3d(a, 4, threads=4))
 
 
+def brute_jacobi_periodic(a, steps, edge, wrap):
+    """Pure-Python periodic sweep: a coordinate past a wrapped face is taken
+    modulo the extent, past an unwrapped face the neighbour is the edge."""
+    nz, ny, nx = a.shape
+    n = (nx, ny, nz)
+    for _ in range(steps):
+        b = np.empty_like(a)
+        for z in range(nz):
+            for y in range(ny):
+                for x in range(nx):
+                    def at(*c):
+                        c = [c[d] % n[d] if wrap >> d & 1 else c[d] for d in range(3)]
+                        if all(0 <= c[d] < n[d] for d in range(3)):
+                            return a[c[2], c[1], c[0]]
+                        return edge
+                    s = at(x, y, z - 1)
+                    s += at(x, y - 1, z)
+                    s += at(x - 1, y, z)
+                    s += at(x, y, z)
+                    s += at(x + 1, y, z)
+                    s += at(x, y + 1, z)
+                    s += at(x, y, z + 1)
+                    b[z, y, x] = s * (1.0 / 7.0)
+        a = b
+    return a
+
+
+@pytest.mark.parametrize("wrap", range(8))
+def test_jacobi_periodic_brute_force_tiny(oracle, wrap):
+    a = synthetic.jacobi_initial(5, 4, 3, seed=11)
+    assert np.array_equal(oracle.jacobi3d(a, 3, edge=0.25, wrap=wrap), brute_jacobi_periodic(a, 3, 0.25, wrap))
+    # one-cell axes: the periodic neighbours are the cell itself
+    t = synthetic.jacobi_initial(1, 3, 2, seed=12)
+    assert np.array_equal(oracle.jacobi3d(t, 2, edge=0.5, wrap=wrap), brute_jacobi_periodic(t, 2, 0.5, wrap))
+
+
+def test_jacobi_periodic_wrap0_is_cube(oracle):
+    a = synthetic.jacobi_initial(13, 9, 7, seed=13)
+    assert np.array_equal(oracle.jacobi3d(a, 4, edge=0.5, wrap=0), oracle.jacobi3d(a, 4, edge=0.5))
+
+
+@pytest.mark.parametrize("wrap", [1, 2, 4, 5, 7])
+def test_jacobi_periodic_shift_invariance(oracle, wrap):
+    """Shifting the input cyclically along a wrapped axis shifts the output
+    the same way, bit for bit (each cell sums the same values in the same
+    order); an unwrapped axis has no such symmetry, so a wrap applied to the
+    wrong axis or a neighbour taken from the wrong face fails."""
+    a = synthetic.jacobi_initial(11, 8, 9, seed=14)
+    out = oracle.jacobi3d(a, 3, edge=0.5, wrap=wrap)
+    for d, axis in ((0, 2), (1, 1), (2, 0)):
+        if wrap >> d & 1:
+            shifted = oracle.jacobi3d(np.roll(a, 3, axis=axis), 3, edge=0.5, wrap=wrap)
+            assert np.array_equal(shifted, np.roll(out, 3, axis=axis))
+    if wrap != 7:
+        d = [d for d in range(3) if not wrap >> d & 1][0]
+        axis = 2 - d
+        shifted = oracle.jacobi3d(np.roll(a, 3, axis=axis), 3, edge=0.5, wrap=wrap)
+        assert not np.array_equal(shifted, np.roll(out, 3, axis=axis))
+
+
+def test_jacobi_torus_plane_wave_and_conservation(oracle):
+    """On a full torus cos(2 pi k x / nx) is an eigenvector of the sweep with
+    eigenvalue (5 + 2 cos(2 pi k / nx)) / 7 (its y and z neighbours equal
+    it); the grid sum is conserved (every cell's value goes to 7 cells with
+    weight 1/7)."""
+    nx, ny, nz = 24, 5, 6
+    for k, axis in ((3, 0), (2, 1), (1, 2)):
+        n = (nx, ny, nz)[axis]
+        c = np.cos(2 * np.pi * k * np.arange(n) / n)
+        shape = [1, 1, 1]
+        shape[2 - axis] = n
+        a = np.broadcast_to(c.reshape(shape), (nz, ny, nx)).copy()
+        lam = (5 + 2 * np.cos(2 * np.pi * k / n)) / 7
+        out = oracle.jacobi3d(a, 6, wrap=7)
+        np.testing.assert_allclose(out, a * lam ** 6, rtol=0, atol=1e-14)
+    r = synthetic.jacobi_initial(nx, ny, nz, seed=15)
+    out = oracle.jacobi3d(r, 10, wrap=7)
+    assert abs(out.sum() - r.sum()) < 1e-12 * r.size
+
+
 # ----------------------------------------------------------------------
 # Game of Life
 # ----------------------------------------------------------------------
@@ -167,6 +247,51 @@ def test_life_matches_numpy_brute_force(oracle, edge):
 def test_life_openmp_matches_serial(oracle):
     g = synthetic.life_initial(200, 130, seed=2)
     assert np.array_equal(oracle.life2d(g, 12, threads=1), oracle.life2d(g, 12, threads=4))
+
+
+def numpy_life_periodic(g, steps, edge, wrap):
+    """The 3x3 box sum over a grid padded by one cell: periodic images on
+    wrapped axes (np.pad mode "wrap"), the edge state on the others."""
+    g = g.astype(np.int32)
+    ny, nx = g.shape
+    for _ in range(steps):
+        p = np.pad(g, ((0, 0), (1, 1)), mode="wrap") if wrap & 1 else \
+            np.pad(g, ((0, 0), (1, 1)), constant_values=edge)
+        p = np.pad(p, ((1, 1), (0, 0)), mode="wrap") if wrap & 2 else \
+            np.pad(p, ((1, 1), (0, 0)), constant_values=edge)
+        s9 = sum(p[1 + dy:1 + dy + ny, 1 + dx:1 + dx + nx] for dy in (-1, 0, 1) for dx in (-1, 0, 1))
+        g = ((s9 == 3) | ((g == 1) & (s9 == 4))).astype(np.int32)
+    return g.astype(np.uint8)
+
+
+@pytest.mark.parametrize("wrap", [0, 1, 2, 3])
+@pytest.mark.parametrize("edge", [0, 1])
+def test_life_periodic_matches_numpy(oracle, wrap, edge):
+    g = synthetic.life_initial(45, 29, seed=21)
+    assert np.array_equal(oracle.life2d(g, 7, edge=edge, wrap=wrap), numpy_life_periodic(g, 7, edge, wrap))
+    if wrap == 0:
+        assert np.array_equal(oracle.life2d(g, 7, edge=edge, wrap=0), oracle.life2d(g, 7, edge=edge))
+
+
+def test_life_torus_glider_crosses_both_seams(oracle):
+    """A glider moves (1, 1) every 4 generations: on a 16 x 16 torus it is
+    back in place after 64 and shifted by (8, 8) after 32 (it has crossed
+    both seams); on the Cube it dies at the edge."""
+    glider = [(1, 0), (2, 1), (0, 2), (1, 2), (2, 2)]
+    g = place(glider, (16, 16), at=(5, 5))
+    assert np.array_equal(oracle.life2d(g, 64, wrap=3), g)
+    assert np.array_equal(oracle.life2d(g, 32, wrap=3), np.roll(g, (8, 8), axis=(0, 1)))
+    assert not np.array_equal(oracle.life2d(g, 64), g)
+
+
+@pytest.mark.parametrize("wrap", [1, 2, 3])
+def test_life_periodic_shift_invariance(oracle, wrap):
+    g = synthetic.life_initial(40, 24, seed=22)
+    out = oracle.life2d(g, 6, edge=1, wrap=wrap)
+    for bit, axis in ((1, 1), (2, 0)):
+        if wrap & bit:
+            assert np.array_equal(oracle.life2d(np.roll(g, 5, axis=axis), 6, edge=1, wrap=wrap),
+                                  np.roll(out, 5, axis=axis))
 
 
 # ----------------------------------------------------------------------
