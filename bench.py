@@ -735,7 +735,8 @@ class UNet:
         ms = timed(self.stream, self.net.forward, reps=reps) / reps
         self.fwd_ms = round(ms, 4)
         achieved = self.fwd_flops / (ms * 1e-3) / 1e12
-        return {"kernel": "unet3d_forward (conv3d_fwd_mfma and friends)", "bound": "mfma",
+        return {"kernel": "unet3d_forward (15 convolutions: z-marching, LDS-tile and GEMM-tiled MFMA kernels)",
+                "bound": "mfma",
                 "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
                 "avg_launch_us": round(ms * 1e3, 3), "algorithmic_flops_per_launch": self.fwd_flops}
@@ -882,7 +883,7 @@ def main():
 # configured shape, Infinity-Cache resident), Jacobi3D 512^3 x 100 (the
 # HBM-bound shape), D3Q19 LBM FP32 256^3 x 2000
 SECONDARY = [("jacobi", dict(n=128, steps=100)), ("jacobi", dict(n=512, steps=100)),
-             ("lbm", dict(n=256, steps=2000))]
+             ("lbm", dict(n=256, steps=2000)), ("unet", dict(n=128))]
 
 
 def measure(wl, steps, warmup, cpu, cpu_budget):

@@ -145,6 +145,9 @@ int lgdhip_stream_synchronize(lgdhip_stream_t stream);
 int lgdhip_event_create(lgdhip_event_t *event);
 int lgdhip_event_destroy(lgdhip_event_t event);
 int lgdhip_event_record(lgdhip_event_t event, lgdhip_stream_t stream);
+/* Work queued on `stream` after this call waits for `event` (recorded on
+ * another stream): orders a copy stream after the launches it reads. */
+int lgdhip_stream_wait_event(lgdhip_stream_t stream, lgdhip_event_t event);
 int lgdhip_event_synchronize(lgdhip_event_t event);
 int lgdhip_event_elapsed_ms(float *ms, lgdhip_event_t start, lgdhip_event_t stop);
 
@@ -253,6 +256,16 @@ int lgdhip_jacobi3d_f64_run(double *a, double *b, const lgdhip_grid *g, int step
  * reference's CUDASimulator has no counterpart (its launch shape is the
  * block size it is given, cudasimulator.h:314-429). */
 int lgdhip_jacobi3d_f64_plan(const lgdhip_grid *g, const lgdhip_tile *tile, int *variant, int *zchunk);
+
+/* One launch of a temporal-blocking variant (tile->variant 4-12, or the
+ * default plan's) writing only the planes [z0, z1) of dst: K sweeps of src
+ * for those planes (the rest of dst is not touched), *sweeps = K.  The
+ * building block of a slab stepper that updates its rim planes first, hands
+ * them to the neighbours' ghost zones, and updates the inner planes while
+ * the copies run (the inner-set / rim split of nesting/cudastepper.h:336-380,
+ * 516-575).  Whole grids only. */
+int lgdhip_jacobi3d_f64_run_zrange(const double *src, double *dst, const lgdhip_grid *g, int z0, int z1,
+                                   const lgdhip_tile *tile, int *sweeps, lgdhip_stream_t stream);
 
 /* The same sweeps with periodic axes: wrap_mask bits 1, 2, 4 wrap x, y, z
  * (Topologies::Torus and the selectively wrapped topologies,
@@ -461,6 +474,18 @@ int lgdhip_conv3d_kpad(int cin, int ksize);
 int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
                                const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
                                int relu, int accumulate, lgdhip_stream_t stream);
+/* The same convolution with a workspace for the small volumes: 3x3x3 layers
+ * with cin % 8 == 0 below 64^3 split their k range over work-groups and add
+ * the FP32 partials in a fixed order (deterministic; the k order differs
+ * from the unsplit launch, so results agree to FP32 rounding, not bit for
+ * bit).  workspace_bytes >= lgdhip_conv3d_forward_workspace_bytes(n, cin,
+ * ksize, cout), which is 0 for layers that do not split; a NULL workspace
+ * runs the unsplit kernels, as lgdhip_conv3d_bf16_forward does. */
+size_t lgdhip_conv3d_forward_workspace_bytes(int n, int cin, int ksize, int cout);
+int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
+                                  const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
+                                  int relu, int accumulate, void *workspace, size_t workspace_bytes,
+                                  lgdhip_stream_t stream);
 /* weights of the input-gradient convolution: wt[ci][k'] (Kp' = kpad(cout)),
  * flipped taps, so dx = conv(dy, wt) with cin' = cout, cout' = cin */
 int lgdhip_conv3d_dgrad_weights(const void *w, int cin, int cout, int ksize, void *wt, lgdhip_stream_t stream);

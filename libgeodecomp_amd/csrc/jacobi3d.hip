@@ -12,6 +12,8 @@
 // x/y neighbours come from the same cache lines.
 #include "lgdhip_internal.h"
 
+#include <algorithm>
+
 namespace {
 
 constexpr int kWaveX = 64;  // lanes along x: one 512-byte row segment
@@ -122,6 +124,18 @@ __device__ __forceinline__ double lane_next(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// `inside ? value : edge` as a select: the empty asm makes `value` (and
+// the LDS reads feeding it) unconditional, which keeps the compiler from
+// turning the select into a branch around the sum -- a branch into which it
+// sinks each level's LDS reads and then waits for every read on its own.
+// Used by the two-rows kernel (4% faster at 512^3 with its LDS reads hoisted
+// to the start of the step); the one-row kernel measures faster with the
+// compiler's branch.
+__device__ __forceinline__ double keep_select(double value, bool inside, double edge) {
+    asm volatile("" : "+v"(value));
+    return inside ? value : edge;
+}
+
 // Coordinates of the temporal-blocking kernels.  A tile reaches K cells
 // past the box; on a wrapped axis (Torus topologies) those cells are the
 // periodic images of interior cells and are computed like them, on the
@@ -160,7 +174,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
                                                            int nx, int ny, int nz, int64_t sx, int64_t sxy,
                                                            int64_t origin, int zchunk, unsigned gx, unsigned gy,
                                                            unsigned ntiles, unsigned per, int swizzle, int wrap,
-                                                           int64_t edge_at) {
+                                                           int64_t edge_at, int zbase, int zend) {
     constexpr int OX = 64 - 2 * K, OY = TY - 2 * K;
     static_assert(PF == 3, "level-0 ring of 6: a window of 3 planes plus 3 in flight");
     __shared__ double lds[2][K][TY][64];
@@ -169,8 +183,8 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x = (int)(t % gx) * OX - K + tx;
     const int y = (int)((t / gx) % gy) * OY - K + ty;
-    const int z0 = (int)(t / (gx * gy)) * zchunk;
-    const int z1 = min(z0 + zchunk, nz);
+    const int z0 = zbase + (int)(t / (gx * gy)) * zchunk;  // output planes [zbase, zend) in chunks
+    const int z1 = min(z0 + zchunk, zend);
     // a cell counts as inside when every coordinate is in range or its axis wraps
     const bool inxy = in_axis(x, nx, wrap & 1) && in_axis(y, ny, wrap & 2);
     const bool writer = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && tx >= K && tx < 64 - K &&
@@ -235,6 +249,8 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb(const double *__restr
             s += xp;       // ( 1,  0,  0)
             s += yp;       // ( 0,  1,  0)
             s += b;        // ( 0,  0,  1)
+            // a plain select: the compiler's branch around the sum measures
+            // faster here than the forced select of the two-rows kernel
             const double v = (inxy && in_axis(q, nz, wrap & 4)) ? s * (1.0 / 7.0) : edge;
             if (l < K) {
                 w[l][NEW] = v;
@@ -283,7 +299,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
                                                             int nx, int ny, int nz, int64_t sx, int64_t sxy,
                                                             int64_t origin, int zchunk, unsigned gx, unsigned gy,
                                                             unsigned ntiles, unsigned per, int swizzle, int wrap,
-                                                            int64_t edge_at) {
+                                                            int64_t edge_at, int zbase, int zend) {
     constexpr int PF = 3;
     constexpr int CY = 2 * TY;  // tile rows
     constexpr int OX = 64 - 2 * K, OY = CY - 2 * K;
@@ -295,8 +311,8 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
     const int tx = threadIdx.x, ty = threadIdx.y;
     const int x = (int)(t % gx) * OX - K + tx;
     const int y0 = (int)((t / gx) % gy) * OY - K + 2 * ty;  // rows y0, y0 + 1
-    const int z0 = (int)(t / (gx * gy)) * zchunk;
-    const int z1 = min(z0 + zchunk, nz);
+    const int z0 = zbase + (int)(t / (gx * gy)) * zchunk;
+    const int z1 = min(z0 + zchunk, zend);
     const bool xin = in_axis(x, nx, wrap & 1);
     const bool inxy[2] = {xin && in_axis(y0, ny, wrap & 2), xin && in_axis(y0 + 1, ny, wrap & 2)};
     const bool xw = (unsigned)x < (unsigned)nx && tx >= K && tx < 64 - K;
@@ -337,6 +353,15 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
         constexpr int B = JM & 1;
 #pragma unroll
         for (int r = 0; r < 2; ++r) lv[r][(JM + PF) % 6] = load0(r, p + PF);
+        // the pair's outer y neighbours of every level (previous step's LDS
+        // planes), read before the level chain
+        double ymv[K], ypv[K];
+#pragma unroll
+        for (int l = 1; l <= K; ++l) {
+            if (2 * l > JR) continue;
+            ymv[l - 1] = lds[B][l - 1][tyd][1][tx];
+            ypv[l - 1] = lds[B][l - 1][tyu][0][tx];
+        }
 #pragma unroll
         for (int l = 1; l <= K; ++l) {
             if (2 * l > JR) continue;
@@ -349,8 +374,8 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
                 c[r] = l == 1 ? lv[r][(JM + 5) % 6] : w[l - 1][r][MID];
                 b[r] = l == 1 ? lv[r][JM] : w[l - 1][r][NEW];
             }
-            const double ym[2] = {lds[B][l - 1][tyd][1][tx], c[0]};
-            const double yp[2] = {c[1], lds[B][l - 1][tyu][0][tx]};
+            const double ym[2] = {ymv[l - 1], c[0]};
+            const double yp[2] = {c[1], ypv[l - 1]};
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
                 double s = a[r];          // ( 0,  0, -1)
@@ -360,7 +385,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
                 s += lane_next(c[r]);     // ( 1,  0,  0)
                 s += yp[r];               // ( 0,  1,  0)
                 s += b[r];                // ( 0,  0,  1)
-                const double v = (inxy[r] && zin) ? s * (1.0 / 7.0) : edge;
+                const double v = keep_select(s * (1.0 / 7.0), inxy[r] && zin, edge);
                 if (l < K) {
                     w[l][r][NEW] = v;
                 } else if (writer[r]) {
@@ -442,6 +467,7 @@ struct JacobiLaunch {
     int variant;
     unsigned gx, gy, ntiles, per;  // temporal blocking: tile grid and XCD swizzle
     int swizzle;
+    int zbase = 0, zend = 0;       // temporal blocking: output planes [zbase, zend) (zend 0: all)
 };
 
 // Temporal blocking: tiles of (64-2K) x (TY*rows-2K) output columns; a z
@@ -536,8 +562,10 @@ int launch_tb(const double *src, double *dst, const lgdhip_grid &g, const Jacobi
     const int64_t origin = lgdhip::origin_index(g);
     if (edge_at < 0) edge_at = origin - 1;
     const int nx = (int)g.nx, ny = (int)g.ny, nz = (int)g.nz;
+    const int zend = L.zend > 0 ? L.zend : nz;
 #define LGD_TB_ARGS                                                                                                \
-    src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx, L.gy, L.ntiles, L.per, L.swizzle, wrap, edge_at
+    src, dst, nx, ny, nz, g.sx, sxy, origin, L.zchunk, L.gx, L.gy, L.ntiles, L.per, L.swizzle, wrap, edge_at,     \
+        L.zbase, zend
     switch (L.variant) {
     case 4: jacobi3d_f64_tb<2, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
     case 5: jacobi3d_f64_tb<3, 16, 3><<<L.grid, L.block, 0, s>>>(LGD_TB_ARGS); break;
@@ -727,5 +755,33 @@ extern "C" int lgdhip_jacobi3d_f64_run_periodic(double *a, double *b, const lgdh
         nxt = tmp;
     }
     if (result_in_b) *result_in_b = (cur == b) ? 1 : 0;
+    return 0;
+}
+
+// One blocked launch writing only the planes [z0, z1) of dst (all x, y),
+// K sweeps of src (K of the tile's variant; the default plan's otherwise):
+// the building block of a stepper that updates a slab's rim planes first,
+// hands them to a copy, and then updates the inner planes.
+extern "C" int lgdhip_jacobi3d_f64_run_zrange(const double *src, double *dst, const lgdhip_grid *g, int z0, int z1,
+                                              const lgdhip_tile *tile, int *sweeps, lgdhip_stream_t stream) {
+    int rc = lgdhip::check_grid(g, 3, 8, 1, true);
+    if (rc) return rc;
+    LGD_REQUIRE(src && dst && src != dst, "need two distinct device buffers");
+    LGD_REQUIRE(sweeps, "NULL output pointer");
+    if (g->flags & LGDHIP_GRID_VIEW)
+        return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "a z range of a blocked launch needs a whole grid");
+    LGD_REQUIRE(0 <= z0 && z0 < z1 && z1 <= g->nz, "need 0 <= z0 < z1 <= nz");
+    JacobiLaunch L = plan(*g, tile);
+    const int k = tb_shape(L.variant).k;
+    if (k < 1) return lgdhip::set_error(LGDHIP_ERR_UNSUPPORTED, "a z range needs a temporal-blocking variant");
+    if (!tile || tile->bz <= 0) L.zchunk = std::min(L.zchunk, z1 - z0);
+    L.zbase = z0;
+    L.zend = z1;
+    L.ntiles = L.gx * L.gy * lgdhip::ceil_div(z1 - z0, L.zchunk);
+    L.per = lgdhip::xcd_per(L.ntiles);
+    L.grid = dim3(L.swizzle ? lgdhip::xcd_groups(L.ntiles) : L.ntiles, 1, 1);
+    rc = launch_tb(src, dst, *g, L, lgdhip::as_stream(stream));
+    if (rc) return rc;
+    *sweeps = k;
     return 0;
 }

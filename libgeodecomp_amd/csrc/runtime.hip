@@ -315,6 +315,12 @@ int lgdhip_event_record(lgdhip_event_t event, lgdhip_stream_t stream) {
     return 0;
 }
 
+int lgdhip_stream_wait_event(lgdhip_stream_t stream, lgdhip_event_t event) {
+    LGD_REQUIRE(event, "event is NULL");
+    LGD_HIP(hipStreamWaitEvent(as_stream(stream), (hipEvent_t)event, 0));
+    return 0;
+}
+
 int lgdhip_event_synchronize(lgdhip_event_t event) {
     LGD_REQUIRE(event, "event is NULL");
     LGD_HIP(hipEventSynchronize((hipEvent_t)event));

@@ -19,6 +19,8 @@
 
 #include <hip/hip_bf16.h>
 
+#include <algorithm>
+
 namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -601,6 +603,183 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     }
 }
 
+// GEMM-tiled implicit convolution for 3x3x3 layers with cin % 8 == 0 on the
+// small levels (n < 64) and for cout > 32: Y[v][co] = sum_k A(v, k) W[co][k]
+// over a 64-voxel x 64-channel block per work-group.  k runs in stages of 64
+// (eight 8-channel chunks of one or more taps); a stage's A block (64 voxels
+// x 64 k, gathered from the channels-last input with zeros outside the
+// volume) and B block (64 channels x 64 k of the weights) are staged in LDS,
+// double buffered: the next stage's global loads are in flight in registers
+// while the current one is computed.  Wave w computes the 32 x 32 quarter
+// (voxel half w & 1, channel half w >> 1) as 2 x 2 MFMA tiles, two k steps
+// of 32 per stage, every fragment one ds_read_b128 (rows padded to 72
+// elements: the 16 rows of a fragment read hit distinct banks).  A block
+// reads its A and B data from L2 once instead of once per wave.
+// With `slices` > 1 the stage range is split over work-groups (a slice
+// each, for small volumes whose 64 x 64 blocks alone leave CUs idle) and
+// every slice writes FP32 partials P[slice][v][co]; conv3d_gemm_reduce adds
+// them in slice order.  k order: stage-major, then k within the stage (the
+// tap-major order of the weights), per slice.
+constexpr int kGemmBM = 64, kGemmBN = 64, kGemmBK = 64;
+constexpr int kGemmLd = kGemmBK + 8;  // LDS row stride in elements (144 bytes)
+
+template <bool SPLIT>
+__global__ __launch_bounds__(256) void conv3d_fwd_gemm(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
+                                                       const u16 *__restrict__ W, int Kp,
+                                                       const float *__restrict__ bias, int cout, u16 *Y, int ldy,
+                                                       int offy, float *Y32, int ld32, int relu, int accumulate,
+                                                       int mblocks, int nblocks, int slices, int stages,
+                                                       float *__restrict__ P) {
+    __shared__ uint4 lds[2][2][kGemmBM * kGemmLd / 8];  // [buffer][A, B][row * kGemmLd / 8 + chunk]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int i = lane & 15, q = lane >> 4;
+    int b = blockIdx.x;
+    const int mb = b % mblocks;
+    b /= mblocks;
+    const int nb = b % nblocks;
+    const int slice = b / nblocks;
+    const int64_t nvox = (int64_t)n * n * n;
+    const int64_t v0 = (int64_t)mb * kGemmBM;
+    const int co0 = nb * kGemmBN;
+    const int st0 = (int)((int64_t)slice * stages / slices), st1 = (int)((int64_t)(slice + 1) * stages / slices);
+    const int K = 27 * cin;
+
+    // loader roles: chunk c = tid + 256 u (u = 0, 1) of the 512 16-byte
+    // chunks of each block: row c / 8, k chunk c % 8 (the same for both)
+    const int j = tid & 7;
+    const gh_ptr zero = (gh_ptr)(const void *)kZero16;
+    const gh_ptr xg = (gh_ptr)X + offx;
+    int vx[2], vy[2], vz[2];
+    bool vok[2];
+    const gh_ptr wrow[2] = {(gh_ptr)W + (int64_t)(co0 + (tid >> 3)) * Kp,
+                            (gh_ptr)W + (int64_t)(co0 + (tid >> 3) + 32) * Kp};
+    const bool cok[2] = {co0 + (tid >> 3) < cout, co0 + (tid >> 3) + 32 < cout};
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int64_t v = v0 + (tid >> 3) + 32 * u;
+        vok[u] = v < nvox;
+        vx[u] = (int)(v % n);
+        vy[u] = (int)((v / n) % n);
+        vz[u] = (int)(v / ((int64_t)n * n));
+    }
+    const float inv_cin = 1.0f / (float)cin;
+    uint4 ra[2], rb[2];
+    auto load_stage = [&](int st) {
+        const int k0 = st * kGemmBK + 8 * j;
+        // tap = k0 / cin from a float reciprocal, corrected (k0 < 2^24)
+        int tap = (int)((float)k0 * inv_cin);
+        tap -= tap * cin > k0;
+        tap += (tap + 1) * cin <= k0;
+        const int ci = k0 - tap * cin;
+        const bool kin = k0 < K;
+        int dz, dy, dx;
+        tap_offset(kin ? tap : 13, 3, dz, dy, dx);
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int zz = vz[u] + dz, yy = vy[u] + dy, xx = vx[u] + dx;
+            const bool ok = kin && vok[u] && (unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n &&
+                            (unsigned)xx < (unsigned)n;
+            ra[u] = load_global16(ok ? xg + ((int64_t)(zz * n + yy) * n + xx) * ldx + ci : zero);
+            rb[u] = load_global16(cok[u] && k0 < Kp ? wrow[u] + k0 : zero);
+        }
+    };
+    auto store_stage = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int row = (tid >> 3) + 32 * u;
+            lds[buf][0][row * (kGemmLd / 8) + j] = ra[u];
+            lds[buf][1][row * (kGemmLd / 8) + j] = rb[u];
+        }
+    };
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int t = 0; t < 2; ++t) acc[m][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int wm = wave & 1, wn = wave >> 1;
+    if (st0 < st1) {
+        load_stage(st0);
+        store_stage(0);
+        __syncthreads();
+        for (int st = st0; st < st1; ++st) {
+            const int cur = (st - st0) & 1;
+            // the next stage's loads go out first (the last stage re-loads
+            // its own, never stored, so every iteration waits alike)
+            load_stage(st + 1 < st1 ? st + 1 : st);
+#pragma unroll
+            for (int ks = 0; ks < 2; ++ks) {
+                Frag a[2], bf[2];
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+                    a[m].q = lds[cur][0][(32 * wm + 16 * m + i) * (kGemmLd / 8) + 4 * ks + q];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    bf[t].q = lds[cur][1][(32 * wn + 16 * t + i) * (kGemmLd / 8) + 4 * ks + q];
+#pragma unroll
+                for (int m = 0; m < 2; ++m)
+#pragma unroll
+                    for (int t = 0; t < 2; ++t)
+                        acc[m][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m].v, bf[t].v, acc[m][t], 0, 0, 0);
+            }
+            if (st + 1 < st1) store_stage(cur ^ 1);
+            __syncthreads();
+        }
+    }
+    // D[r][c] in lane c + 16 (r / 4), element r % 4: voxel row 4q + e of the
+    // tile, channel i
+    const int ldp = nblocks * kGemmBN;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int co = co0 + 32 * wn + 16 * t + i;
+        const float bv = !SPLIT && bias && co < cout ? bias[co] : 0.0f;
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t v = v0 + 32 * wm + 16 * m + 4 * q + e;
+                if (v >= nvox) continue;
+                if (SPLIT) {
+                    P[((int64_t)slice * nvox + v) * ldp + co] = acc[m][t][e];
+                    continue;
+                }
+                if (co >= cout) continue;
+                float r = acc[m][t][e] + bv;
+                if (relu) r = r > 0.0f ? r : 0.0f;
+                if (Y) {
+                    u16 *pp = Y + v * ldy + offy + co;
+                    if (accumulate) r += bf2f(*pp);
+                    *pp = f2bf(r);
+                }
+                if (Y32) Y32[v * ld32 + co] = r;
+            }
+    }
+}
+
+// the slices' partials of conv3d_fwd_gemm summed in slice order, then bias,
+// ReLU, accumulation and the BF16 / FP32 stores of the unsplit epilogue
+__global__ __launch_bounds__(256) void conv3d_gemm_reduce(const float *__restrict__ P, int64_t nvox, int ldp,
+                                                          int slices, const float *__restrict__ bias, int cout,
+                                                          u16 *Y, int ldy, int offy, float *Y32, int ld32, int relu,
+                                                          int accumulate) {
+    // a split launch has fewer than 1024 blocks of 64 x 64 outputs, so
+    // nvox * cout < 2^22 and 32-bit index arithmetic holds
+    const uint32_t idx = blockIdx.x * 256u + threadIdx.x;
+    if (idx >= (uint32_t)(nvox * cout)) return;
+    const uint32_t v = idx / (uint32_t)cout;
+    const int co = (int)(idx - v * (uint32_t)cout);
+    float r = 0.0f;
+    for (int sl = 0; sl < slices; ++sl) r += P[((int64_t)sl * nvox + v) * ldp + co];
+    r += bias ? bias[co] : 0.0f;
+    if (relu) r = r > 0.0f ? r : 0.0f;
+    if (Y) {
+        u16 *pp = Y + (int64_t)v * ldy + offy + co;
+        if (accumulate) r += bf2f(*pp);
+        *pp = f2bf(r);
+    }
+    if (Y32) Y32[(int64_t)v * ld32 + co] = r;
+}
+
 // 1x1x1 convolution to a single output channel (the U-Net's output layer):
 // one thread per voxel, its cin channels in 16-byte loads (cin % 8 == 0), an
 // FP32 dot product with the cin weights.  An MFMA tile would compute 16
@@ -633,6 +812,41 @@ __global__ __launch_bounds__(256) void conv1x1_to1(const u16 *__restrict__ X, in
 // the LDS-staged kernel takes 3x3x3 layers with up to 32 output channels on
 // volumes of at least 16^3
 constexpr int kLdsMinN = 16;
+
+// The GEMM-tiled kernel's launch: 64 x 64 blocks, stages of 64 k, and the
+// number of slices the stage range is split into when a workspace for the
+// partials is given -- enough work-groups for 4 per CU, at least 4 stages
+// per slice.
+struct GemmPlan {
+    int mblocks, nblocks, stages, slices;
+};
+GemmPlan gemm_plan(int n, int cin, int cout, bool split) {
+    GemmPlan p;
+    const int64_t nvox = (int64_t)n * n * n;
+    p.mblocks = (int)((nvox + kGemmBM - 1) / kGemmBM);
+    p.nblocks = (cout + kGemmBN - 1) / kGemmBN;
+    p.stages = (lgdhip_conv3d_kpad(cin, 3) + kGemmBK - 1) / kGemmBK;
+    p.slices = 1;
+    const int64_t base = (int64_t)p.mblocks * p.nblocks, target = 4 * lgdhip::kCUs;
+    if (split && base < target) {
+        const int64_t want = (target + base - 1) / base;
+        p.slices = (int)std::max<int64_t>(1, std::min<int64_t>(want, p.stages / 4));
+    }
+    return p;
+}
+size_t gemm_workspace_bytes(int n, const GemmPlan &p) {
+    if (p.slices <= 1) return 0;
+    return (size_t)p.slices * (size_t)n * n * n * (size_t)p.nblocks * kGemmBN * 4;
+}
+// the layers the GEMM-tiled kernel takes (16-byte input rows; the caller
+// has checked ldx, offx and the alignment when cin % 8 == 0): 3x3x3 with
+// cin % 8 == 0 and more than 16 output channels below 64^3, and any size
+// with more than 32 (the LDS-tile kernel takes 32 channels from 64^3 on;
+// 16-channel layers stay with the z-marching kernel and, for the launches it
+// does not take, the LDS-tile kernel, which sums in the same k order)
+bool gemm_applies(int n, int cin, int ksize, int cout) {
+    return ksize == 3 && (cin & 7) == 0 && cout > 16 && (n < 64 || cout > 32);
+}
 
 // voxel tiles per wave of the forward kernel: weight-fragment reuse bounded
 // by the accumulator registers (MT x NT x 4)
@@ -1028,13 +1242,26 @@ struct alignas(16) Bf8 {
 __device__ __forceinline__ Bf8 ld8(const u16 *p) { return *reinterpret_cast<const Bf8 *>(p); }
 __device__ __forceinline__ void st8(u16 *p, const Bf8 &v) { *reinterpret_cast<Bf8 *>(p) = v; }
 
+// Thread coordinates of the 8-channel resampling kernels over a w^3 volume:
+// a launch row_grid(w, c8) gives each (y, z) row blockIdx.y / .z, and the
+// threads along x cover its (x, channel chunk) pairs, the chunk fastest
+// (consecutive threads, consecutive 16-byte pieces of a voxel's channels).
+// 32-bit arithmetic throughout (a flat 64-bit index divided by run-time
+// extents costs hundreds of instructions per thread).
+__device__ __forceinline__ bool row_thread(int w, int c8, int &x, int &ch) {
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= w * c8) return false;
+    x = t / c8;
+    ch = (t - x * c8) * 8;
+    return true;
+}
+
 __global__ void avgpool2_v8(const u16 *__restrict__ X, int n, int ldx, int offx, u16 *Y, int ldy, int offy, int c8) {
     const int m = n / 2;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)m * m * m * c8) return;
-    const int ch = (int)(idx % c8) * 8;
-    const int64_t v = idx / c8;
-    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    int x, ch;
+    if (!row_thread(m, c8, x, ch)) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const int64_t v = ((int64_t)z * m + y) * m + x;
     float s[8] = {};
     for (int d = 0; d < 8; ++d) {
         const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
@@ -1051,11 +1278,10 @@ __global__ void avgpool2_v8(const u16 *__restrict__ X, int n, int ldx, int offx,
 __global__ void avgpool2_backward_v8(const u16 *__restrict__ dY, int n, int ldy, int offy, u16 *dX, int ldx, int offx,
                                      int c8, int accumulate) {
     const int m = n / 2;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * n * n * c8) return;
-    const int ch = (int)(idx % c8) * 8;
-    const int64_t v = idx / c8;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    int x, ch;
+    if (!row_thread(n, c8, x, ch)) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const int64_t v = ((int64_t)z * n + y) * n + x;
     const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
     const Bf8 a = ld8(dY + p * ldy + offy + ch);
     u16 *q = dX + v * ldx + offx + ch;
@@ -1073,11 +1299,10 @@ __global__ void avgpool2_backward_v8(const u16 *__restrict__ dY, int n, int ldy,
 
 __global__ void upsample2_v8(const u16 *__restrict__ X, int m, int ldx, int offx, u16 *Y, int ldy, int offy, int c8) {
     const int n = 2 * m;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)n * n * n * c8) return;
-    const int ch = (int)(idx % c8) * 8;
-    const int64_t v = idx / c8;
-    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    int x, ch;
+    if (!row_thread(n, c8, x, ch)) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const int64_t v = ((int64_t)z * n + y) * n + x;
     const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
     st8(Y + v * ldy + offy + ch, ld8(X + p * ldx + offx + ch));
 }
@@ -1085,11 +1310,10 @@ __global__ void upsample2_v8(const u16 *__restrict__ X, int m, int ldx, int offx
 __global__ void upsample2_backward_v8(const u16 *__restrict__ dY, int m, int ldy, int offy, u16 *dX, int ldx, int offx,
                                       int c8, int accumulate) {
     const int n = 2 * m;
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)m * m * m * c8) return;
-    const int ch = (int)(idx % c8) * 8;
-    const int64_t v = idx / c8;
-    const int x = (int)(v % m), y = (int)((v / m) % m), z = (int)(v / ((int64_t)m * m));
+    int x, ch;
+    if (!row_thread(m, c8, x, ch)) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const int64_t v = ((int64_t)z * m + y) * m + x;
     float s[8] = {};
     for (int d = 0; d < 8; ++d) {
         const int zz = 2 * z + (d >> 2), yy = 2 * y + ((d >> 1) & 1), xx = 2 * x + (d & 1);
@@ -1155,6 +1379,11 @@ __global__ void adam_step(float *__restrict__ w, float *__restrict__ m, float *_
 }
 
 unsigned blocks_for(int64_t n, int per = 256) { return (unsigned)((n + per - 1) / per); }
+// the launch of a row_thread kernel over a w^3 volume with c8 chunks of 8
+// channels (w <= kMaxRowGrid: the grid's y and z extents; larger volumes take
+// the per-element kernels)
+constexpr int64_t kMaxRowGrid = 65535;
+dim3 row_grid(int w, int c8) { return dim3(blocks_for((int64_t)w * c8), (unsigned)w, (unsigned)w); }
 // 8-channel (16-byte) resampling applies: channel count, strides and channel
 // offsets multiples of 8 elements, both base pointers 16-byte aligned
 bool vec8(const void *a, const void *b, int c, int lda, int offa, int ldb, int offb) {
@@ -1181,9 +1410,22 @@ extern "C" {
 
 int lgdhip_conv3d_kpad(int cin, int ksize) { return (((ksize == 1 ? 1 : 27) * cin + 31) / 32) * 32; }
 
+size_t lgdhip_conv3d_forward_workspace_bytes(int n, int cin, int ksize, int cout) {
+    if (n < 1 || cin < 1 || cout < 1 || !gemm_applies(n, cin, ksize, cout)) return 0;
+    return gemm_workspace_bytes(n, gemm_plan(n, cin, cout, true));
+}
+
 int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
                                const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
                                int relu, int accumulate, lgdhip_stream_t stream) {
+    return lgdhip_conv3d_bf16_forward_ws(x, n, ldx, offx, cin, ksize, w, bias, cout, y, ldy, offy, y32, ld32, relu,
+                                         accumulate, nullptr, 0, stream);
+}
+
+int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *w,
+                                  const float *bias, int cout, void *y, int ldy, int offy, float *y32, int ld32,
+                                  int relu, int accumulate, void *workspace, size_t workspace_bytes,
+                                  lgdhip_stream_t stream) {
     LGD_REQUIRE(x && w && (y || y32), "NULL pointer");
     LGD_REQUIRE(ksize == 1 || ksize == 3, "ksize must be 1 or 3");
     LGD_REQUIRE(cout >= 1 && cout <= 16 * kMaxNTiles, "cout must be in [1, 192]");
@@ -1233,6 +1475,31 @@ int lgdhip_conv3d_bf16_forward(const void *x, int n, int ldx, int offx, int cin,
             LGD_LAUNCHED("conv3d_fwd_zm");
             return 0;
         }
+    }
+    if (gemm_applies(n, cin, ksize, cout)) {
+        const GemmPlan p = gemm_plan(n, cin, cout, workspace != nullptr);
+        if (workspace && workspace_bytes < gemm_workspace_bytes(n, p))
+            return lgdhip::set_error(LGDHIP_ERR_WORKSPACE,
+                                     "forward workspace too small (lgdhip_conv3d_forward_workspace_bytes)");
+        const unsigned blocks = (unsigned)((int64_t)p.mblocks * p.nblocks * p.slices);
+        if (p.slices > 1) {
+            // (gemm_plan splits only launches of fewer than 4 x 256 blocks)
+            float *P = (float *)workspace;
+            conv3d_fwd_gemm<true><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, y32,
+                                                         ld32, relu, accumulate, p.mblocks, p.nblocks, p.slices,
+                                                         p.stages, P);
+            LGD_LAUNCHED("conv3d_fwd_gemm");
+            conv3d_gemm_reduce<<<blocks_for(nvox * cout), 256, 0, s>>>(P, nvox, p.nblocks * kGemmBN, p.slices, bias,
+                                                                       cout, Y, ldy, offy, y32, ld32, relu,
+                                                                       accumulate);
+            LGD_LAUNCHED("conv3d_gemm_reduce");
+        } else {
+            conv3d_fwd_gemm<false><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, y32,
+                                                          ld32, relu, accumulate, p.mblocks, p.nblocks, 1, p.stages,
+                                                          nullptr);
+            LGD_LAUNCHED("conv3d_fwd_gemm");
+        }
+        return 0;
     }
     if (ksize == 3 && nt <= 2 && n >= kLdsMinN) {
         const int bx = (n + kLdsTX - 1) / kLdsTX, by = (n + kLdsTY - 1) / kLdsTY;
@@ -1421,8 +1688,8 @@ int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int l
                          lgdhip_stream_t stream) {
     LGD_REQUIRE(x && y && n % 2 == 0, "bad arguments");
     const int64_t m = n / 2;
-    if (vec8(x, y, c, ldx, offx, ldy, offy)) {
-        avgpool2_v8<<<blocks_for(m * m * m * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+    if (vec8(x, y, c, ldx, offx, ldy, offy) && n <= kMaxRowGrid) {
+        avgpool2_v8<<<row_grid((int)m, c / 8), 256, 0, lgdhip::as_stream(stream)>>>(
             (const u16 *)x, n, ldx, offx, (u16 *)y, ldy, offy, c / 8);
         LGD_LAUNCHED("avgpool2_v8");
         return 0;
@@ -1436,8 +1703,8 @@ int lgdhip_avgpool2_bf16(const void *x, int n, int ldx, int offx, void *y, int l
 int lgdhip_avgpool2_backward_bf16(const void *dy, int n, int ldy, int offy, void *dx, int ldx, int offx, int c,
                                   int accumulate, lgdhip_stream_t stream) {
     LGD_REQUIRE(dy && dx && n % 2 == 0, "bad arguments");
-    if (vec8(dy, dx, c, ldy, offy, ldx, offx)) {
-        avgpool2_backward_v8<<<blocks_for((int64_t)n * n * n * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+    if (vec8(dy, dx, c, ldy, offy, ldx, offx) && n <= kMaxRowGrid) {
+        avgpool2_backward_v8<<<row_grid(n, c / 8), 256, 0, lgdhip::as_stream(stream)>>>(
             (const u16 *)dy, n, ldy, offy, (u16 *)dx, ldx, offx, c / 8, accumulate);
         LGD_LAUNCHED("avgpool2_backward_v8");
         return 0;
@@ -1452,8 +1719,8 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
                           lgdhip_stream_t stream) {
     LGD_REQUIRE(x && y && m >= 1, "bad arguments");
     const int64_t n = 2 * (int64_t)m;
-    if (vec8(x, y, c, ldx, offx, ldy, offy)) {
-        upsample2_v8<<<blocks_for(n * n * n * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+    if (vec8(x, y, c, ldx, offx, ldy, offy) && n <= kMaxRowGrid) {
+        upsample2_v8<<<row_grid((int)n, c / 8), 256, 0, lgdhip::as_stream(stream)>>>(
             (const u16 *)x, m, ldx, offx, (u16 *)y, ldy, offy, c / 8);
         LGD_LAUNCHED("upsample2_v8");
         return 0;
@@ -1467,8 +1734,8 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
 int lgdhip_upsample2_backward_bf16(const void *dy, int m, int ldy, int offy, void *dx, int ldx, int offx, int c,
                                    int accumulate, lgdhip_stream_t stream) {
     LGD_REQUIRE(dy && dx && m >= 1, "bad arguments");
-    if (vec8(dy, dx, c, ldy, offy, ldx, offx)) {
-        upsample2_backward_v8<<<blocks_for((int64_t)m * m * m * (c / 8)), 256, 0, lgdhip::as_stream(stream)>>>(
+    if (vec8(dy, dx, c, ldy, offy, ldx, offx) && 2 * (int64_t)m <= kMaxRowGrid) {
+        upsample2_backward_v8<<<row_grid(m, c / 8), 256, 0, lgdhip::as_stream(stream)>>>(
             (const u16 *)dy, m, ldy, offy, (u16 *)dx, ldx, offx, c / 8, accumulate);
         LGD_LAUNCHED("upsample2_backward_v8");
         return 0;

@@ -115,6 +115,7 @@ SIGNATURES = {
     "lgdhip_event_create": (_I, [C.POINTER(_P)]),
     "lgdhip_event_destroy": (_I, [_P]),
     "lgdhip_event_record": (_I, [_P, _P]),
+    "lgdhip_stream_wait_event": (_I, [_P, _P]),
     "lgdhip_event_synchronize": (_I, [_P]),
     "lgdhip_event_elapsed_ms": (_I, [C.POINTER(C.c_float), _P, _P]),
     "lgdhip_graph_begin": (_I, [_P]),
@@ -137,6 +138,7 @@ SIGNATURES = {
     "lgdhip_grid_store": (_I, [_P, _GP, _P, _P]),
     "lgdhip_jacobi3d_f64_run": (_I, [_P, _P, _GP, _I, _TP, _IP, _P]),
     "lgdhip_jacobi3d_f64_plan": (_I, [_GP, _TP, _IP, _IP]),
+    "lgdhip_jacobi3d_f64_run_zrange": (_I, [_P, _P, _GP, _I, _I, _TP, _IP, _P]),
     "lgdhip_jacobi3d_f64_run_periodic": (_I, [_P, _P, _GP, _I, _I, C.c_double, _TP, _IP, _P]),
     "lgdhip_life2d_u8_run": (_I, [_P, _P, _GP, _I, _IP, _P]),
     "lgdhip_life2d_packed_words": (_I64, [_GP]),
@@ -167,6 +169,9 @@ SIGNATURES = {
                                         C.POINTER(lgdhip_poisson_result), _P]),
     "lgdhip_conv3d_kpad": (_I, [_I, _I]),
     "lgdhip_conv3d_bf16_forward": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _I, _P, _I, _I, _I, _P]),
+    "lgdhip_conv3d_forward_workspace_bytes": (_SZ, [_I, _I, _I, _I]),
+    "lgdhip_conv3d_bf16_forward_ws": (_I, [_P, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _I, _P, _I, _I, _I, _P, _SZ,
+                                           _P]),
     "lgdhip_conv3d_dgrad_weights": (_I, [_P, _I, _I, _I, _P, _P]),
     "lgdhip_conv3d_wgrad_workspace_bytes": (_SZ, [_I, _I, _I, _I, _I]),
     "lgdhip_conv3d_bf16_wgrad": (_I, [_P, _I, _I, _I, _I, _I, _P, _I, _I, _I, _P, _P, _I, _I, _P, _SZ, _P]),

@@ -161,7 +161,10 @@ class UNet3D:
         self.gpred = Tensor(s[0], 1)
         ws = 0
         for l in L:
-            ws = max(ws, self.lib.lgdhip_conv3d_wgrad_workspace_bytes(l.n, l.cin, l.ksize, l.cout, 4096))
+            ws = max(ws, self.lib.lgdhip_conv3d_wgrad_workspace_bytes(l.n, l.cin, l.ksize, l.cout, 4096),
+                     # the forward convolution's and the input gradient's (cin and cout swapped) split partials
+                     self.lib.lgdhip_conv3d_forward_workspace_bytes(l.n, l.cin, l.ksize, l.cout),
+                     self.lib.lgdhip_conv3d_forward_workspace_bytes(l.n, l.cout, l.ksize, l.cin))
         self.ws_bytes = ws
         self.ws = lgdhip.DeviceBuffer(ws)
         self.loss_partial = lgdhip.DeviceBuffer(self.lib.lgdhip_mse_partials(s[0] ** 3) * 4)
@@ -203,10 +206,10 @@ class UNet3D:
     # ------------------------------------------------------------------
     def _conv(self, name, x: Tensor, xoff, y: Tensor | None, yoff, y32=None, accumulate=0):
         l = self.layers[name]
-        lgdhip.call("lgdhip_conv3d_bf16_forward", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, l.wb.ptr,
+        lgdhip.call("lgdhip_conv3d_bf16_forward_ws", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, l.wb.ptr,
                     l.bias.ptr if self.use_bias else None,
                     l.cout, y.ptr if y is not None else None, y.ld if y is not None else 1, yoff,
-                    y32, 1, 1 if l.relu else 0, accumulate, self.stream)
+                    y32, 1, 1 if l.relu else 0, accumulate, self.ws.ptr, self.ws_bytes, self.stream)
 
     def load_input(self, fields: list):
         """fields: FP32 device buffers (n^3 each) -> input tensor channels."""
@@ -265,8 +268,9 @@ class UNet3D:
                     l.cout, l.gw.ptr, l.gb.ptr if self.use_bias else None, accumulate_w, 4096, self.ws.ptr,
                     self.ws_bytes, S)
         if g_in is not None:
-            lgdhip.call("lgdhip_conv3d_bf16_forward", g_out.ptr, l.n, g_out.ld, goff, l.cout, l.ksize, l.wt.ptr,
-                        None, l.cin, g_in.ptr, g_in.ld, ginoff, None, 1, 0, accumulate_in, S)
+            lgdhip.call("lgdhip_conv3d_bf16_forward_ws", g_out.ptr, l.n, g_out.ld, goff, l.cout, l.ksize,
+                        l.wt.ptr, None, l.cin, g_in.ptr, g_in.ld, ginoff, None, 1, 0, accumulate_in, self.ws.ptr,
+                        self.ws_bytes, S)
 
     def backward(self, accumulate: int = 0):
         """Gradients of the last loss() w.r.t. every weight (accumulate adds

@@ -15,6 +15,7 @@
 #include <libgeodecomp/misc/patternoptimizer.h>
 #include <libgeodecomp/parallelization/autotuningsimulator.h>
 #include <libgeodecomp/parallelization/hipsimulator.h>
+#include <libgeodecomp/parallelization/nesting/hipslabstepper.h>
 #include <serialsimulator.h>
 
 #include <cmath>
@@ -488,6 +489,65 @@ bool sameGrid(const GridBase<CELL, DIM> *a, const GridBase<CELL, DIM> *b)
         }
     }
     return true;
+}
+
+// ---------------------------------------------------------------------
+// slab stepper: z slabs with ghost zones, one blocked launch per slab and
+// round split into rim and inner planes, rims copied to the neighbours'
+// ghost zones on a copy stream (the multi-GPU step loop on one GPU)
+// ---------------------------------------------------------------------
+static void slabCase(Coord<3> dim, int slabs, int width, unsigned steps, double edgeValue)
+{
+    std::printf("  slabs %d, ghost zone %d, %u steps, %dx%dx%d\n", slabs, width, steps, dim.x(), dim.y(), dim.z());
+    typedef HipSlabStepper<JacobiCell>::GridType JacobiGrid;
+    JacobiInitializer init(dim, steps, 7, edgeValue);
+    JacobiGrid initial(CoordBox<3>(Coord<3>(), dim));
+    init.grid(&initial);
+    HipSlabStepper<JacobiCell> stepper(initial, slabs, width);
+    CHECK_EQ(stepper.slabs(), (std::size_t)slabs);
+    stepper.update(steps);
+    JacobiGrid got = initial;
+    stepper.grid(&got);
+    SerialSimulator<JacobiCell> ref(new JacobiInitializer(dim, steps, 7, edgeValue));
+    ref.run();
+    CHECK(sameGrid(&got, ref.getGrid()));
+    CHECK_EQ(stepper.currentStep(), steps);
+    const unsigned long long rounds = (steps + width - 1) / width;
+    CHECK_EQ(stepper.roundsRun(), rounds);
+    CHECK_EQ(stepper.planesExchanged(), rounds * 2 * (unsigned long long)(slabs - 1) * width);
+    // every slab updates its rim next to each neighbour separately from its
+    // inner planes unless the slab is too thin to have inner planes
+    CHECK(stepper.rimUpdates() <= rounds * 2 * (unsigned long long)(slabs - 1));
+    // the stepper continues from its state: more steps equal a longer run
+    stepper.update(3);
+    stepper.grid(&got);
+    SerialSimulator<JacobiCell> ref2(new JacobiInitializer(dim, steps + 3, 7, edgeValue));
+    ref2.run();
+    CHECK(sameGrid(&got, ref2.getGrid()));
+}
+
+LGD_TEST(slab_stepper_equals_oracle)
+{
+    slabCase(Coord<3>(45, 38, 40), 1, 4, 9, 0.25);  // one slab: no exchange
+    slabCase(Coord<3>(45, 38, 40), 2, 4, 12, 0.25);
+    slabCase(Coord<3>(64, 20, 37), 3, 4, 11, 0.5);  // uneven slabs, a partial round
+    slabCase(Coord<3>(30, 33, 16), 4, 4, 8, 0.0);   // slabs one ghost zone thick: no inner planes
+    slabCase(Coord<3>(30, 33, 21), 5, 2, 7, 0.5);   // two-sweep rounds
+    slabCase(Coord<3>(17, 9, 12), 4, 3, 10, 0.25);  // three-sweep rounds
+    slabCase(Coord<3>(17, 9, 12), 6, 1, 5, 0.25);   // one sweep per round (the one-sweep tile kernel)
+    slabCase(Coord<3>(130, 70, 96), 3, 4, 8, 0.0);  // the two-rows-per-thread default for larger slabs
+}
+
+LGD_TEST(slab_stepper_rejects_thin_slabs)
+{
+    HipSlabStepper<JacobiCell>::GridType initial(CoordBox<3>(Coord<3>(), Coord<3>(8, 8, 10)));
+    bool thrown = false;
+    try {
+        HipSlabStepper<JacobiCell> stepper(initial, 3, 4);  // slabs of 3 planes, ghost zones of 4
+    } catch (const std::invalid_argument&) {
+        thrown = true;
+    }
+    CHECK(thrown);
 }
 
 LGD_TEST(hip_simulation_factory_every_jacobi_variant)

@@ -99,6 +99,35 @@ def test_life_periodic_rejects_ragged_wrapped_x_without_gpu():
     assert rc == -1 and "wrap_mask" in lib.lgdhip_last_error().decode()
 
 
+def test_conv_forward_workspace_without_gpu():
+    """The split partials' workspace (host-only query and check): the small
+    levels of the U-Net ask for one, the 64^3 and 128^3 levels (LDS-tile
+    and z-marching kernels) do not; a workspace smaller than asked for is
+    refused before any launch."""
+    lib = lgdhip.load()
+    q = lib.lgdhip_conv3d_forward_workspace_bytes
+    assert q(32, 192, 3, 64) > 0 and q(16, 128, 3, 128) > 0 and q(8, 192, 3, 64) > 0
+    assert q(128, 16, 3, 16) == 0 and q(64, 32, 3, 32) == 0 and q(16, 3, 3, 16) == 0 and q(16, 16, 1, 1) == 0
+    assert q(16, 64, 3, 16) == 0  # 16 output channels: not the GEMM kernel
+    fake = C.c_void_p(0x1000)
+    rc = lib.lgdhip_conv3d_bf16_forward_ws(fake, 16, 64, 0, 64, 3, fake, None, 64, fake, 64, 0, None, 1, 0, 0,
+                                           fake, 16, None)
+    assert rc == -3 and "workspace" in lib.lgdhip_last_error().decode()
+
+
+def test_jacobi_zrange_rejects_bad_ranges_without_gpu():
+    lib = lgdhip.load()
+    g = lgdhip.make_grid(3, 16, 16, 12, halo=1, elem_bytes=8)
+    fake = C.c_void_p(0x1000), C.c_void_p(0x2000)
+    k = C.c_int(0)
+    for z0, z1 in ((0, 0), (5, 3), (-1, 4), (0, 13)):
+        rc = lib.lgdhip_jacobi3d_f64_run_zrange(fake[0], fake[1], C.byref(g), z0, z1, None, C.byref(k), None)
+        assert rc == -1 and "z0" in lib.lgdhip_last_error().decode()
+    t = lgdhip.tile(variant=2)  # a single-sweep variant has no z range
+    rc = lib.lgdhip_jacobi3d_f64_run_zrange(fake[0], fake[1], C.byref(g), 0, 4, C.byref(t), C.byref(k), None)
+    assert rc == -2 and "temporal-blocking" in lib.lgdhip_last_error().decode()
+
+
 def test_jacobi_launch_plan_without_gpu():
     """lgdhip_jacobi3d_f64_plan (host-only): the default picks 4 sweeps per
     launch with one row per thread on a 128^3 cube (5 chunks of 26 planes)

@@ -126,6 +126,33 @@ def test_jacobi_periodic_bit_exact(hip, oracle, shape, steps):
             assert np.array_equal(got, exp), (wrap, variant, bz)
 
 
+def test_jacobi_zrange_launch_writes_only_its_planes(hip, oracle):
+    """lgdhip_jacobi3d_f64_run_zrange: one blocked launch writes the planes
+    [z0, z1) of dst, equal to K oracle sweeps there, and leaves every other
+    plane of dst as it was (the rim / inner split of the slab stepper)."""
+    import ctypes as C
+    nx, ny, nz = 70, 45, 40
+    a = synthetic.jacobi_initial(nx, ny, nz, seed=5)
+    sentinel = np.full((nz, ny, nx), -7.25)
+    g = hip.make_grid(3, nx, ny, nz, halo=1, elem_bytes=8)
+    src, dst = hip.DeviceGrid(g, np.float64), hip.DeviceGrid(g, np.float64)
+    for d in (src, dst):
+        d.fill_edge(0.5)
+    src.load(a)
+    for variant, k in ((0, 4), (4, 2), (5, 3), (11, 1), (10, 4), (12, 1)):
+        exp = oracle.jacobi3d(a, k, edge=0.5, threads=4)
+        for z0, z1 in ((0, 4), (4, 36), (36, 40), (13, 14), (0, 40)):
+            say("jacobi zrange", variant, z0, z1)
+            dst.load(sentinel)
+            sweeps = C.c_int(0)
+            hip.call("lgdhip_jacobi3d_f64_run_zrange", src.ptr, dst.ptr, C.byref(g), z0, z1,
+                     C.byref(hip.tile(variant=variant)), C.byref(sweeps), None)
+            out = dst.store()
+            assert sweeps.value == k
+            assert np.array_equal(out[z0:z1], exp[z0:z1])
+            assert np.array_equal(out[:z0], sentinel[:z0]) and np.array_equal(out[z1:], sentinel[z1:])
+
+
 def test_jacobi_periodic_zero_steps(hip):
     a = synthetic.jacobi_initial(12, 10, 9, seed=3)
     assert np.array_equal(gpu_jacobi_periodic(hip, a, 0, 7), a)

@@ -89,11 +89,14 @@ def run_conv(hip, x, w, bias, ksize, ld_out=None, off_out=0, relu=False, y_init=
     return out32.reshape(n, n, n, cout), from_bf16_bits(outb).reshape(n, n, n, ld)
 
 
-# n >= 16 with up to 32 output channels runs the LDS-staged kernel (odd
-# sizes leave partial blocks; cin 2 and 3 take its element-load path)
+# 3x3x3 with cin % 8 == 0 below 64^3 (and any layer with more than 32
+# output channels) runs the GEMM-tiled kernel, unsplit through this entry
+# point; 16 output channels from 16^3 on the z-marching kernel; the LDS tile
+# kernel takes 32 channels from 64^3 on and cin 2 / 3 (its element-load
+# path); odd sizes leave partial blocks
 CONV_CASES = [(8, 2, 16, 3), (16, 16, 16, 3), (12, 48, 16, 3), (8, 192, 64, 3), (16, 16, 1, 1), (4, 64, 128, 3),
               (8, 32, 192, 3), (6, 8, 24, 3), (32, 16, 16, 3), (33, 48, 16, 3), (20, 3, 16, 3), (16, 2, 16, 3),
-              (24, 32, 32, 3), (17, 16, 24, 3)]
+              (24, 32, 32, 3), (17, 16, 24, 3), (64, 16, 32, 3), (9, 24, 72, 3)]
 
 
 @pytest.mark.gpu
@@ -141,14 +144,16 @@ def test_conv_input_row_wider_than_cin(hip, oracle):
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,cin,cout,ld,off,xld", [(17, 16, 16, 16, 0, 16), (19, 48, 16, 48, 32, 48),
                                                    (33, 40, 16, 16, 0, 40), (21, 3, 16, 16, 0, 8),
-                                                   (18, 32, 32, 32, 0, 32), (16, 16, 32, 96, 64, 16)])
+                                                   (18, 32, 32, 32, 0, 32), (16, 16, 32, 96, 64, 16),
+                                                   (64, 16, 32, 48, 16, 16)])
 def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout, ld, off, xld):
-    """BF16 output alone (16 output channels: the z-marching kernel; 32: the
-    tile kernel's LDS-transposed 16-byte stores) is byte-for-byte the output
-    written next to an FP32 copy (the tile kernel with per-element stores),
-    bias and ReLU included, partial z chunks (n = 33) and a 3-channel input
-    in an 8-channel row of junk (xld = 8) too; the slice's other channels
-    are untouched."""
+    """BF16 output alone (16 output channels: the z-marching kernel; 32 at
+    64^3: the tile kernel's LDS-transposed 16-byte stores) is byte-for-byte
+    the output written next to an FP32 copy (the tile kernel with
+    per-element stores), bias and ReLU included, partial z chunks (n = 33)
+    and a 3-channel input in an 8-channel row of junk (xld = 8) too; the
+    slice's other channels are untouched (32 channels below 64^3: the GEMM
+    kernel, the same launch either way)."""
     rng = np.random.default_rng(n + cin + cout + ld)
     row = bf16_round(rng.standard_normal((n, n, n, xld)).astype(np.float32) * 50)
     row[..., :cin] = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
@@ -178,6 +183,78 @@ def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout,
     assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-3)
 
 
+def run_conv_split(hip, x, w, bias, ld_out=None, off_out=0, relu=False, y_init=None):
+    """run_conv through lgdhip_conv3d_bf16_forward_ws with the workspace the
+    layer asks for; returns (y32, y_bf16, workspace bytes)."""
+    n, cin = x.shape[0], x.shape[-1]
+    cout = w.shape[0]
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, 3)
+    wsb = lib.lgdhip_conv3d_forward_workspace_bytes(n, cin, 3, cout)
+    ws = hip.DeviceBuffer(max(wsb, 16))
+    xb = dev(hip, to_bf16_bits(x))
+    wb = dev(hip, to_bf16_bits(conv_weights_layout(w.astype(np.float32), kp)))
+    bb = dev(hip, bias.astype(np.float32))
+    ld = ld_out or cout
+    y = dev(hip, to_bf16_bits(y_init) if y_init is not None else np.zeros(n ** 3 * ld, np.uint16))
+    y32 = hip.DeviceBuffer(n ** 3 * cout * 4)
+    hip.call("lgdhip_conv3d_bf16_forward_ws", xb.ptr, n, cin, 0, cin, 3, wb.ptr, bb.ptr, cout, y.ptr, ld, off_out,
+             y32.ptr, cout, 1 if relu else 0, 1 if y_init is not None else 0, ws.ptr, wsb, None)
+    out32 = np.empty(n ** 3 * cout, np.float32)
+    y32.download(out32)
+    outb = np.empty(n ** 3 * ld, np.uint16)
+    y.download(outb)
+    return out32.reshape(n, n, n, cout), from_bf16_bits(outb).reshape(n, n, n, ld), wsb
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout", [(8, 192, 64), (16, 64, 64), (16, 128, 128), (12, 48, 80)])
+def test_conv_split_k_vs_oracle(hip, oracle, n, cin, cout):
+    """Small volumes with a workspace: the GEMM kernel's k range split over
+    work-groups (3-20 slices here) and the partials added in slice order.
+    Same bound as the unsplit kernels (each output is still an FP32 sum of
+    the same products, grouped by slice); the BF16 output is the FP32
+    result rounded; two runs are identical (fixed-order reduction)."""
+    print("[case] conv split-k", n, cin, cout, flush=True)
+    rng = np.random.default_rng(n * 7 + cin + cout)
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / np.sqrt(27 * cin)).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
+    y32, yb, wsb = run_conv_split(hip, x, w, b)
+    assert wsb > 0  # the layer splits
+    ref = oracle.conv3d_forward(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64), 3)
+    mag = oracle.conv3d_forward(np.abs(x).astype(np.float64), np.abs(w).astype(np.float64), np.abs(b), 3)
+    K = 27 * cin
+    assert np.all(np.abs(y32 - ref) <= (K + 2) * 2.0 ** -24 * mag + 1e-30)
+    assert np.array_equal(yb, bf16_round(y32))
+    y32b, _, _ = run_conv_split(hip, x, w, b)
+    assert np.array_equal(y32, y32b)
+    # against the unsplit launch: FP32 rounding apart
+    u32, _ = run_conv(hip, x, w, b, 3)
+    assert np.all(np.abs(u32 - y32) <= 2 * (K + 2) * 2.0 ** -24 * mag + 1e-30)
+
+
+@pytest.mark.gpu
+def test_conv_split_k_relu_slice_accumulate(hip, oracle):
+    """The split path's reduction applies ReLU, the accumulation into an
+    existing BF16 slice and the FP32 copy like the unsplit epilogue, and
+    leaves the slice's other channels alone."""
+    rng = np.random.default_rng(44)
+    n, cin, cout, ld, off = 16, 64, 48, 96, 32
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / 40).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
+    init = bf16_round(rng.standard_normal((n, n, n, ld)).astype(np.float32))
+    y32, yb, wsb = run_conv_split(hip, x, w, b, ld_out=ld, off_out=off, relu=True, y_init=init)
+    assert wsb > 0
+    ref = np.maximum(oracle.conv3d_forward(x, w.astype(np.float64), b.astype(np.float64), 3), 0)
+    got = y32.astype(np.float64) - init[..., off:off + cout]
+    assert np.all(np.abs(got - ref) <= 1e-5 * (1 + np.abs(init[..., off:off + cout]) + ref))
+    assert np.array_equal(yb[..., :off], init[..., :off])
+    assert np.array_equal(yb[..., off + cout:], init[..., off + cout:])
+    assert np.array_equal(yb[..., off:off + cout], bf16_round(y32))
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("n", [8, 19])
 def test_conv_relu_slice_accumulate(hip, oracle, n):
@@ -198,7 +275,7 @@ def test_conv_relu_slice_accumulate(hip, oracle, n):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n,cin,cout", [(8, 16, 16), (8, 48, 16), (4, 192, 64), (8, 16, 1), (16, 16, 16),
-                                        (18, 32, 16)])
+                                        (18, 32, 16), (16, 192, 64), (12, 64, 128)])
 def test_conv_dgrad_vs_oracle(hip, oracle, n, cin, cout):
     print("[case] conv dgrad", n, cin, cout, flush=True)
     rng = np.random.default_rng(cin + cout)
