@@ -913,20 +913,42 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__
 
     const int v_begin = chunk * chunk_vox;
     const int v_end = min(v_begin + chunk_vox, nvox);
+    // coordinates of this lane group's first voxel vb + 8 vq, carried along
+    // (one voxel per element, 32 per iteration) instead of divided out per
+    // element
+    int x0 = (v_begin + 8 * vq) % n, y0 = ((v_begin + 8 * vq) / n) % n, z0 = (v_begin + 8 * vq) / (n * n);
+    auto advance = [n](int &xa, int &ya, int &za, int by) {
+        xa += by;
+        while (xa >= n) {
+            xa -= n;
+            if (++ya == n) {
+                ya = 0;
+                ++za;
+            }
+        }
+    };
     for (int vb = v_begin; vb < v_end; vb += 32) {
         Frag b;
+        int x = x0, y = y0, z = z0;
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int v = vb + 8 * vq + e;
             u16 val = 0;
             if (v < v_end && k < K) {
-                const int x = v % n, y = (v / n) % n, z = v / (n * n);
                 const int zz = z + dz, yy = y + dy, xx = x + dx;
                 if ((unsigned)zz < (unsigned)n && (unsigned)yy < (unsigned)n && (unsigned)xx < (unsigned)n)
                     val = X[((int64_t)((zz * n + yy) * n + xx)) * ldx + offx + ci];
             }
             b.h[e] = val;
+            if (++x == n) {  // the next voxel
+                x = 0;
+                if (++y == n) {
+                    y = 0;
+                    ++z;
+                }
+            }
         }
+        advance(x0, y0, z0, 32);
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             Frag a;
@@ -954,15 +976,21 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__
 // multiples of 8): one thread per (voxel, group of 8 channels), a 16-byte
 // read and 8 writes that are contiguous across the wave
 __global__ __launch_bounds__(256) void to_channels_first(const u16 *__restrict__ src, int64_t nvox, int ld, int off,
-                                                         u16 *__restrict__ dst) {
+                                                         u16 *__restrict__ dst, int64_t ldc) {
     const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int g = blockIdx.y;
     if (v >= nvox) return;
     Frag f;
     f.q = *(const uint4 *)(src + v * ld + off + 8 * g);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) dst[(int64_t)(8 * g + e) * nvox + v] = f.h[e];
+    for (int e = 0; e < 8; ++e) dst[(int64_t)(8 * g + e) * ldc + v] = f.h[e];
 }
+
+// Row stride of the channels-first copies: nvox padded by 256 bytes.  With
+// an unpadded stride of a power of two (4 MiB at 128^3) the 16 channel rows
+// a fragment load touches map to the same L2 channel.
+constexpr int64_t kCfPad = 128;
+inline int64_t cf_stride(int64_t nvox) { return nvox + kCfPad; }
 
 // 8 BF16 of a channels-first row shifted by one voxel: dx = -1 gives
 // element e = c[e-1] with element 0 from the previous 8 (prev[7]); dx = +1
@@ -990,7 +1018,7 @@ __device__ __forceinline__ uint4 shift_from_next(uint4 c, uint4 next) {
 template <int CT, int IT, int DXT>
 __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ Xc, const u16 *__restrict__ Gc, int n,
                                                        int cin, int cout, int Kp, int chunk_vox, int cotg, int citg,
-                                                       float *__restrict__ P) {
+                                                       float *__restrict__ P, int64_t ldc) {
     constexpr int kGroups = 27 / DXT;  // (dz, dy) rows, or single taps
     const int lane = threadIdx.x & 63;
     const int64_t nvox = (int64_t)n * n * n;
@@ -1017,38 +1045,79 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ X
 
     const int64_t v_begin = chunk * chunk_vox;
     const int64_t v_end = v_begin + chunk_vox < nvox ? v_begin + chunk_vox : nvox;
-    for (int64_t vb = v_begin; vb < v_end; vb += 32) {
+    // this lane group's voxel v = vb + 8q and its coordinates, stepped along
+    // with v (32 voxels per iteration) instead of divided out each time
+    int x, y, z;
+    {
+        const int64_t v0 = v_begin + 8 * q;
+        x = (int)(v0 % n);
+        const int64_t r0 = v0 / n;
+        y = (int)(r0 % n);
+        z = (int)(r0 / n);
+    }
+    // One 32-voxel step's fragments: dY (A) and each X row's centre 8 with
+    // its left and right neighbours.  Loads outside the volume read a zero
+    // block instead of being skipped, so they need no branch, and the next
+    // step's loads are issued before this step's MFMAs (the last step
+    // re-loads its own), keeping a step of loads in flight behind the MFMAs.
+    const gh_ptr zero = (gh_ptr)(const void *)kZero16;
+    struct Step {
+        uint4 a[CT], c[IT], prev[IT], next[IT];
+    };
+    auto fetch = [&](int64_t vb, int xs, int ys, int zs, Step &f) {
         const int64_t v = vb + 8 * q;
-        const int x = (int)(v % n);
-        const int64_t r = v / n;
-        const int y = (int)(r % n), z = (int)(r / n);
-        Frag a[CT];
 #pragma unroll
         for (int t = 0; t < CT; ++t) {
             const int co = (cog * CT + t) * 16 + i;
-            a[t].q = *(const uint4 *)(Gc + (int64_t)co * nvox + v);
+            f.a[t] = load_global16((gh_ptr)Gc + (int64_t)co * ldc + v);
         }
-        const int yy = y + dy, zz = z + dz;
+        const int yy = ys + dy, zz = zs + dz;
         const bool rowok = (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n;
-        const int64_t base = ((int64_t)zz * n + yy) * n + x;
-        Frag b[DXT][IT];
+        const int64_t base = ((int64_t)zz * n + yy) * n + xs;
+        const bool lo = rowok && (DXT == 3 || dx0 < 0) && xs > 0;
+        const bool hi = rowok && (DXT == 3 || dx0 > 0) && xs + 8 < n;
 #pragma unroll
         for (int u = 0; u < IT; ++u) {
             const int ci = (cig * IT + u) * 16 + i;
-            const u16 *row = Xc + (int64_t)ci * nvox;
-            const uint4 zero = make_uint4(0, 0, 0, 0);
-            uint4 c = zero, prev = zero, next = zero;
-            if (rowok) {
-                c = *(const uint4 *)(row + base);
-                if ((DXT == 3 || dx0 < 0) && x > 0) prev = *(const uint4 *)(row + base - 8);
-                if ((DXT == 3 || dx0 > 0) && x + 8 < n) next = *(const uint4 *)(row + base + 8);
+            const gh_ptr row = (gh_ptr)Xc + (int64_t)ci * ldc;
+            f.c[u] = load_global16(rowok ? row + base : zero);
+            f.prev[u] = load_global16(lo ? row + base - 8 : zero);
+            f.next[u] = load_global16(hi ? row + base + 8 : zero);
+        }
+    };
+    auto advance = [n](int &xa, int &ya, int &za) {
+        xa += 32;
+        while (xa >= n) {  // once per row for n >= 32
+            xa -= n;
+            if (++ya == n) {
+                ya = 0;
+                ++za;
             }
+        }
+    };
+    Step cur, nxt;
+    fetch(v_begin, x, y, z, cur);
+    for (int64_t vb = v_begin; vb < v_end; vb += 32) {
+        int xn = x, yn = y, zn = z;
+        advance(xn, yn, zn);
+        if (vb + 32 < v_end) {
+            fetch(vb + 32, xn, yn, zn, nxt);
+        } else {
+            fetch(vb, x, y, z, nxt);
+        }
+        Frag a[CT], b[DXT][IT];
+#pragma unroll
+        for (int t = 0; t < CT; ++t) a[t].q = cur.a[t];
+#pragma unroll
+        for (int u = 0; u < IT; ++u) {
             if (DXT == 3) {
-                b[0][u].q = shift_from_prev(c, prev);
-                b[1 % DXT][u].q = c;
-                b[2 % DXT][u].q = shift_from_next(c, next);
+                b[0][u].q = shift_from_prev(cur.c[u], cur.prev[u]);
+                b[1 % DXT][u].q = cur.c[u];
+                b[2 % DXT][u].q = shift_from_next(cur.c[u], cur.next[u]);
             } else {
-                b[0][u].q = dx0 == 0 ? c : (dx0 < 0 ? shift_from_prev(c, prev) : shift_from_next(c, next));
+                b[0][u].q = dx0 == 0 ? cur.c[u]
+                                     : (dx0 < 0 ? shift_from_prev(cur.c[u], cur.prev[u])
+                                                : shift_from_next(cur.c[u], cur.next[u]));
             }
         }
 #pragma unroll
@@ -1058,6 +1127,10 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ X
 #pragma unroll
                 for (int u = 0; u < IT; ++u)
                     acc[d][t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t].v, b[d][u].v, acc[d][t][u], 0, 0, 0);
+        cur = nxt;
+        x = xn;
+        y = yn;
+        z = zn;
     }
     // D lane layout: row (co) 4*(lane/16) + e, column (ci) lane % 16
 #pragma unroll
@@ -1076,30 +1149,46 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ X
 
 // dW[co][k] = sum over chunks of P (fixed order) for k < K; the padding
 // columns k >= K stay zero
-__global__ void conv3d_wgrad_reduce(const float *__restrict__ P, int chunks, int cout, int Kp, int K,
-                                    float *__restrict__ dW, int accumulate) {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (int64_t)cout * Kp) return;
-    if (idx % Kp >= K) {
-        dW[idx] = accumulate ? dW[idx] : 0.0f;
+// A work-group of 64 x 4 threads takes 64 consecutive outputs; thread row j
+// sums the chunks j, j + 4, j + 8, ... (eight loads in flight, coalesced
+// across the 64 outputs) and the four row sums are added in row order:
+// a fixed order for every output.
+constexpr int kWgradReduceOutputs = 64, kWgradReduceRows = 4;
+__global__ __launch_bounds__(256) void conv3d_wgrad_reduce(const float *__restrict__ P, int chunks, int cout, int Kp,
+                                                           int K, float *__restrict__ dW, int accumulate) {
+    __shared__ float part[kWgradReduceRows][kWgradReduceOutputs];
+    const int total = cout * Kp;  // < 2^31: cout <= 192, Kp <= 27 * 192 + 31
+    const int o = (int)blockIdx.x * kWgradReduceOutputs + (int)threadIdx.x;
+    const int j = (int)threadIdx.y;
+    float s = 0.0f;
+    if (o < total && o % Kp < K) {
+#pragma unroll 8
+        for (int c = j; c < chunks; c += kWgradReduceRows) s += P[(int64_t)c * total + o];
+    }
+    part[j][threadIdx.x] = s;
+    __syncthreads();
+    if (j != 0 || o >= total) return;
+    float r = part[0][threadIdx.x];
+#pragma unroll
+    for (int k = 1; k < kWgradReduceRows; ++k) r += part[k][threadIdx.x];
+    if (o % Kp >= K) {  // padding columns stay zero
+        dW[o] = accumulate ? dW[o] : 0.0f;
         return;
     }
-    float s = 0.0f;
-    for (int c = 0; c < chunks; ++c) s += P[(int64_t)c * cout * Kp + idx];
-    dW[idx] = accumulate ? dW[idx] + s : s;
+    dW[o] = accumulate ? dW[o] + r : r;
 }
 
 // db from the channels-first gradient: partial sums of each row over
 // chunks of 8192 voxels (coalesced), then a fixed-order sum of the chunks
 constexpr int kBiasChunk = 8192;
 __global__ __launch_bounds__(256) void row_sum_partial(const u16 *__restrict__ Gc, int64_t nvox, float *__restrict__ Pb,
-                                                       int bchunks) {
+                                                       int bchunks, int64_t ldc) {
     __shared__ float red[256];
     const int co = blockIdx.x, c = blockIdx.y;
     const int64_t v0 = (int64_t)c * kBiasChunk;
     const int64_t v1 = v0 + kBiasChunk < nvox ? v0 + kBiasChunk : nvox;
     float s = 0.0f;
-    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) s += bf2f(Gc[(int64_t)co * nvox + v]);
+    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) s += bf2f(Gc[(int64_t)co * ldc + v]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -1558,7 +1647,7 @@ size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, 
     // [P: chunk partials][bias partials][x and dy channels-first copies]
     size_t bytes = (size_t)(chunks * cout * lgdhip_conv3d_kpad(cin, ksize) * 4);
     bytes += (size_t)cout * (size_t)((nvox + kBiasChunk - 1) / kBiasChunk) * 4;
-    if (wgrad_cf_shape(n, cin, ksize, cout)) bytes += 256 + (size_t)(cin + cout) * (size_t)nvox * 2;
+    if (wgrad_cf_shape(n, cin, ksize, cout)) bytes += 256 + (size_t)(cin + cout) * (size_t)cf_stride(nvox) * 2;
     return bytes;
 }
 
@@ -1591,9 +1680,10 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
         char *base = (char *)(Pb + (size_t)cout * bchunks0);
         base += (256 - (uintptr_t)base % 256) % 256;
         u16 *Xc = (u16 *)base;
-        u16 *Gc = Xc + (size_t)cin * nvox;
-        to_channels_first<<<dim3(blocks_for(nvox), cin / 8), 256, 0, s>>>((const u16 *)x, nvox, ldx, offx, Xc);
-        to_channels_first<<<dim3(blocks_for(nvox), cout / 8), 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, Gc);
+        const int64_t ldc = cf_stride(nvox);
+        u16 *Gc = Xc + (size_t)cin * ldc;
+        to_channels_first<<<dim3(blocks_for(nvox), cin / 8), 256, 0, s>>>((const u16 *)x, nvox, ldx, offx, Xc, ldc);
+        to_channels_first<<<dim3(blocks_for(nvox), cout / 8), 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, Gc, ldc);
         LGD_LAUNCHED("to_channels_first");
         const int cot = cout / 16, cit = cin / 16;
         const int CT = cot % 4 == 0 ? 4 : (cot % 2 == 0 ? 2 : 1);
@@ -1606,7 +1696,7 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
         const unsigned grid = (unsigned)((waves + 3) / 4);
 #define LGD_CF_CASE(A, B, D)                                                                                 \
     if (CT == A && IT == B && DXT == D)                                                                      \
-        conv3d_wgrad_cf<A, B, D><<<grid, 256, 0, s>>>(Xc, Gc, n, cin, cout, Kp, chunk_vox, cotg, citg, P);
+        conv3d_wgrad_cf<A, B, D><<<grid, 256, 0, s>>>(Xc, Gc, n, cin, cout, Kp, chunk_vox, cotg, citg, P, ldc);
         LGD_CF_CASE(1, 1, 3)
         LGD_CF_CASE(1, 2, 3)
         LGD_CF_CASE(1, 4, 3)
@@ -1618,11 +1708,13 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
         LGD_CF_CASE(4, 4, 1)
 #undef LGD_CF_CASE
         LGD_LAUNCHED("conv3d_wgrad_cf");
-        conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, K, dw, accumulate);
+        conv3d_wgrad_reduce<<<(unsigned)((cout * Kp + kWgradReduceOutputs - 1) / kWgradReduceOutputs),
+                              dim3(kWgradReduceOutputs, kWgradReduceRows), 0, s>>>(P, chunks, cout, Kp, K, dw,
+                                                                                    accumulate);
         LGD_LAUNCHED("conv3d_wgrad_reduce");
         if (db) {
             const int bchunks = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
-            row_sum_partial<<<dim3(cout, bchunks), 256, 0, s>>>(Gc, nvox, Pb, bchunks);
+            row_sum_partial<<<dim3(cout, bchunks), 256, 0, s>>>(Gc, nvox, Pb, bchunks, ldc);
             row_sum_final<<<blocks_for(cout), 256, 0, s>>>(Pb, bchunks, cout, db, accumulate);
             LGD_LAUNCHED("row_sum");
         }
@@ -1652,7 +1744,9 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
     }
 #undef LGD_WG_CASE
     LGD_LAUNCHED("conv3d_wgrad_mfma");
-    conv3d_wgrad_reduce<<<blocks_for((int64_t)cout * Kp), 256, 0, s>>>(P, chunks, cout, Kp, K, dw, accumulate);
+    conv3d_wgrad_reduce<<<(unsigned)((cout * Kp + kWgradReduceOutputs - 1) / kWgradReduceOutputs),
+                              dim3(kWgradReduceOutputs, kWgradReduceRows), 0, s>>>(P, chunks, cout, Kp, K, dw,
+                                                                                    accumulate);
     LGD_LAUNCHED("conv3d_wgrad_reduce");
     if (db) {
         const int bchunks = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
