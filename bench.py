@@ -587,7 +587,7 @@ class Learned:
     dtype = "bf16"   # the network (BF16 operands, FP32 accumulation); polishing is FP32/FP64
     bytes_per_lup = 12.0
 
-    def __init__(self, n=128, steps=20000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, lr=5e-4, **_):
+    def __init__(self, n=128, steps=30000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, lr=5e-4, **_):
         from libgeodecomp_amd import learned_guess as lgm
         self.lgm = lgm
         self.n, self.train_steps, self.seed, self.ntrain, self.neval = n, steps, seed, train, eval

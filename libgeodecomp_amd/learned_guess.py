@@ -26,7 +26,7 @@ learned guess, so the network's own contribution is visible.
 No reference counterpart.  Everything on the device is a liblgdhip kernel;
 this module only orchestrates (and times) them.
 
-    python -m libgeodecomp_amd.learned_guess --n 128 --train 512 --eval 16 --steps 20000 --lr 5e-4
+    python -m libgeodecomp_amd.learned_guess --n 128 --train 512 --eval 16 --steps 30000 --lr 5e-4
 """
 from __future__ import annotations
 
@@ -305,7 +305,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--train", type=int, default=512, help="solved training problems")
     ap.add_argument("--eval", type=int, default=16, help="held-out problems")
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=30000)
     ap.add_argument("--batch", type=int, default=2)
     ap.add_argument("--lr", type=float, default=5e-4, help="peak Adam learning rate (cosine decay)")
     ap.add_argument("--tol", type=float, default=1e-6)
