@@ -9,9 +9,13 @@
 // slab's box are taken as the edge, which corrupts only the outermost K
 // planes of the ghost zones after K sweeps: exactly what the next exchange
 // overwrites).  Within a round the launch is split by z range: the rim
-// (the ghostZoneWidth() planes next to each neighbour) first; its planes go
-// to the neighbours' ghost zones on a copy stream, ordered after the rim by
-// an event, while the inner planes are computed on the compute stream.
+// (the ghostZoneWidth() planes next to each neighbour) runs on a rim stream
+// and its planes go to the neighbours' ghost zones on a copy stream, ordered
+// after the rim by an event, while the inner planes are computed on the
+// compute stream.  Rim and inner launches of one round read the same buffer
+// and write disjoint planes, so they overlap; the next round's inner
+// launches wait for this round's copies (which waited for the rims), its
+// rims for this round's inner launches and copies.
 // This is the step loop a multi-GPU run would use with the copies crossing
 // to other GPUs; here all slabs share one GPU, so the copies are
 // device-to-device.  Results equal HipSimulator's (and the oracle's) bit
@@ -142,6 +146,7 @@ public:
     {
         // the buffers outlive the streams' queued work
         lgdhip_stream_synchronize(compute.get());
+        lgdhip_stream_synchronize(rims.get());
         lgdhip_stream_synchronize(copies.get());
     }
 
@@ -160,8 +165,7 @@ public:
     // the current state of every slab's own planes
     void grid(GridType *out)
     {
-        compute.sync();
-        copies.sync();
+        sync();
         for (Slab& slab : slabList) {
             std::vector<double> dense((std::size_t)nx * ny * (slab.z1 - slab.z0 + slab.lo + slab.hi));
             HipKernels::check(lgdhip_grid_store(slab.cur->get(), &slab.g, dense.data(), compute.get()),
@@ -176,6 +180,14 @@ public:
                 }
             }
         }
+    }
+
+    // wait for every queued round (for timing; grid() waits by itself)
+    void sync()
+    {
+        compute.sync();
+        rims.sync();
+        copies.sync();
     }
 
     unsigned currentStep() const { return step; }
@@ -204,14 +216,14 @@ private:
     unsigned step;
     unsigned long long planesCopied, rimLaunches, innerLaunches, rounds;
     std::vector<Slab> slabList;
-    HipKernels::Stream compute, copies;
-    HipKernels::Event copiesDone;
+    HipKernels::Stream compute, rims, copies;
+    HipKernels::Event copiesDone, innerDone;
 
     // variant of the blocked kernels for k sweeps per launch (4: the
     // default plan's choice for the slab's shape)
     static int variantFor(int k) { return k == 1 ? 11 : (k == 2 ? 4 : (k == 3 ? 5 : 0)); }
 
-    void launch(Slab& slab, int z0, int z1, int k, unsigned long long *counter)
+    void launch(Slab& slab, int z0, int z1, int k, lgdhip_stream_t stream, unsigned long long *counter)
     {
         if (z0 >= z1) {
             return;
@@ -221,7 +233,7 @@ private:
         tile.variant = variantFor(k);
         int sweeps = 0;
         HipKernels::check(lgdhip_jacobi3d_f64_run_zrange((const double *)slab.cur->get(), (double *)slab.nxt->get(),
-                                                         &slab.g, z0, z1, &tile, &sweeps, compute.get()),
+                                                         &slab.g, z0, z1, &tile, &sweeps, stream),
                           "lgdhip_jacobi3d_f64_run_zrange");
         if (sweeps != k) {
             throw std::logic_error("blocked launch ran an unexpected number of sweeps");
@@ -241,22 +253,29 @@ private:
         planesCopied += (unsigned long long)count;
     }
 
-    // k <= width steps on every slab: rim planes, their copies to the
-    // neighbours' ghost zones (copy stream), then the inner planes
+    // k <= width steps on every slab: rim planes (rim stream) and their
+    // copies to the neighbours' ghost zones (copy stream), next to the inner
+    // planes (compute stream)
     void round(int k)
     {
         const std::size_t n = slabList.size();
+        // the rims read the ghost zones the last round's copies wrote and the
+        // planes its inner launches wrote, and overwrite what those read
+        if (rounds > 0) {
+            HipKernels::check(lgdhip_stream_wait_event(rims.get(), copiesDone.get()), "lgdhip_stream_wait_event");
+            HipKernels::check(lgdhip_stream_wait_event(rims.get(), innerDone.get()), "lgdhip_stream_wait_event");
+        }
         for (std::size_t s = 0; s < n; ++s) {
             Slab& slab = slabList[s];
             const int own0 = slab.lo, own1 = slab.lo + (slab.z1 - slab.z0);
             // rim: the planes each neighbour keeps as its ghost zone
             const int rimLo = s > 0 ? std::min(own0 + slabList[s - 1].hi, own1) : own0;
             const int rimHi = s + 1 < n ? std::max(own1 - slabList[s + 1].lo, rimLo) : own1;
-            launch(slab, own0, rimLo, k, &rimLaunches);
+            launch(slab, own0, rimLo, k, rims.get(), &rimLaunches);
             if (rimHi < own1) {
-                launch(slab, std::max(rimHi, rimLo), own1, k, &rimLaunches);
+                launch(slab, std::max(rimHi, rimLo), own1, k, rims.get(), &rimLaunches);
             }
-            HipKernels::check(lgdhip_event_record(slab.rimDone->get(), compute.get()), "lgdhip_event_record");
+            HipKernels::check(lgdhip_event_record(slab.rimDone->get(), rims.get()), "lgdhip_event_record");
             HipKernels::check(lgdhip_stream_wait_event(copies.get(), slab.rimDone->get()),
                               "lgdhip_stream_wait_event");
             if (s > 0) {
@@ -267,12 +286,14 @@ private:
                 Slab& above = slabList[s + 1];
                 copyPlanes(slab, own1 - above.lo, above, 0, above.lo);
             }
-            // inner planes while the copies run
-            launch(slab, rimLo, std::max(rimHi, rimLo), k, &innerLaunches);
+            // inner planes while the rims and copies run
+            launch(slab, rimLo, std::max(rimHi, rimLo), k, compute.get(), &innerLaunches);
         }
-        // the next round reads the ghost zones: the compute stream waits
-        // for this round's copies
+        // the next round's inner launches read the ghost zones and rim
+        // planes (the copies waited for every rim) and overwrite what the
+        // rims read
         HipKernels::check(lgdhip_event_record(copiesDone.get(), copies.get()), "lgdhip_event_record");
+        HipKernels::check(lgdhip_event_record(innerDone.get(), compute.get()), "lgdhip_event_record");
         HipKernels::check(lgdhip_stream_wait_event(compute.get(), copiesDone.get()), "lgdhip_stream_wait_event");
         for (Slab& slab : slabList) {
             std::swap(slab.cur, slab.nxt);

@@ -18,6 +18,7 @@
 #include <libgeodecomp/parallelization/nesting/hipslabstepper.h>
 #include <serialsimulator.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
@@ -536,6 +537,46 @@ LGD_TEST(slab_stepper_equals_oracle)
     slabCase(Coord<3>(17, 9, 12), 4, 3, 10, 0.25);  // three-sweep rounds
     slabCase(Coord<3>(17, 9, 12), 6, 1, 5, 0.25);   // one sweep per round (the one-sweep tile kernel)
     slabCase(Coord<3>(130, 70, 96), 3, 4, 8, 0.0);  // the two-rows-per-thread default for larger slabs
+}
+
+// the stepper's rate against the single-grid blocked sweeps on the same
+// 256^3 state (printed, not asserted: a measurement, not a contract)
+LGD_TEST(slab_stepper_rate)
+{
+    typedef HipSlabStepper<JacobiCell>::GridType JacobiGrid;
+    const Coord<3> dim(256, 256, 256);
+    const unsigned steps = 40;
+    JacobiInitializer init(dim, steps, 9, 0.0);
+    JacobiGrid initial(CoordBox<3>(Coord<3>(), dim));
+    init.grid(&initial);
+    const double cells = (double)dim.x() * dim.y() * dim.z();
+    // single grid: lgdhip_jacobi3d_f64_run on two device buffers
+    lgdhip_grid g;
+    CHECK_EQ(lgdhip_grid_make(&g, 3, dim.x(), dim.y(), dim.z(), 1, 8, 1), 0);
+    HipKernels::DeviceMemory a((std::size_t)lgdhip_grid_bytes(&g)), b((std::size_t)lgdhip_grid_bytes(&g));
+    const double zero = 0.0;
+    CHECK_EQ(lgdhip_grid_fill_edge(a.get(), &g, &zero, nullptr), 0);
+    CHECK_EQ(lgdhip_grid_fill_edge(b.get(), &g, &zero, nullptr), 0);
+    int inB = 0;
+    CHECK_EQ(lgdhip_jacobi3d_f64_run((double *)a.get(), (double *)b.get(), &g, steps, nullptr, &inB, nullptr), 0);
+    CHECK_EQ(lgdhip_device_synchronize(), 0);
+    auto t0 = std::chrono::steady_clock::now();
+    CHECK_EQ(lgdhip_jacobi3d_f64_run((double *)a.get(), (double *)b.get(), &g, steps, nullptr, &inB, nullptr), 0);
+    CHECK_EQ(lgdhip_device_synchronize(), 0);
+    const double single = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (int slabs : {2, 4, 8}) {
+        HipSlabStepper<JacobiCell> stepper(initial, slabs, 4);
+        stepper.update(steps);  // warm-up
+        stepper.sync();
+        auto t1 = std::chrono::steady_clock::now();
+        stepper.update(steps);
+        stepper.sync();
+        const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        std::printf("  256^3 x %u steps: single grid %.1f GLUPS, %d slabs %.1f GLUPS (%llu planes exchanged)\n", steps,
+                    cells * steps / single * 1e-9, slabs, cells * steps / dt * 1e-9,
+                    stepper.planesExchanged() / 2);
+        CHECK(dt > 0);
+    }
 }
 
 LGD_TEST(slab_stepper_rejects_thin_slabs)
