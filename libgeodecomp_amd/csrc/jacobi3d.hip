@@ -434,7 +434,7 @@ __global__ __launch_bounds__(64 * TY) void jacobi3d_f64_tb2(const double *__rest
 // launch (a plane fetched once per tile, y neighbours through LDS, x by
 // DPP): 11 one row per thread, 12 two rows.  The default blocks 4 sweeps per
 // launch with variant 6 or 10, whichever the launch cost model rates cheaper
-// for the grid (6 at 128^3, 10 at 512^3, the faster ones there on MI355X);
+// for the grid (10 from 96^3, 6 at 64^3, the faster ones there on MI355X);
 // step counts not divisible by K finish with single sweeps of
 // kSingleSweepVariant.
 constexpr int kJacobiDefaultVariant = 6;
@@ -471,24 +471,29 @@ struct JacobiLaunch {
 };
 
 // Temporal blocking: tiles of (64-2K) x (TY*rows-2K) output columns; a z
-// chunk of C planes costs its work-group C full pipeline steps plus a 2K-step
-// ramp of about K - 1 steps' work.  One work-group fits a CU at a time, so a
-// launch takes about ceil(work-groups / CUs) rounds of C + K plane steps, a
-// step of a two-rows-per-thread kernel costing about 2.2 single-row steps
-// (two rows' arithmetic, half the LDS reads per row; 2.2 matches the MI355X
-// timings: equal at 128^3, the taller tile 15% faster at 512^3).  Returns
-// that cost for the best chunk (any C up to 64, the measured best at 512^3)
-// and sets *zchunk: at 128^3, variant 6 takes 5 chunks of 26 planes.
+// chunk of C planes costs its work-group C + 2K pipeline steps.  Two
+// one-row work-groups share a CU (64 KB of LDS each at K = 4, TY = 16), a
+// two-rows work-group (128 KB, 102 VGPRs) has it to itself.  A step costs
+// (MI355X, 7-point FP64, 128^3 and 512^3 chunk sweeps): one row 0.78 us
+// with the CU to itself and 1.17 us with two work-groups on it; two rows
+// 1.0 us alone and 1.17 us once the grid is larger than the chip (memory
+// traffic of every CU).  A launch takes ceil(work-groups / slots) such
+// rounds.  Returns that cost for the best chunk (any C up to 64) and sets
+// *zchunk: at 128^3 two rows in chunks of 10 planes (234 work-groups, one
+// round of 18 steps, against one row's 480 work-groups in chunks of 13 at
+// 21 shared steps); at 512^3 two rows in chunks of 64.
 double tb_cost(const lgdhip_grid &g, const TbShape &tb, int *zchunk) {
     const unsigned gx = lgdhip::ceil_div(g.nx, 64 - 2 * tb.k);
     const unsigned gy = lgdhip::ceil_div(g.ny, tb.ty * tb.rows - 2 * tb.k);
+    const uint64_t slots = (uint64_t)lgdhip::kCUs * (tb.rows == 2 ? 1 : 2);
     double best = 1e300;
     for (int64_t n = lgdhip::ceil_div(g.nz, 64); n <= g.nz; ++n) {
         const int c = (int)lgdhip::ceil_div(g.nz, n);
         if (lgdhip::ceil_div(g.nz, c) != n) continue;  // same chunk as a smaller n
         const uint64_t groups = (uint64_t)gx * gy * n;
-        const double cost = (double)((groups + lgdhip::kCUs - 1) / lgdhip::kCUs) * (c + tb.k) *
-                            (tb.rows == 2 ? 2.2 : 1.0);
+        const bool full = groups > (uint64_t)lgdhip::kCUs;
+        const double step = tb.rows == 2 ? (full ? 1.17 : 1.0) : (full ? 1.17 : 0.78);
+        const double cost = (double)((groups + slots - 1) / slots) * (c + 2 * tb.k) * step;
         if (cost < best) {
             best = cost;
             *zchunk = c;

@@ -130,8 +130,10 @@ def test_jacobi_zrange_rejects_bad_ranges_without_gpu():
 
 def test_jacobi_launch_plan_without_gpu():
     """lgdhip_jacobi3d_f64_plan (host-only): the default picks 4 sweeps per
-    launch with one row per thread on a 128^3 cube (5 chunks of 26 planes)
-    and two rows per thread on 512^3 (64-plane chunks); an explicit variant
+    launch with two rows per thread on a 128^3 cube (13 chunks of 10
+    planes: 234 work-groups, one per CU) and on 512^3 (64-plane chunks), one
+    row per thread on 64^3 (its 2 x 8 tiles need 4-plane chunks to give the
+    CUs 256 work-groups); an explicit variant
     and z chunk are kept; a view falls back to single sweeps and rejects
     temporal blocking, as lgdhip_jacobi3d_f64_run does."""
     lib = lgdhip.load()
@@ -141,7 +143,8 @@ def test_jacobi_launch_plan_without_gpu():
         rc = lib.lgdhip_jacobi3d_f64_plan(C.byref(g), C.byref(tile) if tile else None, C.byref(v), C.byref(z))
         return rc, v.value, z.value
 
-    assert plan(lgdhip.make_grid(3, 128, 128, 128, halo=1, elem_bytes=8)) == (0, 6, 26)
+    assert plan(lgdhip.make_grid(3, 128, 128, 128, halo=1, elem_bytes=8)) == (0, 10, 10)
+    assert plan(lgdhip.make_grid(3, 64, 64, 64, halo=1, elem_bytes=8)) == (0, 6, 4)
     assert plan(lgdhip.make_grid(3, 512, 512, 512, halo=1, elem_bytes=8)) == (0, 10, 64)
     g = lgdhip.make_grid(3, 96, 70, 40, halo=1, elem_bytes=8)
     assert plan(g, lgdhip.tile(variant=9, bz=7)) == (0, 9, 7)
