@@ -735,11 +735,45 @@ class UNet:
         ms = timed(self.stream, self.net.forward, reps=reps) / reps
         self.fwd_ms = round(ms, 4)
         achieved = self.fwd_flops / (ms * 1e-3) / 1e12
+        bound_us, mfma_us = self.layer_bound_us()
         return {"kernel": "unet3d_forward (15 convolutions: z-marching, LDS-tile and GEMM-tiled MFMA kernels)",
                 "bound": "mfma",
                 "achieved": round(achieved, 2), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 4), "traffic": None,
-                "avg_launch_us": round(ms * 1e3, 3), "algorithmic_flops_per_launch": self.fwd_flops}
+                "avg_launch_us": round(ms * 1e3, 3), "algorithmic_flops_per_launch": self.fwd_flops,
+                "layers": {"bound_us": round(bound_us, 2), "mfma_only_us": round(mfma_us, 2),
+                           "frac": round(bound_us / (ms * 1e3), 4),
+                           "meaning": "sum over the forward's 15 convolutions, 3 poolings and 3 upsamplings of "
+                                      "max(FLOPs / 2.5 PFLOP/s, bytes each must move at least once / 8 TB/s), "
+                                      "over the measured forward time: most 128^3 and 64^3 layers are "
+                                      "HBM-bound at these channel counts"}}
+
+    def layer_bound_us(self):
+        """The forward's per-layer roofline: each op costs at least the larger
+        of its FLOPs at the MFMA peak and its compulsory bytes (BF16 inputs
+        read once, outputs written once; the last layer writes FP32) at the
+        HBM peak.  Returns (that sum, the MFMA-only sum) in us."""
+        net = self.net
+        c0, c1, c2, c3 = net.ch
+        n = self.n
+        total = mfma = 0.0
+
+        def op(flops, nbytes):
+            nonlocal total, mfma
+            t_mfma = flops / (MFMA_BF16_PEAK_TFLOPS * 1e12) * 1e6
+            total += max(t_mfma, nbytes / (HBM_PEAK_GBS * 1e9) * 1e6)
+            mfma += t_mfma
+
+        for name in net.order:
+            l = net.layers[name]
+            taps = 27 if l.ksize == 3 else 1
+            out_bytes = 4 if name == "out" else 2
+            op(2.0 * taps * l.cin * l.cout * l.n ** 3, l.n ** 3 * (l.cin * 2 + l.cout * out_bytes))
+        for m, c in ((n, c0), (n // 2, c1), (n // 4, c2)):   # poolings
+            op(0.0, m ** 3 * c * 2 + (m // 2) ** 3 * c * 2)
+        for m, c in ((n // 8, c3), (n // 4, c2), (n // 2, c1)):   # upsamplings
+            op(0.0, m ** 3 * c * 2 + (2 * m) ** 3 * c * 2)
+        return total, mfma
 
     def cpu_baseline(self, budget_s=0.0):
         return None

@@ -490,7 +490,10 @@ int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int c
  * flipped taps, so dx = conv(dy, wt) with cin' = cout, cout' = cin */
 int lgdhip_conv3d_dgrad_weights(const void *w, int cin, int cout, int ksize, void *wt, lgdhip_stream_t stream);
 /* dw[co][k] (+)= sum_v dy[v][co] x[v+tap(k)][ci(k)] (FP32, fixed-order
- * reduction over voxel chunks), db[co] = sum_v dy[v][co] (if non-NULL) */
+ * reduction over voxel chunks of chunk_vox voxels, a multiple of 32; <= 0:
+ * the library's choice for the shape, the largest power of two <= 4096 and
+ * >= 128 that gives 2048 waves -- pass the same value to the workspace
+ * query), db[co] = sum_v dy[v][co] (if non-NULL) */
 size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, int chunk_vox);
 int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, int ksize, const void *dy, int ldy,
                              int offy, int cout, float *dw, float *db, int accumulate, int chunk_vox,

@@ -222,6 +222,12 @@ __device__ __forceinline__ uint4 load_global16(gh_ptr p) {
     const u32x4 v = *(gq_ptr)p;
     return make_uint4(v.x, v.y, v.z, v.w);
 }
+// one 4-byte word (two BF16) where only that word is used: a 16-byte load
+// of which the code reads one word would still write all four registers,
+// and the compiler, reusing the three it considers dead, then waits for
+// the load before going on
+typedef const unsigned int __attribute__((address_space(1))) *gw_ptr;
+__device__ __forceinline__ unsigned load_global4(gh_ptr p) { return *(gw_ptr)p; }
 
 // IN: how the input is staged -- kInVec, 16-byte loads (cin % 8 == 0);
 // kInMasked, 16-byte loads of a row whose channels past cin (< 8) are
@@ -884,14 +890,17 @@ FwdShape fwd_shape(int64_t nvox, int nt) {
 }
 
 // dW partial over a chunk of voxels: P[chunk][co][k] = sum_v dY[v][co] X[v+tap(k)][ci(k)]
-// one wave per (chunk, k tile of 16), all co tiles
+// one wave per (chunk, k tile of 16), all co tiles; work-groups in XCD
+// order (XCD c takes the contiguous range [c per, (c + 1) per): a chunk's
+// waves and its neighbours share one L2)
 template <int NT>
 __global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__ X, int n, int ldx, int offx, int cin,
                                                          int ksize, const u16 *__restrict__ dY, int ldy, int offy,
-                                                         int cout, int Kp, int chunk_vox, float *__restrict__ P) {
+                                                         int cout, int Kp, int chunk_vox, float *__restrict__ P,
+                                                         unsigned per) {
     const int lane = threadIdx.x & 63;
     const int ktiles = Kp / 16;
-    const int wave = (int)(blockIdx.x * 4 + threadIdx.x / 64);
+    const int wave = (int)(lgdhip::xcd_tile(blockIdx.x, per, true) * 4 + threadIdx.x / 64);
     const int nvox = n * n * n;
     const int chunks = (nvox + chunk_vox - 1) / chunk_vox;
     if (wave >= chunks * ktiles) return;
@@ -972,81 +981,97 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_mfma(const u16 *__restrict__
     }
 }
 
-// channels-last (ld, off, c) -> channels-first [c][nvox] (c, ld, off
-// multiples of 8): one thread per (voxel, group of 8 channels), a 16-byte
-// read and 8 writes that are contiguous across the wave
-__global__ __launch_bounds__(256) void to_channels_first(const u16 *__restrict__ src, int64_t nvox, int ld, int off,
-                                                         u16 *__restrict__ dst, int64_t ldc) {
-    const int64_t v = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int g = blockIdx.y;
-    if (v >= nvox) return;
+// channels-last (ld, off, C) -> voxel-blocked [nvox / 8][C][8] (C, ld, off
+// and nvox multiples of 8): element (v, c) at ((v / 8) C + c) 8 + v % 8, so
+// a channel's 8 consecutive voxels are 16 contiguous bytes and the 16
+// channels of an MFMA fragment at one 8-voxel group 256 contiguous bytes.
+// A work-group takes 256 voxels and 8 channels: coalesced 16-byte reads of
+// each voxel's 8 channels, a transpose through LDS, 16-byte writes of each
+// channel's 8 voxels (128 contiguous bytes per 8 lanes).
+__global__ __launch_bounds__(256) void to_voxel_blocked(const u16 *__restrict__ src, int64_t nvox, int ld, int off,
+                                                        int C, u16 *__restrict__ dst) {
+    __shared__ uint4 tile[256];  // [voxel of the block][8 channels]
+    const int64_t v0 = (int64_t)blockIdx.x * 256;
+    const int g = blockIdx.y;  // channels 8g .. 8g + 7
+    const int t = threadIdx.x;
+    if (v0 + t < nvox) tile[t] = *(const uint4 *)(src + (v0 + t) * ld + off + 8 * g);
+    __syncthreads();
+    const int vg = t >> 3, c = t & 7;  // voxel group of the block, channel
+    const int64_t vb = v0 + 8 * vg;
+    if (vb >= nvox) return;  // after the only barrier; nvox % 8 == 0
+    const u16 *tl = reinterpret_cast<const u16 *>(tile);
     Frag f;
-    f.q = *(const uint4 *)(src + v * ld + off + 8 * g);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) dst[(int64_t)(8 * g + e) * ldc + v] = f.h[e];
+    for (int e = 0; e < 8; ++e) f.h[e] = tl[(8 * vg + e) * 8 + c];
+    *reinterpret_cast<uint4 *>(dst + (((vb >> 3) * C + 8 * g + c) << 3)) = f.q;
 }
 
-// Row stride of the channels-first copies: nvox padded by 256 bytes.  With
-// an unpadded stride of a power of two (4 MiB at 128^3) the 16 channel rows
-// a fragment load touches map to the same L2 channel.
-constexpr int64_t kCfPad = 128;
-inline int64_t cf_stride(int64_t nvox) { return nvox + kCfPad; }
-
-// 8 BF16 of a channels-first row shifted by one voxel: dx = -1 gives
-// element e = c[e-1] with element 0 from the previous 8 (prev[7]); dx = +1
-// gives c[e+1] with element 7 from the next 8 (next[0]); little-endian
+// 8 BF16 of one channel's voxels shifted by one voxel: dx = -1 gives
+// element e = c[e-1] with element 0 from the previous 8 (their last word,
+// elements 6 and 7: prev_w); dx = +1 gives c[e+1] with element 7 from the
+// next 8 (their first word, elements 0 and 1: next_x); little-endian
 // halves, so each 32-bit word is an alignbit of two neighbours
-__device__ __forceinline__ uint4 shift_from_prev(uint4 c, uint4 prev) {
-    return make_uint4(__builtin_amdgcn_alignbit(c.x, prev.w, 16), __builtin_amdgcn_alignbit(c.y, c.x, 16),
+__device__ __forceinline__ uint4 shift_from_prev(uint4 c, unsigned prev_w) {
+    return make_uint4(__builtin_amdgcn_alignbit(c.x, prev_w, 16), __builtin_amdgcn_alignbit(c.y, c.x, 16),
                       __builtin_amdgcn_alignbit(c.z, c.y, 16), __builtin_amdgcn_alignbit(c.w, c.z, 16));
 }
-__device__ __forceinline__ uint4 shift_from_next(uint4 c, uint4 next) {
+__device__ __forceinline__ uint4 shift_from_next(uint4 c, unsigned next_x) {
     return make_uint4(__builtin_amdgcn_alignbit(c.y, c.x, 16), __builtin_amdgcn_alignbit(c.z, c.y, 16),
-                      __builtin_amdgcn_alignbit(c.w, c.z, 16), __builtin_amdgcn_alignbit(next.x, c.w, 16));
+                      __builtin_amdgcn_alignbit(c.w, c.z, 16), __builtin_amdgcn_alignbit(next_x, c.w, 16));
 }
 
-// dW partial from channels-first copies: P[chunk][co][tap*cin + ci] =
+// dW partial from voxel-blocked copies: P[chunk][co][tap*cin + ci] =
 // sum over the chunk's voxels v of G[co][v] X[ci][v + tap].  One wave per
-// (chunk, taps, group of CT co tiles, group of IT ci tiles): with DXT = 3 a
-// wave takes the three x-shifted taps of one (dz, dy) row, whose B
-// fragments all come from the same three aligned loads (previous, current,
-// next 8 voxels), and reuses each A fragment for all of them; with DXT = 1
-// one tap.  Per 32 voxels (4 groups of 8 consecutive voxels of one x row,
-// n % 8 == 0) every fragment is one 16-byte load, and the CT x IT x DXT
-// MFMAs share them.  Needs ksize 3, cin and cout multiples of 16, chunk_vox
-// a multiple of 32.
-template <int CT, int IT, int DXT>
+// (chunk, taps, group of CT co tiles, group of IT ci tiles).  A wave's
+// taps: DXT = 3 takes the three x-shifted taps of a (dz, dy) row, whose B
+// fragments come from the row's centre 8 voxels and one word of each
+// neighbouring 8; DYT = 3 (with DXT = 3) takes the three rows of one dz as
+// well, so 9 taps share the A fragment, the voxel stepping and the
+// validity tests (per MFMA about a third of the bookkeeping of one row);
+// DXT = 1: one tap.  Per 32 voxels (4 groups of 8 consecutive voxels of one
+// x row, n % 8 == 0) every fragment is one 16-byte load per lane, the 16
+// lanes of a voxel group reading 256 contiguous bytes of the voxel-blocked
+// copy.  Needs ksize 3, cin and cout multiples of 16, chunk_vox a multiple
+// of 32.  Work-groups (4 waves) run in XCD order: XCD c takes the
+// contiguous range [c per, (c + 1) per), so the waves of one chunk and of
+// the chunks above and beside it (whose rows its dz / dy taps read) share
+// one L2 instead of every XCD fetching every chunk's rows.
+template <int CT, int IT, int DXT, int DYT>
 __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ Xc, const u16 *__restrict__ Gc, int n,
                                                        int cin, int cout, int Kp, int chunk_vox, int cotg, int citg,
-                                                       float *__restrict__ P, int64_t ldc) {
-    constexpr int kGroups = 27 / DXT;  // (dz, dy) rows, or single taps
+                                                       float *__restrict__ P, unsigned per) {
+    static_assert(DYT == 1 || DXT == 3, "three y rows take the three x taps of each");
+    constexpr int kGroups = 27 / (DXT * DYT);  // dz planes, (dz, dy) rows or single taps
     const int lane = threadIdx.x & 63;
     const int64_t nvox = (int64_t)n * n * n;
     const int64_t chunks = (nvox + chunk_vox - 1) / chunk_vox;
     const int64_t tasks = (int64_t)kGroups * cotg * citg;
-    const int64_t wave = (int64_t)blockIdx.x * 4 + threadIdx.x / 64;
-    if (wave >= chunks * tasks) return;
+    const int64_t wave = (int64_t)lgdhip::xcd_tile(blockIdx.x, per, true) * 4 + threadIdx.x / 64;
+    if (wave >= chunks * tasks) return;  // no barriers in this kernel
     const int64_t chunk = wave / tasks;
     int64_t rem = wave % tasks;
     const int grp = (int)(rem % kGroups);  // taps fastest: neighbouring waves share the G fragments
     rem /= kGroups;
     const int cig = (int)(rem % citg), cog = (int)(rem / citg);
-    const int tap0 = grp * DXT;            // DXT = 3: taps (dz, dy, -1), (dz, dy, 0), (dz, dy, +1)
+    const int tap0 = grp * DXT * DYT;  // taps tap0 + 3 r + e: row dy + r, x offset e - 1 (DXT = 3)
     int dz, dy, dx0;
     tap_offset(tap0, 3, dz, dy, dx0);
     const int i = lane & 15, q = lane >> 4;
-    f32x4 acc[DXT][CT][IT];
+    f32x4 acc[DYT][DXT][CT][IT];
 #pragma unroll
-    for (int d = 0; d < DXT; ++d)
+    for (int r = 0; r < DYT; ++r)
 #pragma unroll
-        for (int t = 0; t < CT; ++t)
+        for (int d = 0; d < DXT; ++d)
 #pragma unroll
-            for (int u = 0; u < IT; ++u) acc[d][t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int t = 0; t < CT; ++t)
+#pragma unroll
+                for (int u = 0; u < IT; ++u) acc[r][d][t][u] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int64_t v_begin = chunk * chunk_vox;
     const int64_t v_end = v_begin + chunk_vox < nvox ? v_begin + chunk_vox : nvox;
-    // this lane group's voxel v = vb + 8q and its coordinates, stepped along
-    // with v (32 voxels per iteration) instead of divided out each time
+    // the fetch front's voxel v = vb + 8q of this lane group and its
+    // coordinates, stepped along with v (32 voxels per step) instead of
+    // divided out each time
     int x, y, z;
     {
         const int64_t v0 = v_begin + 8 * q;
@@ -1056,95 +1081,136 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_cf(const u16 *__restrict__ X
         z = (int)(r0 / n);
     }
     // One 32-voxel step's fragments: dY (A) and each X row's centre 8 with
-    // its left and right neighbours.  Loads outside the volume read a zero
-    // block instead of being skipped, so they need no branch, and the next
-    // step's loads are issued before this step's MFMAs (the last step
-    // re-loads its own), keeping a step of loads in flight behind the MFMAs.
+    // the adjacent words of its left and right neighbours.  Loads outside
+    // the volume read a zero block instead of being skipped, so they need
+    // no branch.  Addresses are stepped, not recomputed: the voxel-blocked
+    // copies put the front's 8-voxel group g at element (g C + channel) 8,
+    // a step is 4 groups, tap (dz, dy) reads group g + (dz n^2 + dy n) / 8
+    // (exact, n % 8 == 0) and the next row n cin elements further; the
+    // coordinates only decide which loads are inside the volume.
     const gh_ptr zero = (gh_ptr)(const void *)kZero16;
     struct Step {
-        uint4 a[CT], c[IT], prev[IT], next[IT];
+        uint4 a[CT], c[DYT][IT];
+        unsigned prev[DYT][IT], next[DYT][IT];  // the neighbouring 8 voxels' adjacent word
     };
-    auto fetch = [&](int64_t vb, int xs, int ys, int zs, Step &f) {
-        const int64_t v = vb + 8 * q;
+    const int64_t g0 = (v_begin + 8 * q) >> 3;
+    const int64_t tap_groups = ((int64_t)dz * n * n + (int64_t)dy * n) >> 3;
+    const int64_t row_elems = (int64_t)n * cin;
+    gh_ptr pa[CT], px[IT];
 #pragma unroll
-        for (int t = 0; t < CT; ++t) {
-            const int co = (cog * CT + t) * 16 + i;
-            f.a[t] = load_global16((gh_ptr)Gc + (int64_t)co * ldc + v);
-        }
-        const int yy = ys + dy, zz = zs + dz;
-        const bool rowok = (unsigned)yy < (unsigned)n && (unsigned)zz < (unsigned)n;
-        const int64_t base = ((int64_t)zz * n + yy) * n + xs;
-        const bool lo = rowok && (DXT == 3 || dx0 < 0) && xs > 0;
-        const bool hi = rowok && (DXT == 3 || dx0 > 0) && xs + 8 < n;
+    for (int t = 0; t < CT; ++t) pa[t] = (gh_ptr)Gc + ((g0 * cout + (cog * CT + t) * 16 + i) << 3);
 #pragma unroll
-        for (int u = 0; u < IT; ++u) {
-            const int ci = (cig * IT + u) * 16 + i;
-            const gh_ptr row = (gh_ptr)Xc + (int64_t)ci * ldc;
-            f.c[u] = load_global16(rowok ? row + base : zero);
-            f.prev[u] = load_global16(lo ? row + base - 8 : zero);
-            f.next[u] = load_global16(hi ? row + base + 8 : zero);
+    for (int u = 0; u < IT; ++u) px[u] = (gh_ptr)Xc + (((g0 + tap_groups) * cin + (cig * IT + u) * 16 + i) << 3);
+    const int64_t step_a = (int64_t)32 * cout, step_x = (int64_t)32 * cin;  // 4 groups of 8 voxels
+    auto fetch = [&](Step &f) {
+#pragma unroll
+        for (int t = 0; t < CT; ++t) f.a[t] = load_global16(pa[t]);
+        const bool zok = (unsigned)(z + dz) < (unsigned)n;
+        const bool xlo = (DXT == 3 || dx0 < 0) && x > 0;
+        const bool xhi = (DXT == 3 || dx0 > 0) && x + 8 < n;
+#pragma unroll
+        for (int r = 0; r < DYT; ++r) {
+            const bool rowok = zok && (unsigned)(y + dy + r) < (unsigned)n;
+            const bool lo = rowok && xlo, hi = rowok && xhi;
+#pragma unroll
+            for (int u = 0; u < IT; ++u) {
+                const gh_ptr pr = px[u] + r * row_elems;
+                f.c[r][u] = load_global16(rowok ? pr : zero);
+                f.prev[r][u] = load_global4(lo ? pr - 8 * cin + 6 : zero);
+                f.next[r][u] = load_global4(hi ? pr + 8 * cin : zero);
+            }
         }
     };
+    // 32 voxels on: at most 32 / n <= 4 row wraps (n >= 8) and one plane
+    // wrap (n^2 >= 64), as selects -- no data-dependent loop between a
+    // step's loads and the MFMAs that wait for them
     auto advance = [n](int &xa, int &ya, int &za) {
         xa += 32;
-        while (xa >= n) {  // once per row for n >= 32
-            xa -= n;
-            if (++ya == n) {
-                ya = 0;
-                ++za;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const bool w = xa >= n;
+            xa = w ? xa - n : xa;
+            ya = w ? ya + 1 : ya;
+        }
+        const bool wz = ya >= n;
+        ya = wz ? ya - n : ya;
+        za = wz ? za + 1 : za;
+    };
+    // Two step buffers, unrolled by two with fixed roles: step k + 1's
+    // loads go out before step k's MFMAs, and step k + 2's before step
+    // k + 1's, so a step's loads have a whole other step to arrive (a copy
+    // from a "next" buffer would wait for them at the end of every step).
+    // The fetch front stops at the chunk's last step, which later fetches
+    // re-load; an odd step count ends with one step of a zero A fragment
+    // (adding exact zeros), so the body has no branch.
+    const int S = (int)((v_end - v_begin) / 32);
+    int front = 0;  // step the next fetch loads
+    auto fetch_front = [&](Step &f) {
+        fetch(f);
+        const bool on = front + 1 < S;
+        int xa = x, ya = y, za = z;
+        advance(xa, ya, za);
+        front += on ? 1 : 0;
+        x = on ? xa : x;
+        y = on ? ya : y;
+        z = on ? za : z;
+#pragma unroll
+        for (int t = 0; t < CT; ++t) pa[t] += on ? step_a : 0;
+#pragma unroll
+        for (int u = 0; u < IT; ++u) px[u] += on ? step_x : 0;
+    };
+    auto compute = [&](const Step &cur, bool live) {
+        Frag a[CT];
+#pragma unroll
+        for (int t = 0; t < CT; ++t) a[t].q = live ? cur.a[t] : make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+        for (int r = 0; r < DYT; ++r) {
+            Frag b[DXT][IT];
+#pragma unroll
+            for (int u = 0; u < IT; ++u) {
+                if (DXT == 3) {
+                    b[0][u].q = shift_from_prev(cur.c[r][u], cur.prev[r][u]);
+                    b[1 % DXT][u].q = cur.c[r][u];
+                    b[2 % DXT][u].q = shift_from_next(cur.c[r][u], cur.next[r][u]);
+                } else {
+                    b[0][u].q = dx0 == 0 ? cur.c[r][u]
+                                         : (dx0 < 0 ? shift_from_prev(cur.c[r][u], cur.prev[r][u])
+                                                    : shift_from_next(cur.c[r][u], cur.next[r][u]));
+                }
             }
+#pragma unroll
+            for (int e = 0; e < DXT; ++e)
+#pragma unroll
+                for (int t = 0; t < CT; ++t)
+#pragma unroll
+                    for (int u = 0; u < IT; ++u)
+                        acc[r][e][t][u] =
+                            __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t].v, b[e][u].v, acc[r][e][t][u], 0, 0, 0);
         }
     };
-    Step cur, nxt;
-    fetch(v_begin, x, y, z, cur);
-    for (int64_t vb = v_begin; vb < v_end; vb += 32) {
-        int xn = x, yn = y, zn = z;
-        advance(xn, yn, zn);
-        if (vb + 32 < v_end) {
-            fetch(vb + 32, xn, yn, zn, nxt);
-        } else {
-            fetch(vb, x, y, z, nxt);
-        }
-        Frag a[CT], b[DXT][IT];
+    Step s0, s1;
+    fetch_front(s0);
+    for (int k = 0; k < S; k += 2) {
+        fetch_front(s1);  // step k + 1
+        compute(s0, true);
+        fetch_front(s0);  // step k + 2
+        compute(s1, k + 1 < S);
+    }
+    // D lane layout: row (co) 4*(lane/16) + e, column (ci) lane % 16
 #pragma unroll
-        for (int t = 0; t < CT; ++t) a[t].q = cur.a[t];
-#pragma unroll
-        for (int u = 0; u < IT; ++u) {
-            if (DXT == 3) {
-                b[0][u].q = shift_from_prev(cur.c[u], cur.prev[u]);
-                b[1 % DXT][u].q = cur.c[u];
-                b[2 % DXT][u].q = shift_from_next(cur.c[u], cur.next[u]);
-            } else {
-                b[0][u].q = dx0 == 0 ? cur.c[u]
-                                     : (dx0 < 0 ? shift_from_prev(cur.c[u], cur.prev[u])
-                                                : shift_from_next(cur.c[u], cur.next[u]));
-            }
-        }
+    for (int r = 0; r < DYT; ++r)
 #pragma unroll
         for (int d = 0; d < DXT; ++d)
 #pragma unroll
             for (int t = 0; t < CT; ++t)
 #pragma unroll
                 for (int u = 0; u < IT; ++u)
-                    acc[d][t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t].v, b[d][u].v, acc[d][t][u], 0, 0, 0);
-        cur = nxt;
-        x = xn;
-        y = yn;
-        z = zn;
-    }
-    // D lane layout: row (co) 4*(lane/16) + e, column (ci) lane % 16
 #pragma unroll
-    for (int d = 0; d < DXT; ++d)
-#pragma unroll
-        for (int t = 0; t < CT; ++t)
-#pragma unroll
-            for (int u = 0; u < IT; ++u)
-#pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    const int co = (cog * CT + t) * 16 + 4 * q + e;
-                    const int ci = (cig * IT + u) * 16 + i;
-                    P[((int64_t)chunk * cout + co) * Kp + (tap0 + d) * cin + ci] = acc[d][t][u][e];
-                }
+                    for (int e = 0; e < 4; ++e) {
+                        const int co = (cog * CT + t) * 16 + 4 * q + e;
+                        const int ci = (cig * IT + u) * 16 + i;
+                        P[((int64_t)chunk * cout + co) * Kp + (tap0 + 3 * r + d) * cin + ci] = acc[r][d][t][u][e];
+                    }
 }
 
 // dW[co][k] = sum over chunks of P (fixed order) for k < K; the padding
@@ -1178,17 +1244,17 @@ __global__ __launch_bounds__(256) void conv3d_wgrad_reduce(const float *__restri
     dW[o] = accumulate ? dW[o] + r : r;
 }
 
-// db from the channels-first gradient: partial sums of each row over
-// chunks of 8192 voxels (coalesced), then a fixed-order sum of the chunks
+// db from the voxel-blocked gradient (C channels): partial sums of each
+// channel over chunks of 8192 voxels, then a fixed-order sum of the chunks
 constexpr int kBiasChunk = 8192;
 __global__ __launch_bounds__(256) void row_sum_partial(const u16 *__restrict__ Gc, int64_t nvox, float *__restrict__ Pb,
-                                                       int bchunks, int64_t ldc) {
+                                                       int bchunks, int C) {
     __shared__ float red[256];
     const int co = blockIdx.x, c = blockIdx.y;
     const int64_t v0 = (int64_t)c * kBiasChunk;
     const int64_t v1 = v0 + kBiasChunk < nvox ? v0 + kBiasChunk : nvox;
     float s = 0.0f;
-    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) s += bf2f(Gc[(int64_t)co * ldc + v]);
+    for (int64_t v = v0 + threadIdx.x; v < v1; v += 256) s += bf2f(Gc[((((v >> 3) * C) + co) << 3) + (v & 7)]);
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -1480,9 +1546,43 @@ bool vec8(const void *a, const void *b, int c, int lda, int offa, int ldb, int o
            ((uintptr_t)a & 15) == 0 && ((uintptr_t)b & 15) == 0;
 }
 
-// shapes the channels-first weight gradient handles
+// shapes the voxel-blocked weight gradient (conv3d_wgrad_cf) handles
 bool wgrad_cf_shape(int n, int cin, int ksize, int cout) {
     return ksize == 3 && cin % 16 == 0 && cout % 16 == 0 && n % 8 == 0;
+}
+
+// conv3d_wgrad_cf's tiling: CT x IT tiles of 16 co x 16 ci per wave, the
+// three x taps of a row at once (DXT = 3) while the accumulators stay
+// within 48 registers, and for a single tile the three rows of a dz too
+// (DYT = 3, 36 accumulator registers)
+struct CfTiling {
+    int CT, IT, DXT, DYT, cotg, citg;
+    int tasks() const { return 27 / (DXT * DYT) * cotg * citg; }  // waves per chunk
+};
+CfTiling cf_tiling(int cin, int cout) {
+    const int cot = cout / 16, cit = cin / 16;
+    CfTiling t;
+    t.CT = cot % 4 == 0 ? 4 : (cot % 2 == 0 ? 2 : 1);
+    t.IT = cit % 4 == 0 ? 4 : (cit % 2 == 0 ? 2 : 1);
+    t.cotg = cot / t.CT;
+    t.citg = cit / t.IT;
+    t.DXT = t.CT * t.IT <= 4 ? 3 : 1;
+    t.DYT = t.CT * t.IT == 1 ? 3 : 1;  // one co and one ci tile: all three rows of a dz
+    return t;
+}
+
+// Voxels per chunk when the caller leaves it to the library (chunk_vox <=
+// 0): the largest power of two <= 4096 (and >= 128) that still gives the
+// launch 2048 waves (8 per CU), since each wave marches its chunk 32 voxels
+// per dependent step.  4096 at 128^3 (16 channels: 4608 waves of 128
+// steps); the 32^3 and 16^3 levels would otherwise run 54-162 work-groups.
+int wgrad_chunk(int n, int cin, int ksize, int cout) {
+    const int64_t nvox = (int64_t)n * n * n;
+    const int64_t tasks = wgrad_cf_shape(n, cin, ksize, cout) ? cf_tiling(cin, cout).tasks()
+                                                               : lgdhip_conv3d_kpad(cin, ksize) / 16;
+    int c = 4096;
+    while (c > 128 && (nvox + c - 1) / c * tasks < 2048) c /= 2;
+    return c;
 }
 
 int check_tensor(int n, int ld, int off, int c) {
@@ -1641,13 +1741,14 @@ int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int c
 }
 
 size_t lgdhip_conv3d_wgrad_workspace_bytes(int n, int cin, int ksize, int cout, int chunk_vox) {
-    if (chunk_vox <= 0) chunk_vox = 4096;
+    if (n < 1 || cin < 1 || cout < 1 || (ksize != 1 && ksize != 3)) return 0;
+    if (chunk_vox <= 0) chunk_vox = wgrad_chunk(n, cin, ksize, cout);
     const int64_t nvox = (int64_t)n * n * n;
     const int64_t chunks = (nvox + chunk_vox - 1) / chunk_vox;
-    // [P: chunk partials][bias partials][x and dy channels-first copies]
+    // [P: chunk partials][bias partials][x and dy voxel-blocked copies]
     size_t bytes = (size_t)(chunks * cout * lgdhip_conv3d_kpad(cin, ksize) * 4);
     bytes += (size_t)cout * (size_t)((nvox + kBiasChunk - 1) / kBiasChunk) * 4;
-    if (wgrad_cf_shape(n, cin, ksize, cout)) bytes += 256 + (size_t)(cin + cout) * (size_t)cf_stride(nvox) * 2;
+    if (wgrad_cf_shape(n, cin, ksize, cout)) bytes += 256 + (size_t)(cin + cout) * (size_t)nvox * 2;
     return bytes;
 }
 
@@ -1661,7 +1762,7 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
     if (rc) return rc;
     rc = check_tensor(n, ldy, offy, cout);
     if (rc) return rc;
-    if (chunk_vox <= 0) chunk_vox = 4096;
+    if (chunk_vox <= 0) chunk_vox = wgrad_chunk(n, cin, ksize, cout);
     LGD_REQUIRE(chunk_vox % 32 == 0, "chunk_vox must be a multiple of 32");
     if (workspace_bytes < lgdhip_conv3d_wgrad_workspace_bytes(n, cin, ksize, cout, chunk_vox))
         return lgdhip::set_error(LGDHIP_ERR_WORKSPACE, "wgrad workspace too small");
@@ -1674,38 +1775,33 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
     const bool cf = wgrad_cf_shape(n, cin, ksize, cout) && ldx % 8 == 0 && offx % 8 == 0 && ldy % 8 == 0 &&
                     offy % 8 == 0 && (uintptr_t)x % 16 == 0 && (uintptr_t)dy % 16 == 0;
     if (cf) {
-        // channels-first copies: every MFMA fragment becomes one 16-byte load
+        // voxel-blocked copies: every MFMA fragment becomes one 16-byte load per lane
         const int bchunks0 = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
         float *Pb = P + (size_t)chunks * cout * Kp;
         char *base = (char *)(Pb + (size_t)cout * bchunks0);
         base += (256 - (uintptr_t)base % 256) % 256;
         u16 *Xc = (u16 *)base;
-        const int64_t ldc = cf_stride(nvox);
-        u16 *Gc = Xc + (size_t)cin * ldc;
-        to_channels_first<<<dim3(blocks_for(nvox), cin / 8), 256, 0, s>>>((const u16 *)x, nvox, ldx, offx, Xc, ldc);
-        to_channels_first<<<dim3(blocks_for(nvox), cout / 8), 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, Gc, ldc);
-        LGD_LAUNCHED("to_channels_first");
-        const int cot = cout / 16, cit = cin / 16;
-        const int CT = cot % 4 == 0 ? 4 : (cot % 2 == 0 ? 2 : 1);
-        const int IT = cit % 4 == 0 ? 4 : (cit % 2 == 0 ? 2 : 1);
-        const int cotg = cot / CT, citg = cit / IT;
-        // small tile groups take the three x taps of a row at once (the
-        // accumulators stay within 48 registers)
-        const int DXT = CT * IT <= 4 ? 3 : 1;
-        const int64_t waves = (int64_t)chunks * (27 / DXT) * cotg * citg;
-        const unsigned grid = (unsigned)((waves + 3) / 4);
-#define LGD_CF_CASE(A, B, D)                                                                                 \
-    if (CT == A && IT == B && DXT == D)                                                                      \
-        conv3d_wgrad_cf<A, B, D><<<grid, 256, 0, s>>>(Xc, Gc, n, cin, cout, Kp, chunk_vox, cotg, citg, P, ldc);
-        LGD_CF_CASE(1, 1, 3)
-        LGD_CF_CASE(1, 2, 3)
-        LGD_CF_CASE(1, 4, 3)
-        LGD_CF_CASE(2, 1, 3)
-        LGD_CF_CASE(2, 2, 3)
-        LGD_CF_CASE(2, 4, 1)
-        LGD_CF_CASE(4, 1, 3)
-        LGD_CF_CASE(4, 2, 1)
-        LGD_CF_CASE(4, 4, 1)
+        u16 *Gc = Xc + (size_t)cin * nvox;
+        to_voxel_blocked<<<dim3(blocks_for(nvox), cin / 8), 256, 0, s>>>((const u16 *)x, nvox, ldx, offx, cin, Xc);
+        to_voxel_blocked<<<dim3(blocks_for(nvox), cout / 8), 256, 0, s>>>((const u16 *)dy, nvox, ldy, offy, cout, Gc);
+        LGD_LAUNCHED("to_voxel_blocked");
+        const CfTiling tl = cf_tiling(cin, cout);
+        const int CT = tl.CT, IT = tl.IT, DXT = tl.DXT, DYT = tl.DYT, cotg = tl.cotg, citg = tl.citg;
+        const int64_t waves = (int64_t)chunks * tl.tasks();
+        const unsigned wgs = (unsigned)((waves + 3) / 4);
+        const unsigned per = lgdhip::xcd_per(wgs), grid = lgdhip::xcd_groups(wgs);
+#define LGD_CF_CASE(A, B, D, R)                                                                              \
+    if (CT == A && IT == B && DXT == D && DYT == R)                                                          \
+        conv3d_wgrad_cf<A, B, D, R><<<grid, 256, 0, s>>>(Xc, Gc, n, cin, cout, Kp, chunk_vox, cotg, citg, P, per);
+        LGD_CF_CASE(1, 1, 3, 3)
+        LGD_CF_CASE(1, 2, 3, 1)
+        LGD_CF_CASE(1, 4, 3, 1)
+        LGD_CF_CASE(2, 1, 3, 1)
+        LGD_CF_CASE(2, 2, 3, 1)
+        LGD_CF_CASE(2, 4, 1, 1)
+        LGD_CF_CASE(4, 1, 3, 1)
+        LGD_CF_CASE(4, 2, 1, 1)
+        LGD_CF_CASE(4, 4, 1, 1)
 #undef LGD_CF_CASE
         LGD_LAUNCHED("conv3d_wgrad_cf");
         conv3d_wgrad_reduce<<<(unsigned)((cout * Kp + kWgradReduceOutputs - 1) / kWgradReduceOutputs),
@@ -1714,19 +1810,20 @@ int lgdhip_conv3d_bf16_wgrad(const void *x, int n, int ldx, int offx, int cin, i
         LGD_LAUNCHED("conv3d_wgrad_reduce");
         if (db) {
             const int bchunks = (int)((nvox + kBiasChunk - 1) / kBiasChunk);
-            row_sum_partial<<<dim3(cout, bchunks), 256, 0, s>>>(Gc, nvox, Pb, bchunks, ldc);
+            row_sum_partial<<<dim3(cout, bchunks), 256, 0, s>>>(Gc, nvox, Pb, bchunks, cout);
             row_sum_final<<<blocks_for(cout), 256, 0, s>>>(Pb, bchunks, cout, db, accumulate);
             LGD_LAUNCHED("row_sum");
         }
         return 0;
     }
     const int64_t waves = (int64_t)chunks * (Kp / 16);
-    const unsigned grid = (unsigned)((waves + 3) / 4);
+    const unsigned wgs = (unsigned)((waves + 3) / 4);
+    const unsigned per = lgdhip::xcd_per(wgs), grid = lgdhip::xcd_groups(wgs);
     const int nt = (cout + 15) / 16;
 #define LGD_WG_CASE(T)                                                                                       \
     case T:                                                                                                  \
         conv3d_wgrad_mfma<T><<<grid, 256, 0, s>>>((const u16 *)x, n, ldx, offx, cin, ksize, (const u16 *)dy, \
-                                                  ldy, offy, cout, Kp, chunk_vox, P);                        \
+                                                  ldy, offy, cout, Kp, chunk_vox, P, per);                   \
         break;
     switch (nt) {
         LGD_WG_CASE(1)

@@ -161,7 +161,7 @@ class UNet3D:
         self.gpred = Tensor(s[0], 1)
         ws = 0
         for l in L:
-            ws = max(ws, self.lib.lgdhip_conv3d_wgrad_workspace_bytes(l.n, l.cin, l.ksize, l.cout, 4096),
+            ws = max(ws, self.lib.lgdhip_conv3d_wgrad_workspace_bytes(l.n, l.cin, l.ksize, l.cout, 0),
                      # the forward convolution's and the input gradient's (cin and cout swapped) split partials
                      self.lib.lgdhip_conv3d_forward_workspace_bytes(l.n, l.cin, l.ksize, l.cout),
                      self.lib.lgdhip_conv3d_forward_workspace_bytes(l.n, l.cout, l.ksize, l.cin))
@@ -265,7 +265,7 @@ class UNet3D:
             lgdhip.call("lgdhip_relu_backward_bf16", g_out.ptr, g_out.ld, goff, y_act.ptr, y_act.ld, yoff, l.cout,
                         l.n ** 3, S)
         lgdhip.call("lgdhip_conv3d_bf16_wgrad", x.ptr, l.n, x.ld, xoff, l.cin, l.ksize, g_out.ptr, g_out.ld, goff,
-                    l.cout, l.gw.ptr, l.gb.ptr if self.use_bias else None, accumulate_w, 4096, self.ws.ptr,
+                    l.cout, l.gw.ptr, l.gb.ptr if self.use_bias else None, accumulate_w, 0, self.ws.ptr,
                     self.ws_bytes, S)
         if g_in is not None:
             lgdhip.call("lgdhip_conv3d_bf16_forward_ws", g_out.ptr, l.n, g_out.ld, goff, l.cout, l.ksize,

@@ -5,6 +5,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from libgeodecomp_amd import lgdhip
 from libgeodecomp_amd.unet import bf16_round, conv_weights_layout, from_bf16_bits, to_bf16_bits
 
 
@@ -306,16 +307,40 @@ def test_conv_dgrad_vs_oracle(hip, oracle, n, cin, cout):
                                               (8, 16, 32, 3), (8, 32, 32, 3), (8, 16, 64, 3)])
 def test_conv_wgrad_vs_oracle(hip, oracle, n, cin, cout, ksize):
     """Element-gather kernel (odd channel counts, 1x1x1, n % 8 != 0) and the
-    channels-first kernel (3x3x3, channels multiples of 16, n % 8 == 0, every
+    voxel-blocked kernel (3x3x3, channels multiples of 16, n % 8 == 0, every
     tile grouping CT x IT in {1, 2, 4}^2 reached by some case)."""
-    print("[case] conv wgrad", n, cin, cout, ksize, flush=True)
+    run_wgrad_case(hip, oracle, n, cin, cout, ksize, 256)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n,cin,cout,ksize", [(12, 2, 16, 3), (16, 32, 64, 3), (8, 192, 128, 3), (24, 16, 48, 3)])
+def test_conv_wgrad_default_chunk(hip, oracle, n, cin, cout, ksize):
+    """chunk_vox = 0: the library's chunk for the shape (128-voxel chunks
+    here), in the workspace query and the call alike."""
+    run_wgrad_case(hip, oracle, n, cin, cout, ksize, 0)
+
+
+def test_conv_wgrad_default_chunk_sizes():
+    """The library's chunk (chunk_vox = 0) for the U-Net's levels, read off
+    the workspace size: 4096 voxels while that gives 2048 waves (128^3, 48
+    -> 16 channels: 512 chunks x 9 waves), smaller elsewhere (128^3 16 ->
+    16, one wave per dz: 2048; 32^3 64 -> 64: 256; 16^3 128 -> 128: 128),
+    never below 128."""
+    lib = lgdhip.load()
+    for n, cin, cout, chunk in [(128, 16, 16, 2048), (128, 48, 16, 4096), (64, 32, 32, 1024), (32, 64, 64, 256),
+                                (16, 128, 128, 128), (128, 3, 16, 4096), (8, 16, 16, 128)]:
+        assert (lib.lgdhip_conv3d_wgrad_workspace_bytes(n, cin, 3, cout, 0) ==
+                lib.lgdhip_conv3d_wgrad_workspace_bytes(n, cin, 3, cout, chunk)), (n, cin, cout, chunk)
+
+
+def run_wgrad_case(hip, oracle, n, cin, cout, ksize, chunk):
+    print("[case] conv wgrad", n, cin, cout, ksize, "chunk", chunk, flush=True)
     rng = np.random.default_rng(7 * cin + cout)
     taps = 27 if ksize == 3 else 1
     x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
     g = bf16_round(rng.standard_normal((n, n, n, cout)).astype(np.float32))
     lib = hip.load()
     kp = lib.lgdhip_conv3d_kpad(cin, ksize)
-    chunk = 256
     wsb = lib.lgdhip_conv3d_wgrad_workspace_bytes(n, cin, ksize, cout, chunk)
     ws = hip.DeviceBuffer(wsb)
     xb, gb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(g))
