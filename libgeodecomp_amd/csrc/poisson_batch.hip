@@ -564,7 +564,8 @@ void append_tiles(std::vector<Tile> *out, int p, int n) {
 }
 
 // temporal blocking shapes: variant -> (K, TY); tiles write (64-2K) x (TY-2K)
-// columns and march z chunks of kTbZ planes
+// columns and march z chunks of kTbZ planes (fewer on launches of few
+// tiles, tb_zchunk)
 struct TbShape {
     int k, ty, rows;  // sweeps per launch, work-group rows, tile rows per work-group row
 };
@@ -593,16 +594,38 @@ TbShape tb_shape(int variant) {
     }
 }
 
-int64_t tb_tiles_of(int n, TbShape sh) {
+int64_t tb_tiles_of(int n, TbShape sh, int zchunk = kTbZ) {
     const int ox = 64 - 2 * sh.k, oy = sh.ty * sh.rows - 2 * sh.k;
-    return (int64_t)((n + ox - 1) / ox) * ((n + oy - 1) / oy) * ((n + kTbZ - 1) / kTbZ);
+    return (int64_t)((n + ox - 1) / ox) * ((n + oy - 1) / oy) * ((n + zchunk - 1) / zchunk);
 }
 
-void append_tb_tiles(std::vector<Tile> *out, int p, int n, TbShape sh) {
+void append_tb_tiles(std::vector<Tile> *out, int p, int n, TbShape sh, int zchunk) {
     const int ox = 64 - 2 * sh.k, oy = sh.ty * sh.rows - 2 * sh.k;
-    for (int z = 0; z < n; z += kTbZ)
+    for (int z = 0; z < n; z += zchunk)
         for (int y = 0; y < n; y += oy)
             for (int x = 0; x < n; x += ox) out->push_back(Tile{p, x, y, z});
+}
+
+// z planes per blocked work-group for a launch over `set`: kTbZ, halved
+// (down to 4) while the launch has fewer than kTbMinGroups tiles.  A tile
+// is a z march of chunk + 2K plane steps, so a launch of small problems
+// (16 coarse grids of 31^3: 64 tiles of 39 steps at kTbZ) takes as long as
+// its longest march; 4-plane chunks give 512 tiles of 12 steps.  Blocked
+// results do not depend on the chunk.  Halving stops once the count
+// reaches kTbMinGroups, so a reduced list holds fewer than
+// kTbSmallListTiles tiles (twice the count before the last halving plus
+// one rounding tile per (problem, xy tile), itself under kTbMinGroups).
+constexpr int64_t kTbMinGroups = 512;  // two one-row work-groups per CU
+constexpr int64_t kTbSmallListTiles = 3 * kTbMinGroups;
+int tb_zchunk(const lgdhip_poisson_problem *probs, const std::vector<int> &set, TbShape sh) {
+    auto tiles = [&](int zc) {
+        int64_t t = 0;
+        for (int k : set) t += tb_tiles_of(probs[k].n, sh, zc);
+        return t;
+    };
+    int zc = kTbZ;
+    while (zc > 4 && tiles(zc) < kTbMinGroups) zc /= 2;
+    return zc;
 }
 
 // the blocked sweep shape of a launch over the problems in `set`: the
@@ -678,7 +701,8 @@ int64_t total_tiles(const lgdhip_poisson_problem *probs, int count) {
     return t;
 }
 
-// tile capacity for every temporal-blocking shape
+// tile capacity for every temporal-blocking shape: kTbZ chunks, or a
+// list of reduced chunks (under kTbSmallListTiles, tb_zchunk)
 int64_t total_tb_tiles(const lgdhip_poisson_problem *probs, int count) {
     int64_t t = 0;
     for (int k = 0; k < count; ++k) {
@@ -686,22 +710,24 @@ int64_t total_tb_tiles(const lgdhip_poisson_problem *probs, int count) {
         for (int v = 2; v <= kSweepVariants; ++v) t_max = std::max(t_max, tb_tiles_of(probs[k].n, tb_shape(v)));
         t += t_max;
     }
-    return t;
+    return std::max(t, kTbSmallListTiles);
 }
 
 Workspace carve_for(void *base, const lgdhip_poisson_problem *probs, int count) {
     return carve(base, pool_extent(probs, count), total_tiles(probs, count), total_tb_tiles(probs, count), count);
 }
 
-// Uploads the temporal-blocking tile list of the problems in `set`;
+// Uploads the temporal-blocking tile list of the problems in `set` (z
+// chunks of tb_zchunk planes, returned in *zchunk for the launches);
 // tb_begin/tb_end index the tiles of set[k].  Returns the tile count.
 int upload_tb_set(const Workspace &w, const lgdhip_poisson_problem *probs, const std::vector<int> &set, TbShape sh,
-                  int64_t *ntiles, hipStream_t s) {
+                  int64_t *ntiles, int *zchunk, hipStream_t s) {
     std::vector<Tile> tiles;
     std::vector<int> b(set.size()), e(set.size());
+    *zchunk = tb_zchunk(probs, set, sh);
     for (size_t k = 0; k < set.size(); ++k) {
         b[k] = (int)tiles.size();
-        append_tb_tiles(&tiles, set[k], probs[set[k]].n, sh);
+        append_tb_tiles(&tiles, set[k], probs[set[k]].n, sh, *zchunk);
         e[k] = (int)tiles.size();
     }
     *ntiles = (int64_t)tiles.size();
@@ -715,34 +741,35 @@ int upload_tb_set(const Workspace &w, const lgdhip_poisson_problem *probs, const
 
 template <bool DAMPED>
 void launch_tb_kind(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
-                    double *partial, float omega, hipStream_t s) {
+                    int zchunk, double *partial, float omega, hipStream_t s) {
     const unsigned nt = (unsigned)ntiles, per = lgdhip::xcd_per(nt), groups = lgdhip::xcd_groups(nt);
     if (sh.rows == 3)
         poisson_sweep_tb2<4, 16, 3, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial,
-                                                                            kTbZ, nt, per, omega);
+                                                                            zchunk, nt, per, omega);
     else if (sh.rows == 2 && sh.ty == 16)
         poisson_sweep_tb2<4, 16, 2, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial,
-                                                                            kTbZ, nt, per, omega);
+                                                                            zchunk, nt, per, omega);
     else if (sh.rows == 2)
-        poisson_sweep_tb2<4, 8, 2, DAMPED><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ,
+        poisson_sweep_tb2<4, 8, 2, DAMPED><<<groups, dim3(64, 8), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, zchunk,
                                                                           nt, per, omega);
     else if (sh.k == 2)
-        poisson_sweep_tb<2, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+        poisson_sweep_tb<2, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, zchunk, nt,
                                                                         per, omega);
     else
-        poisson_sweep_tb<4, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, kTbZ, nt,
+        poisson_sweep_tb<4, 16, DAMPED><<<groups, dim3(64, 16), 0, s>>>(a, b, r, w.tb_tiles, w.probs, partial, zchunk, nt,
                                                                         per, omega);
 }
 
-// one launch of K blocked sweeps a -> b over the uploaded tile list; omega
-// > 0 runs weighted sweeps e' = e + omega (J(e) - e)
+// one launch of K blocked sweeps a -> b over the uploaded tile list (z
+// chunks of `zchunk` planes, as uploaded); omega > 0 runs weighted sweeps
+// e' = e + omega (J(e) - e)
 void launch_tb(TbShape sh, const float *a, float *b, const float *r, const Workspace &w, int64_t ntiles,
-               double *partial, hipStream_t s, float omega = 0.0f) {
+               int zchunk, double *partial, hipStream_t s, float omega = 0.0f) {
     if (ntiles <= 0) return;
     if (omega > 0.0f)
-        launch_tb_kind<true>(sh, a, b, r, w, ntiles, partial, omega, s);
+        launch_tb_kind<true>(sh, a, b, r, w, ntiles, zchunk, partial, omega, s);
     else
-        launch_tb_kind<false>(sh, a, b, r, w, ntiles, partial, 0.0f, s);
+        launch_tb_kind<false>(sh, a, b, r, w, ntiles, zchunk, partial, 0.0f, s);
 }
 
 // Uploads problem descriptors and the tile list of the problems in `set`;
@@ -800,10 +827,11 @@ int run_sweeps_in(float *ea, float *eb, const float *r, const lgdhip_poisson_pro
     int t = 0;
     if (sh.k > 0 && sweeps >= sh.k) {
         int64_t ntb = 0;
-        rc = upload_tb_set(w, probs, all, sh, &ntb, s);
+        int zc = kTbZ;
+        rc = upload_tb_set(w, probs, all, sh, &ntb, &zc, s);
         if (rc) return rc;
         for (; t + sh.k <= sweeps; t += sh.k) {
-            launch_tb(sh, cur, nxt, r, w, ntb, nullptr, s, omega);
+            launch_tb(sh, cur, nxt, r, w, ntb, zc, nullptr, s, omega);
             LGD_LAUNCHED("poisson_sweep_tb");
             std::swap(cur, nxt);
         }
@@ -964,7 +992,7 @@ int lgdhip_poisson_sweeps_f32_noalloc(float *ea, float *eb, const float *r, cons
     const TbShape sh = sweep_shape(tile ? tile->variant : 0, probs, all);
     int64_t ntb = 0;
     if (sh.k > 0)
-        for (int k = 0; k < count; ++k) ntb += tb_tiles_of(probs[k].n, sh);
+        for (int k = 0; k < count; ++k) ntb += tb_tiles_of(probs[k].n, sh, tb_zchunk(probs, all, sh));
     const size_t need = carve_lists(nullptr, total_tiles(probs, count), ntb, count).bytes;
     if (need > kNoAllocScratchBytes)
         return lgdhip::set_error(LGDHIP_ERR_WORKSPACE,
@@ -1120,11 +1148,12 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             int t = 0;
             if (ssh.k > 0 && smooth >= 2 * ssh.k) {
                 int64_t ntb = 0;
-                rc = upload_tb_set(w, probs, active, ssh, &ntb, s);
+                int zc = kTbZ;
+                rc = upload_tb_set(w, probs, active, ssh, &ntb, &zc, s);
                 if (rc) return rc;
                 for (; t + 2 * ssh.k <= smooth; t += 2 * ssh.k) {
                     for (int half = 0; half < 2; ++half) {
-                        launch_tb(ssh, ecur, enxt, w.r, w, ntb, nullptr, s, opt->smooth_omega);
+                        launch_tb(ssh, ecur, enxt, w.r, w, ntb, zc, nullptr, s, opt->smooth_omega);
                         LGD_LAUNCHED("poisson_sweep_tb");
                         std::swap(ecur, enxt);
                     }
@@ -1160,9 +1189,10 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
         rc = upload_set(w, probs, active, &tiles, s);
         if (rc) return rc;
         int64_t ntb = 0;
+        int zc = kTbZ;
         sh = sweep_shape(opt->sweep_variant, probs, active);
         if (sh.k > 0) {
-            rc = upload_tb_set(w, probs, active, sh, &ntb, s);
+            rc = upload_tb_set(w, probs, active, sh, &ntb, &zc, s);
             if (rc) return rc;
         }
         graph_valid = false;
@@ -1175,7 +1205,7 @@ int lgdhip_poisson_batch_solve(double *u, const float *f, const lgdhip_poisson_p
             auto enqueue = [&](float *a, float *b) -> int {
                 if (sh.k > 0) {
                     for (int t = 0; t < C; t += sh.k) {
-                        launch_tb(sh, a, b, w.r, w, ntb, t + sh.k == C ? w.partial : nullptr, s);
+                        launch_tb(sh, a, b, w.r, w, ntb, zc, t + sh.k == C ? w.partial : nullptr, s);
                         std::swap(a, b);
                     }
                     poisson_reduce<<<(unsigned)active.size(), 256, 0, s>>>(w.partial, w.tb_begin, w.tb_end,
