@@ -818,6 +818,9 @@ __global__ __launch_bounds__(256) void conv1x1_to1(const u16 *__restrict__ X, in
 // the LDS-staged kernel takes 3x3x3 layers with up to 32 output channels on
 // volumes of at least 16^3
 constexpr int kLdsMinN = 16;
+// z-marching in 16-channel slices for wider outputs from this edge on (96^3:
+// 432 work-groups per slice launch; 64^3 would leave half the CUs idle)
+constexpr int kZmSliceMinN = 96;
 
 // The GEMM-tiled kernel's launch: 64 x 64 blocks, stages of 64 k, and the
 // number of slices the stage range is split into when a workspace for the
@@ -1645,8 +1648,14 @@ int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int c
     }
     // z-marching kernel: 16 output channels written as plain BF16 into an
     // aligned slice, input of at most 6 groups of 8 channels (or a masked
-    // row of fewer than 8)
-    if (ksize == 3 && cout == 16 && n >= kLdsMinN && y && !y32 && !accumulate && (ldy & 7) == 0 &&
+    // row of fewer than 8).  Wider outputs on volumes of at least
+    // kZmSliceMinN^3 run one launch per 16-channel slice (weights rows 16 j
+    // .. 16 j + 15, output channels offy + 16 j ..): each launch still has
+    // hundreds of work-groups, each input plane is staged once per slice,
+    // and the slices together beat one GEMM-tiled launch (the input
+    // gradient of a 16 -> 48 layer at 128^3: 412 us there)
+    const bool zm_width = cout == 16 || (cout % 16 == 0 && cout <= 16 * kMaxNTiles && n >= kZmSliceMinN);
+    if (ksize == 3 && zm_width && n >= kLdsMinN && y && !y32 && !accumulate && (ldy & 7) == 0 &&
         (offy & 7) == 0 && ((uintptr_t)y & 15) == 0) {
         const bool masked = cin < 8 && (ldx & 7) == 0 && (offx & 7) == 0 && offx + 8 <= ldx &&
                             ((uintptr_t)x & 15) == 0;
@@ -1654,14 +1663,18 @@ int lgdhip_conv3d_bf16_forward_ws(const void *x, int n, int ldx, int offx, int c
         if (groups >= 1 && groups <= 6) {
             const int bx = (n + 15) / 16, by = (n + 7) / 8, bz = (n + kZmChunk - 1) / kZmChunk;
             const unsigned blocks = (unsigned)((int64_t)bx * by * bz);
+            for (int j = 0; j < cout / 16; ++j) {
+                const u16 *Wj = Wb + (size_t)16 * j * Kp;  // Kp % 32 == 0: 16-byte aligned
+                const float *bj = bias ? bias + 16 * j : nullptr;
+                const int offj = offy + 16 * j;
 #define LGD_ZM_CASE(G, I)                                                                                    \
     if (groups == G && (I == kInMasked) == masked)                                                           \
-        conv3d_fwd_zm<G, I><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wb, Kp, bias, cout, Y, ldy, offy, relu, \
-                                                   bx, by);
-            LGD_ZM_CASE(1, kInMasked) LGD_ZM_CASE(1, kInVec) LGD_ZM_CASE(2, kInVec) LGD_ZM_CASE(3, kInVec)
-            LGD_ZM_CASE(4, kInVec) LGD_ZM_CASE(5, kInVec) LGD_ZM_CASE(6, kInVec)
+        conv3d_fwd_zm<G, I><<<blocks, 256, 0, s>>>(X, n, ldx, offx, cin, Wj, Kp, bj, 16, Y, ldy, offj, relu, bx, by);
+                LGD_ZM_CASE(1, kInMasked) LGD_ZM_CASE(1, kInVec) LGD_ZM_CASE(2, kInVec) LGD_ZM_CASE(3, kInVec)
+                LGD_ZM_CASE(4, kInVec) LGD_ZM_CASE(5, kInVec) LGD_ZM_CASE(6, kInVec)
 #undef LGD_ZM_CASE
-            LGD_LAUNCHED("conv3d_fwd_zm");
+                LGD_LAUNCHED("conv3d_fwd_zm");
+            }
             return 0;
         }
     }

@@ -184,6 +184,52 @@ def test_conv_bf16_output_alone_equals_with_fp32_copy(hip, oracle, n, cin, cout,
     assert np.all(np.abs(got - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-3)
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("cout,ld,off", [(32, 32, 0), (48, 64, 8)])
+def test_conv_zm_slices(hip, oracle, cout, ld, off):
+    """Wider outputs on volumes of at least 96^3 run the z-marching kernel
+    once per 16-channel slice (the BF16-only output).  32 channels: the same
+    k order as the LDS tile kernel that writes the FP32 copy, so both outputs
+    are byte-equal; 48 channels (the FP32 copy from the GEMM-tiled kernel)
+    into a 64-channel row at channel 8: the other channels untouched.  Both
+    checked against the FP64 oracle on two corner cubes (the oracle's
+    convolution of the corner 10^3 of the input equals the full one on the
+    inner 9^3 next to the volume's faces)."""
+    n, cin = 96, 16
+    rng = np.random.default_rng(cout * 7 + ld)
+    x = bf16_round(rng.standard_normal((n, n, n, cin)).astype(np.float32))
+    w = bf16_round((rng.standard_normal((cout, 27, cin)) / 12).astype(np.float32))
+    b = rng.standard_normal(cout).astype(np.float32)
+    lib = hip.load()
+    kp = lib.lgdhip_conv3d_kpad(cin, 3)
+    xb, wb, bb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(conv_weights_layout(w, kp))), dev(hip, b)
+    init = to_bf16_bits(rng.standard_normal((n, n, n, ld)).astype(np.float32))
+    outs = []
+    for with32 in (False, True):
+        y = dev(hip, init)
+        y32 = hip.DeviceBuffer(n ** 3 * cout * 4) if with32 else None
+        print("[case] conv z-march slices", n, cin, cout, ld, off, with32, flush=True)
+        hip.call("lgdhip_conv3d_bf16_forward", xb.ptr, n, cin, 0, cin, 3, wb.ptr, bb.ptr, cout, y.ptr, ld, off,
+                 y32.ptr if with32 else None, cout, 1, 0, None)
+        host = np.empty(n ** 3 * ld, np.uint16)
+        y.download(host)
+        outs.append(host.reshape(n, n, n, ld))
+    if cout == 32:
+        assert np.array_equal(outs[0], outs[1])
+    other = np.ones(ld, bool)
+    other[off:off + cout] = False
+    for o in outs:
+        assert np.array_equal(o[..., other], init.reshape(n, n, n, ld)[..., other])
+    for lo in (0, n - 10):
+        c = x[lo:lo + 10, lo:lo + 10, lo:lo + 10]
+        ref = np.maximum(oracle.conv3d_forward(c.astype(np.float64), w.astype(np.float64), b.astype(np.float64), 3), 0)
+        keep = slice(0, 9) if lo == 0 else slice(1, 10)
+        ref = ref[keep, keep, keep]
+        for o in outs:
+            got = from_bf16_bits(o[lo:lo + 10, lo:lo + 10, lo:lo + 10, off:off + cout])[keep, keep, keep]
+            assert np.all(np.abs(got.astype(np.float64) - ref) <= 2.0 ** -8 * np.abs(ref) + 1e-3)
+
+
 def run_conv_split(hip, x, w, bias, ld_out=None, off_out=0, relu=False, y_init=None):
     """run_conv through lgdhip_conv3d_bf16_forward_ws with the workspace the
     layer asks for; returns (y32, y_bf16, workspace bytes)."""
