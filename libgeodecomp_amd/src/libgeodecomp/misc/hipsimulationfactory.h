@@ -3,8 +3,9 @@
 // parameters are the tuning space of the cell's kernel family (the
 // reference's three block dimensions become what each family can actually
 // vary: work-group shape, temporal-blocking depth, kernel variant):
-//   Jacobi3DF64   Variant in the order 1 2 3 11 12 7 4 8 5 6 9 10: single
-//                 sweeps (plane marches, then the tile pipeline at K = 1
+//   Jacobi3DF64   Variant in the order 1 2 3 13 11 12 7 4 8 5 6 9 10: single
+//                 sweeps (plane marches, one and two cells per thread, then
+//                 the tile pipeline at K = 1
 //                 with one and two rows per thread), temporal blocking of
 //                 growing depth (K = 2, 2, 3, 3, 4), then K = 4 with two rows
 //                 per thread (16-, 32-row tiles);
@@ -58,7 +59,7 @@ class TuningSpace<Jacobi3DF64<CELL, VALUE, MEMBER> >
 public:
     static void add(SimulationParameters *p)
     {
-        p->addParameter("Variant", std::vector<int>{1, 2, 3, 11, 12, 7, 4, 8, 5, 6, 9, 10});
+        p->addParameter("Variant", std::vector<int>{1, 2, 3, 13, 11, 12, 7, 4, 8, 5, 6, 9, 10});
         p->addParameter("ZChunk", std::vector<int>{0, 8, 16, 26, 32, 48, 64});
     }
     static LaunchChoice choose(const SimulationParameters& p)

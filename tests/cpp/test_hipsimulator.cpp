@@ -601,7 +601,7 @@ LGD_TEST(hip_simulation_factory_every_jacobi_variant)
     SerialSimulator<JacobiCell> ref(proxy->clone());
     ref.run();
     SimulationParameters p = factory.parameters();
-    for (int k = 0; k < 12; ++k) {  // every index of the Variant set
+    for (int k = 0; k < 13; ++k) {  // every index of the Variant set
         p["Variant"].setValue(k);
         const double fit = factory(p);
         CHECK(fit < 0 && fit > -60);

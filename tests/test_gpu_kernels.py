@@ -36,7 +36,7 @@ def gpu_jacobi(hip, host, steps, edge=0.0, tile=None):
     # one launch swaps the buffers once: K sweeps per launch for the
     # temporal-blocking variants (and the default, which blocks 4 sweeps),
     # so the result's buffer follows the launches
-    if tile is not None and 1 <= tile.variant <= 3:
+    if tile is not None and (1 <= tile.variant <= 3 or tile.variant == 13):
         assert in_b.value == steps % 2
     assert in_b.value in (0, 1)
     return (b if in_b.value else a).store()
@@ -61,12 +61,26 @@ def test_jacobi_bit_exact_vs_oracle(hip, oracle, shape, steps):
 def test_jacobi_tiles_agree(hip, oracle):
     a = synthetic.jacobi_initial(96, 70, 40, seed=9)
     exp = oracle.jacobi3d(a, 6, edge=0.5, threads=4)
-    for variant in (1, 2, 3):
+    for variant in (1, 2, 3, 13):
         for by in (1, 2, 4, 8):
             for bz in (1, 3, 9, 64):
                 say("jacobi tile", variant, by, bz)
                 t = hip.tile(by=by, bz=bz, variant=variant)
                 assert np.array_equal(gpu_jacobi(hip, a, 6, edge=0.5, tile=t), exp)
+
+
+@pytest.mark.parametrize("shape", [(97, 70, 40), (1, 5, 3), (129, 3, 17), (128, 9, 6)])
+def test_jacobi_pair_march_bit_exact(hip, oracle, shape):
+    """Variant 13 (two cells per thread, 16-byte loads, the x neighbours as
+    unaligned pairs): odd widths leave a last pair whose second cell is not
+    stored; a one-cell row; widths at and past one 128-cell segment."""
+    nx, ny, nz = shape
+    a = synthetic.jacobi_initial(nx, ny, nz, seed=nx + ny)
+    exp = oracle.jacobi3d(a, 5, edge=0.25, threads=4)
+    for bz in (2, 64):
+        say("jacobi pair", shape, bz)
+        t = hip.tile(by=4, bz=bz, variant=13)
+        assert np.array_equal(gpu_jacobi(hip, a, 5, edge=0.25, tile=t), exp)
 
 
 @pytest.mark.parametrize("shape,steps", [((96, 70, 40), 7), ((130, 17, 9), 9), ((1, 1, 1), 5), ((61, 1, 30), 4),
