@@ -511,6 +511,12 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
                           lgdhip_stream_t stream);
 int lgdhip_upsample2_backward_bf16(const void *dy, int m, int ldy, int offy, void *dx, int ldx, int offx, int c,
                                    int accumulate, lgdhip_stream_t stream);
+/* nearest 2x upsample of x (m^3, cu channels) and a skip tensor ((2m)^3, cs
+ * channels) written as whole concatenation rows y[v][offy .. offy+cu+cs)
+ * (the upsampled channels first); no reference counterpart: the U-Net of
+ * the learned initial guess */
+int lgdhip_upsample2_concat_bf16(const void *x, int m, int ldx, int offx, int cu, const void *skip, int lds,
+                                 int offs, int cs, void *y, int ldy, int offy, lgdhip_stream_t stream);
 /* FP32 field -> one BF16 channel of a channels-last tensor */
 int lgdhip_pack_bf16(const float *src, int64_t nvox, void *y, int ldy, int offy, lgdhip_stream_t stream);
 /* MSE: grad[v*ldg] = 2 (pred - target) / nvox (BF16), partial[b] = per-block

@@ -1465,6 +1465,50 @@ __global__ void upsample2_v8(const u16 *__restrict__ X, int m, int ldx, int offx
     st8(Y + v * ldy + offy + ch, ld8(X + p * ldx + offx + ch));
 }
 
+// Nearest 2x upsampling of x (m^3, cu channels) written together with a
+// skip tensor (n^3 = (2m)^3, cs channels) as whole concatenation rows:
+//   y[v][offy + c] = c < cu ? x[parent(v)][offx + c] : s[v][offs + c - cu]
+// so every 128-byte line of y is written by one launch in full (the
+// encoder writing its channels into the rows and the upsampling writing the
+// others leave each line partly written twice, and a partial line costs
+// the memory a read-modify-write).  8 channels per thread; consecutive
+// threads write consecutive 16-byte pieces of a row.
+__global__ void upsample2_concat_v8(const u16 *__restrict__ X, int m, int ldx, int offx, int cu8,
+                                    const u16 *__restrict__ S, int lds, int offs, u16 *Y, int ldy, int offy, int c8) {
+    const int n = 2 * m;
+    int x, ch;
+    if (!row_thread(n, c8, x, ch)) return;
+    const int y = blockIdx.y, z = blockIdx.z;
+    const int64_t v = ((int64_t)z * n + y) * n + x;
+    Bf8 val;
+    if (ch < 8 * cu8) {
+        const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+        val = ld8(X + p * ldx + offx + ch);
+    } else {
+        val = ld8(S + v * lds + offs + ch - 8 * cu8);
+    }
+    st8(Y + v * ldy + offy + ch, val);
+}
+
+// the same, one element per thread (any channel counts, strides, offsets)
+__global__ void upsample2_concat(const u16 *__restrict__ X, int m, int ldx, int offx, int cu,
+                                 const u16 *__restrict__ S, int lds, int offs, int cs, u16 *Y, int ldy, int offy) {
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = cu + cs, n = 2 * m;
+    if (idx >= (int64_t)n * n * n * c) return;
+    const int ch = (int)(idx % c);
+    const int64_t v = idx / c;
+    const int x = (int)(v % n), y = (int)((v / n) % n), z = (int)(v / ((int64_t)n * n));
+    u16 val;
+    if (ch < cu) {
+        const int64_t p = ((int64_t)(z / 2) * m + y / 2) * m + x / 2;
+        val = X[p * ldx + offx + ch];
+    } else {
+        val = S[v * lds + offs + ch - cu];
+    }
+    Y[v * ldy + offy + ch] = val;
+}
+
 __global__ void upsample2_backward_v8(const u16 *__restrict__ dY, int m, int ldy, int offy, u16 *dX, int ldx, int offx,
                                       int c8, int accumulate) {
     const int n = 2 * m;
@@ -1932,6 +1976,22 @@ int lgdhip_upsample2_bf16(const void *x, int m, int ldx, int offx, void *y, int 
     upsample2<<<blocks_for(n * n * n * c), 256, 0, lgdhip::as_stream(stream)>>>((const u16 *)x, m, ldx, offx, (u16 *)y,
                                                                                 ldy, offy, c);
     LGD_LAUNCHED("upsample2");
+    return 0;
+}
+
+int lgdhip_upsample2_concat_bf16(const void *x, int m, int ldx, int offx, int cu, const void *skip, int lds,
+                                 int offs, int cs, void *y, int ldy, int offy, lgdhip_stream_t stream) {
+    LGD_REQUIRE(x && skip && y && m >= 1 && cu > 0 && cs > 0, "bad arguments");
+    const int64_t n = 2 * (int64_t)m;
+    if (vec8(x, y, cu, ldx, offx, ldy, offy) && vec8(skip, y, cs, lds, offs, ldy, offy) && n <= kMaxRowGrid) {
+        upsample2_concat_v8<<<row_grid((int)n, (cu + cs) / 8), 256, 0, lgdhip::as_stream(stream)>>>(
+            (const u16 *)x, m, ldx, offx, cu / 8, (const u16 *)skip, lds, offs, (u16 *)y, ldy, offy, (cu + cs) / 8);
+        LGD_LAUNCHED("upsample2_concat_v8");
+        return 0;
+    }
+    upsample2_concat<<<blocks_for(n * n * n * (cu + cs)), 256, 0, lgdhip::as_stream(stream)>>>(
+        (const u16 *)x, m, ldx, offx, cu, (const u16 *)skip, lds, offs, cs, (u16 *)y, ldy, offy);
+    LGD_LAUNCHED("upsample2_concat");
     return 0;
 }
 

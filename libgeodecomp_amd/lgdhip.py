@@ -180,6 +180,7 @@ SIGNATURES = {
     "lgdhip_avgpool2_backward_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P]),
     "lgdhip_upsample2_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _P]),
     "lgdhip_upsample2_backward_bf16": (_I, [_P, _I, _I, _I, _P, _I, _I, _I, _I, _P]),
+    "lgdhip_upsample2_concat_bf16": (_I, [_P, _I, _I, _I, _I, _P, _I, _I, _I, _P, _I, _I, _P]),
     "lgdhip_pack_bf16": (_I, [_P, _I64, _P, _I, _I, _P]),
     "lgdhip_mse_partials": (_I64, [_I64]),
     "lgdhip_mse_loss": (_I, [_P, _P, _I64, _P, _I, _P, _P]),

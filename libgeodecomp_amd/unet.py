@@ -149,10 +149,13 @@ class UNet3D:
         self.t = 0
         # activations (forward) and their gradients
         self.x = Tensor(s[0], 8)                     # input: channels 0..in_ch-1, padded to 8
-        self.a0, self.cat0 = Tensor(s[0], c0), Tensor(s[0], c1 + c0)   # cat0 = [up(d1b) | e0b]
+        self.a0, self.cat0 = Tensor(s[0], c0), Tensor(s[0], c1 + c0)   # cat0 = [up(d1b) | e0b (= s0)]
         self.p1, self.a1, self.cat1 = Tensor(s[1], c0), Tensor(s[1], c1), Tensor(s[1], c2 + c1)
         self.p2, self.a2, self.cat2 = Tensor(s[2], c1), Tensor(s[2], c2), Tensor(s[2], c3 + c2)
         self.p3, self.a3, self.b3 = Tensor(s[3], c2), Tensor(s[3], c3), Tensor(s[3], c3)
+        # the encoder's skip outputs as dense tensors: pooled from, and copied
+        # into the concatenations by the upsampling as whole rows
+        self.s0, self.s1, self.s2 = Tensor(s[0], c0), Tensor(s[1], c1), Tensor(s[2], c2)
         self.a2d, self.b2d = Tensor(s[2], c2), Tensor(s[2], c2)
         self.a1d, self.b1d = Tensor(s[1], c1), Tensor(s[1], c1)
         self.a0d, self.b0d = Tensor(s[0], c0), Tensor(s[0], c0)
@@ -221,23 +224,26 @@ class UNet3D:
         c0, c1, c2, c3 = self.ch
         S = self.stream
         self._conv("e0a", self.x, 0, self.a0, 0)
-        self._conv("e0b", self.a0, 0, self.cat0, c1)
-        lgdhip.call("lgdhip_avgpool2_bf16", self.cat0.ptr, self.n, self.cat0.ld, c1, self.p1.ptr, c0, 0, c0, S)
+        self._conv("e0b", self.a0, 0, self.s0, 0)
+        lgdhip.call("lgdhip_avgpool2_bf16", self.s0.ptr, self.n, c0, 0, self.p1.ptr, c0, 0, c0, S)
         self._conv("e1a", self.p1, 0, self.a1, 0)
-        self._conv("e1b", self.a1, 0, self.cat1, c2)
-        lgdhip.call("lgdhip_avgpool2_bf16", self.cat1.ptr, self.n // 2, self.cat1.ld, c2, self.p2.ptr, c1, 0, c1, S)
+        self._conv("e1b", self.a1, 0, self.s1, 0)
+        lgdhip.call("lgdhip_avgpool2_bf16", self.s1.ptr, self.n // 2, c1, 0, self.p2.ptr, c1, 0, c1, S)
         self._conv("e2a", self.p2, 0, self.a2, 0)
-        self._conv("e2b", self.a2, 0, self.cat2, c3)
-        lgdhip.call("lgdhip_avgpool2_bf16", self.cat2.ptr, self.n // 4, self.cat2.ld, c3, self.p3.ptr, c2, 0, c2, S)
+        self._conv("e2b", self.a2, 0, self.s2, 0)
+        lgdhip.call("lgdhip_avgpool2_bf16", self.s2.ptr, self.n // 4, c2, 0, self.p3.ptr, c2, 0, c2, S)
         self._conv("e3a", self.p3, 0, self.a3, 0)
         self._conv("e3b", self.a3, 0, self.b3, 0)
-        lgdhip.call("lgdhip_upsample2_bf16", self.b3.ptr, self.n // 8, c3, 0, self.cat2.ptr, self.cat2.ld, 0, c3, S)
+        lgdhip.call("lgdhip_upsample2_concat_bf16", self.b3.ptr, self.n // 8, c3, 0, c3, self.s2.ptr, c2, 0, c2,
+                    self.cat2.ptr, self.cat2.ld, 0, S)
         self._conv("d2a", self.cat2, 0, self.a2d, 0)
         self._conv("d2b", self.a2d, 0, self.b2d, 0)
-        lgdhip.call("lgdhip_upsample2_bf16", self.b2d.ptr, self.n // 4, c2, 0, self.cat1.ptr, self.cat1.ld, 0, c2, S)
+        lgdhip.call("lgdhip_upsample2_concat_bf16", self.b2d.ptr, self.n // 4, c2, 0, c2, self.s1.ptr, c1, 0, c1,
+                    self.cat1.ptr, self.cat1.ld, 0, S)
         self._conv("d1a", self.cat1, 0, self.a1d, 0)
         self._conv("d1b", self.a1d, 0, self.b1d, 0)
-        lgdhip.call("lgdhip_upsample2_bf16", self.b1d.ptr, self.n // 2, c1, 0, self.cat0.ptr, self.cat0.ld, 0, c1, S)
+        lgdhip.call("lgdhip_upsample2_concat_bf16", self.b1d.ptr, self.n // 2, c1, 0, c1, self.s0.ptr, c0, 0, c0,
+                    self.cat0.ptr, self.cat0.ld, 0, S)
         self._conv("d0a", self.cat0, 0, self.a0d, 0)
         self._conv("d0b", self.a0d, 0, self.b0d, 0)
         self._conv("out", self.b0d, 0, None, 0, y32=self.pred.ptr)
@@ -296,15 +302,15 @@ class UNet3D:
         self._back("e3a", self.p3, 0, G["a3"], 0, self.a3, 0, G["p3"], 0, acc)
         lgdhip.call("lgdhip_avgpool2_backward_bf16", G["p3"].ptr, self.n // 4, c2, 0, G["cat2"].ptr,
                     G["cat2"].ld, c3, c2, 1, S)
-        self._back("e2b", self.a2, 0, G["cat2"], c3, self.cat2, c3, G["a2"], 0, acc)
+        self._back("e2b", self.a2, 0, G["cat2"], c3, self.s2, 0, G["a2"], 0, acc)
         self._back("e2a", self.p2, 0, G["a2"], 0, self.a2, 0, G["p2"], 0, acc)
         lgdhip.call("lgdhip_avgpool2_backward_bf16", G["p2"].ptr, self.n // 2, c1, 0, G["cat1"].ptr,
                     G["cat1"].ld, c2, c1, 1, S)
-        self._back("e1b", self.a1, 0, G["cat1"], c2, self.cat1, c2, G["a1"], 0, acc)
+        self._back("e1b", self.a1, 0, G["cat1"], c2, self.s1, 0, G["a1"], 0, acc)
         self._back("e1a", self.p1, 0, G["a1"], 0, self.a1, 0, G["p1"], 0, acc)
         lgdhip.call("lgdhip_avgpool2_backward_bf16", G["p1"].ptr, self.n, c0, 0, G["cat0"].ptr, G["cat0"].ld, c1,
                     c0, 1, S)
-        self._back("e0b", self.a0, 0, G["cat0"], c1, self.cat0, c1, G["a0"], 0, acc)
+        self._back("e0b", self.a0, 0, G["cat0"], c1, self.s0, 0, G["a0"], 0, acc)
         self._back("e0a", self.x, 0, G["a0"], 0, self.a0, 0, None, 0, acc)
 
     def adam(self, lr: float = 1e-3, b1: float = 0.9, b2: float = 0.999, eps: float = 1e-8):

@@ -501,6 +501,35 @@ def test_pool_upsample_forward_backward(hip, c, pad, off):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("cu,cs,ldx,offx,lds,offs,ldy,offy", [(32, 16, 32, 0, 16, 0, 48, 0),
+                                                             (16, 8, 24, 8, 16, 8, 40, 8),
+                                                             (4, 3, 5, 1, 3, 0, 9, 2)])
+def test_upsample_concat(hip, cu, cs, ldx, offx, lds, offs, ldy, offy):
+    """Upsampled channels and a skip tensor written as whole concatenation
+    rows: y[v][offy:offy+cu] = x[parent(v)], y[v][offy+cu:offy+cu+cs] =
+    skip[v], byte for byte, the row's other channels untouched (the
+    8-channel path, and the per-element path for counts and offsets that are
+    not multiples of 8; odd m)."""
+    rng = np.random.default_rng(cu * 7 + cs + ldy)
+    for m in (4, 3):
+        n = 2 * m
+        x = bf16_round(rng.standard_normal((m, m, m, ldx)).astype(np.float32))
+        sk = bf16_round(rng.standard_normal((n, n, n, lds)).astype(np.float32))
+        y0 = bf16_round(rng.standard_normal((n, n, n, ldy)).astype(np.float32))
+        xb, sb, yb = dev(hip, to_bf16_bits(x)), dev(hip, to_bf16_bits(sk)), dev(hip, to_bf16_bits(y0))
+        print(f"[case] upsample concat m {m} cu {cu} cs {cs} ldy {ldy} offy {offy}", flush=True)
+        hip.call("lgdhip_upsample2_concat_bf16", xb.ptr, m, ldx, offx, cu, sb.ptr, lds, offs, cs, yb.ptr, ldy, offy,
+                 None)
+        out = np.empty(y0.size, np.uint16)
+        yb.download(out)
+        exp = y0.copy()
+        up = np.repeat(np.repeat(np.repeat(x[..., offx:offx + cu], 2, 0), 2, 1), 2, 2)
+        exp[..., offy:offy + cu] = up
+        exp[..., offy + cu:offy + cu + cs] = sk[..., offs:offs + cs]
+        assert np.array_equal(from_bf16_bits(out).reshape(y0.shape), exp)
+
+
+@pytest.mark.gpu
 def test_relu_mse_adam(hip):
     rng = np.random.default_rng(4)
     n, c = 8, 16
