@@ -587,13 +587,30 @@ class Learned:
     dtype = "bf16"   # the network (BF16 operands, FP32 accumulation); polishing is FP32/FP64
     bytes_per_lup = 12.0
 
-    def __init__(self, n=128, steps=30000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, lr=5e-4, **_):
+    def __init__(self, n=128, steps=30000, seed=synthetic.DEFAULT_SEED, train=512, eval=16, lr=5e-4, weights="",
+                 **_):
         from libgeodecomp_amd import learned_guess as lgm
         self.lgm = lgm
         self.n, self.train_steps, self.seed, self.ntrain, self.neval = n, steps, seed, train, eval
         self.lr = lr
-        self.m = lgm.default_coarse_size(n)
         self.stream = lgdhip.Stream()
+        self.weights = weights
+        self.result = None
+        if weights:
+            # a network this library trained earlier (learned_guess --save):
+            # no generation, no training; the timed step is the same
+            self.model, meta = lgm.LearnedGuess.load(weights, stream=self.stream.h)
+            self.weights_meta = meta
+            self.n = self.model.n
+            self.m = meta.get("coarse_m", lgm.default_coarse_size(self.n))
+            self.ntrain, self.train_steps = meta.get("train_problems"), meta.get("train_steps")
+            self.lr = meta.get("peak_lr")
+            self.generate_s, self.train_s = meta.get("generate_s"), meta.get("train_s")
+            self.log = [(self.train_steps, meta.get("final_loss"))]
+            self.eval = lgm.Ensemble(synthetic.poisson_ensemble(eval, sizes=(self.n,), seed=seed + 7919),
+                                     stream=self.stream.h)
+            return
+        self.m = lgm.default_coarse_size(n)
         t0 = time.perf_counter()
         data = lgm.Ensemble(synthetic.poisson_ensemble(train, sizes=(n,), seed=seed), stream=self.stream.h)
         res = data.solve(1e-6)
@@ -610,20 +627,23 @@ class Learned:
         self.train_s = time.perf_counter() - t0
         del base, data
         self.eval = lgm.Ensemble(synthetic.poisson_ensemble(eval, sizes=(n,), seed=seed + 7919), stream=self.stream.h)
-        self.result = None
 
     def config(self):
         r = self.result or {}
-        return {"workload": f"learned initial guess vs zero guess, {self.neval} held-out Poisson problems {self.n}^3 "
+        c = {"workload": f"learned initial guess vs zero guess, {self.neval} held-out Poisson problems {self.n}^3 "
                             f"(Gaussian sources, Dirichlet faces), Jacobi polishing to 1e-6 relative FP64 residual",
                 "network": "3D U-Net 16-32-64-128, 3x3x3 BF16 convs on MFMA, FP32 accumulation, bias-free",
                 "coarse_grid": self.m, "train_problems": self.ntrain, "train_steps": self.train_steps,
                 "peak_lr": self.lr,
-                "generate_s": round(self.generate_s, 1), "train_s": round(self.train_s, 1),
+                "generate_s": self.generate_s and round(self.generate_s, 1),
+                "train_s": self.train_s and round(self.train_s, 1),
                 "final_loss": self.log[-1][1] if self.log else None, "arms": r.get("arms"),
                 "sweep_ratio": r.get("sweep_ratio"), "fp64_polish_sweeps": r.get("fp64_polish_sweeps"),
                 "network_gain_over_coarse_base": self.network_gain(), "seed": self.seed,
                 "parallelism": "single GPU"}
+        if self.weights:
+            c["weights"] = {"file": os.path.relpath(self.weights, REPO), "trained_by": self.weights_meta.get("command")}
+        return c
 
     def network_gain(self):
         """The network's own share: wall time of the coarse-base guess alone
@@ -839,6 +859,7 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6, help="poisson: relative residual to solve to")
     ap.add_argument("--wrap", type=int, default=0,
                     help="jacobi (bits 1, 2, 4) / life (bits 1, 2): periodic axes (0: the Cube)")
+    ap.add_argument("--weights", default="", help="learned: evaluate a saved network instead of training one")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true", help="default run: the main workload only")
     args = ap.parse_args()
@@ -876,6 +897,8 @@ def main():
             kw.update(layout=lgdhip.LAYOUT_Z_SLABS if args.layout == "zslabs" else lgdhip.LAYOUT_MEMBER_PLANES)
     if args.workload == "poisson":
         kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol, variant=max(0, args.variant))
+    if args.workload == "learned" and args.weights:
+        kw.update(weights=args.weights)
     wl = WORKLOADS[args.workload](**kw)
     main_entry = measure(wl, args.steps, args.warmup, not args.no_cpu_baseline, 10.0)
     secondary = []
@@ -918,6 +941,12 @@ def main():
 # HBM-bound shape), D3Q19 LBM FP32 256^3 x 2000
 SECONDARY = [("jacobi", dict(n=128, steps=100)), ("jacobi", dict(n=512, steps=100)),
              ("lbm", dict(n=256, steps=2000)), ("unet", dict(n=128))]
+# the learned initial guess against the zero guess on 16 held-out 128^3
+# problems, with the network this library trained and saved (training takes
+# minutes; the command that made the file is in the entry's config)
+LEARNED_WEIGHTS = REPO / "libgeodecomp_amd" / "weights" / "unet_guess_128.npz"
+if LEARNED_WEIGHTS.exists():
+    SECONDARY.append(("learned", dict(weights=str(LEARNED_WEIGHTS))))
 
 
 def measure(wl, steps, warmup, cpu, cpu_budget):

@@ -34,12 +34,13 @@ import argparse
 import ctypes as C
 import json
 import math
+import sys
 import time
 
 import numpy as np
 
 from . import lgdhip, synthetic
-from .unet import UNet3D
+from .unet import UNet3D, from_bf16_bits
 
 
 def _sync():
@@ -223,6 +224,36 @@ class LearnedGuess:
             self.net.adam(lr=lr_t)
         return log
 
+    def save(self, path, meta: dict | None = None):
+        """The network's BF16 weights (the copies the forward pass reads) and
+        its shape in an .npz file; `meta` (JSON-serialisable) is kept beside
+        them (how the weights were trained)."""
+        arrays = {}
+        for name in self.net.order:
+            l = self.net.layers[name]
+            wb = np.empty(l.cout * l.kp, np.uint16)
+            l.wb.download(wb, self.stream)
+            arrays["w_" + name] = wb.reshape(l.cout, l.kp)
+        info = {"n": self.n, "channels": list(self.net.ch), "use_base": self.use_base, "meta": meta or {}}
+        np.savez_compressed(path, info=np.frombuffer(json.dumps(info).encode(), np.uint8), **arrays)
+
+    @classmethod
+    def load(cls, path, stream=None):
+        """A LearnedGuess with the weights `save` wrote (bit-identical BF16
+        weights, so the same guesses); returns (model, meta)."""
+        with np.load(path, allow_pickle=False) as z:
+            info = json.loads(bytes(z["info"]).decode())
+            model = cls(info["n"], channels=tuple(info["channels"]), use_base=info["use_base"], stream=stream)
+            for name in model.net.order:
+                l = model.net.layers[name]
+                taps = 27 if l.ksize == 3 else 1
+                wb = z["w_" + name]
+                if wb.shape != (l.cout, l.kp):
+                    raise ValueError(f"{path}: layer {name} has shape {wb.shape}, expected {(l.cout, l.kp)}")
+                w = from_bf16_bits(wb[:, :taps * l.cin]).reshape(l.cout, taps, l.cin)
+                model.net.set_weights(name, w, np.zeros(l.cout, np.float32))
+        return model, info["meta"]
+
     def propose(self, data: Ensemble, base: CoarseBase | None, k: int, s: float):
         """Writes the guess for problem k into data.u."""
         self._input(data, base, k, 0, 1.0 / s)
@@ -313,6 +344,8 @@ def main(argv=None):
     ap.add_argument("--smooth", type=int, default=128, help="weighted-Jacobi sweeps before polishing a guess")
     ap.add_argument("--seed", type=int, default=synthetic.DEFAULT_SEED)
     ap.add_argument("--log-every", type=int, default=250)
+    ap.add_argument("--save", default="", help="write the trained network's weights to this .npz file")
+    ap.add_argument("--load", default="", help="evaluate the network in this .npz file (no generation, no training)")
     a = ap.parse_args(argv)
     lgdhip.load()
     lgdhip.call("lgdhip_set_device", 0)
@@ -320,6 +353,16 @@ def main(argv=None):
     m = default_coarse_size(a.n) if a.coarse < 0 else a.coarse
     out = {"n": a.n, "coarse_m": m, "train_problems": a.train, "eval_problems": a.eval, "steps": a.steps,
            "batch": a.batch}
+
+    if a.load:
+        model, meta = LearnedGuess.load(a.load, stream=stream.h)
+        out["weights"] = {"file": a.load, **meta}
+        ev = Ensemble(synthetic.poisson_ensemble(a.eval, sizes=(model.n,), seed=a.seed + 7919), stream=stream.h)
+        out["eval"] = evaluate(model, ev, m, a.tol, smooth_sweeps=a.smooth)
+        for arm in out["eval"]["arms"]:
+            print(f"[learned] {arm['arm']:8s} sweeps {arm['sweeps_mean']:9.0f}  wall {arm['wall_s']:.3f} s", flush=True)
+        print(json.dumps(out), flush=True)
+        return out
 
     t0 = time.perf_counter()
     train = Ensemble(synthetic.poisson_ensemble(a.train, sizes=(a.n,), seed=a.seed), stream=stream.h)
@@ -344,6 +387,12 @@ def main(argv=None):
     print(f"[learned] trained {a.steps} steps in {out['train_s']:.1f} s, loss {log[0][1]:.3e} -> {log[-1][1]:.3e}",
           flush=True)
     del base, train
+    if a.save:
+        model.save(a.save, {"train_problems": a.train, "train_steps": a.steps, "batch": a.batch, "peak_lr": a.lr,
+                            "coarse_m": m, "seed": a.seed, "final_loss": log[-1][1] if log else None,
+                            "generate_s": round(out["generate_s"], 1), "train_s": round(out["train_s"], 1),
+                            "command": "python -m libgeodecomp_amd.learned_guess " + " ".join(argv or sys.argv[1:])})
+        print(f"[learned] weights written to {a.save}", flush=True)
 
     ev = Ensemble(synthetic.poisson_ensemble(a.eval, sizes=(a.n,), seed=a.seed + 7919), stream=stream.h)
     out["eval"] = evaluate(model, ev, m, a.tol, smooth_sweeps=a.smooth)

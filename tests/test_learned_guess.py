@@ -318,7 +318,7 @@ def test_smoothing_phase_damps_high_frequency_guess_error(hip):
 
 
 @pytest.mark.gpu
-def test_trained_network_beats_zero_guess(hip):
+def test_trained_network_beats_zero_guess(hip, tmp_path):
     """End to end at 32^3: solutions generated by the batched Jacobi solver,
     the U-Net trained on them by this library's kernels on top of the
     prolongated 7^3 coarse solution, then held-out problems solved from the
@@ -348,3 +348,20 @@ def test_trained_network_beats_zero_guess(hip):
         assert a["converged"] == 8 and a["final_rel_residual_max"] <= 1e-6
     assert arms["learned"]["guess_rel_error_median"] < arms["coarse"]["guess_rel_error_median"]
     assert arms["learned"]["sweeps_total"] <= 0.75 * arms["zero"]["sweeps_total"]
+    # saved and loaded (the bench's form of a trained network): the same BF16
+    # weights, so the same guess bit for bit
+    path = tmp_path / "net.npz"
+    model.save(path, {"train_steps": 600})
+    loaded, meta = LearnedGuess.load(path)
+    assert meta == {"train_steps": 600}
+    bev = CoarseBase(ev, m)
+    bev.compute()
+    s = LearnedGuess.scales(ev)
+    guesses = []
+    for mdl in (model, loaded):
+        ev.reset()
+        mdl.propose(ev, bev, 3, s[3])
+        u = np.empty(ev.elems, np.float64)
+        ev.u.download(u)
+        guesses.append(u)
+    assert np.array_equal(guesses[0], guesses[1])
