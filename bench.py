@@ -898,7 +898,8 @@ def main():
     if args.workload == "poisson":
         kw.update(use_graph=0 if args.no_graph else 1, tol=args.tol, variant=max(0, args.variant))
     if args.workload == "learned" and args.weights:
-        kw.update(weights=args.weights)
+        w = Path(args.weights)
+        kw.update(weights=str(w if w.is_absolute() or w.exists() else REPO / w))  # else relative to the repository
     wl = WORKLOADS[args.workload](**kw)
     main_entry = measure(wl, args.steps, args.warmup, not args.no_cpu_baseline, 10.0)
     secondary = []
